@@ -1,0 +1,206 @@
+"""Benchmark: particle-steps/s of the PY-BEMCS time step (Poisson included) on
+one MI355X.
+
+Workload (default "c2"): two-grid Xe+ extraction (screen +1000 V, 0.38 mm,
+r 0.95 mm; accel -200 V, 0.5 mm, r 0.55 mm; 0.6 mm gap), 1024 x 384 nodes,
+8M macro-ions + 8M macro-electrons preloaded (synthetic, seed 0), Ti 0.2 eV,
+electrons Maxwellian 5 eV with m_e = M_Xe/200, uniform neutral Xe 1e18 m^-3
+at 300 K, log-linear CEX cross-section, dt 5e-10 s, particles sorted by cell
+every 20 steps, MG-PCG Poisson to 1e-13 relative residual, 5 Picard
+iterations every 2nd step.
+
+Prints one JSON line.  The timed region is `--steps` steps after `--warmup`
+steps, bracketed by device synchronisation; `value` counts the particles
+pushed in each timed step (after injection) divided by the wall time.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(name, ions=None, electrons=None):
+    mesh = {"c2": (1024, 384, 8_000_000), "c1": (256, 96, 100_000)}[name]
+    nx, ny, npre = mesh
+    return {
+        "grids": [{"V": 1000.0, "t": 0.38, "gap": 0.6, "r": 0.95, "cham": 0.0},
+                  {"V": -200.0, "t": 0.5, "gap": 1.0, "r": 0.55, "cham": 0.0}],
+        "pitch_mm": 6.1447, "geometry": "half_hole", "V_plasma_offset": 20.0,
+        "n0_plasma": 1e17, "Te_up": 3.0, "Ti": 0.2, "Te": 5.0, "Tn": 300.0,
+        "n0": 1e18, "neutral_model": "uniform", "cx_model": "loglinear",
+        "neut_rate": 30, "sim_mode": "Trajectories", "mesh": {"nx": nx, "ny": ny},
+        "preload": {"ions": npre if ions is None else ions,
+                    "electrons": npre if electrons is None else electrons},
+        "seed": 0,
+    }
+
+
+# ------------------------------------------------------------- CPU baseline
+def cpu_worker(workload, ions, electrons, max_seconds, max_steps):
+    """Time the NumPy oracle (reference dtype semantics) on one core."""
+    from oracle.sim import OracleSimulator
+    p = workload_params(workload, ions, electrons)
+    o = OracleSimulator(mode="ref32", mcc_math="ref32")
+    o.m_e = o.m_ion / 200.0
+    o.build_domain(p)
+    pushed = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < max_seconds:
+        o.step(p)
+        pushed += o.pushed
+        steps += 1
+    wall = time.perf_counter() - t0
+    print(json.dumps({"value": pushed / wall, "steps": steps, "wall_s": wall, "pushed": pushed}))
+
+
+def cpu_baseline(workload, ions, electrons, max_seconds=20.0, max_steps=200):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    env["HIP_VISIBLE_DEVICES"] = ""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", workload, str(ions), str(electrons),
+           str(max_seconds), str(max_steps)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=max_seconds * 20 + 600)
+    if out.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    cpu = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": rec["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        "sample": (f"NumPy oracle (reference dtype semantics, SciPy SuperLU) on the same "
+                   f"{workload} mesh and physics with {ions}+{electrons} preloaded particles, "
+                   f"{rec['steps']} steps in {rec['wall_s']:.1f} s on 1 core of {cpu}"),
+    }
+
+
+# ---------------------------------------------------------------- GPU bench
+def algorithmic_bytes(name, n_ions, n_elec, n_nodes):
+    """HBM bytes one launch must move at minimum.
+    ion_step: read and write x, y, vx, vy, vz, tag (24 + 24 B per ion), read
+    the (Ex, Ey) field (8 B/node), write NGP and PPC counts (4 + 4 B/node).
+    electron_step: 20 + 20 B per electron, 8 B/node field, 4 B/node counts.
+    poisson_picard (one call = 5 Picard iterations + E field): per node,
+    counts 8 B, 5 x (V read + V write 16 B, rho read 4 B), E write 8 B,
+    V read 8 B."""
+    if name == "ion_step":
+        return 48 * n_ions + 16 * n_nodes
+    if name == "electron_step":
+        return 40 * n_elec + 12 * n_nodes
+    if name == "poisson_picard":
+        return n_nodes * (8 + 5 * 20 + 16)
+    return 0
+
+
+def run(args):
+    import numpy as np
+    import torch
+    from thruster_cx_code_amd import DigitalTwinSimulator
+
+    p = workload_params(args.workload)
+    sim = DigitalTwinSimulator()
+    sim.m_e = sim.m_ion / 200.0
+    t_build = time.perf_counter()
+    sim.build_domain(p)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    sim.advance(p, args.warmup)
+    torch.cuda.synchronize()
+    first = sim.iteration + 1
+    sim.kernel_timing = {}
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    sim.advance(p, args.steps)
+    end.record()
+    torch.cuda.synchronize()
+    wall_ms = start.elapsed_time(end)
+    from thruster_cx_code_amd.physics_engine import RING
+    slots = [(first + k) % RING for k in range(args.steps)]
+    nin = sim.ring_nin.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
+    pushed = int(nin.sum())
+    value = pushed / (wall_ms * 1e-3)
+    n_nodes = sim.nx * sim.ny
+    kstats = {}
+    for name, evs in sim.kernel_timing.items():
+        durs = [a.elapsed_time(b) for a, b in evs]
+        kstats[name] = {"launches": len(durs), "total_ms": float(sum(durs)), "avg_ms": float(np.mean(durs))}
+    dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
+    if dom in ("ion_step", "electron_step"):
+        col = 0 if dom == "ion_step" else 1
+        byts = sum(algorithmic_bytes(dom, int(r[0]), int(r[1]), n_nodes) for r in nin)
+    else:
+        byts = kstats[dom]["launches"] * algorithmic_bytes(dom, 0, 0, n_nodes)
+    achieved = byts / (kstats[dom]["total_ms"] * 1e-3) / 1e9
+    rec = sim.record()
+    out = {
+        "metric": "particle-steps/s incl. Poisson at 1024x384",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 particle state, fp64 gather/kick and Poisson (CG), fp32 MG preconditioner",
+        "data": "synthetic (seeded preload, Philox streams)",
+        "config": {"workload": ("c2: two-grid Xe+ extraction, 1024x384 nodes, 8M ions + 8M electrons "
+                                "preloaded, dt 5e-10 s, MG-PCG to 1e-13, Poisson every 2nd step")
+                   if args.workload == "c2" else "c1: 256x96 nodes, 1e5 + 1e5 preloaded",
+                   "nodes": n_nodes, "particles_pushed": pushed, "build_s": t_build,
+                   "n_sub_last": [rec["n_sub_ion"], rec["n_sub_e"]],
+                   "num_p_last": rec["num_p"], "num_e_last": rec["num_e"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kstats,
+    }
+    if not args.no_cpu_baseline:
+        ions = electrons = 100_000 if args.workload == "c2" else 100_000
+        out["cpu_baseline"] = cpu_baseline(args.workload, ions, electrons, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        w, ni, ne, secs, steps = sys.argv[2:7]
+        cpu_worker(w, int(ni), int(ne), float(secs), int(steps))
+        return
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time step shards "
+              f"across GPUs by case or by particles, which this build does not run.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps > 1000:
+        sys.exit("--steps must be <= 1000 (per-step records ring)")
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

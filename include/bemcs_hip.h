@@ -1,0 +1,282 @@
+/*
+ * bemcs_hip.h -- C ABI of the MI355X (gfx950) backend for the PY-BEMCS
+ * DigitalTwinSimulator time step.
+ *
+ * Every entry point takes plain device pointers, sizes and an explicit
+ * hipStream_t (as void*), returns 0 on success or a negative BEMCS_E* code,
+ * never throws, and records a message readable with bemcs_last_error()
+ * (thread-local).  The library keeps no global mutable state: calls on
+ * distinct buffers/handles are re-entrant.  Particle and mesh buffers are
+ * owned by the caller; only Poisson solver handles own device memory.
+ *
+ * Units follow the reference: positions in mm, velocities in m/s, fields in
+ * V/m, potentials in V, times in s.
+ */
+#ifndef BEMCS_HIP_H
+#define BEMCS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEMCS_ABI_VERSION 1
+
+#define BEMCS_OK 0
+#define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
+#define BEMCS_EHIP -2       /* HIP runtime error */
+#define BEMCS_ENOCONV -3    /* Poisson solve did not reach the tolerance */
+#define BEMCS_EOVERFLOW -4  /* a particle buffer ran out of capacity */
+
+typedef void* bemcs_stream_t; /* hipStream_t */
+
+int bemcs_abi_version(void);
+const char* bemcs_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in kernels.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces push_particles_boris_cpu / push_particles_boris_taichi
+ * (Python/physics_engine.py:188-254, :63-144): bilinear gather of E (and B
+ * when Bx/By/Bz are non-null; null means a zero field, as the reference
+ * never writes B, :432-434), Boris rotation, drift x += v*dt*1000.  Applied
+ * n_rep times in one launch, equal to n_rep successive reference calls with
+ * the same dt.  The gather and kick use the reference NumPy path's dtype
+ * promotion (float32 state, float64 arithmetic), so results are bit-identical
+ * to it on the same inputs. */
+int bemcs_push_boris(float* x, float* y, float* vx, float* vy, float* vz, int64_t n,
+                     const float* Ex, const float* Ey, const float* Bx, const float* By,
+                     const float* Bz, int32_t nx, int32_t ny, double dx, double dy,
+                     double dt, double q_m, int32_t n_rep, bemcs_stream_t stream);
+
+/* NGP cell counts (Python/physics_engine.py:178-185, :38-60): counts[iy*nx+ix]
+ * += 1 with ix = clip(rint(x/f32(dx)), 1, nx-2), likewise iy.  counts is not
+ * cleared. */
+int bemcs_deposit_ngp_counts(const float* x, const float* y, int64_t n, double dx,
+                             double dy, int32_t nx, int32_t ny, uint32_t* counts,
+                             bemcs_stream_t stream);
+
+/* rho[c] = f32(f64(rho[c]) + charge_density) applied counts[c] times, which
+ * is bit-identical to np.add.at(rho, idx, charge_density) with a Python-float
+ * charge density (:185): NumPy evaluates each indexed add that way, and every
+ * added value of one deposit is the same. */
+int bemcs_counts_to_rho(const uint32_t* counts, int64_t n_nodes, double charge_density,
+                        float* rho, bemcs_stream_t stream);
+
+/* compute_particle_substeps (:928-966): folds max(ds) of the first *n_dev
+ * particles into *ds_max_bits (the bit pattern of a non-negative double; zero
+ * it first) and writes n_sub = max(1, ceil(ds_max / ds_lim)) to *n_sub_out,
+ * ds_lim = frac*min(dx,dy)*1e-3.  dt2 is the caller's dt**2 (Python's pow,
+ * which is not always dt*dt). */
+int bemcs_substeps(const float* x, const float* y, const float* vx, const float* vy,
+                   const float* vz, const int32_t* n_dev, int64_t n_bound,
+                   const float* Exy, int32_t nx, int32_t ny, double dx, double dy,
+                   double q_m, double dt, double dt2, double frac, uint64_t* ds_max_bits,
+                   int32_t* n_sub_out, bemcs_stream_t stream);
+
+/* Poisson(lam) sample of stream (seed, purpose) at `step` -- host function,
+ * the injected-particle count of Python/physics_engine.py:1015-1018. */
+int64_t bemcs_poisson_count(uint32_t seed, uint32_t purpose, uint32_t step, double lam);
+
+/* Deterministic math, host build (for checks). */
+double bemcs_det_exp(double x);
+double bemcs_det_log(double x);
+void bemcs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ------------------------------------------------------------------------
+ * Device-resident time step.
+ * ---------------------------------------------------------------------- */
+
+/* One species, structure of arrays, capacity `cap`.  tag is the ion's
+ * persistent id with bit 31 = isCEX (Python/physics_engine.py:413); it is
+ * null for electrons.  count points to a device int32. */
+typedef struct {
+  float* x;
+  float* y;
+  float* vx;
+  float* vy;
+  float* vz;
+  uint32_t* tag;
+  int32_t* count;
+  int64_t cap;
+} bemcs_species_t;
+
+/* Per-step ion diagnostics accumulated on the device (Python/physics_engine.py
+ * :1316-1497). */
+typedef struct {
+  uint32_t entered;        /* entered_optics_step */
+  uint32_t crossed;        /* exit_count_step (also the theta95 sample size) */
+  uint32_t exited;         /* any(exited_mask) test */
+  uint32_t lost_to_grid;   /* hit & ~OOB */
+  uint32_t see_spawned;    /* SEE electrons created */
+  uint32_t see_kept;       /* ... that survived the electron purge */
+  uint32_t mcc_events;     /* CEX collisions */
+  uint32_t crossed_primary;
+  uint32_t overflow;       /* non-zero if an append exceeded capacity */
+  uint32_t pad_;
+  double sum_vx, sum_vx2, sum_v, sum_v2, sum_E;
+  double sum_v_primary, sum_v_cex;
+} bemcs_ion_diag_t;
+
+/* Fused ion step: n_sub pushes, periodic wrap, first-hit test along the
+ * 8-sample segment, OOB, entered/crossed diagnostics, SEE spawning (with the
+ * electron purge applied to the spawned electron), removal, CEX collisions,
+ * NGP counts and PPC counts of the survivors.  (:1211-1246, :1289-1529,
+ * :1682-1758, :1880-1901) */
+typedef struct {
+  const float* Exy;          /* interleaved (Ex, Ey), [ny][nx][2] */
+  const uint8_t* is_bound;   /* [ny][nx] */
+  const int32_t* n_sub;      /* device scalar */
+  uint32_t* cnt_next;        /* NGP counts of survivors [ny*nx] */
+  uint32_t* ppc;             /* PPC counts (trunc index) [ny*nx] */
+  float* crossed_angle;      /* |atan2(vy,vx)| in degrees, capacity = ion cap */
+  bemcs_ion_diag_t* diag;
+  uint32_t* e_cnt_next;      /* electron NGP counts (SEE electrons) */
+  double dt;                 /* full step; the substep is dt / n_sub */
+  double q_m;
+  double Lx, Ly;             /* mm */
+  double m_ion, q;
+  double see_vth;            /* sqrt(2 q T_see / m_e) */
+  double mcc_n0;             /* params n0 (neutral density scale) */
+  double mcc_n0a;            /* n0 * a_corr (Roy model) */
+  double mcc_rT;             /* Ly * 1e-3 */
+  double mcc_vth;            /* sqrt(2 kB Tn / m_ion) */
+  float dx32, dy32;          /* float32(dx), float32(dy) in mm */
+  float x_entry32, x_exit32, x_plume32;
+  float Lx32, Ly32;
+  float dt32;                /* float32(dt) for the SEE back-off */
+  int32_t nx, ny, periodic_y;
+  int32_t neutral_model;     /* 0 = Roy plume, 1 = uniform */
+  int32_t cx_model;          /* 0 = reference fit, 1 = log-linear */
+  int32_t mcc_enabled;
+  uint32_t seed, step;
+} bemcs_ion_step_t;
+
+int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
+                   const bemcs_species_t* e_out, const bemcs_ion_step_t* p,
+                   int64_t n_bound, bemcs_stream_t stream);
+
+/* Fused electron step: n_sub pushes, periodic wrap, purge (nearest-node hit
+ * or outside [0,Lx]x[0,Ly] or NaN, :1534-1561), removal, NGP counts. */
+typedef struct {
+  const float* Exy;
+  const uint8_t* is_bound;
+  const int32_t* n_sub;
+  uint32_t* cnt_next;
+  uint32_t* overflow;
+  double dt;
+  double q_m;
+  double Ly;
+  float dx32, dy32;
+  float Lx32, Ly32;
+  int32_t nx, ny, periodic_y;
+  int32_t pad_;
+} bemcs_electron_step_t;
+
+int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
+                        const bemcs_electron_step_t* p, int64_t n_bound,
+                        bemcs_stream_t stream);
+
+/* Injection of `n` particles at the end of `sp` (count is advanced), adding
+ * their NGP counts to cnt.  kind: 0 = beam ions (:1020-1064), 1 = neutraliser
+ * electrons (:1085-1099), 2 = RF source electrons (:1066-1083), 3 = preloaded
+ * ions, 4 = preloaded electrons (uniform over non-electrode nodes). */
+typedef struct {
+  uint8_t const* is_bound;
+  uint32_t* cnt;
+  double y_lo[2], y_hi[2];   /* sampling band(s); two_holes uses both */
+  double x0;                 /* injection x (mm) */
+  double v_drift;            /* vx offset (v_bohm) */
+  double v_spread;           /* thermal speed multiplying the normals */
+  double Lx, Ly;             /* preload box */
+  float dx32, dy32;
+  int32_t nx, ny;
+  int32_t kind;
+  int32_t two_band;          /* 1: choose band by a uniform (two_holes) */
+  int32_t band_f32;          /* 1: band limits are float32 values */
+  uint32_t seed, step;
+  uint32_t pid_base;         /* ions: persistent id of the first particle */
+  uint32_t pad_;
+} bemcs_inject_t;
+
+int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
+                 int64_t n_bound_before, bemcs_stream_t stream);
+
+/* Counting sort of a species by gather cell (iy0*nx + ix0); writes to `out`
+ * and sets out->count.  scratch must hold 2*nx*ny + 1024 uint32. */
+int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
+                       int32_t nx, int32_t ny, double dx, double dy, uint32_t* scratch,
+                       bemcs_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Poisson: matrix-free MG-PCG on the reference operator.
+ * ---------------------------------------------------------------------- */
+
+typedef struct bemcs_poisson_s* bemcs_poisson_t;
+
+/* Builds the reduced SPD system of build_sparse_matrix (:525-608) from the
+ * host isBound map (uint8 [ny][nx], 1 = electrode/Dirichlet) and the geometry
+ * mode, plus its multigrid hierarchy; allocates device memory. */
+int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny,
+                         int32_t periodic_y, bemcs_poisson_t* out);
+int bemcs_poisson_free(bemcs_poisson_t h);
+int bemcs_poisson_levels(bemcs_poisson_t h);
+
+/* Solves the full reference system A V = b (laplacian_lu(b), :900) for
+ * device f64 vectors; V holds the initial guess on entry.  Stops when
+ * ||r||_2 <= rtol * ||b_reduced||_2 on the eliminated system. */
+int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rtol,
+                        int32_t max_iter, int32_t* iters, double* relres,
+                        bemcs_stream_t stream);
+
+/* recalc_poisson (:850-902): `iterations` Picard sweeps with Boltzmann
+ * electrons, rho = ion/electron NGP counts converted as bemcs_counts_to_rho
+ * (cd_ion first, then cd_e), SuperLU replaced by MG-PCG, then E = -grad V
+ * (np.gradient, periodic fix) written interleaved to Exy (float32).  V is
+ * f64 [ny][nx] in/out; V_fixed f64; rho_out (float32) receives rho. */
+typedef struct {
+  const uint32_t* cnt_ion;
+  const uint32_t* cnt_e;
+  const double* V_fixed;
+  double* V;
+  float* rho_out;
+  float* Exy;
+  double cd_ion, cd_e;
+  double coeff;       /* (dx*1e-3)^2 / eps0 */
+  double V_plasma, Te_up, n0, q;
+  double omega;
+  double dx_m, dy_m;  /* mesh spacing in metres for the gradient */
+  double rtol;
+  int32_t iterations;
+  int32_t max_iter;
+} bemcs_picard_t;
+
+int bemcs_poisson_picard(bemcs_poisson_t h, const bemcs_picard_t* p, int32_t* iters_total,
+                         double* worst_relres, bemcs_stream_t stream);
+
+/* E = -grad V only (:871-884). */
+int bemcs_efield(const double* V, int32_t nx, int32_t ny, double dx_m, double dy_m,
+                 int32_t periodic_y, float* Exy, bemcs_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Diagnostics.
+ * ---------------------------------------------------------------------- */
+
+/* np.percentile(angles[:n], q_percent) with linear interpolation evaluated in
+ * float32 as NumPy does for float32 data (bit-identical); n read from the
+ * device; result written to *out (double), NaN when n <= min_count.
+ * scratch is unused (may be null). */
+int bemcs_percentile(const float* angles, const uint32_t* n_dev, int64_t n_bound, double q_percent,
+                     uint32_t min_count, uint32_t* scratch, double* out, bemcs_stream_t stream);
+
+/* accum += ppc; stats[0] = #cells with ppc > 0, stats[1] = #cells with
+ * 0 < ppc < 3 (:1755-1767). */
+int bemcs_ppc_stats(const uint32_t* ppc, int64_t n_nodes, double* accum, uint32_t* stats,
+                    bemcs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEMCS_HIP_H */

@@ -1,0 +1,86 @@
+"""Keyed random streams (test infrastructure, oracle side).
+
+Every random draw of the reference time step (``np.random.poisson``
+``Python/physics_engine.py:1018``; ``np.random.uniform``/``randn`` at
+``:1042-1059`` and ``:1093-1098``; SEE ``rand``/``randn`` at ``:1400-1416``;
+CEX ``rand`` at ``:1719-1734``) is replaced by a Philox4x32-10 block keyed by
+
+    key = (seed, purpose)          counter = (id, step, block, 0)
+
+so a draw depends only on what it is for, whom it is for and when.  The HIP
+kernels build the identical key/counter and convert words to numbers with the
+identical arithmetic (``thruster_cx_code_amd/csrc/bemcs_rng.h``).
+
+Pinned by: Philox known-answer vectors, moment / chi-square tests of the
+derived distributions (tests/test_oracle_rng.py) and bit equality with the HIP
+host build of the Poisson sampler.
+"""
+import math
+import numpy as np
+
+from .philox import philox4x32_10, u01_pair
+from .dmath import det_exp, box_muller
+
+# purpose tags (the same numbers appear in the HIP header)
+P_INIT_ION = 0x10
+P_INIT_ELEC = 0x11
+P_INJ_COUNT = 0x20
+P_INJ_ION = 0x21
+P_NEUT = 0x22
+P_RF_COUNT = 0x23
+P_RF_ELEC = 0x24
+P_SEE = 0x30
+P_MCC = 0x40
+
+POISSON_CHUNK = 16.0
+
+
+def block(seed, purpose, a, b, c):
+    """Philox block(s) for counter (a, b, c, 0); a/b/c broadcast."""
+    a, b, c = np.broadcast_arrays(np.asarray(a, np.uint64), np.asarray(b, np.uint64),
+                                  np.asarray(c, np.uint64))
+    ctr = np.stack([a & 0xFFFFFFFF, b & 0xFFFFFFFF, c & 0xFFFFFFFF,
+                    np.zeros_like(a)], axis=-1).astype(np.uint32)
+    key = np.array([seed & 0xFFFFFFFF, purpose & 0xFFFFFFFF], dtype=np.uint32)
+    return philox4x32_10(ctr, key)
+
+
+def uniforms(seed, purpose, a, b, c):
+    return u01_pair(block(seed, purpose, a, b, c))
+
+
+def normals(seed, purpose, a, b, c):
+    u0, u1 = uniforms(seed, purpose, a, b, c)
+    return box_muller(u0, u1)
+
+
+def poisson_count(seed, purpose, step, lam):
+    """Poisson(lam) sample as a sum of ceil(lam/16) Knuth product samples.
+
+    Uniform j of the stream is half (j & 1) of block counter (j >> 1, step, 0).
+    """
+    lam = float(lam)
+    if not lam > 0.0:
+        return 0
+    nchunk = int(math.ceil(lam / POISSON_CHUNK))
+    lam_c = lam / nchunk
+    L = float(det_exp(-lam_c))
+    count = 0
+    j = 0
+    cache_blk = -1
+    pair = (0.0, 0.0)
+    for _ in range(nchunk):
+        p = 1.0
+        while True:
+            blk = j >> 1
+            if blk != cache_blk:
+                u0, u1 = uniforms(seed, purpose, blk, step, 0)
+                pair = (float(u0), float(u1))
+                cache_blk = blk
+            u = pair[j & 1]
+            j += 1
+            p = p * u
+            if p <= L:
+                break
+            count += 1
+    return count

@@ -1,0 +1,85 @@
+"""The C-ABI library loads, exports every declared symbol, and its host-side
+functions agree bit-for-bit with the oracle (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from thruster_cx_code_amd import _lib
+from oracle.dmath import det_exp, det_log
+from oracle.philox import philox4x32_10
+from oracle import rng
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "bemcs_hip.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libbemcs_hip.so not built: run python -m thruster_cx_code_amd.build")
+    return _lib.load()
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(bemcs_[a-z0-9_]+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.EXPORTED, f"{n} has no ctypes signature"
+
+
+def test_abi_version(lib):
+    assert lib.bemcs_abi_version() == _lib.ABI_VERSION
+
+
+def test_struct_sizes_match_header_layout():
+    # sizes implied by the field lists in include/bemcs_hip.h (LP64)
+    assert C.sizeof(_lib.Species) == 7 * 8 + 8
+    assert C.sizeof(_lib.IonDiag) == 10 * 4 + 7 * 8
+    assert C.sizeof(_lib.IonStep) == 8 * 8 + 11 * 8 + 8 * 4 + 8 * 4
+    assert C.sizeof(_lib.ElectronStep) == 5 * 8 + 3 * 8 + 4 * 4 + 4 * 4
+    assert C.sizeof(_lib.Inject) == 2 * 8 + 4 * 8 + 5 * 8 + 2 * 4 + 9 * 4 + 4
+    assert C.sizeof(_lib.Picard) == 6 * 8 + 11 * 8 + 2 * 4
+
+
+def test_host_det_math_bit_equal(lib):
+    x = np.concatenate([np.linspace(-800, 800, 8001), np.random.default_rng(0).normal(0, 30, 4000)])
+    got = np.array([lib.bemcs_det_exp(float(v)) for v in x])
+    assert np.array_equal(got.view(np.uint64), det_exp(x).view(np.uint64))
+    y = np.exp(np.linspace(-700, 700, 8001))
+    got = np.array([lib.bemcs_det_log(float(v)) for v in y])
+    assert np.array_equal(got.view(np.uint64), det_log(y).view(np.uint64))
+
+
+def test_host_philox_bit_equal(lib):
+    r = np.random.default_rng(1)
+    for _ in range(200):
+        ctr = r.integers(0, 2 ** 32, 4, dtype=np.uint64).astype(np.uint32)
+        key = r.integers(0, 2 ** 32, 2, dtype=np.uint64).astype(np.uint32)
+        out = (C.c_uint32 * 4)()
+        lib.bemcs_philox4x32_10((C.c_uint32 * 4)(*ctr.tolist()), (C.c_uint32 * 2)(*key.tolist()), out)
+        assert list(out) == [int(v) for v in philox4x32_10(ctr, key)]
+
+
+@pytest.mark.parametrize("lam", [0.1, 2.5, 16.0, 16.0001, 97.3, 524.0])
+def test_host_poisson_count_bit_equal(lib, lam):
+    for step in range(60):
+        assert lib.bemcs_poisson_count(3, rng.P_INJ_COUNT, step, lam) == \
+            rng.poisson_count(3, rng.P_INJ_COUNT, step, lam)
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        _lib.load()

@@ -1,0 +1,149 @@
+// Internal helpers shared by the translation units of libbemcs_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/bemcs_hip.h"
+#include "bemcs_dmath.h"
+
+namespace bemcs {
+
+void set_error(const char* fmt, ...);
+
+#define BEMCS_CHECK_HIP(expr)                                                   \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ::bemcs::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,              \
+                         hipGetErrorString(e_));                                 \
+      return BEMCS_EHIP;                                                         \
+    }                                                                            \
+  } while (0)
+
+#define BEMCS_CHECK_LAUNCH()                                                    \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) {                                                      \
+      ::bemcs::set_error("%s:%d kernel launch: %s", __FILE__, __LINE__,          \
+                         hipGetErrorString(e_));                                 \
+      return BEMCS_EHIP;                                                         \
+    }                                                                            \
+  } while (0)
+
+#define BEMCS_REQUIRE(cond, msg)                                                \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      ::bemcs::set_error("%s", msg);                                             \
+      return BEMCS_EINVAL;                                                       \
+    }                                                                            \
+  } while (0)
+
+inline hipStream_t as_stream(bemcs_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ----------------------------------------------------------------- indices
+// NumPy cast semantics on x86: a float->int conversion that is out of range
+// or NaN yields the most negative integer, which the reference's np.clip then
+// maps to the lower bound.
+
+// np.clip(np.floor(g).astype(np.int32), 0, hi)
+__device__ __forceinline__ int floor_i32_np(float g, int hi) {
+  float f = floorf(g);
+  int i = (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT32_MIN;
+  return i < 0 ? 0 : (i > hi ? hi : i);
+}
+
+// np.clip(np.floor(g).astype(int), 0, hi)   (int64)
+__device__ __forceinline__ int floor_i64_np(float g, int hi) {
+  float f = floorf(g);
+  if (!(f >= -9.2e18f && f < 9.2e18f)) return 0;
+  return f < 0.0f ? 0 : (f > (float)hi ? hi : (int)f);
+}
+
+// np.clip(np.round(g).astype(int), lo, hi)   (int64, half to even)
+__device__ __forceinline__ int round_i64_np(float g, int lo, int hi) {
+  float f = rintf(g);
+  if (!(f >= -9.2e18f && f < 9.2e18f)) return lo;
+  return f < (float)lo ? lo : (f > (float)hi ? hi : (int)f);
+}
+
+// np.clip(np.round(g).astype(np.int32), lo, hi)
+__device__ __forceinline__ int round_i32_np(float g, int lo, int hi) {
+  float f = rintf(g);
+  int i = (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT32_MIN;
+  return i < lo ? lo : (i > hi ? hi : i);
+}
+
+// np.clip((g).astype(int), 0, hi)   (truncation, int64)
+__device__ __forceinline__ int trunc_i64_np(float g, int hi) {
+  if (!(g >= -9.2e18f && g < 9.2e18f)) return 0;
+  float t = truncf(g);
+  return t < 0.0f ? 0 : (t > (float)hi ? hi : (int)t);
+}
+
+// np.maximum(a, b) for a possibly NaN, b not NaN
+__host__ __device__ __forceinline__ double nmax(double a, double b) { return (a < b) ? b : a; }
+
+// np.mod(a, b) for float32 (npy_divmodf)
+__device__ __forceinline__ float np_modf(float a, float b) {
+  float m = fmodf(a, b);
+  if (m != 0.0f) {
+    if ((b < 0.0f) != (m < 0.0f)) m = m + b;
+  } else {
+    m = copysignf(0.0f, b);
+  }
+  return m;
+}
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Reserve one slot per lane with pred set; returns the slot or -1.
+template <typename T>
+__device__ __forceinline__ int64_t wave_append(T* counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return -1;
+  int lane = lane_id();
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  int total = __popcll(mask);
+  T base = 0;
+  if (lane == leader) base = atomicAdd(counter, (T)total);
+  base = __shfl(base, leader);
+  uint64_t below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+  int off = __popcll(below);
+  return pred ? (int64_t)base + off : -1;
+}
+
+// Count of lanes with pred set, added once per wave.
+__device__ __forceinline__ void wave_count(uint32_t* counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  if (lane_id() == (__ffsll((unsigned long long)mask) - 1)) atomicAdd(counter, (uint32_t)__popcll(mask));
+}
+
+// Histogram add of key (key < 0: none).  Runs of equal keys in consecutive
+// lanes (cell-sorted particles) are merged into one atomic.
+__device__ __forceinline__ void wave_hist_add(uint32_t* hist, int key) {
+  int lane = lane_id();
+  int prev = __shfl_up(key, 1);
+  bool head = (lane == 0) || (key != prev);
+  uint64_t heads = __ballot(head);
+  if (head && key >= 0) {
+    uint64_t after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+    int run = after ? (__ffsll((unsigned long long)after)) : (64 - lane);
+    atomicAdd(&hist[key], (uint32_t)run);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+}  // namespace bemcs

@@ -1,0 +1,829 @@
+// Particle kernels of the MI355X time step: gather + Boris push, NGP
+// deposit, substep count, fused per-species step (push, hits, SEE, CEX MCC,
+// removal, deposit and PPC counts), injection and cell sort.
+//
+// Layout: one species is a structure of float32 arrays (x, y, vx, vy, vz)
+// plus a uint32 tag (ion id | isCEX<<31); one thread owns one particle and
+// keeps it in registers for all n_sub substeps, so a step reads and writes
+// the particle state from HBM exactly once.  The field is interleaved
+// (Ex, Ey) float32 so one 8-byte load fetches both components of a corner.
+//
+// Arithmetic follows the reference NumPy path (Python/physics_engine.py
+// :188-254): float32 positions and velocities, float64 interpolation weights
+// and kicks (float32 - int32 promotes to float64 there), float32 drift.  With
+// -ffp-contract=off every result is bit-identical to it on the same inputs.
+#include "bemcs_internal.h"
+
+namespace bemcs {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ push
+template <bool INTERLEAVED>
+struct EField {
+  const float* ex;
+  const float* ey;
+  __device__ __forceinline__ void corner(int idx, float& a, float& b) const {
+    if (INTERLEAVED) {
+      float2 v = reinterpret_cast<const float2*>(ex)[idx];
+      a = v.x;
+      b = v.y;
+    } else {
+      a = ex[idx];
+      b = ey[idx];
+    }
+  }
+};
+
+// One gather + Boris + drift substep.  h = 0.5*q_m*dt (f64), dt32 = f32(dt).
+template <bool INTERLEAVED, bool HAS_B>
+__device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, float& vy, float& vz,
+                                              const EField<INTERLEAVED>& E, const float* Bx,
+                                              const float* By, const float* Bz, int nx, int ny,
+                                              float dx32, float dy32, double h, float dt32) {
+  float gx = x / dx32;
+  float gy = y / dy32;
+  int i0 = floor_i32_np(gx, nx - 2);
+  int j0 = floor_i32_np(gy, ny - 2);
+  double fx = (double)gx - (double)i0;
+  double fy = (double)gy - (double)j0;
+  int c00 = j0 * nx + i0;
+  int c10 = c00 + 1;
+  int c01 = c00 + nx;
+  int c11 = c01 + 1;
+  double a00 = (1.0 - fx) * (1.0 - fy);
+  double a10 = fx * (1.0 - fy);
+  double a01 = (1.0 - fx) * fy;
+  double a11 = fx * fy;
+  float e00x, e00y, e10x, e10y, e01x, e01y, e11x, e11y;
+  E.corner(c00, e00x, e00y);
+  E.corner(c10, e10x, e10y);
+  E.corner(c01, e01x, e01y);
+  E.corner(c11, e11x, e11y);
+  double ex = (double)e00x * a00 + (double)e10x * a10 + (double)e01x * a01 + (double)e11x * a11;
+  double ey = (double)e00y * a00 + (double)e10y * a10 + (double)e01y * a01 + (double)e11y * a11;
+  double kx = h * ex;
+  double ky = h * ey;
+  double ux = (double)vx + kx;
+  double uy = (double)vy + ky;
+  if (!HAS_B) {
+    // with B = 0 the rotation adds exact zeros: v+ = v-
+    vx = (float)(ux + kx);
+    vy = (float)(uy + ky);
+  } else {
+    double bx = (double)Bx[c00] * a00 + (double)Bx[c10] * a10 + (double)Bx[c01] * a01 +
+                (double)Bx[c11] * a11;
+    double by = (double)By[c00] * a00 + (double)By[c10] * a10 + (double)By[c01] * a01 +
+                (double)By[c11] * a11;
+    double bz = (double)Bz[c00] * a00 + (double)Bz[c10] * a10 + (double)Bz[c01] * a01 +
+                (double)Bz[c11] * a11;
+    double uz = (double)vz;
+    double tx = h * bx, ty = h * by, tz = h * bz;
+    double tt = tx * tx + ty * ty + tz * tz;
+    double sx = 2.0 * tx / (1.0 + tt);
+    double sy = 2.0 * ty / (1.0 + tt);
+    double sz = 2.0 * tz / (1.0 + tt);
+    double wx = ux + (uy * tz - uz * ty);
+    double wy = uy + (uz * tx - ux * tz);
+    double wz = uz + (ux * ty - uy * tx);
+    double px = ux + (wy * sz - wz * sy);
+    double py = uy + (wz * sx - wx * sz);
+    double pz = uz + (wx * sy - wy * sx);
+    vx = (float)(px + kx);
+    vy = (float)(py + ky);
+    vz = (float)pz;
+  }
+  x = x + (vx * dt32) * 1000.0f;
+  y = y + (vy * dt32) * 1000.0f;
+}
+
+template <bool HAS_B>
+__global__ __launch_bounds__(kBlock) void k_push_boris(float* __restrict__ x, float* __restrict__ y,
+                                                        float* __restrict__ vx, float* __restrict__ vy,
+                                                        float* __restrict__ vz, int64_t n,
+                                                        const float* Ex, const float* Ey,
+                                                        const float* Bx, const float* By,
+                                                        const float* Bz, int nx, int ny, float dx32,
+                                                        float dy32, double h, float dt32, int n_rep) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float px = x[i], py = y[i], pvx = vx[i], pvy = vy[i], pvz = vz[i];
+  EField<false> E{Ex, Ey};
+  for (int r = 0; r < n_rep; ++r)
+    boris_substep<false, HAS_B>(px, py, pvx, pvy, pvz, E, Bx, By, Bz, nx, ny, dx32, dy32, h, dt32);
+  x[i] = px;
+  y[i] = py;
+  vx[i] = pvx;
+  vy[i] = pvy;
+  vz[i] = pvz;
+}
+
+// --------------------------------------------------------------- deposit
+__global__ __launch_bounds__(kBlock) void k_deposit(const float* __restrict__ x,
+                                                     const float* __restrict__ y, int64_t n,
+                                                     const int32_t* n_dev, float dx32, float dy32,
+                                                     int nx, int ny, uint32_t* counts) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t nn = n_dev ? (int64_t)*n_dev : n;
+  int key = -1;
+  if (i < nn) {
+    int ix = round_i32_np(x[i] / dx32, 1, nx - 2);
+    int iy = round_i32_np(y[i] / dy32, 1, ny - 2);
+    key = iy * nx + ix;
+  }
+  wave_hist_add(counts, key);
+}
+
+// rho[c] = f32(f64(rho[c]) + cd), counts[c] times (np.add.at semantics)
+__global__ __launch_bounds__(kBlock) void k_counts_to_rho(const uint32_t* __restrict__ cnt,
+                                                           int64_t n, double cd, float* rho) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cnt[i];
+  if (c == 0) return;
+  float r = rho[i];
+  for (uint32_t k = 0; k < c; ++k) r = (float)((double)r + cd);
+  rho[i] = r;
+}
+
+// -------------------------------------------------------------- substeps
+// ds = |v| dt + 0.5 |qm E| dt^2 with the reference's mixed float32/float64
+// evaluation (:940-962); max over particles folded into *ds_bits (ds >= 0,
+// so the float64 bit pattern orders like the value).
+__global__ __launch_bounds__(kBlock) void k_substep_ds(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ vx,
+    const float* __restrict__ vy, const float* __restrict__ vz, const int32_t* n_dev,
+    const float* __restrict__ Exy, int nx, int ny, float dx32, float dy32, double qm, float dt32,
+    double dt2, unsigned long long* ds_bits) {
+  __shared__ unsigned long long smax[kBlock / 64];
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t n = *n_dev;
+  double ds = 0.0;
+  if (i < n) {
+    float px = x[i], py = y[i];
+    float gx = px / dx32, gy = py / dy32;
+    int i0 = floor_i64_np(gx, nx - 2);
+    int j0 = floor_i64_np(gy, ny - 2);
+    double fx = (double)gx - (double)i0;
+    double fy = (double)gy - (double)j0;
+    const float2* E2 = reinterpret_cast<const float2*>(Exy);
+    float2 e00 = E2[j0 * nx + i0], e10 = E2[j0 * nx + i0 + 1];
+    float2 e01 = E2[(j0 + 1) * nx + i0], e11 = E2[(j0 + 1) * nx + i0 + 1];
+    double gxm = 1.0 - fx, gym = 1.0 - fy;
+    double ex = (double)e00.x * gxm * gym + (double)e10.x * fx * gym + (double)e01.x * gxm * fy +
+                (double)e11.x * fx * fy;
+    double ey = (double)e00.y * gxm * gym + (double)e10.y * fx * gym + (double)e01.y * gxm * fy +
+                (double)e11.y * fx * fy;
+    double ax = qm * ex, ay = qm * ey;
+    double a_mag = sqrt(ax * ax + ay * ay);
+    float a = vx[i], b = vy[i], c = vz[i];
+    float v_mag = sqrtf(a * a + b * b + c * c);
+    ds = (double)(v_mag * dt32) + 0.5 * a_mag * dt2;
+    if (!(ds >= 0.0)) ds = 0.0;  // NaN particles do not set the substep count
+  }
+  unsigned long long bits = (unsigned long long)__double_as_longlong(ds);
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long other = __shfl_xor(bits, o);
+    bits = other > bits ? other : bits;
+  }
+  if (lane_id() == 0) smax[threadIdx.x / 64] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = smax[0];
+    for (int w = 1; w < kBlock / 64; ++w) m = smax[w] > m ? smax[w] : m;
+    if (m) atomicMax(ds_bits, m);
+  }
+}
+
+__global__ void k_substep_finalize(const unsigned long long* ds_bits, const int32_t* n_dev,
+                                   double ds_lim, int32_t* n_sub) {
+  double ds_max = __longlong_as_double((long long)*ds_bits);
+  int ns = 1;
+  if (*n_dev > 0) {
+    double lim = ds_lim > 1e-30 ? ds_lim : 1e-30;
+    double c = ceil(ds_max / lim);
+    ns = c > 1.0 ? (int)c : 1;
+  }
+  *n_sub = ns;
+}
+
+// ---------------------------------------------------------- fused ion step
+template <bool PERIODIC>
+__global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_species_t out,
+                                                      bemcs_species_t eout, bemcs_ion_step_t p) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t n = *in.count;
+  const bool active = i < n;
+  float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+  uint32_t tag = 0;
+  if (active) {
+    x = in.x[i];
+    y = in.y[i];
+    vx = in.vx[i];
+    vy = in.vy[i];
+    vz = in.vz[i];
+    tag = in.tag[i];
+  }
+  const float x0 = x, y0 = y;
+  const int nx = p.nx, ny = p.ny;
+  const uint8_t* __restrict__ bound = p.is_bound;
+
+  // ---- C. push (n_sub substeps, dt/n_sub)
+  {
+    const int nsub = *p.n_sub;
+    const double dts = p.dt / (double)nsub;
+    const double h = 0.5 * p.q_m * dts;
+    const float dts32 = (float)dts;
+    EField<true> E{p.Exy, p.Exy + 1};
+    if (active)
+      for (int s = 0; s < nsub; ++s)
+        boris_substep<true, false>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, p.dx32,
+                                   p.dy32, h, dts32);
+  }
+  if (PERIODIC && active) y = np_modf(y, p.Ly32);
+
+  // ---- D. hits, OOB, diagnostics
+  bool hit = false, oob = false;
+  if (active) {
+    int ix = round_i64_np(x / p.dx32, 0, nx - 1);
+    int iy = round_i64_np(y / p.dy32, 0, ny - 1);
+    hit = bound[iy * nx + ix] != 0;
+    if (!hit) {
+      for (int k = 1; k <= 8; ++k) {
+        float f = (float)k / 8.0f;
+        float xi = x0 + (x - x0) * f;
+        float yi = y0 + (y - y0) * f;
+        int cx = round_i64_np(xi / p.dx32, 0, nx - 1);
+        int cy = round_i64_np(yi / p.dy32, 0, ny - 1);
+        if (bound[cy * nx + cx]) {
+          hit = true;
+          break;
+        }
+      }
+    }
+    oob = (x < 0.0f) || (x > p.x_plume32) || isnan(x);
+    if (!PERIODIC) oob = oob || (y < 0.0f) || (y > p.Ly32);
+  }
+  const bool entered = active && (x0 < p.x_entry32) && (x >= p.x_entry32) && (vx > 0.0f);
+  wave_count(&p.diag->entered, entered);
+  wave_count(&p.diag->lost_to_grid, active && hit && !oob);
+
+  // SEE: electrons spawned by grid hits (:1390-1417), purge-checked (:1541-1551)
+  {
+    bool spawn = false;
+    float ex = 0.f, ey = 0.f, evx = 0.f, evy = 0.f, evz = 0.f;
+    if (active && hit && x > 0.25f) {
+      double a = vx, b = vy, c = vz;
+      double v2 = a * a + b * b + c * c;
+      double E_eV = (0.5 * p.m_ion * v2) / p.q;
+      double gamma = 0.05 + 1e-4 * E_eV;
+      gamma = gamma < 0.0 ? 0.0 : (gamma > 1.0 ? 1.0 : gamma);
+      const uint32_t pid = tag & 0x7FFFFFFFu;
+      u01x2 u = uniforms(p.seed, P_SEE, pid, p.step, 0u);
+      if (u.a < gamma) {
+        spawn = true;
+        ex = x - ((vx * p.dt32) * 1000.0f) / 1.5f;
+        ey = y - ((vy * p.dt32) * 1000.0f) / 1.5f;
+        u01x2 z01 = normals(p.seed, P_SEE, pid, p.step, 1u);
+        u01x2 z23 = normals(p.seed, P_SEE, pid, p.step, 2u);
+        evx = (float)(z01.a * p.see_vth);
+        evy = (float)(z01.b * p.see_vth);
+        evz = (float)(z23.a * p.see_vth);
+      }
+    }
+    wave_count(&p.diag->see_spawned, spawn);
+    bool keep_e = false;
+    int ekey = -1;
+    if (spawn) {
+      int eix = round_i64_np(ex / p.dx32, 0, nx - 1);
+      int eiy = round_i64_np(ey / p.dy32, 0, ny - 1);
+      bool ehit = bound[eiy * nx + eix] != 0;
+      bool eout = (ex < 0.0f) || (ex > p.Lx32) || (ey < 0.0f) || (ey > p.Ly32) || isnan(ex);
+      keep_e = !(ehit || eout);
+      if (keep_e) {
+        int dix = round_i32_np(ex / p.dx32, 1, nx - 2);
+        int diy = round_i32_np(ey / p.dy32, 1, ny - 2);
+        ekey = diy * nx + dix;
+      }
+    }
+    int64_t slot = wave_append(eout.count, keep_e);
+    if (keep_e) {
+      if (slot < eout.cap) {
+        eout.x[slot] = ex;
+        eout.y[slot] = ey;
+        eout.vx[slot] = evx;
+        eout.vy[slot] = evy;
+        eout.vz[slot] = evz;
+      } else {
+        atomicOr(&p.diag->overflow, 1u);
+        ekey = -1;
+      }
+    }
+    wave_count(&p.diag->see_kept, keep_e);
+    wave_hist_add(p.e_cnt_next, ekey);
+  }
+
+  // exit statistics and theta95 samples (:1451-1488)
+  const bool crossed = active && (x0 <= p.x_exit32) && (x > p.x_exit32) && (vx > 0.0f);
+  const bool exited = active && (x > p.x_exit32) && (vx > 0.0f);
+  wave_count(&p.diag->exited, exited);
+  {
+    int64_t slot = wave_append(&p.diag->crossed, crossed);
+    double s_vx = 0, s_vx2 = 0, s_v = 0, s_v2 = 0, s_E = 0, s_vp = 0, s_vc = 0;
+    bool prim_c = false;
+    if (crossed) {
+      p.crossed_angle[slot] = (fabsf(atan2f(vy, vx)) * 180.0f) / 3.14159265358979323846f;
+      float v = sqrtf(vx * vx + vy * vy + vz * vz);
+      float E = ((float)(0.5 * p.m_ion) * (v * v)) / (float)p.q;
+      s_vx = vx;
+      s_vx2 = (double)vx * vx;
+      s_v = v;
+      s_v2 = (double)v * v;
+      s_E = E;
+      prim_c = (tag >> 31) == 0;
+      if (prim_c) s_vp = v; else s_vc = v;
+    }
+    wave_count(&p.diag->crossed_primary, prim_c);
+    if (__ballot(crossed)) {
+      s_vx = wave_sum(s_vx);
+      s_vx2 = wave_sum(s_vx2);
+      s_v = wave_sum(s_v);
+      s_v2 = wave_sum(s_v2);
+      s_E = wave_sum(s_E);
+      s_vp = wave_sum(s_vp);
+      s_vc = wave_sum(s_vc);
+      if (lane_id() == 0) {
+        atomicAdd(&p.diag->sum_vx, s_vx);
+        atomicAdd(&p.diag->sum_vx2, s_vx2);
+        atomicAdd(&p.diag->sum_v, s_v);
+        atomicAdd(&p.diag->sum_v2, s_v2);
+        atomicAdd(&p.diag->sum_E, s_E);
+        atomicAdd(&p.diag->sum_v_primary, s_vp);
+        atomicAdd(&p.diag->sum_v_cex, s_vc);
+      }
+    }
+  }
+
+  // removal (:1499-1529); the keep test is implied by the OOB test but kept
+  bool keep = active && !(hit || oob);
+  keep = keep && ((x <= p.x_plume32) || (vx < 0.0f));
+
+  // ---- H. CEX collisions (:1682-1739), float64 deterministic probability
+  bool coll = false;
+  if (keep && p.mcc_enabled && (tag >> 31) == 0 && x >= 1.0f && x <= p.Lx32) {
+    double x64 = x, y64 = y;
+    double nloc;
+    if (p.neutral_model == 1) {
+      nloc = p.mcc_n0;
+    } else {
+      double z = nmax((x64 - 1.0) * 1e-3, 0.0);
+      double r = y64 * 1e-3;
+      double zr = z + p.mcc_rT;
+      double R = sqrt(r * r + zr * zr);
+      double cos_t = zr / R;
+      double qq = p.mcc_rT / nmax(R, 1e-12);
+      nloc = (p.mcc_n0a * (1.0 - 1.0 / sqrt(1.0 + qq * qq))) * cos_t;
+      nloc = nmax(nloc, 0.0);
+    }
+    double a = vx, b = vy, c = vz;
+    double g = nmax(sqrt(a * a + b * b + c * c), 1.0);
+    double sigma;
+    if (p.cx_model == 1) {
+      double E = (0.5 * p.m_ion * (g * g)) / p.q;
+      sigma = nmax((87.3 - 13.6 * (det_log(E) * kInvLn10)) * 1e-20, 0.0);
+    } else {
+      double t = -0.8821 * det_log(g) + 15.1262;
+      sigma = (t * t) * 1e-20;
+    }
+    double P = 1.0 - det_exp(-(((nloc * sigma) * g) * p.dt));
+    const uint32_t pid = tag & 0x7FFFFFFFu;
+    u01x2 u = uniforms(p.seed, P_MCC, pid, p.step, 0u);
+    if (u.a < P) {
+      coll = true;
+      double R[10];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        u01x2 w = uniforms(p.seed, P_MCC, pid, p.step, (uint32_t)(k + 1));
+        R[2 * k] = w.a;
+        R[2 * k + 1] = w.b;
+      }
+      vx = (float)(p.mcc_vth * (2.0 * (R[0] + R[1] + R[2] - 1.5)));
+      vy = (float)(p.mcc_vth * (2.0 * (R[3] + R[4] + R[5] - 1.5)));
+      vz = (float)(p.mcc_vth * (2.0 * (R[6] + R[7] + R[8] - 1.5)));
+      tag |= 0x80000000u;
+    }
+  }
+  wave_count(&p.diag->mcc_events, coll);
+
+  // ---- write survivors, NGP counts for the next deposit, PPC counts
+  int64_t slot = wave_append(out.count, keep);
+  int dkey = -1, pkey = -1;
+  if (keep) {
+    if (slot < out.cap) {
+      out.x[slot] = x;
+      out.y[slot] = y;
+      out.vx[slot] = vx;
+      out.vy[slot] = vy;
+      out.vz[slot] = vz;
+      out.tag[slot] = tag;
+      int dix = round_i32_np(x / p.dx32, 1, nx - 2);
+      int diy = round_i32_np(y / p.dy32, 1, ny - 2);
+      dkey = diy * nx + dix;
+      int pix = trunc_i64_np(x / p.dx32, nx - 1);
+      int piy = trunc_i64_np(y / p.dy32, ny - 1);
+      pkey = piy * nx + pix;
+    } else {
+      atomicOr(&p.diag->overflow, 2u);
+    }
+  }
+  wave_hist_add(p.cnt_next, dkey);
+  wave_hist_add(p.ppc, pkey);
+}
+
+// ------------------------------------------------------ fused electron step
+template <bool PERIODIC>
+__global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, bemcs_species_t out,
+                                                           bemcs_electron_step_t p) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t n = *in.count;
+  const bool active = i < n;
+  float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+  if (active) {
+    x = in.x[i];
+    y = in.y[i];
+    vx = in.vx[i];
+    vy = in.vy[i];
+    vz = in.vz[i];
+  }
+  const int nx = p.nx, ny = p.ny;
+  {
+    const int nsub = *p.n_sub;
+    const double dts = p.dt / (double)nsub;
+    const double h = 0.5 * p.q_m * dts;
+    const float dts32 = (float)dts;
+    EField<true> E{p.Exy, p.Exy + 1};
+    if (active)
+      for (int s = 0; s < nsub; ++s)
+        boris_substep<true, false>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, p.dx32,
+                                   p.dy32, h, dts32);
+  }
+  if (PERIODIC && active) y = np_modf(y, p.Ly32);
+  bool keep = false;
+  if (active) {
+    int ix = round_i64_np(x / p.dx32, 0, nx - 1);
+    int iy = round_i64_np(y / p.dy32, 0, ny - 1);
+    bool hit = p.is_bound[iy * nx + ix] != 0;
+    bool outb = (x < 0.0f) || (x > p.Lx32) || (y < 0.0f) || (y > p.Ly32) || isnan(x);
+    keep = !(hit || outb);
+  }
+  int64_t slot = wave_append(out.count, keep);
+  int dkey = -1;
+  if (keep) {
+    if (slot < out.cap) {
+      out.x[slot] = x;
+      out.y[slot] = y;
+      out.vx[slot] = vx;
+      out.vy[slot] = vy;
+      out.vz[slot] = vz;
+      int dix = round_i32_np(x / p.dx32, 1, nx - 2);
+      int diy = round_i32_np(y / p.dy32, 1, ny - 2);
+      dkey = diy * nx + dix;
+    } else {
+      atomicOr(p.overflow, 4u);
+    }
+  }
+  wave_hist_add(p.cnt_next, dkey);
+}
+
+// --------------------------------------------------------------- injection
+__global__ __launch_bounds__(kBlock) void k_inject(bemcs_species_t sp, bemcs_inject_t p, int64_t n,
+                                                    int64_t base_max) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t base = *sp.count;
+  int key = -1;
+  if (i < n && base + i < sp.cap) {
+    const uint32_t id = (uint32_t)i;
+    float x, y;
+    double vx, vy, vz;
+    if (p.kind == 3 || p.kind == 4) {
+      const uint32_t purpose = p.kind == 3 ? P_INIT_ION : P_INIT_ELEC;
+      x = 0.f;
+      y = 0.f;
+      for (uint32_t t = 0; t < 32; ++t) {
+        u01x2 u = uniforms(p.seed, purpose, id, t, 0u);
+        float cx = (float)(p.Lx * u.a);
+        float cy = (float)(p.Ly * u.b);
+        int bi = round_i64_np(cx / p.dx32, 0, p.nx - 1);
+        int bj = round_i64_np(cy / p.dy32, 0, p.ny - 1);
+        x = cx;
+        y = cy;
+        if (!p.is_bound[bj * p.nx + bi]) break;
+      }
+      u01x2 z01 = normals(p.seed, purpose, id, 0u, 1u);
+      u01x2 z23 = normals(p.seed, purpose, id, 0u, 2u);
+      if (p.kind == 3) {
+        vx = p.v_drift + z01.a * p.v_spread;
+        vy = z01.b * p.v_spread;
+        vz = z23.a * p.v_spread;
+      } else {
+        vx = z01.a * p.v_spread;
+        vy = z01.b * p.v_spread;
+        vz = z23.a * p.v_spread;
+      }
+    } else {
+      const uint32_t purpose = p.kind == 0 ? P_INJ_ION : (p.kind == 1 ? P_NEUT : P_RF_ELEC);
+      u01x2 u = uniforms(p.seed, purpose, id, p.step, 0u);
+      int band = (p.two_band && u.b >= 0.5) ? 1 : 0;
+      double lo = p.y_lo[band], hi = p.y_hi[band];
+      double span = p.band_f32 ? (double)((float)hi - (float)lo) : hi - lo;
+      y = (float)(lo + span * u.a);
+      x = (float)p.x0;
+      u01x2 z01 = normals(p.seed, purpose, id, p.step, 1u);
+      u01x2 z23 = normals(p.seed, purpose, id, p.step, 2u);
+      if (p.kind == 0) {
+        vx = p.v_drift + z01.a * p.v_spread;
+      } else if (p.kind == 2) {
+        vx = fabs(z01.a) * p.v_spread + p.v_drift;
+      } else {
+        vx = z01.a * p.v_spread;
+      }
+      vy = z01.b * p.v_spread;
+      vz = z23.a * p.v_spread;
+    }
+    const int64_t s = base + i;
+    sp.x[s] = x;
+    sp.y[s] = y;
+    sp.vx[s] = (float)vx;
+    sp.vy[s] = (float)vy;
+    sp.vz[s] = (float)vz;
+    if (sp.tag) sp.tag[s] = p.pid_base + id;
+    if (p.cnt) {
+      int dix = round_i32_np(x / p.dx32, 1, p.nx - 2);
+      int diy = round_i32_np(y / p.dy32, 1, p.ny - 2);
+      key = diy * p.nx + dix;
+    }
+  }
+  if (p.cnt) wave_hist_add(p.cnt, key);
+}
+
+__global__ void k_add_count(int32_t* count, int64_t n, int64_t cap, uint32_t* overflow) {
+  int64_t c = (int64_t)*count + n;
+  if (c > cap) {
+    if (overflow) atomicOr(overflow, 8u);
+    c = cap;
+  }
+  *count = (int32_t)c;
+}
+
+// --------------------------------------------------------------- cell sort
+__global__ __launch_bounds__(kBlock) void k_cell_hist(const float* __restrict__ x,
+                                                       const float* __restrict__ y,
+                                                       const int32_t* n_dev, float dx32, float dy32,
+                                                       int nx, int ny, uint32_t* hist,
+                                                       uint32_t* keys) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int key = -1;
+  if (i < *n_dev) {
+    int i0 = floor_i32_np(x[i] / dx32, nx - 2);
+    int j0 = floor_i32_np(y[i] / dy32, ny - 2);
+    key = j0 * nx + i0;
+    keys[i] = (uint32_t)key;
+  }
+  wave_hist_add(hist, key);
+}
+
+// exclusive scan in place over n entries: per-block scan of 1024 entries
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* data, int64_t n, uint32_t* sums) {
+  __shared__ uint32_t s[1024];
+  int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint32_t v = i < n ? data[i] : 0u;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) data[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* sums, int nb) {
+  __shared__ uint32_t s[1024];
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + (int)threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = carry + s[threadIdx.x] - v;
+    uint32_t tot = s[1023];
+    __syncthreads();
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(uint32_t* data, int64_t n, const uint32_t* sums) {
+  int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) data[i] += sums[blockIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_species_t in, bemcs_species_t out,
+                                                          const uint32_t* __restrict__ keys,
+                                                          uint32_t* offs) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t n = *in.count;
+  if (i >= n) return;
+  uint32_t d = atomicAdd(&offs[keys[i]], 1u);
+  out.x[d] = in.x[i];
+  out.y[d] = in.y[i];
+  out.vx[d] = in.vx[i];
+  out.vy[d] = in.vy[i];
+  out.vz[d] = in.vz[i];
+  if (in.tag) out.tag[d] = in.tag[i];
+}
+
+__global__ void k_copy_count(const int32_t* src, int32_t* dst) { *dst = *src; }
+
+int scan_exclusive(uint32_t* data, int64_t n, uint32_t* sums, hipStream_t s) {
+  int nb = (int)((n + 1023) / 1024);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, s, data, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, data, n, sums);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+}  // namespace bemcs
+
+using namespace bemcs;
+
+extern "C" {
+
+int bemcs_push_boris(float* x, float* y, float* vx, float* vy, float* vz, int64_t n,
+                     const float* Ex, const float* Ey, const float* Bx, const float* By,
+                     const float* Bz, int32_t nx, int32_t ny, double dx, double dy, double dt,
+                     double q_m, int32_t n_rep, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n >= 0 && nx >= 2 && ny >= 2 && n_rep >= 0, "bemcs_push_boris: bad sizes");
+  if (n == 0 || n_rep == 0) return BEMCS_OK;
+  BEMCS_REQUIRE(x && y && vx && vy && vz && Ex && Ey, "bemcs_push_boris: null pointer");
+  bool has_b = Bx || By || Bz;
+  BEMCS_REQUIRE(!has_b || (Bx && By && Bz), "bemcs_push_boris: B needs all three components");
+  double h = 0.5 * q_m * dt;
+  float dt32 = (float)dt;
+  hipStream_t s = as_stream(stream);
+  if (has_b)
+    hipLaunchKernelGGL(k_push_boris<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, x, y, vx,
+                       vy, vz, n, Ex, Ey, Bx, By, Bz, nx, ny, (float)dx, (float)dy, h, dt32, n_rep);
+  else
+    hipLaunchKernelGGL(k_push_boris<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, x, y, vx,
+                       vy, vz, n, Ex, Ey, Bx, By, Bz, nx, ny, (float)dx, (float)dy, h, dt32, n_rep);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_deposit_ngp_counts(const float* x, const float* y, int64_t n, double dx, double dy,
+                             int32_t nx, int32_t ny, uint32_t* counts, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n >= 0 && nx >= 3 && ny >= 3, "bemcs_deposit_ngp_counts: bad sizes");
+  if (n == 0) return BEMCS_OK;
+  BEMCS_REQUIRE(x && y && counts, "bemcs_deposit_ngp_counts: null pointer");
+  hipLaunchKernelGGL(k_deposit, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), x, y,
+                     n, (const int32_t*)nullptr, (float)dx, (float)dy, nx, ny, counts);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_counts_to_rho(const uint32_t* counts, int64_t n_nodes, double charge_density, float* rho,
+                        bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n_nodes >= 0 && counts && rho, "bemcs_counts_to_rho: bad arguments");
+  if (n_nodes == 0) return BEMCS_OK;
+  hipLaunchKernelGGL(k_counts_to_rho, dim3(grid_for(n_nodes, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), counts, n_nodes, charge_density, rho);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_substeps(const float* x, const float* y, const float* vx, const float* vy,
+                   const float* vz, const int32_t* n_dev, int64_t n_bound, const float* Exy,
+                   int32_t nx, int32_t ny, double dx, double dy, double q_m, double dt,
+                   double dt2, double frac, uint64_t* ds_max_bits, int32_t* n_sub_out,
+                   bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n_dev && Exy && ds_max_bits && n_sub_out && nx >= 2 && ny >= 2,
+                "bemcs_substeps: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (n_bound > 0)
+    hipLaunchKernelGGL(k_substep_ds, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, x, y, vx,
+                       vy, vz, n_dev, Exy, nx, ny, (float)dx, (float)dy, q_m, (float)dt, dt2,
+                       (unsigned long long*)ds_max_bits);
+  double ds_lim = frac * (dx < dy ? dx : dy) * 1e-3;
+  hipLaunchKernelGGL(k_substep_finalize, dim3(1), dim3(1), 0, s,
+                     (const unsigned long long*)ds_max_bits, n_dev, ds_lim, n_sub_out);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int64_t bemcs_poisson_count(uint32_t seed, uint32_t purpose, uint32_t step, double lam) {
+  return poisson_count(seed, purpose, step, lam);
+}
+
+double bemcs_det_exp(double x) { return det_exp(x); }
+double bemcs_det_log(double x) { return det_log(x); }
+
+void bemcs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  u32x4 r = philox4x32_10(u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  out[3] = r.w;
+}
+
+int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
+                   const bemcs_species_t* e_out, const bemcs_ion_step_t* p, int64_t n_bound,
+                   bemcs_stream_t stream) {
+  BEMCS_REQUIRE(in && out && e_out && p, "bemcs_ion_step: null argument");
+  BEMCS_REQUIRE(in->tag && out->tag && in->count && out->count && e_out->count,
+                "bemcs_ion_step: species buffers incomplete");
+  BEMCS_REQUIRE(p->Exy && p->is_bound && p->n_sub && p->cnt_next && p->ppc && p->crossed_angle &&
+                    p->diag && p->e_cnt_next,
+                "bemcs_ion_step: null buffer in parameters");
+  BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_ion_step: mesh too small");
+  BEMCS_REQUIRE(n_bound <= in->cap && out->cap >= n_bound, "bemcs_ion_step: capacity below bound");
+  if (n_bound <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  if (p->periodic_y)
+    hipLaunchKernelGGL(k_ion_step<true>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in,
+                       *out, *e_out, *p);
+  else
+    hipLaunchKernelGGL(k_ion_step<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in,
+                       *out, *e_out, *p);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
+                        const bemcs_electron_step_t* p, int64_t n_bound, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(in && out && p && in->count && out->count, "bemcs_electron_step: null argument");
+  BEMCS_REQUIRE(p->Exy && p->is_bound && p->n_sub && p->cnt_next && p->overflow,
+                "bemcs_electron_step: null buffer in parameters");
+  BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_electron_step: mesh too small");
+  BEMCS_REQUIRE(n_bound <= in->cap, "bemcs_electron_step: capacity below bound");
+  if (n_bound <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  if (p->periodic_y)
+    hipLaunchKernelGGL(k_electron_step<true>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s,
+                       *in, *out, *p);
+  else
+    hipLaunchKernelGGL(k_electron_step<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s,
+                       *in, *out, *p);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
+                 int64_t n_bound_before, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(sp && p && sp->count, "bemcs_inject: null argument");
+  BEMCS_REQUIRE(p->kind >= 0 && p->kind <= 4, "bemcs_inject: bad kind");
+  BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_inject: mesh too small");
+  BEMCS_REQUIRE((p->kind != 3 && p->kind != 4) || p->is_bound, "bemcs_inject: preload needs is_bound");
+  BEMCS_REQUIRE(n_bound_before + n <= sp->cap, "bemcs_inject: capacity exceeded");
+  if (n <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_inject, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, *sp, *p, n,
+                     n_bound_before);
+  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, sp->count, n, sp->cap,
+                     (uint32_t*)nullptr);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
+                       int32_t nx, int32_t ny, double dx, double dy, uint32_t* scratch,
+                       bemcs_stream_t stream) {
+  BEMCS_REQUIRE(in && out && scratch && in->count && out->count, "bemcs_sort_by_cell: null argument");
+  BEMCS_REQUIRE(out->cap >= n_bound && in->cap >= n_bound, "bemcs_sort_by_cell: capacity");
+  hipStream_t s = as_stream(stream);
+  int64_t ncell = (int64_t)nx * ny;
+  uint32_t* hist = scratch;
+  uint32_t* sums = scratch + ncell;
+  uint32_t* keys = sums + 1024 + ((ncell + 1023) / 1024);
+  BEMCS_CHECK_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * ncell, s));
+  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, s, in->count, out->count);
+  if (n_bound <= 0) return BEMCS_OK;
+  hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, in->x, in->y,
+                     in->count, (float)dx, (float)dy, nx, ny, hist, keys);
+  int rc = scan_exclusive(hist, ncell, sums, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out,
+                     keys, hist);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+}  // extern "C"

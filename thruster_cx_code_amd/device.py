@@ -1,0 +1,72 @@
+"""Device plumbing: torch-ROCm buffers and raw pointers for the C ABI."""
+import ctypes as C
+
+import torch
+
+from . import _lib
+
+
+def require_gpu():
+    """The backend has no CPU fallback: fail loudly without a HIP device."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("thruster_cx_code_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    _lib.load()
+
+
+def ptr(t):
+    """Raw device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def empty(n, dtype, device):
+    return torch.empty(int(n), dtype=dtype, device=device)
+
+
+def zeros(n, dtype, device):
+    return torch.zeros(int(n), dtype=dtype, device=device)
+
+
+class SpeciesBuffers:
+    """Structure-of-arrays particle storage of one species (capacity `cap`)
+    with a device count.  Ions carry a uint32 tag: persistent id | isCEX<<31."""
+
+    def __init__(self, cap, device, with_tag):
+        self.cap = int(cap)
+        self.device = device
+        self.x = empty(cap, torch.float32, device)
+        self.y = empty(cap, torch.float32, device)
+        self.vx = empty(cap, torch.float32, device)
+        self.vy = empty(cap, torch.float32, device)
+        self.vz = empty(cap, torch.float32, device)
+        self.tag = empty(cap, torch.int32, device) if with_tag else None
+        self.count = zeros(1, torch.int32, device)
+
+    def struct(self):
+        return _lib.Species(ptr(self.x), ptr(self.y), ptr(self.vx), ptr(self.vy), ptr(self.vz),
+                            ptr(self.tag), ptr(self.count), self.cap)
+
+    def arrays(self):
+        out = [self.x, self.y, self.vx, self.vy, self.vz]
+        if self.tag is not None:
+            out.append(self.tag)
+        return out
+
+    def grow(self, new_cap, n_valid):
+        """Reallocate with a larger capacity, keeping the first n_valid entries."""
+        new_cap = int(new_cap)
+        for name in ("x", "y", "vx", "vy", "vz", "tag"):
+            old = getattr(self, name)
+            if old is None:
+                continue
+            new = torch.empty(new_cap, dtype=old.dtype, device=self.device)
+            if n_valid:
+                new[:n_valid].copy_(old[:n_valid])
+            setattr(self, name, new)
+        self.cap = new_cap

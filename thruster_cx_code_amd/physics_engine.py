@@ -1,0 +1,836 @@
+"""DigitalTwinSimulator on MI355X: the PY-BEMCS time step with device-resident
+state (the "hip" kernel backend).
+
+Same public API as the reference class (Python/physics_engine.py:309-1963):
+``build_domain(params, preserve_state=False)``, ``step(params)`` returning
+``(remeshed, min_pot, current_div, T_grids, trans_last_frame)``, and the state
+attributes ``p_x .. p_vz, p_isCEX, num_p, e_*, num_e, V, Ex, Ey, rho, isBound,
+mask_grids`` (host copies made on access).  ``advance(params, n)`` runs n steps
+without any host synchronisation for throughput use.
+
+Per step on the device (one HIP stream):
+  inject (counts drawn on the host from the keyed Philox stream)
+  -> Poisson every 2nd step (MG-PCG Picard loop, E field)
+  -> substep counts -> fused ion kernel and fused electron kernel
+  -> PPC statistics, theta95 percentile, saddle potential into a ring of
+     per-step records -> periodic cell sort.
+The random streams, the CEX probability in float64 and the additive
+``params`` keys (mesh, preload, seed, neutral_model, cx_model) are described
+in DESIGN.md-independent terms in the module docstrings of the kernels.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+from .device import SpeciesBuffers, ptr, stream_ptr, require_gpu, zeros
+
+_NP_FP = np.float32
+
+P_INJ_COUNT = 0x20
+P_RF_COUNT = 0x23
+
+RING = 1024  # per-step diagnostic records kept on the device
+
+MATERIAL_PRESETS = {
+    'Molybdenum': {'k': 138.0, 'rho': 10280.0, 'cp': 250.0, 'emissivity': 0.80,
+                   'alpha': 4.8e-6, 'E_mod': 329e9, 'Y_coeff': 1.05e-4, 'E_th': 30.0},
+    'Steel (SS316)': {'k': 16.3, 'rho': 8000.0, 'cp': 500.0, 'emissivity': 0.60,
+                      'alpha': 16.0e-6, 'E_mod': 193e9, 'Y_coeff': 2.8e-4, 'E_th': 25.0},
+    'Titanium': {'k': 21.9, 'rho': 4507.0, 'cp': 520.0, 'emissivity': 0.50,
+                 'alpha': 8.6e-6, 'E_mod': 116e9, 'Y_coeff': 1.8e-4, 'E_th': 20.0},
+    'Graphite': {'k': 120.0, 'rho': 2200.0, 'cp': 710.0, 'emissivity': 0.85,
+                 'alpha': 3.0e-6, 'E_mod': 11e9, 'Y_coeff': 3.5e-4, 'E_th': 15.0},
+}
+
+
+class DigitalTwinSimulator:
+    """Device-resident PY-BEMCS simulator (reference: physics_engine.py:309)."""
+
+    kernel_backend = "hip"
+
+    def __init__(self, device=None, sort_interval=20, poisson_rtol=1e-13, poisson_max_iter=200):
+        require_gpu()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.lib = _lib.load()
+        self.sort_interval = int(sort_interval)
+        self.poisson_rtol = float(poisson_rtol)
+        self.poisson_max_iter = int(poisson_max_iter)
+        # physical constants and defaults as the reference (:310-401)
+        self.dt = 5e-10
+        self.injection_stop_time = None
+        self.injection_enabled = True
+        self.q = 1.602e-19
+        self.m_XE = 131.293 * 1.6605e-27
+        self.m_ion = self.m_XE
+        self.Z_ion = 1
+        self.q_ion = self.q
+        self.kB = 1.380649e-23
+        self.eps0 = 8.854e-12
+        self.m_e = self.m_XE / 100.0
+        self.user_cs = {}
+        self.MATERIAL_PRESETS = MATERIAL_PRESETS
+        self.set_material('Molybdenum')
+        self.macro_weight = 3e5
+        self.sb_sigma = 5.67e-8
+        self.thermal_accel = 1e7
+        self.Lx, self.Ly = 3, 3
+        self.dx, self.dy = 0.02, 0.02
+        self.nx = int(self.Lx / self.dx) + 1
+        self.ny = int(self.Ly / self.dy) + 1
+        self.dx = self.Lx / (self.nx - 1)
+        self.dy = self.Ly / (self.ny - 1)
+        self.T_grids = []
+        self.mask_grids = []
+        self.iteration = 0
+        self.seed = 0
+        self._poisson = None
+        self._built = False
+        self.injected_ions = 0.0
+        self.transmitted_ions = 0.0
+        self.entered_optics = 0.0
+        self.ppc_steps_count = 0
+        self.last_poisson_iters = 0
+        self.last_poisson_relres = 0.0
+        # optional {kernel name: [(start_event, end_event), ...]} filled per step
+        self.kernel_timing = None
+
+    # ----------------------------------------------------------------- misc
+    def set_material(self, name=None, props=None):
+        if name and name in MATERIAL_PRESETS:
+            mat = MATERIAL_PRESETS[name]
+        elif props:
+            mat = props
+        else:
+            return
+        self.mat_k, self.mat_rho, self.mat_cp = mat['k'], mat['rho'], mat['cp']
+        self.emissivity, self.alpha_thermal = mat['emissivity'], mat['alpha']
+        self.E_modulus = mat['E_mod']
+        self.sputter_Y_coeff, self.sputter_E_th = mat['Y_coeff'], mat['E_th']
+        self.C_cell = self.mat_rho * (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3 * self.mat_cp if hasattr(self, 'dx') else None
+
+    def __del__(self):
+        try:
+            if self._poisson:
+                self.lib.bemcs_poisson_free(self._poisson)
+        except Exception:
+            pass
+
+    # --------------------------------------------------------- domain build
+    def build_domain(self, params, preserve_state=False):
+        """Geometry, numerical checks, masks, operator set-up and the initial
+        Poisson solve (reference :629-847), then device allocation."""
+        grids = params.get("grids", [])
+        total_t = sum(g['t'] + g['gap'] for g in grids) if grids else 0.0
+        dc = params.get("discharge_chamber")
+        pitch = params.get("pitch_mm", dc.get("pitch_mm", 0.0) if isinstance(dc, dict) else 0.0)
+        Te_up = params.get('Te_up', 3.0)
+        n0_n = params.get("n0_plasma", 1e17)
+        bulk = params.get("entire_bulk_plasma", False)
+        self.seed = int(params.get("seed", 0)) & 0xFFFFFFFF
+        debye = np.sqrt(self.eps0 * Te_up / (self.q * n0_n * 0.61))
+        if bulk:
+            n_debye = 80 if n0_n <= 1e17 else (40 if n0_n <= 4e17 else 30)
+            self.recommended_upstream_gap_mm = n_debye * debye * 1e3
+        else:
+            self.recommended_upstream_gap_mm = 0.75 * (grids[0]['r'] if grids else 0.80)
+        user_gap = float(params.get('upstream_gap_mm', 0.0))
+        gap = user_gap if user_gap > 0.0 else self.recommended_upstream_gap_mm
+        self.upstream_gap_mm = max(gap, self.dx * 2)
+        self.Lx = self.upstream_gap_mm + total_t + 3.0 if grids else params.get('Lx', self.Lx)
+        geometry = params.get('geometry', 'half_hole')
+        self.geometry = geometry
+        self.periodic_y = geometry in ('one_hole', 'two_holes')
+        if grids:
+            sr = grids[0]['r']
+            self.Ly = {'two_holes': sr + 2.0 * sr + pitch, 'one_hole': 3.0 * sr}.get(
+                geometry, 0.5 * sr + 0.30 * pitch)
+        else:
+            self.Ly = params.get('Ly', self.Ly)
+        mesh = params.get('mesh')
+        if mesh:
+            self.nx, self.ny = int(mesh['nx']), int(mesh['ny'])
+        else:
+            self.nx = int(self.Lx / self.dx) + 1
+            self.ny = int(self.Ly / self.dy) + 1
+        if self.nx < 4 or self.ny < 4:
+            raise ValueError(f"mesh too small: {self.nx} x {self.ny}")
+        self.dx = self.Lx / (self.nx - 1)
+        self.dy = self.Ly / (self.ny - 1)
+        if self.dx * 1e-3 > debye or self.dy * 1e-3 > debye:
+            raise ValueError(
+                f"Grid spacing too large for Debye resolution: dx={self.dx:.6f} mm, "
+                f"dy={self.dy:.6f} mm, lambda_D={debye * 1e3:.6f} mm. Choose dx, dy <= lambda_D.")
+        wpi = np.sqrt(n0_n * self.q ** 2 / (self.m_ion * self.eps0))
+        if 2 * 3.14159 / wpi < self.dt:
+            raise ValueError(f"Too low time step: dt used ={self.dt:.8e} s, "
+                             f"dt minimum ={2 * 3.14159 / wpi:.6e} s, ")
+        Ti = params.get('Ti', 0.1)
+        vmax = np.sqrt(self.q_ion * Te_up / self.m_ion) + 4 * np.sqrt(self.q_ion * Ti / self.m_ion)
+        if self.dx * 1e-3 / self.dt < vmax or self.dy * 1e-3 / self.dt < vmax:
+            raise ValueError(
+                f"Grid spacing and time step too large for velocity resolution: "
+                f"dx/dt={self.dx * 1e-3 / self.dt:.2e} m/s, dy/dt={self.dy * 1e-3 / self.dt:.2e} m/s, "
+                f"vmax={vmax:.2e} m/s.")
+        self.set_material(props={'k': self.mat_k, 'rho': self.mat_rho, 'cp': self.mat_cp,
+                                 'emissivity': self.emissivity, 'alpha': self.alpha_thermal,
+                                 'E_mod': self.E_modulus, 'Y_coeff': self.sputter_Y_coeff,
+                                 'E_th': self.sputter_E_th})
+        self.xpts = np.linspace(0, self.Lx, self.nx)
+        self.ypts = np.linspace(0, self.Ly, self.ny)
+        X, Y = np.meshgrid(self.xpts, self.ypts)
+        if not preserve_state or not hasattr(self, 'grid_deflections'):
+            self.grid_deflections = [0.0] * len(grids)
+        inj_time = params.get("inj_time", 0.0)
+        self.injection_stop_time = inj_time if inj_time > 0.0 else None
+        self.injection_enabled = True
+        n0 = params.get('n0_plasma', 1e17)
+        cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
+        self.macro_weight = max(n0 * cell_vol / 80.0, 1e3)
+
+        isBound = np.zeros((self.ny, self.nx), bool)
+        V_fixed = np.zeros((self.ny, self.nx))
+        self.mask_grids = []
+        self.T_grids = []
+        if grids:
+            sr = grids[0]['r']
+            centers = {'two_holes': [1.5 * sr, 1.5 * sr + pitch], 'one_hole': [1.5 * sr]}.get(geometry, [0.0])
+        else:
+            centers = []
+        self.hole_centers = centers
+        self.grid_x_starts, self.grid_x_ends = [], []
+        cur = self.upstream_gap_mm
+        for i, g in enumerate(grids):
+            gs, ge = cur, cur + g["t"]
+            self.grid_x_starts.append(gs)
+            self.grid_x_ends.append(ge)
+            d = self.grid_deflections[i] if i < len(self.grid_deflections) else 0.0
+            dxb = 0.0
+            if abs(d) > 1e-6:
+                if geometry == 'two_holes':
+                    eta = np.clip(1.0 - np.abs(Y - (1.5 * sr + 0.5 * pitch)) / max(pitch * 0.5, 1e-6), 0.0, 1.0)
+                    dxb = d * eta ** 2
+                elif geometry == 'one_hole':
+                    eta = np.clip(1.0 - np.abs(Y - 1.5 * sr) / max(self.Ly * 0.5, 1e-6), 0.0, 1.0)
+                    dxb = d * eta ** 2
+                else:
+                    Lc = self.Ly - g["r"]
+                    dxb = d * np.clip((self.Ly - Y) / Lc, 0.0, 1.0) ** 2 if Lc > 0 else 0.0
+            inx = (X >= gs + dxb) & (X <= ge + dxb)
+            mask = inx.copy()
+            for yc in centers:
+                lr = np.maximum(g["r"] - np.maximum(0.0, X - gs - dxb) * np.tan(np.radians(g["cham"])), 0.0)
+                mask &= ~(inx & (np.abs(Y - yc) <= lr))
+            isBound[mask] = True
+            V_fixed[mask] = g["V"]
+            self.mask_grids.append(mask)
+            self.T_grids.append(300.0)
+            cur = ge + g["gap"]
+        self.Vdc = V_fixed.copy()
+        vpb = (grids[0]["V"] if grids else 1000.0) + params.get("V_plasma_offset", 20.0)
+        V_fixed[:, 0] = vpb
+        isBound[:, 0] = True
+        self._isBound_host = isBound
+        self._V_fixed_host = V_fixed
+        self._grids = grids
+        self._alloc_mesh(preserve_state)
+        if not preserve_state:
+            self._alloc_particles(params)
+        self._set_geometry_constants(params)
+        self._built = True
+        self.recalc_poisson(30 if not preserve_state else 10, params)
+        pre = params.get('preload')
+        if pre and not preserve_state:
+            self._preload(params, int(pre.get('ions', 0)), int(pre.get('electrons', 0)))
+
+    def _alloc_mesh(self, preserve_state):
+        dev = self.device
+        nn = self.nx * self.ny
+        if self._poisson:
+            self.lib.bemcs_poisson_free(self._poisson)
+            self._poisson = None
+        h = C.c_void_p()
+        ib = np.ascontiguousarray(self._isBound_host.astype(np.uint8))
+        check(self.lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
+                                            int(self.periodic_y), C.byref(h)), "poisson_create")
+        self._poisson = h
+        self.is_bound_d = torch.from_numpy(ib.ravel()).to(dev)
+        self.V_fixed_d = torch.from_numpy(self._V_fixed_host.ravel().copy()).to(dev)
+        keep_V = preserve_state and hasattr(self, 'V_d') and self.V_d.numel() == nn
+        if not keep_V:
+            self.V_d = zeros(nn, torch.float64, dev)
+        self.Exy = zeros(2 * nn, torch.float32, dev)
+        self.rho_d = zeros(nn, torch.float32, dev)
+        if not preserve_state or not hasattr(self, 'cnt_i') or self.cnt_i.numel() != nn:
+            self.cnt_i = zeros(nn, torch.int32, dev)
+            self.cnt_e = zeros(nn, torch.int32, dev)
+            self.cnt_i_next = zeros(nn, torch.int32, dev)
+            self.cnt_e_next = zeros(nn, torch.int32, dev)
+            self.ppc_d = zeros(nn, torch.int32, dev)
+            self.accum_ppc_d = zeros(nn, torch.float64, dev)
+        self.sort_scratch = zeros(nn + (nn + 1023) // 1024 + 1024, torch.int32, dev)
+        self.ds_bits = zeros(2, torch.int64, dev)
+        self.n_sub_d = torch.ones(2, dtype=torch.int32, device=dev)
+        self.overflow_d = zeros(1, torch.int32, dev)
+        sz = C.sizeof(_lib.IonDiag)
+        self.ring_diag = zeros(RING * sz, torch.uint8, dev)
+        self.ring_theta = zeros(RING, torch.float64, dev)
+        self.ring_minpot = zeros(RING, torch.float64, dev)
+        self.ring_ppc = zeros(2 * RING, torch.int32, dev)
+        self.ring_counts = zeros(2 * RING, torch.int32, dev)
+        self.ring_nsub = zeros(2 * RING, torch.int32, dev)
+        self.ring_nin = zeros(2 * RING, torch.int32, dev)
+
+    def _alloc_particles(self, params):
+        pre = params.get('preload') or {}
+        ni = int(pre.get('ions', 0))
+        ne = int(pre.get('electrons', 0))
+        cap_i = max(100000, int(ni * 1.25) + 65536)
+        cap_e = max(100000, int(ne * 1.25) + 65536) + cap_i
+        dev = self.device
+        self.ions = [SpeciesBuffers(cap_i, dev, True), SpeciesBuffers(cap_i, dev, True)]
+        self.elec = [SpeciesBuffers(cap_e, dev, False), SpeciesBuffers(cap_e, dev, False)]
+        self.crossed_angle = torch.empty(cap_i, dtype=torch.float32, device=dev)
+        self.cnt_i.zero_()
+        self.cnt_e.zero_()
+        self.iteration = 0
+        self.next_ion_pid = 0
+        self.bound_i = 0
+        self.bound_e = 0
+        self.injected_ions = self.transmitted_ions = self.entered_optics = 0.0
+        self.ppc_steps_count = 0
+        self._pending = []
+
+    def _set_geometry_constants(self, params):
+        grids = self._grids
+        if grids:
+            self.x_entry = self.grid_x_starts[0]
+            self.x_exit = self.grid_x_ends[-1]
+            self.x_plume = self.grid_x_ends[-1] + grids[-1]["gap"]
+        else:
+            self.x_entry, self.x_exit, self.x_plume = 0.5, 3.0, 3.0
+        if len(grids) >= 2:
+            xg = 0.5 * (self.grid_x_starts[1] + self.grid_x_ends[1])
+        elif len(grids) == 1:
+            xg = 0.5 * (self.grid_x_starts[0] + self.grid_x_ends[0])
+        else:
+            xg = self.Lx * 0.5
+        xi = int(np.clip(round(xg / self.dx), 0, self.nx - 1))
+        if self.hole_centers:
+            idx = [int(np.clip(round(yc / self.dy), 0, self.ny - 1)) * self.nx + xi for yc in self.hole_centers]
+        else:
+            idx = [(self.ny // 2) * self.nx + xi]
+        self._saddle_idx = torch.tensor(idx, dtype=torch.int64, device=self.device)
+
+    # ------------------------------------------------------------ Poisson
+    def recalc_poisson(self, iterations=5, params=None):
+        """Picard loop with Boltzmann electrons and E field (reference :850-902)."""
+        if not self._built:
+            return
+        params = params or {}
+        grids = params.get('grids', [{'V': 1000}])
+        cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
+        cd = self.q * self.macro_weight / cell_vol
+        p = _lib.Picard()
+        p.cnt_ion, p.cnt_e = ptr(self.cnt_i), ptr(self.cnt_e)
+        p.V_fixed, p.V = ptr(self.V_fixed_d), ptr(self.V_d)
+        p.rho_out, p.Exy = ptr(self.rho_d), ptr(self.Exy)
+        p.cd_ion, p.cd_e = cd, -cd
+        p.coeff = (self.dx * 1e-3) ** 2 / self.eps0
+        p.V_plasma = grids[0]['V'] + params.get('V_plasma_offset', 20.0)
+        p.Te_up = params.get('Te_up', 3.0)
+        p.n0 = params.get('n0_plasma', 1e17)
+        p.q = self.q
+        p.omega = 0.2
+        p.dx_m, p.dy_m = self.dx * 1e-3, self.dy * 1e-3
+        p.rtol = self.poisson_rtol
+        p.iterations = int(iterations)
+        p.max_iter = self.poisson_max_iter
+        its = C.c_int32(0)
+        rel = C.c_double(0.0)
+        check(self.lib.bemcs_poisson_picard(self._poisson, C.byref(p), C.byref(its), C.byref(rel),
+                                            stream_ptr()), "poisson_picard")
+        self.last_poisson_iters = its.value
+        self.last_poisson_relres = rel.value
+
+    # ------------------------------------------------------------ particles
+    def _ensure_capacity(self, need_i, need_e):
+        """Grow particle buffers when the host bound could exceed capacity."""
+        if need_i <= self.ions[0].cap and need_e <= self.elec[0].cap:
+            return
+        torch.cuda.synchronize(self.device)
+        ni = int(self.ions[0].count.item())
+        ne = int(self.elec[0].count.item())
+        extra_i, extra_e = need_i - self.bound_i, need_e - self.bound_e
+        self.bound_i, self.bound_e = ni, ne
+        self._pending.clear()
+        need_i, need_e = ni + extra_i, ne + extra_e
+        if need_i > self.ions[0].cap:
+            cap = max(int(self.ions[0].cap * 1.5), need_i + 65536)
+            self.ions[0].grow(cap, ni)
+            self.ions[1].grow(cap, 0)
+            self.crossed_angle = torch.empty(cap, dtype=torch.float32, device=self.device)
+        cap_e_need = max(need_e, ne) + self.ions[0].cap
+        if cap_e_need > self.elec[0].cap:
+            cap = max(int(self.elec[0].cap * 1.5), cap_e_need + 65536)
+            self.elec[0].grow(cap, ne)
+            self.elec[1].grow(cap, 0)
+
+    def _inject_struct(self, kind, cnt):
+        p = _lib.Inject()
+        p.is_bound = ptr(self.is_bound_d)
+        p.cnt = ptr(cnt)
+        p.dx32, p.dy32 = float(np.float32(self.dx)), float(np.float32(self.dy))
+        p.nx, p.ny = self.nx, self.ny
+        p.kind = kind
+        p.seed = self.seed
+        p.step = self.iteration & 0xFFFFFFFF
+        p.Lx, p.Ly = self.Lx, self.Ly
+        return p
+
+    def _launch_inject(self, sp_idx_ions, p, n):
+        sp = self.ions[0] if sp_idx_ions else self.elec[0]
+        bound = self.bound_i if sp_idx_ions else self.bound_e
+        check(self.lib.bemcs_inject(C.byref(sp.struct()), C.byref(p), int(n), int(bound), stream_ptr()),
+              "inject")
+        if sp_idx_ions:
+            self.bound_i += n
+        else:
+            self.bound_e += n
+
+    def _preload(self, params, n_i, n_e):
+        Te_up = params.get('Te_up', 3.0)
+        Ti = params.get('Ti', 0.1)
+        if n_i > 0:
+            self._ensure_capacity(self.bound_i + n_i, self.bound_e)
+            p = self._inject_struct(3, self.cnt_i)
+            p.Lx = self.x_plume  # ions beyond x_plume are out of bounds (:1349-1355)
+            p.v_drift = float(np.sqrt(self.q_ion * Te_up / self.m_ion))
+            p.v_spread = float(np.sqrt(self.q_ion * Ti / self.m_ion))
+            p.pid_base = 0
+            self._launch_inject(True, p, n_i)
+            self.next_ion_pid = n_i
+        if n_e > 0:
+            self._ensure_capacity(self.bound_i, self.bound_e + n_e)
+            p = self._inject_struct(4, self.cnt_e)
+            p.v_spread = float(np.sqrt(2.0 * self.q * params.get('Te', 5.0) / self.m_e))
+            self._launch_inject(False, p, n_e)
+
+    def _inject(self, params, t_current):
+        grids = params.get('grids', [])
+        if self.injection_enabled and (self.injection_stop_time is None or t_current <= self.injection_stop_time):
+            n0 = params.get('n0_plasma', 1e17)
+            Te_up = params.get('Te_up', 3.0)
+            Ti = params.get('Ti', 0.1)
+            vb = np.sqrt(self.q_ion * Te_up / self.m_ion)
+            area = self.Ly * 1e-3 * 0.005
+            bf = 1.0 if params.get('entire_bulk_plasma', False) else 0.61
+            I_ion = self.q_ion * bf * n0 * vb * area
+            cpm = self.q_ion * self.macro_weight
+            lam = (I_ion * self.dt) / cpm
+            n = int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, self.iteration & 0xFFFFFFFF, lam))
+            self.injected_ions_step = float(n)
+            if n > 0:
+                self._ensure_capacity(self.bound_i + n, self.bound_e)
+                p = self._inject_struct(0, self.cnt_i)
+                if grids:
+                    sr = grids[0]['r']
+                    if self.geometry == 'two_holes':
+                        pitch = params.get('pitch_mm', 0.0)
+                        lim = np.array([[max(self.dy, 0.5 * sr), min(2.5 * sr, self.Ly - self.dy)],
+                                        [max(self.dy, 0.5 * sr + pitch), min(2.5 * sr + pitch, self.Ly - self.dy)]],
+                                       dtype=_NP_FP)
+                        p.y_lo[0], p.y_hi[0] = float(lim[0, 0]), float(lim[0, 1])
+                        p.y_lo[1], p.y_hi[1] = float(lim[1, 0]), float(lim[1, 1])
+                        p.two_band, p.band_f32 = 1, 1
+                    elif self.geometry == 'one_hole':
+                        p.y_lo[0], p.y_hi[0] = max(self.dy, 0.5 * sr), min(2.5 * sr, self.Ly - self.dy)
+                    else:
+                        p.y_lo[0], p.y_hi[0] = 0.0, min(sr, self.Ly - self.dy)
+                else:
+                    p.y_lo[0], p.y_hi[0] = self.dy, self.Ly - self.dy
+                p.x0 = self.dx * 1.5
+                p.v_drift = float(vb)
+                p.v_spread = float(np.sqrt(self.q_ion * Ti / self.m_ion))
+                p.pid_base = self.next_ion_pid & 0xFFFFFFFF
+                self._launch_inject(True, p, n)
+                self.next_ion_pid += n
+                self.injected_ions += float(n)
+            if params.get('rf_enable'):
+                vth_s = np.sqrt(2.0 * self.q * Te_up / self.m_e)
+                I_e = self.q * 0.25 * n0 * vth_s * area
+                nef = (I_e * self.dt) / cpm
+                ne = int(nef)
+                u = self._uniform_host(P_RF_COUNT, 0, self.iteration, 0)
+                if u < (nef - ne):
+                    ne += 1
+                if ne > 0:
+                    self._ensure_capacity(self.bound_i, self.bound_e + ne)
+                    p = self._inject_struct(2, self.cnt_e)
+                    ylo = max(self.dy, 1e-9)
+                    p.y_lo[0], p.y_hi[0] = ylo, max(ylo, self.Ly - ylo)
+                    p.x0 = 0.1
+                    p.v_drift = float(vb)
+                    p.v_spread = float(vth_s)
+                    self._launch_inject(False, p, ne)
+        ne = int(params.get('neut_rate', 30))
+        if ne > 0:
+            Te = params.get('Te', 5.0)
+            nxp = float(np.clip(params.get('neut_x', self.Lx - 0.5), self.dx, self.Lx - self.dx))
+            nrp = float(np.clip(params.get('neut_r', self.Ly), self.dy, self.Ly))
+            self._ensure_capacity(self.bound_i, self.bound_e + ne)
+            p = self._inject_struct(1, self.cnt_e)
+            p.y_lo[0], p.y_hi[0] = 0.0, nrp
+            p.x0 = nxp
+            p.v_spread = float(np.sqrt(2.0 * self.q * Te / self.m_e))
+            self._launch_inject(False, p, ne)
+
+    def _uniform_host(self, purpose, a, b, c):
+        out = (C.c_uint32 * 4)()
+        ctr = (C.c_uint32 * 4)(a & 0xFFFFFFFF, b & 0xFFFFFFFF, c & 0xFFFFFFFF, 0)
+        key = (C.c_uint32 * 2)(self.seed, purpose)
+        self.lib.bemcs_philox4x32_10(ctr, key, out)
+        w = ((out[1] << 32) | out[0]) >> 11
+        return float(w) * 2.0 ** -53
+
+    # -------------------------------------------------------------- step
+    def _refresh_bounds(self, block=False):
+        """Tighten the host bounds with particle counts already copied back."""
+        while self._pending:
+            ev, cnts, bi_after, be_after, step = self._pending[0]
+            if not block and not ev.query():
+                break
+            ev.synchronize()
+            self._pending.pop(0)
+            ni, ne = int(cnts[0]), int(cnts[1])
+            # bounds grew by bi_after/be_after minus the bound at that step
+            self.bound_i = min(self.bound_i, ni + (self.bound_i - bi_after))
+            self.bound_e = min(self.bound_e, ne + (self.bound_e - be_after))
+
+    def _one_step(self, params):
+        self.iteration += 1
+        it = self.iteration
+        t_current = it * self.dt
+        s = stream_ptr()
+        lib = self.lib
+        self._refresh_bounds()
+        self._inject(params, t_current)
+        if it % 2 == 0:
+            kt = self.kernel_timing
+            ev = self._timed_begin(kt)
+            self.recalc_poisson(5, params)
+            self._timed_end(kt, "poisson_picard", ev)
+        # the ion step may append one SEE electron per ion
+        self._ensure_capacity(self.bound_i, self.bound_e + self.bound_i)
+        # substep counts for both species (:1177-1209)
+        self.ds_bits.zero_()
+        I0, I1 = self.ions
+        E0, E1 = self.elec
+        dsb = self.ds_bits.data_ptr()
+        nsb = self.n_sub_d.data_ptr()
+        check(lib.bemcs_substeps(ptr(I0.x), ptr(I0.y), ptr(I0.vx), ptr(I0.vy), ptr(I0.vz), ptr(I0.count),
+                                 self.bound_i, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
+                                 self.q_ion / self.m_ion, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb), C.c_void_p(nsb), s),
+              "substeps(ions)")
+        check(lib.bemcs_substeps(ptr(E0.x), ptr(E0.y), ptr(E0.vx), ptr(E0.vy), ptr(E0.vz), ptr(E0.count),
+                                 self.bound_e, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
+                                 -self.q / self.m_e, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb + 8), C.c_void_p(nsb + 4), s),
+              "substeps(electrons)")
+        # outputs
+        slot = it % RING
+        dsz = C.sizeof(_lib.IonDiag)
+        diag_ptr = self.ring_diag.data_ptr() + slot * dsz
+        self.ring_diag[slot * dsz:(slot + 1) * dsz].zero_()
+        I1.count.zero_()
+        E1.count.zero_()
+        self.cnt_i_next.zero_()
+        self.cnt_e_next.zero_()
+        self.ppc_d.zero_()
+        ip = self._ion_params
+        ip.Exy, ip.is_bound, ip.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb)
+        ip.cnt_next, ip.ppc = ptr(self.cnt_i_next), ptr(self.ppc_d)
+        ip.crossed_angle, ip.diag, ip.e_cnt_next = ptr(self.crossed_angle), C.c_void_p(diag_ptr), ptr(self.cnt_e_next)
+        ip.step = it & 0xFFFFFFFF
+        self.ring_nin[2 * slot].copy_(I0.count[0])
+        self.ring_nin[2 * slot + 1].copy_(E0.count[0])
+        kt = self.kernel_timing
+        ev = self._timed_begin(kt)
+        check(lib.bemcs_ion_step(C.byref(I0.struct()), C.byref(I1.struct()), C.byref(E1.struct()),
+                                 C.byref(ip), self.bound_i, s), "ion_step")
+        self._timed_end(kt, "ion_step", ev)
+        ep = self._elec_params
+        ep.Exy, ep.is_bound, ep.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb + 4)
+        ep.cnt_next, ep.overflow = ptr(self.cnt_e_next), ptr(self.overflow_d)
+        ev = self._timed_begin(kt)
+        check(lib.bemcs_electron_step(C.byref(E0.struct()), C.byref(E1.struct()), C.byref(ep),
+                                      self.bound_e, s), "electron_step")
+        self._timed_end(kt, "electron_step", ev)
+        self.bound_e = self.bound_e + self.bound_i  # SEE electrons
+        self.ions.reverse()
+        self.elec.reverse()
+        self.cnt_i, self.cnt_i_next = self.cnt_i_next, self.cnt_i
+        self.cnt_e, self.cnt_e_next = self.cnt_e_next, self.cnt_e
+        # per-step records
+        self.ring_ppc[2 * slot:2 * slot + 2].zero_()
+        check(lib.bemcs_ppc_stats(ptr(self.ppc_d), self.nx * self.ny, ptr(self.accum_ppc_d),
+                                  C.c_void_p(self.ring_ppc.data_ptr() + 8 * slot), s), "ppc_stats")
+        self.ppc_steps_count += 1
+        check(lib.bemcs_percentile(ptr(self.crossed_angle), C.c_void_p(diag_ptr + 4), self.bound_i, 95.0, 5,
+                                   None, C.c_void_p(self.ring_theta.data_ptr() + 8 * slot), s), "percentile")
+        self.ring_minpot[slot] = torch.min(self.V_d[self._saddle_idx])
+        self.ring_counts[2 * slot].copy_(self.ions[0].count[0])
+        self.ring_counts[2 * slot + 1].copy_(self.elec[0].count[0])
+        self.ring_nsub[2 * slot:2 * slot + 2].copy_(self.n_sub_d)
+        if self.sort_interval > 0 and it % self.sort_interval == 0:
+            self._sort()
+        # asynchronous count readback for the host bounds
+        cnts = torch.empty(2, dtype=torch.int32, pin_memory=True)
+        cnts.copy_(self.ring_counts[2 * slot:2 * slot + 2], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._pending.append((ev, cnts, self.bound_i, self.bound_e, it))
+        return slot
+
+    @staticmethod
+    def _timed_begin(kt):
+        if kt is None:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    @staticmethod
+    def _timed_end(kt, name, start):
+        if kt is None:
+            return
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        kt.setdefault(name, []).append((start, e))
+
+    def _sort(self):
+        s = stream_ptr()
+        for pair in (self.ions, self.elec):
+            bound = self.bound_i if pair is self.ions else self.bound_e
+            check(self.lib.bemcs_sort_by_cell(C.byref(pair[0].struct()), C.byref(pair[1].struct()), bound,
+                                              self.nx, self.ny, self.dx, self.dy, ptr(self.sort_scratch_for(bound)), s),
+                  "sort_by_cell")
+            pair.reverse()
+
+    def sort_scratch_for(self, n_bound):
+        nn = self.nx * self.ny
+        need = nn + (nn + 1023) // 1024 + 1024 + int(n_bound)
+        if self.sort_scratch.numel() < need:
+            self.sort_scratch = torch.empty(int(need * 1.25), dtype=torch.int32, device=self.device)
+        return self.sort_scratch
+
+    def _prepare_step_params(self, params):
+        grids = params.get('grids', [])
+        ip = _lib.IonStep()
+        ip.dt = self.dt
+        ip.q_m = self.q_ion / self.m_ion
+        ip.Lx, ip.Ly = self.Lx, self.Ly
+        ip.m_ion, ip.q = self.m_ion, self.q
+        ip.see_vth = float(np.sqrt(2.0 * self.q * 2.0 / self.m_e))
+        ip.mcc_n0 = float(params.get('n0', 1e20))
+        a_corr = 1.0 / (1.0 - 1.0 / np.sqrt(2.0))
+        ip.mcc_n0a = float(params.get('n0', 1e20) * a_corr)
+        ip.mcc_rT = self.Ly * 1e-3
+        ip.mcc_vth = float(np.sqrt(2.0 * self.kB * params.get('Tn', 300.0) / self.m_ion))
+        f = lambda v: float(np.float32(v))
+        ip.dx32, ip.dy32 = f(self.dx), f(self.dy)
+        ip.x_entry32, ip.x_exit32, ip.x_plume32 = f(self.x_entry), f(self.x_exit), f(self.x_plume)
+        ip.Lx32, ip.Ly32 = f(self.Lx), f(self.Ly)
+        ip.dt32 = f(self.dt)
+        ip.nx, ip.ny, ip.periodic_y = self.nx, self.ny, int(self.periodic_y)
+        ip.neutral_model = 1 if params.get('neutral_model', 'roy') == 'uniform' else 0
+        ip.cx_model = 1 if params.get('cx_model', 'reference') == 'loglinear' else 0
+        ip.mcc_enabled = 1
+        ip.seed = self.seed
+        self._ion_params = ip
+        ep = _lib.ElectronStep()
+        ep.dt = self.dt
+        ep.q_m = -self.q / self.m_e
+        ep.Ly = self.Ly
+        ep.dx32, ep.dy32 = f(self.dx), f(self.dy)
+        ep.Lx32, ep.Ly32 = f(self.Lx), f(self.Ly)
+        ep.nx, ep.ny, ep.periodic_y = self.nx, self.ny, int(self.periodic_y)
+        self._elec_params = ep
+        if grids and params.get('sim_mode', 'Both') != 'Trajectories' and not getattr(self, '_warned_mode', False):
+            self._warned_mode = True
+
+    def advance(self, params, n_steps):
+        """Run n_steps time steps without synchronising the host."""
+        if not self._built:
+            return
+        self._prepare_step_params(params)
+        for _ in range(int(n_steps)):
+            self._one_step(params)
+
+    def step(self, params):
+        """One time step; returns (remeshed, min_pot, current_div, T_grids,
+        trans_last_frame) like the reference (:969-1784)."""
+        if not self._built:
+            return False, np.nan, np.nan, self.T_grids, 0.0
+        self._prepare_step_params(params)
+        slot = self._one_step(params)
+        rec = self.record(slot)
+        self.transmitted_ions += rec['transmitted']
+        self.entered_optics += rec['entered']
+        return False, rec['min_pot'], rec['current_div'], self.T_grids, rec['trans_last_frame']
+
+    # ------------------------------------------------------- diagnostics
+    def record(self, slot=None):
+        """Diagnostics of one step (default: the last), synchronising."""
+        if slot is None:
+            slot = self.iteration % RING
+        torch.cuda.current_stream().synchronize()
+        dsz = C.sizeof(_lib.IonDiag)
+        raw = self.ring_diag[slot * dsz:(slot + 1) * dsz].cpu().numpy().tobytes()
+        d = _lib.IonDiag.from_buffer_copy(raw)
+        if d.overflow:
+            raise _lib.BemcsError(_lib.EOVERFLOW, f"particle buffer overflow (flags {d.overflow})")
+        nc = d.crossed
+        rec = dict(entered=float(d.entered), crossed=nc, lost_to_grid=float(d.lost_to_grid),
+                   see_spawned=d.see_spawned, see_kept=d.see_kept, mcc_events=d.mcc_events,
+                   transmitted=float(nc) if d.exited else 0.0)
+        rec['current_div'] = float(self.ring_theta[slot].item()) if nc > 5 else float('nan')
+        rec['min_pot'] = float(self.ring_minpot[slot].item())
+        rec['trans_last_frame'] = (rec['transmitted'] / rec['entered']) if rec['entered'] > 0 else 0.0
+        if nc > 0:
+            rec['exit_vx_mean'] = d.sum_vx / nc
+            rec['exit_v_mean'] = d.sum_v / nc
+            rec['exit_vx_std'] = math.sqrt(max(d.sum_vx2 / nc - (d.sum_vx / nc) ** 2, 0.0))
+            rec['exit_v_std'] = math.sqrt(max(d.sum_v2 / nc - (d.sum_v / nc) ** 2, 0.0))
+            rec['exit_energy_mean_eV'] = d.sum_E / nc
+        counts = self.ring_counts[2 * slot:2 * slot + 2].cpu().numpy()
+        rec['num_p'], rec['num_e'] = int(counts[0]), int(counts[1])
+        ppc = self.ring_ppc[2 * slot:2 * slot + 2].cpu().numpy()
+        rec['total_active_cells'], rec['low_ppc_cells'] = int(ppc[0]), int(ppc[1])
+        nsub = self.ring_nsub[2 * slot:2 * slot + 2].cpu().numpy()
+        rec['n_sub_ion'], rec['n_sub_e'] = int(nsub[0]), int(nsub[1])
+        for k in ('exit_vx_mean', 'exit_v_mean', 'exit_vx_std', 'exit_v_std', 'exit_energy_mean_eV'):
+            setattr(self, k, rec.get(k, float('nan')))
+        self.exit_count_step = nc
+        self.total_active_cells = rec['total_active_cells']
+        self.low_ppc_cells = rec['low_ppc_cells']
+        self.entered_optics_step = rec['entered']
+        self.transmitted_ions_step = rec['transmitted']
+        self.lost_to_grid_step = rec['lost_to_grid']
+        return rec
+
+    def particle_counts(self):
+        torch.cuda.current_stream().synchronize()
+        return int(self.ions[0].count.item()), int(self.elec[0].count.item())
+
+    # ---------------------------------------------------- host mirrors
+    def _ion_host(self, name):
+        n = self.num_p
+        return getattr(self.ions[0], name)[:n].cpu().numpy()
+
+    @property
+    def num_p(self):
+        return int(self.ions[0].count.item()) if self._built else 0
+
+    @property
+    def num_e(self):
+        return int(self.elec[0].count.item()) if self._built else 0
+
+    p_x = property(lambda s: s._ion_host('x'))
+    p_y = property(lambda s: s._ion_host('y'))
+    p_vx = property(lambda s: s._ion_host('vx'))
+    p_vy = property(lambda s: s._ion_host('vy'))
+    p_vz = property(lambda s: s._ion_host('vz'))
+
+    @property
+    def p_isCEX(self):
+        return (self._ion_host('tag').view(np.uint32) >> 31).astype(bool)
+
+    @property
+    def p_pid(self):
+        return self._ion_host('tag').view(np.uint32) & 0x7FFFFFFF
+
+    def _e_host(self, name):
+        n = self.num_e
+        return getattr(self.elec[0], name)[:n].cpu().numpy()
+
+    e_x = property(lambda s: s._e_host('x'))
+    e_y = property(lambda s: s._e_host('y'))
+    e_vx = property(lambda s: s._e_host('vx'))
+    e_vy = property(lambda s: s._e_host('vy'))
+    e_vz = property(lambda s: s._e_host('vz'))
+
+    @property
+    def V(self):
+        return self.V_d.view(self.ny, self.nx).cpu().numpy()
+
+    @property
+    def Ex(self):
+        return self.Exy.view(self.ny, self.nx, 2)[..., 0].cpu().numpy()
+
+    @property
+    def Ey(self):
+        return self.Exy.view(self.ny, self.nx, 2)[..., 1].cpu().numpy()
+
+    @property
+    def rho(self):
+        return self.rho_d.view(self.ny, self.nx).cpu().numpy()
+
+    @property
+    def isBound(self):
+        return self._isBound_host
+
+    @property
+    def V_fixed(self):
+        return self._V_fixed_host
+
+    @property
+    def current_ppc_map(self):
+        return self.ppc_d.view(self.ny, self.nx).cpu().numpy().astype(np.int64)
+
+    @property
+    def accum_ppc_map(self):
+        return self.accum_ppc_d.view(self.ny, self.nx).cpu().numpy()
+
+    def get_avg_ppc_map(self):
+        if self.ppc_steps_count > 0:
+            return self.accum_ppc_map / float(self.ppc_steps_count)
+        return np.zeros((self.ny, self.nx))
+
+    def reset_ppc_accumulator(self):
+        self.accum_ppc_d.zero_()
+        self.ppc_steps_count = 0
+
+    def get_transparency(self):
+        return self.transmitted_ions / self.entered_optics if self.entered_optics > 0 else 0.0
+
+    def has_active_particles(self):
+        return self.num_p > 0 or self.num_e > 0
+
+    def get_total_energy(self):
+        """Kinetic + field energy (reference :1800-1866), summed on the device."""
+        n, ne = self.num_p, self.num_e
+        I, E = self.ions[0], self.elec[0]
+        ke_i = float((I.vx[:n].double() ** 2 + I.vy[:n].double() ** 2 + I.vz[:n].double() ** 2).sum()) \
+            * 0.5 * self.m_ion * self.macro_weight if n else 0.0
+        ke_e = float((E.vx[:ne].double() ** 2 + E.vy[:ne].double() ** 2 + E.vz[:ne].double() ** 2).sum()) \
+            * 0.5 * self.m_e * self.macro_weight if ne else 0.0
+        e_field = float((self.Exy.double() ** 2).sum()) * 0.5 * self.eps0 * (self.dx * 1e-3) * (self.dy * 1e-3)
+        return dict(t_s=self.iteration * self.dt, ke_ions_J=ke_i, ke_elec_J=ke_e, field_J=e_field,
+                    total_J=ke_i + ke_e + e_field)
+
+    def get_particle_kinematics(self):
+        """(time, x, y, vx, vy, vz, energy_eV, type) rows (reference :1934-1963)."""
+        t = self.iteration * self.dt
+        out = []
+        if self.num_p:
+            x, y, vx, vy, vz = self.p_x, self.p_y, self.p_vx, self.p_vy, self.p_vz
+            e = (0.5 * self.m_ion * (vx ** 2 + vy ** 2 + vz ** 2)) / self.q
+            out.append(np.column_stack((np.full(len(x), t), x, y, vx, vy, vz, e, self.p_isCEX.astype(int))))
+        if self.num_e:
+            x, y, vx, vy, vz = self.e_x, self.e_y, self.e_vx, self.e_vy, self.e_vz
+            e = (0.5 * self.m_e * (vx ** 2 + vy ** 2 + vz ** 2)) / self.q
+            out.append(np.column_stack((np.full(len(x), t), x, y, vx, vy, vz, e, np.where(x <= 4.0, 2, 3))))
+        return np.vstack(out) if out else np.empty((0, 8))
