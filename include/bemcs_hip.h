@@ -114,7 +114,8 @@ typedef struct {
   uint32_t see_kept;       /* ... that survived the electron purge */
   uint32_t mcc_events;     /* CEX collisions */
   uint32_t crossed_primary;
-  uint32_t overflow;       /* non-zero if an append exceeded capacity */
+  uint32_t overflow;       /* bit flags: 1 SEE append, 2 survivor append beyond
+                              capacity, 16 count above the launch bound */
   uint32_t pad_;
   double sum_vx, sum_vx2, sum_v, sum_v2, sum_E;
   double sum_v_primary, sum_v_cex;
@@ -205,7 +206,8 @@ int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
                  int64_t n_bound_before, bemcs_stream_t stream);
 
 /* Counting sort of a species by gather cell (iy0*nx + ix0); writes to `out`
- * and sets out->count.  scratch must hold 2*nx*ny + 1024 uint32. */
+ * and sets out->count.  scratch must hold nx*ny + ceil(nx*ny/1024) + 1024 +
+ * n_bound uint32. */
 int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
                        int32_t nx, int32_t ny, double dx, double dy, uint32_t* scratch,
                        bemcs_stream_t stream);

@@ -125,7 +125,11 @@ def test_steps_parity(pair):
             assert np.array_equal(g.cnt_i.cpu().numpy().astype(np.uint32), ci)
             assert np.array_equal(g.cnt_e.cpu().numpy().astype(np.uint32), ce)
             if o.exit_count_step > 5:
-                assert np.float32(rg[2]) == np.float32(ro[2])
+                # NumPy's float32 arctan2 is not correctly rounded (about half of
+                # its results differ from the correctly rounded value by an ulp);
+                # the device angles are correctly rounded, so theta95 agrees to
+                # a few float32 ulps
+                assert abs(rg[2] - ro[2]) <= 4 * np.spacing(np.float32(ro[2]))
             assert rg[1] == ro[1]
             assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
 
@@ -133,7 +137,7 @@ def test_steps_parity(pair):
 def test_mcc_events_happen_and_match():
     """Dense neutrals so that CEX events occur; masks must match exactly."""
     p = c1_params(nx=128, ny=48, ions=20000, electrons=0, neut_rate=0)
-    p["n0"] = 3e21
+    p["n0"] = 3e22
     g, o = _mk(p)
     _sync(o, g)
     g.step(p)

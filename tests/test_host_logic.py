@@ -1,0 +1,44 @@
+"""Host-side logic that needs no GPU."""
+import random
+
+from thruster_cx_code_amd.device import CountBound
+
+
+def test_count_bound_never_below_true_count():
+    """Random growth/removal histories with out-of-order knowledge of past
+    counts: the bound always covers the true count and tightens when counts
+    arrive."""
+    rng = random.Random(0)
+    for trial in range(200):
+        b = CountBound()
+        true = 0
+        history = []  # (count after step, snapshot)
+        for step in range(60):
+            inj = rng.randrange(0, 50)
+            b.grow(inj)
+            true += inj
+            true -= rng.randrange(0, true + 1) if rng.random() < 0.5 else 0
+            see = rng.randrange(0, 20)  # possible appends, only some happen
+            b.grow(see)
+            true += rng.randrange(0, see + 1)
+            history.append((true, b.snapshot()))
+            # deliver some pending observations, in order, possibly several at once
+            while history and rng.random() < 0.4:
+                c, snap = history.pop(0)
+                b.refresh(c, snap)
+            assert b.bound >= true
+        while history:
+            c, snap = history.pop(0)
+            b.refresh(c, snap)
+        assert b.bound == true or b.bound >= true
+
+
+def test_count_bound_tight_after_refresh():
+    b = CountBound()
+    b.grow(100)
+    snap = b.snapshot()
+    b.refresh(40, snap)
+    assert b.bound == 40
+    b.grow(5)
+    b.refresh(40, snap)
+    assert b.bound == 45

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const float* __restrict__ x,
                                                      int nx, int ny, uint32_t* counts) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t nn = n_dev ? (int64_t)*n_dev : n;
+  if (nn > n) nn = n;
   int key = -1;
   if (i < nn) {
     int ix = round_i32_np(x[i] / dx32, 1, nx - 2);
@@ -158,6 +159,8 @@ __global__ __launch_bounds__(kBlock) void k_substep_ds(
   __shared__ unsigned long long smax[kBlock / 64];
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *n_dev;
+  const int64_t lim = (int64_t)gridDim.x * kBlock;
+  if (n > lim) n = lim;
   double ds = 0.0;
   if (i < n) {
     float px = x[i], py = y[i];
@@ -212,7 +215,9 @@ template <bool PERIODIC>
 __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_species_t out,
                                                       bemcs_species_t eout, bemcs_ion_step_t p) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t n = *in.count;
+  const int64_t cnt = *in.count;
+  const int64_t n = cnt < in.cap ? cnt : in.cap;
+  if (i == 0 && cnt > (int64_t)gridDim.x * kBlock) atomicOr(&p.diag->overflow, 16u);
   const bool active = i < n;
   float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
   uint32_t tag = 0;
@@ -332,7 +337,10 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
     double s_vx = 0, s_vx2 = 0, s_v = 0, s_v2 = 0, s_E = 0, s_vp = 0, s_vc = 0;
     bool prim_c = false;
     if (crossed) {
-      p.crossed_angle[slot] = (fabsf(atan2f(vy, vx)) * 180.0f) / 3.14159265358979323846f;
+      // correctly rounded float32 angle (float64 atan2, then rounded); the
+      // scaling to degrees is float32 as in the reference
+      p.crossed_angle[slot] = ((float)fabs(atan2((double)vy, (double)vx)) * 180.0f) /
+                              3.14159265358979323846f;
       float v = sqrtf(vx * vx + vy * vy + vz * vz);
       float E = ((float)(0.5 * p.m_ion) * (v * v)) / (float)p.q;
       s_vx = vx;
@@ -445,7 +453,9 @@ template <bool PERIODIC>
 __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, bemcs_species_t out,
                                                            bemcs_electron_step_t p) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t n = *in.count;
+  const int64_t cnt = *in.count;
+  const int64_t n = cnt < in.cap ? cnt : in.cap;
+  if (i == 0 && cnt > (int64_t)gridDim.x * kBlock) atomicOr(p.overflow, 16u);
   const bool active = i < n;
   float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
   if (active) {
@@ -578,12 +588,14 @@ __global__ void k_add_count(int32_t* count, int64_t n, int64_t cap, uint32_t* ov
 // --------------------------------------------------------------- cell sort
 __global__ __launch_bounds__(kBlock) void k_cell_hist(const float* __restrict__ x,
                                                        const float* __restrict__ y,
-                                                       const int32_t* n_dev, float dx32, float dy32,
-                                                       int nx, int ny, uint32_t* hist,
-                                                       uint32_t* keys) {
+                                                       const int32_t* n_dev, int64_t n_bound,
+                                                       float dx32, float dy32, int nx, int ny,
+                                                       uint32_t* hist, uint32_t* keys) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int key = -1;
-  if (i < *n_dev) {
+  int64_t n = *n_dev;
+  if (n > n_bound) n = n_bound;
+  if (i < n) {
     int i0 = floor_i32_np(x[i] / dx32, nx - 2);
     int j0 = floor_i32_np(y[i] / dy32, ny - 2);
     key = j0 * nx + i0;
@@ -637,9 +649,10 @@ __global__ __launch_bounds__(1024) void k_scan_add(uint32_t* data, int64_t n, co
 
 __global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_species_t in, bemcs_species_t out,
                                                           const uint32_t* __restrict__ keys,
-                                                          uint32_t* offs) {
+                                                          uint32_t* offs, int64_t n_bound) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *in.count;
+  if (n > n_bound) n = n_bound;
   if (i >= n) return;
   uint32_t d = atomicAdd(&offs[keys[i]], 1u);
   out.x[d] = in.x[i];
@@ -650,7 +663,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_species_t in, bem
   if (in.tag) out.tag[d] = in.tag[i];
 }
 
-__global__ void k_copy_count(const int32_t* src, int32_t* dst) { *dst = *src; }
+__global__ void k_copy_count(const int32_t* src, int32_t* dst, int64_t n_bound) {
+  int64_t n = *src;
+  *dst = (int32_t)(n < n_bound ? n : n_bound);
+}
 
 int scan_exclusive(uint32_t* data, int64_t n, uint32_t* sums, hipStream_t s) {
   int nb = (int)((n + 1023) / 1024);
@@ -814,14 +830,15 @@ int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, in
   uint32_t* sums = scratch + ncell;
   uint32_t* keys = sums + 1024 + ((ncell + 1023) / 1024);
   BEMCS_CHECK_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * ncell, s));
-  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, s, in->count, out->count);
+  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, s, in->count, out->count,
+                     n_bound > 0 ? n_bound : (int64_t)0);
   if (n_bound <= 0) return BEMCS_OK;
   hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, in->x, in->y,
-                     in->count, (float)dx, (float)dy, nx, ny, hist, keys);
+                     in->count, n_bound, (float)dx, (float)dy, nx, ny, hist, keys);
   int rc = scan_exclusive(hist, ncell, sums, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out,
-                     keys, hist);
+                     keys, hist, n_bound);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

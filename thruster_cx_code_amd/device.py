@@ -70,3 +70,29 @@ class SpeciesBuffers:
                 new[:n_valid].copy_(old[:n_valid])
             setattr(self, name, new)
         self.cap = new_cap
+
+
+class CountBound:
+    """Host upper bound of a device particle count, kept without per-step
+    synchronisation.  `grow(n)` adds possible new particles; `snapshot()`
+    returns the cumulative growth to store with an asynchronous copy of the
+    count; `refresh(count, snap)` tightens the bound with a count observed
+    when the growth was `snap` (a step never adds particles except through
+    `grow`, so count + growth since then is a valid bound)."""
+
+    def __init__(self):
+        self.bound = 0
+        self.grown = 0
+
+    def grow(self, n):
+        self.bound += int(n)
+        self.grown += int(n)
+
+    def snapshot(self):
+        return self.grown
+
+    def refresh(self, count, snap):
+        self.bound = min(self.bound, int(count) + (self.grown - snap))
+
+    def reset(self, exact):
+        self.bound = int(exact)

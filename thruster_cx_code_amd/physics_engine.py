@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .device import SpeciesBuffers, ptr, stream_ptr, require_gpu, zeros
+from .device import SpeciesBuffers, CountBound, ptr, stream_ptr, require_gpu, zeros
 
 _NP_FP = np.float32
 
@@ -298,8 +298,9 @@ class DigitalTwinSimulator:
         self.cnt_e.zero_()
         self.iteration = 0
         self.next_ion_pid = 0
-        self.bound_i = 0
-        self.bound_e = 0
+        # host upper bounds of the device counts (no per-step synchronisation)
+        self._bi = CountBound()
+        self._be = CountBound()
         self.injected_ions = self.transmitted_ions = self.entered_optics = 0.0
         self.ppc_steps_count = 0
         self._pending = []
@@ -357,6 +358,14 @@ class DigitalTwinSimulator:
         self.last_poisson_relres = rel.value
 
     # ------------------------------------------------------------ particles
+    @property
+    def bound_i(self):
+        return self._bi.bound
+
+    @property
+    def bound_e(self):
+        return self._be.bound
+
     def _ensure_capacity(self, need_i, need_e):
         """Grow particle buffers when the host bound could exceed capacity."""
         if need_i <= self.ions[0].cap and need_e <= self.elec[0].cap:
@@ -365,7 +374,8 @@ class DigitalTwinSimulator:
         ni = int(self.ions[0].count.item())
         ne = int(self.elec[0].count.item())
         extra_i, extra_e = need_i - self.bound_i, need_e - self.bound_e
-        self.bound_i, self.bound_e = ni, ne
+        self._bi.reset(ni)
+        self._be.reset(ne)
         self._pending.clear()
         need_i, need_e = ni + extra_i, ne + extra_e
         if need_i > self.ions[0].cap:
@@ -396,10 +406,7 @@ class DigitalTwinSimulator:
         bound = self.bound_i if sp_idx_ions else self.bound_e
         check(self.lib.bemcs_inject(C.byref(sp.struct()), C.byref(p), int(n), int(bound), stream_ptr()),
               "inject")
-        if sp_idx_ions:
-            self.bound_i += n
-        else:
-            self.bound_e += n
+        (self._bi if sp_idx_ions else self._be).grow(n)
 
     def _preload(self, params, n_i, n_e):
         Te_up = params.get('Te_up', 3.0)
@@ -498,17 +505,17 @@ class DigitalTwinSimulator:
 
     # -------------------------------------------------------------- step
     def _refresh_bounds(self, block=False):
-        """Tighten the host bounds with particle counts already copied back."""
+        """Tighten the host bounds with particle counts already copied back:
+        the count after step k plus every growth applied since step k bounds
+        the current count (steps only remove particles otherwise)."""
         while self._pending:
-            ev, cnts, bi_after, be_after, step = self._pending[0]
+            ev, cnts, gi, ge, step = self._pending[0]
             if not block and not ev.query():
                 break
             ev.synchronize()
             self._pending.pop(0)
-            ni, ne = int(cnts[0]), int(cnts[1])
-            # bounds grew by bi_after/be_after minus the bound at that step
-            self.bound_i = min(self.bound_i, ni + (self.bound_i - bi_after))
-            self.bound_e = min(self.bound_e, ne + (self.bound_e - be_after))
+            self._bi.refresh(int(cnts[0]), gi)
+            self._be.refresh(int(cnts[1]), ge)
 
     def _one_step(self, params):
         self.iteration += 1
@@ -568,7 +575,7 @@ class DigitalTwinSimulator:
         check(lib.bemcs_electron_step(C.byref(E0.struct()), C.byref(E1.struct()), C.byref(ep),
                                       self.bound_e, s), "electron_step")
         self._timed_end(kt, "electron_step", ev)
-        self.bound_e = self.bound_e + self.bound_i  # SEE electrons
+        self._be.grow(self.bound_i)  # at most one SEE electron per ion
         self.ions.reverse()
         self.elec.reverse()
         self.cnt_i, self.cnt_i_next = self.cnt_i_next, self.cnt_i
@@ -591,7 +598,7 @@ class DigitalTwinSimulator:
         cnts.copy_(self.ring_counts[2 * slot:2 * slot + 2], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
-        self._pending.append((ev, cnts, self.bound_i, self.bound_e, it))
+        self._pending.append((ev, cnts, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
 
     @staticmethod
