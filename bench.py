@@ -149,6 +149,15 @@ def run(args):
         byts = kstats[dom]["launches"] * algorithmic_bytes(dom, 0, 0, n_nodes)
     achieved = byts / (kstats[dom]["total_ms"] * 1e-3) / 1e9
     rec = sim.record()
+    nsub = sim.ring_nsub.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
+    for name, col in (("ion_step", 0), ("electron_step", 1)):
+        if name in kstats:
+            subs = int((nin[:, col] * nsub[:, col]).sum())
+            kstats[name]["particle_substeps"] = subs
+            kstats[name]["particle_substeps_per_s"] = subs / (kstats[name]["total_ms"] * 1e-3)
+            kstats[name]["n_sub_range"] = [int(nsub[:, col].min()), int(nsub[:, col].max())]
+    if "poisson_picard" in kstats:
+        kstats["poisson_picard"]["pcg_iterations"] = list(sim.poisson_iter_log[-kstats["poisson_picard"]["launches"]:])
     out = {
         "metric": "particle-steps/s incl. Poisson at 1024x384",
         "value": value,
@@ -180,7 +189,51 @@ def run(args):
     print(json.dumps(out))
 
 
+def poisson_bench(calls=10):
+    """Time recalc_poisson (5 Picard iterations + E field) alone on the c2
+    mesh with the c2 preload's charge (for profiling the solver)."""
+    import numpy as np
+    import torch
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    p = workload_params("c2")
+    sim = DigitalTwinSimulator()
+    sim.m_e = sim.m_ion / 200.0
+    sim.build_domain(p)
+    sim.recalc_poisson(5, p)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(calls):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        sim.recalc_poisson(5, p)
+        b.record()
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b))
+    print(json.dumps({"poisson_ms": times, "median_ms": float(np.median(times)),
+                      "pcg_iterations": sim.poisson_iter_log[-calls:],
+                      "levels": sim.lib.bemcs_poisson_levels(sim._poisson)}))
+
+
+def copy_calibration(mbytes=512, reps=20):
+    """A plain device-to-device copy of known size, to scale the FETCH_SIZE and
+    WRITE_SIZE counters (run it under rocprofv3 --pmc)."""
+    import torch
+    n = mbytes * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "reps": reps}))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-calibration":
+        copy_calibration()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--poisson-bench":
+        poisson_bench()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
         w, ni, ne, secs, steps = sys.argv[2:7]
         cpu_worker(w, int(ni), int(ne), float(secs), int(steps))

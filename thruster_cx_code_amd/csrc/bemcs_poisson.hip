@@ -14,9 +14,9 @@
 // flexible CG keeps x, r, p, Ap and all dot products in float64; the
 // preconditioner is one float32 V-cycle (bilinear prolongation, Galerkin
 // 9-point coarse operators built on the host, damped Jacobi smoothing, dense
-// inverse on the coarsest level).  Levels at or below kTailNodes nodes run in a
-// single work-group kernel so a V-cycle costs few launches.
+// inverse on the coarsest level, at most kCoarseMax nodes).
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <vector>
@@ -26,8 +26,9 @@
 namespace bemcs {
 
 constexpr int kMaxLevels = 20;
-constexpr int kTailNodes = 8192;
-constexpr int kCoarseMax = 600;
+constexpr int kCoarseMax = 512;  // coarsening stops at or below this many nodes
+constexpr int kDenseMax = 4096;  // largest coarsest level the dense solve accepts
+constexpr int kFuseBelow = 32768;  // levels this small fuse residual+restrict, prolong+sweep
 constexpr int kRedBlock = 256;
 constexpr int kMaxRedBlocks = 1024;
 constexpr float kOmega0 = 0.8f;  // Jacobi damping, 5-point fine level
@@ -44,20 +45,13 @@ struct LevelDev {
   float* x;
   float* t;
   float* b;
-  float* r;
 };
 
-struct TailArgs {
-  LevelDev lev[kMaxLevels];
-  int first, last;  // tail covers levels first..last (last = coarsest)
-  const float* cinv;
-  int nc;
-};
 
 }  // namespace bemcs
 
 struct bemcs_poisson_s {
-  int nx, ny, periodic, nlev, tail_first;
+  int nx, ny, periodic, nlev;
   bemcs::LevelDev lev[bemcs::kMaxLevels];
   std::vector<void*> allocs;
   float* cinv;
@@ -71,12 +65,15 @@ struct bemcs_poisson_s {
   int32_t* done;
   double* host_scal; // pinned copy
   bool have_guess;
-  bemcs::TailArgs* tail_dev;
+  float* rscr;        // float32 residual scratch (largest level)
+  double* part2;     // step partials
+  int pred_iters;    // iterations the last solve needed (launch that many first)
+  int post_sweeps;    // Jacobi sweeps after prolongation (1 or 2)
 };
 
 namespace bemcs {
 
-enum { S_RZ = 0, S_PAP, S_RR, S_BB, S_ALPHA, S_BETA, S_TOL2, S_ITERS, S_COUNT = 16 };
+enum { S_RZ = 0, S_PAP, S_RR, S_BB, S_ALPHA, S_BETA, S_TOL2, S_ITERS, S_RTOL, S_COUNT = 16 };
 
 // ------------------------------------------------------------- 1D transfer
 __host__ __device__ __forceinline__ float w1d(int f, int I, int nc, bool periodic) {
@@ -130,20 +127,59 @@ __device__ __forceinline__ float applyL(const LevelDev& L, const float* v, int i
 }
 
 // --------------------------------------------------------- V-cycle phases
-// Each phase is a device function over [begin, n) with stride, used both by
-// the multi-block kernels and by the single-block tail kernel.
-__device__ __forceinline__ void ph_jacobi_first(const LevelDev& L, float omega, int i) {
-  L.x[i] = omega * L.inv_d[i] * L.b[i];
+// Device functions shared by the multi-block kernels and the single
+// work-group tail kernel.  A V-cycle does per level: two Jacobi sweeps from
+// zero (pre), residual fused with restriction, prolongation fused with the
+// first post sweep, second post sweep.  The post sweeps write t and the
+// pointers are swapped twice, so x is the output buffer again afterwards.
+
+// first Jacobi iterate from x = 0
+__device__ __forceinline__ float x1_at(const LevelDev& L, float om, int j) {
+  return om * L.inv_d[j] * L.b[j];
 }
-__device__ __forceinline__ void ph_jacobi(const LevelDev& L, float omega, int i) {
-  float ax = applyL(L, L.x, i);
-  L.t[i] = L.x[i] + omega * L.inv_d[i] * (L.b[i] - ax);
+
+__device__ __forceinline__ float apply_x1(const LevelDev& L, float om, int i) {
+  int x = i % L.nx, y = i / L.nx;
+  if (!L.st) {
+    int d = L.code[i];
+    int s_ = y > 0 ? i - L.nx : (L.periodic ? i + (L.ny - 1) * L.nx : i);
+    int n_ = y < L.ny - 1 ? i + L.nx : (L.periodic ? i - (L.ny - 1) * L.nx : i);
+    float sum = x1_at(L, om, i - 1) + x1_at(L, om, i + 1) + x1_at(L, om, s_) + x1_at(L, om, n_);
+    return (float)d * x1_at(L, om, i) - sum;
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int xx = x + (k % 3 - 1), yy = y + (k / 3 - 1);
+    if (xx < 0 || xx >= L.nx) continue;
+    if (yy < 0 || yy >= L.ny) {
+      if (!L.periodic) continue;
+      yy = (yy + L.ny) % L.ny;
+    }
+    s += L.st[(size_t)k * L.n + i] * x1_at(L, om, yy * L.nx + xx);
+  }
+  return s;
 }
-__device__ __forceinline__ void ph_residual(const LevelDev& L, int i) {
-  float ax = applyL(L, L.x, i);
-  L.r[i] = L.inv_d[i] != 0.0f ? L.b[i] - ax : 0.0f;
+
+__device__ __forceinline__ void ph_pre(const LevelDev& L, float om, int i) {
+  float id = L.inv_d[i];
+  if (id == 0.0f) {
+    L.x[i] = 0.0f;
+    return;
+  }
+  float x1 = om * id * L.b[i];
+  L.x[i] = x1 + om * id * (L.b[i] - apply_x1(L, om, i));
 }
-__device__ __forceinline__ void ph_restrict(const LevelDev& F, const LevelDev& C, int I) {
+
+__device__ __forceinline__ float resid_at(const LevelDev& F, int f) {
+  return F.inv_d[f] != 0.0f ? F.b[f] - applyL(F, F.x, f) : 0.0f;
+}
+
+__device__ __forceinline__ void ph_resid_restrict(const LevelDev& F, const LevelDev& C, int I) {
+  if (C.inv_d[I] == 0.0f) {
+    C.b[I] = 0.0f;
+    return;
+  }
   int cx = I % C.nx, cy = I / C.nx;
   float s = 0.0f;
   for (int oy = -1; oy <= 1; ++oy) {
@@ -159,13 +195,15 @@ __device__ __forceinline__ void ph_restrict(const LevelDev& F, const LevelDev& C
       if (fx < 0 || fx >= F.nx) continue;
       float wx = w1d(fx, cx, C.nx, false);
       if (wx == 0.0f) continue;
-      s += wx * wy * F.r[fy * F.nx + fx];
+      s += wx * wy * resid_at(F, fy * F.nx + fx);
     }
   }
-  C.b[I] = C.inv_d[I] != 0.0f ? s : 0.0f;
+  C.b[I] = s;
 }
-__device__ __forceinline__ void ph_prolong(const LevelDev& F, const LevelDev& C, int i) {
-  if (F.inv_d[i] == 0.0f) return;
+
+// coarse correction P e at fine node i (0 at inactive fine nodes)
+__device__ __forceinline__ float pe_at(const LevelDev& F, const LevelDev& C, int i) {
+  if (F.inv_d[i] == 0.0f) return 0.0f;
   int fx = i % F.nx, fy = i / F.nx;
   int ax0 = fx >> 1, ax1 = -1;
   if (fx & 1) {
@@ -190,87 +228,182 @@ __device__ __forceinline__ void ph_prolong(const LevelDev& F, const LevelDev& C,
       s += wx * wy * C.x[ys[a] * C.nx + xs[b]];
     }
   }
-  F.x[i] += s;
+  return s;
 }
 
-__global__ __launch_bounds__(256) void k_jacobi_first(LevelDev L, float omega, const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) ph_jacobi_first(L, omega, i);
-}
-__global__ __launch_bounds__(256) void k_jacobi(LevelDev L, float omega, const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) ph_jacobi(L, omega, i);
-}
-__global__ __launch_bounds__(256) void k_residual(LevelDev L, const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) ph_residual(L, i);
-}
-__global__ __launch_bounds__(256) void k_restrict(LevelDev F, LevelDev C, const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < C.n) ph_restrict(F, C, i);
-}
-__global__ __launch_bounds__(256) void k_prolong(LevelDev F, LevelDev C, const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < F.n) ph_prolong(F, C, i);
+__device__ __forceinline__ float xnew_at(const LevelDev& F, const LevelDev& C, int j) {
+  return F.x[j] + pe_at(F, C, j);
 }
 
-// All levels first..last in one work-group: pre-smooth, residual, restrict
-// down to the coarsest, dense solve, then prolong and post-smooth back up.
-__global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs* __restrict__ A,
-                                                       const int32_t* done) {
+// x <- x + P e, then one Jacobi sweep; result in t
+__device__ __forceinline__ void ph_post1(const LevelDev& F, const LevelDev& C, float om, int i) {
+  float id = F.inv_d[i];
+  if (id == 0.0f) {
+    F.t[i] = 0.0f;
+    return;
+  }
+  int x = i % F.nx, y = i / F.nx;
+  float xi = xnew_at(F, C, i);
+  float ax;
+  if (!F.st) {
+    int d = F.code[i];
+    int s_ = y > 0 ? i - F.nx : (F.periodic ? i + (F.ny - 1) * F.nx : i);
+    int n_ = y < F.ny - 1 ? i + F.nx : (F.periodic ? i - (F.ny - 1) * F.nx : i);
+    float sum = xnew_at(F, C, i - 1) + xnew_at(F, C, i + 1) + xnew_at(F, C, s_) + xnew_at(F, C, n_);
+    ax = (float)d * xi - sum;
+  } else {
+    ax = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      int xx = x + (k % 3 - 1), yy = y + (k / 3 - 1);
+      if (xx < 0 || xx >= F.nx) continue;
+      if (yy < 0 || yy >= F.ny) {
+        if (!F.periodic) continue;
+        yy = (yy + F.ny) % F.ny;
+      }
+      int j = yy * F.nx + xx;
+      ax += F.st[(size_t)k * F.n + i] * (j == i ? xi : xnew_at(F, C, j));
+    }
+  }
+  F.t[i] = xi + om * id * (F.b[i] - ax);
+}
+
+__device__ __forceinline__ void ph_jacobi(const LevelDev& L, float om, int i) {
+  float id = L.inv_d[i];
+  if (id == 0.0f) {
+    L.t[i] = 0.0f;
+    return;
+  }
+  L.t[i] = L.x[i] + om * id * (L.b[i] - applyL(L, L.x, i));
+}
+
+__device__ __forceinline__ float omega_of(const LevelDev& L) { return L.st ? kOmegaC : kOmega0; }
+
+__global__ __launch_bounds__(256) void k_pre(LevelDev L, const int32_t* __restrict__ done) {
   if (*done) return;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int first = A->first, last = A->last;
-  for (int l = first; l < last; ++l) {
-    LevelDev L = A->lev[l];
-    float om = L.st ? kOmegaC : kOmega0;
-    for (int i = tid; i < L.n; i += nt) ph_jacobi_first(L, om, i);
-    __syncthreads();
-    for (int s = 1; s < kNu; ++s) {
-      for (int i = tid; i < L.n; i += nt) ph_jacobi(L, om, i);
-      __syncthreads();
-      for (int i = tid; i < L.n; i += nt) L.x[i] = L.t[i];
-      __syncthreads();
-    }
-    for (int i = tid; i < L.n; i += nt) ph_residual(L, i);
-    __syncthreads();
-    LevelDev C = A->lev[l + 1];
-    for (int i = tid; i < C.n; i += nt) ph_restrict(L, C, i);
-    __syncthreads();
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < L.n) ph_pre(L, omega_of(L), i);
+}
+__global__ __launch_bounds__(256) void k_resid_restrict(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
+  if (*done) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < C.n) ph_resid_restrict(F, C, i);
+}
+__global__ __launch_bounds__(256) void k_post1(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
+  if (*done) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < F.n) ph_post1(F, C, omega_of(F), i);
+}
+__global__ __launch_bounds__(256) void k_jacobi(LevelDev L, const int32_t* __restrict__ done) {
+  if (*done) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < L.n) ph_jacobi(L, omega_of(L), i);
+}
+
+__global__ __launch_bounds__(256) void k_residual(LevelDev L, float* r, const int32_t* __restrict__ done) {
+  if (*done) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < L.n) r[i] = resid_at(L, i);
+}
+
+// restriction of a stored residual r (fine) into C.b
+__global__ __launch_bounds__(256) void k_restrict(LevelDev F, const float* __restrict__ r, LevelDev C,
+                                                  const int32_t* __restrict__ done) {
+  if (*done) return;
+  int I = blockIdx.x * 256 + threadIdx.x;
+  if (I >= C.n) return;
+  if (C.inv_d[I] == 0.0f) {
+    C.b[I] = 0.0f;
+    return;
   }
-  {
-    LevelDev C = A->lev[last];
-    const int nc = A->nc;
-    for (int i = tid; i < nc; i += nt) {
-      const float* row = A->cinv + (size_t)i * nc;
-      float s = 0.0f;
-      for (int j = 0; j < nc; ++j) s += row[j] * C.b[j];
-      C.x[i] = s;
+  int cx = I % C.nx, cy = I / C.nx;
+  float s = 0.0f;
+  for (int oy = -1; oy <= 1; ++oy) {
+    int fy = 2 * cy + oy;
+    if (fy < 0 || fy >= F.ny) {
+      if (!F.periodic) continue;
+      fy = (fy + F.ny) % F.ny;
     }
-    __syncthreads();
+    float wy = w1d(fy, cy, C.ny, F.periodic != 0);
+    if (wy == 0.0f) continue;
+    for (int ox = -1; ox <= 1; ++ox) {
+      int fx = 2 * cx + ox;
+      if (fx < 0 || fx >= F.nx) continue;
+      float wx = w1d(fx, cx, C.nx, false);
+      if (wx == 0.0f) continue;
+      s += wx * wy * r[fy * F.nx + fx];
+    }
   }
-  for (int l = last - 1; l >= first; --l) {
-    LevelDev L = A->lev[l];
-    LevelDev C = A->lev[l + 1];
-    float om = L.st ? kOmegaC : kOmega0;
-    for (int i = tid; i < L.n; i += nt) ph_prolong(L, C, i);
-    __syncthreads();
-    for (int s = 0; s < kNu; ++s) {
-      for (int i = tid; i < L.n; i += nt) ph_jacobi(L, om, i);
-      __syncthreads();
-      for (int i = tid; i < L.n; i += nt) L.x[i] = L.t[i];
-      __syncthreads();
+  C.b[I] = s;
+}
+
+// x += P e
+__global__ __launch_bounds__(256) void k_prolong(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
+  if (*done) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < F.n) F.x[i] += pe_at(F, C, i);
+}
+
+// coarsest level: x = C^-1 b with the inverse stored column-major.  A block
+// owns 64 rows (one per lane); its 16 waves split the columns, and the partial
+// sums are added in LDS, so each thread runs only nc/16 independent loads.
+constexpr int kCoarseWaves = 16;
+__global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(LevelDev C,
+                                                                    const float* __restrict__ cinv,
+                                                                    const int32_t* __restrict__ done) {
+  __shared__ float sb[kDenseMax];
+  if (*done) return;
+  __shared__ float part[kCoarseWaves][64];
+  const int nc = C.n;
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) sb[j] = C.b[j];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane;
+  float s = 0.0f;
+  if (row < nc) {
+    const int chunk = (nc + kCoarseWaves - 1) / kCoarseWaves;
+    const int j0 = w * chunk, j1 = min(nc, j0 + chunk);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+      s0 += cinv[(size_t)j * nc + row] * sb[j];
+      s1 += cinv[(size_t)(j + 1) * nc + row] * sb[j + 1];
+      s2 += cinv[(size_t)(j + 2) * nc + row] * sb[j + 2];
+      s3 += cinv[(size_t)(j + 3) * nc + row] * sb[j + 3];
     }
+    for (; j < j1; ++j) s0 += cinv[(size_t)j * nc + row] * sb[j];
+    s = (s0 + s1) + (s2 + s3);
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && row < nc) {
+    float t = 0.0f;
+    for (int k = 0; k < kCoarseWaves; ++k) t += part[k][lane];
+    C.x[row] = t;
   }
 }
 
 // ----------------------------------------------------------- CG kernels
-__device__ __forceinline__ void block_reduce2(double a, double b, double* part) {
+// Vector kernels write per-block partial sums (fixed-order wave and block
+// reductions); a one-block kernel adds the partials in a fixed order and
+// updates the scalars, so every reduction is deterministic.
+struct CGArgs {
+  double *x, *r, *rold, *p, *Ap;
+  const double* bred;
+  float* b0;      // level-0 right-hand side of the V-cycle (float32 r)
+  const float* z; // level-0 V-cycle output
+  const uint8_t* code;
+  int nx, ny, per, n;
+  double* part;   // 2 * kMaxRedBlocks: partials of dir and dotz
+  double* part2;  // 2 * kMaxRedBlocks: partials of step (r.r)
+  double* scal;
+  int32_t* done;
+};
+
+// grid-stride loops so the number of partials stays <= kMaxRedBlocks
+#define GRID_LOOP(i, n) for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < (n); i += gridDim.x * kRedBlock)
+
+__device__ __forceinline__ void block_partials(double a, double b, double* part) {
   __shared__ double sa[kRedBlock / 64], sb[kRedBlock / 64];
   a = wave_sum(a);
   b = wave_sum(b);
@@ -280,126 +413,91 @@ __device__ __forceinline__ void block_reduce2(double a, double b, double* part) 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double ta = 0, tb = 0;
+    double pa = 0, pb = 0;
     for (int w = 0; w < kRedBlock / 64; ++w) {
-      ta += sa[w];
-      tb += sb[w];
+      pa += sa[w];
+      pb += sb[w];
     }
-    part[2 * blockIdx.x] = ta;
-    part[2 * blockIdx.x + 1] = tb;
+    part[2 * blockIdx.x] = pa;
+    part[2 * blockIdx.x + 1] = pb;
   }
 }
 
-// grid-stride loops so the number of partials stays <= kMaxRedBlocks
-#define GRID_LOOP(i, n) for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < (n); i += gridDim.x * kRedBlock)
-
-// r = b - A x ; partials (r.r, b.b)
-__global__ __launch_bounds__(kRedBlock) void k_cg_init_residual(const double* __restrict__ b,
-                                                                  const double* __restrict__ x,
-                                                                  double* r, const uint8_t* code,
-                                                                  int nx, int ny, int per, int n,
-                                                                  double* part) {
+// r = b - A x, b0 = f32(r), p = Ap = 0 ; partials (r.r, b.b)
+__global__ __launch_bounds__(kRedBlock) void k_cg_init(CGArgs g) {
   double rr = 0, bb = 0;
-  GRID_LOOP(i, n) {
-    double ri = code[i] ? b[i] - apply5<double>(x, code, i, nx, ny, per != 0) : 0.0;
-    r[i] = ri;
+  GRID_LOOP(i, g.n) {
+    double ri = g.code[i] ? g.bred[i] - apply5<double>(g.x, g.code, i, g.nx, g.ny, g.per != 0) : 0.0;
+    g.r[i] = ri;
+    g.b0[i] = (float)ri;
+    g.p[i] = 0.0;
+    g.Ap[i] = 0.0;
     rr += ri * ri;
-    bb += b[i] * b[i];
+    bb += g.bred[i] * g.bred[i];
   }
-  block_reduce2(rr, bb, part);
+  block_partials(rr, bb, g.part);
 }
 
-// b0 (float) = r ; used before the first V-cycle
-__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ r, float* b0, int n,
-                                                const int32_t* done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) b0[i] = (float)r[i];
-}
-
-// p = z ; partial (r.z, 0)
-__global__ __launch_bounds__(kRedBlock) void k_cg_p_init(const double* __restrict__ r,
-                                                           const float* __restrict__ z, double* p,
-                                                           int n, double* part) {
-  double rz = 0;
-  GRID_LOOP(i, n) {
-    double zi = z[i];
-    p[i] = zi;
-    rz += r[i] * zi;
-  }
-  block_reduce2(rz, 0.0, part);
-}
-
-// Ap = A p ; partial (p.Ap)
-__global__ __launch_bounds__(kRedBlock) void k_cg_apply(const double* __restrict__ p, double* Ap,
-                                                          const uint8_t* code, int nx, int ny,
-                                                          int per, int n, double* part,
-                                                          const int32_t* done) {
-  if (*done) return;
+// p = z + beta p ; Ap = A z + beta Ap ; partial p.Ap
+__global__ __launch_bounds__(kRedBlock) void k_cg_dir(CGArgs g) {
+  if (*g.done) return;
+  const double beta = g.scal[S_BETA];
   double s = 0;
-  GRID_LOOP(i, n) {
-    double a = apply5<double>(p, code, i, nx, ny, per != 0);
-    Ap[i] = a;
-    s += p[i] * a;
+  GRID_LOOP(i, g.n) {
+    double az = 0.0;
+    int d = g.code[i];
+    if (d) {
+      int x = i % g.nx, y = i / g.nx;
+      int sn = y > 0 ? i - g.nx : (g.per ? i + (g.ny - 1) * g.nx : i);
+      int nn = y < g.ny - 1 ? i + g.nx : (g.per ? i - (g.ny - 1) * g.nx : i);
+      double nb = (double)g.z[i - 1] + (double)g.z[i + 1] + (double)g.z[sn] + (double)g.z[nn];
+      az = (double)d * (double)g.z[i] - nb;
+      (void)x;
+    }
+    double pi = (double)g.z[i] + beta * g.p[i];
+    double api = az + beta * g.Ap[i];
+    g.p[i] = pi;
+    g.Ap[i] = api;
+    s += pi * api;
   }
-  block_reduce2(s, 0.0, part);
+  block_partials(s, 0.0, g.part);
 }
 
-// x += a p ; r_old = r ; r -= a Ap ; b0 = f32(r) ; partial (r.r)
-__global__ __launch_bounds__(kRedBlock) void k_cg_update(double* x, double* r, double* rold,
-                                                           const double* __restrict__ p,
-                                                           const double* __restrict__ Ap, float* b0,
-                                                           int n, const double* scal, double* part,
-                                                           const int32_t* done) {
-  if (*done) return;
-  const double a = scal[S_ALPHA];
+// x += a p ; r_old = r ; r -= a Ap ; b0 = f32(r) ; partial r.r
+__global__ __launch_bounds__(kRedBlock) void k_cg_step(CGArgs g) {
+  if (*g.done) return;
+  const double a = g.scal[S_ALPHA];
   double rr = 0;
-  GRID_LOOP(i, n) {
-    x[i] += a * p[i];
-    double ro = r[i];
-    rold[i] = ro;
-    double rn = ro - a * Ap[i];
-    r[i] = rn;
-    b0[i] = (float)rn;
+  GRID_LOOP(i, g.n) {
+    g.x[i] += a * g.p[i];
+    double ro = g.r[i];
+    g.rold[i] = ro;
+    double rn = ro - a * g.Ap[i];
+    g.r[i] = rn;
+    g.b0[i] = (float)rn;
     rr += rn * rn;
   }
-  block_reduce2(rr, 0.0, part);
+  block_partials(rr, 0.0, g.part2);
 }
 
 // partials (r.z, r_old.z)
-__global__ __launch_bounds__(kRedBlock) void k_cg_dot_z(const double* __restrict__ r,
-                                                          const double* __restrict__ rold,
-                                                          const float* __restrict__ z, int n,
-                                                          double* part, const int32_t* done) {
-  if (*done) return;
+__global__ __launch_bounds__(kRedBlock) void k_cg_dotz(CGArgs g) {
+  if (*g.done) return;
   double a = 0, b = 0;
-  GRID_LOOP(i, n) {
-    double zi = z[i];
-    a += r[i] * zi;
-    b += rold[i] * zi;
+  GRID_LOOP(i, g.n) {
+    double zi = g.z[i];
+    a += g.r[i] * zi;
+    b += g.rold[i] * zi;
   }
-  block_reduce2(a, b, part);
+  block_partials(a, b, g.part);
 }
 
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_cg_p_update(double* p, const float* __restrict__ z, int n,
-                                                     const double* scal, const int32_t* done) {
-  if (*done) return;
-  const double beta = scal[S_BETA];
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = (double)z[i] + beta * p[i];
-}
+enum { M_INIT = 0, M_ALPHA = 1, M_BETA0 = 3, M_BETA = 4 };
 
-// Fixed-order sum of the partials and the scalar recurrence.
-// mode 0: after init residual   -> S_RR, S_BB, tolerance, done if converged
-// mode 1: after p init          -> S_RZ
-// mode 2: after apply           -> S_ALPHA = rz / pAp
-// mode 3: after update          -> S_RR, done if converged, iteration count
-// mode 4: after dot_z           -> S_BETA (Polak-Ribiere), S_RZ
-__global__ __launch_bounds__(256) void k_cg_scalar(const double* part, int nparts, double* scal,
-                                                   int32_t* done, int mode, double rtol) {
+// fixed-order sum of n partial pairs by one block
+__device__ __forceinline__ void sum_partials(const double* part, int nparts, double* a_out,
+                                             double* b_out) {
   __shared__ double sa[256], sb[256];
-  if (mode != 0 && mode != 1 && *done) return;
   double a = 0, b = 0;
   for (int k = threadIdx.x; k < nparts; k += 256) {
     a += part[2 * k];
@@ -415,31 +513,51 @@ __global__ __launch_bounds__(256) void k_cg_scalar(const double* part, int npart
     }
     __syncthreads();
   }
+  *a_out = sa[0];
+  *b_out = sb[0];
+}
+
+// scalar recurrence: INIT (tolerance, done), ALPHA = rz / p.Ap, BETA
+// (Polak-Ribiere beta, rz) together with the convergence test on the r.r
+// partials of the preceding step kernel (an iteration that converged still
+// runs its V-cycle, whose output is then unused).
+__global__ __launch_bounds__(256) void k_cg_scalar(const double* part, const double* part2, int nparts,
+                                                   double* scal, int32_t* done, int mode) {
+  if (mode != M_INIT && *done) return;
+  double a, b, rr = 0.0, unused;
+  sum_partials(part, nparts, &a, &b);
+  if (mode == M_BETA) {
+    __syncthreads();
+    sum_partials(part2, nparts, &rr, &unused);
+  }
   if (threadIdx.x != 0) return;
-  a = sa[0];
-  b = sb[0];
-  if (mode == 0) {
+  if (mode == M_INIT) {
+    const double rt = scal[S_RTOL];
     scal[S_RR] = a;
     scal[S_BB] = b;
-    scal[S_TOL2] = rtol * rtol * b;
-    scal[S_ITERS] = 0;
+    scal[S_TOL2] = rt * rt * b;
+    scal[S_ITERS] = 0.0;
+    scal[S_BETA] = 0.0;
     *done = (a <= scal[S_TOL2]) ? 1 : 0;
-  } else if (mode == 1) {
-    scal[S_RZ] = a;
-  } else if (mode == 2) {
+  } else if (mode == M_ALPHA) {
     scal[S_PAP] = a;
     scal[S_ALPHA] = a != 0.0 ? scal[S_RZ] / a : 0.0;
-  } else if (mode == 3) {
-    scal[S_RR] = a;
-    scal[S_ITERS] += 1.0;
-    if (a <= scal[S_TOL2] || !(a == a)) *done = 1;
   } else {
-    double rz_old = scal[S_RZ];
-    double beta = rz_old != 0.0 ? (a - b) / rz_old : 0.0;
-    scal[S_BETA] = beta;
+    if (mode == M_BETA) {
+      scal[S_RR] = rr;
+      scal[S_ITERS] += 1.0;
+      if (rr <= scal[S_TOL2] || !(rr == rr)) {
+        *done = 1;
+        return;
+      }
+    }
+    const double rz_old = scal[S_RZ];
+    scal[S_BETA] = (mode == M_BETA0 || rz_old == 0.0) ? 0.0 : (a - b) / rz_old;
     scal[S_RZ] = a;
   }
 }
+
+__global__ void k_set_rtol(double* scal, double rtol) { scal[S_RTOL] = rtol; }
 
 // --------------------------------------------------- reduced system set-up
 // b_red[P] = -b[P] + sum_{non-unknown nbrs Q} b[Q] at unknown nodes, 0 else;
@@ -675,7 +793,8 @@ static HostLevel coarsen(const HostLevel& F) {
   return C;
 }
 
-// Dense inverse of the active block (Gauss-Jordan with partial pivoting).
+// Dense inverse of the active block (Gauss-Jordan with partial pivoting),
+// returned column-major: element (i, j) at j * n + i.
 static std::vector<float> dense_inverse(const HostLevel& L) {
   const int n = L.nx * L.ny;
   std::vector<int> act;
@@ -724,7 +843,7 @@ static std::vector<float> dense_inverse(const HostLevel& L) {
   }
   std::vector<float> out((size_t)n * n, 0.0f);
   for (int a = 0; a < m; ++a)
-    for (int b = 0; b < m; ++b) out[(size_t)act[a] * n + act[b]] = (float)Inv[(size_t)a * m + b];
+    for (int b = 0; b < m; ++b) out[(size_t)act[b] * n + act[a]] = (float)Inv[(size_t)a * m + b];
   return out;
 }
 
@@ -734,107 +853,136 @@ static int upload(void* dst, const void* src, size_t bytes) {
 
 static void destroy(bemcs_poisson_s* h) {
   if (!h) return;
+
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
   delete h;
 }
 
 // --------------------------------------------------------- V-cycle driver
+// Per level above the coarsest: fused pre-smoothing (2 Jacobi sweeps), then
+// residual + restriction (one fused kernel on small levels, two kernels on
+// large ones where recomputing residuals costs more than a launch), then on
+// the way up prolongation + two Jacobi sweeps (the first fused with the
+// prolongation on small levels).  The coarsest level is a dense solve.
 static void vcycle(bemcs_poisson_s* h, hipStream_t s) {
-  const int tf = h->tail_first;
-  for (int l = 0; l < tf; ++l) {
+  const int last = h->nlev - 1;
+  const int32_t* dn = h->done;
+  for (int l = 0; l < last; ++l) {
     LevelDev& L = h->lev[l];
-    float om = L.st ? kOmegaC : kOmega0;
+    LevelDev& C = h->lev[l + 1];
+    hipLaunchKernelGGL(k_pre, dim3(grid_for(L.n, 256)), dim3(256), 0, s, L, dn);
+    if (L.n <= kFuseBelow) {
+      hipLaunchKernelGGL(k_resid_restrict, dim3(grid_for(C.n, 256)), dim3(256), 0, s, L, C, dn);
+    } else {
+      hipLaunchKernelGGL(k_residual, dim3(grid_for(L.n, 256)), dim3(256), 0, s, L, h->rscr, dn);
+      hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.n, 256)), dim3(256), 0, s, L,
+                         (const float*)h->rscr, C, dn);
+    }
+  }
+  hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[last].n + 63) / 64), dim3(64 * kCoarseWaves), 0, s,
+                     h->lev[last], (const float*)h->cinv, dn);
+  for (int l = last - 1; l >= 0; --l) {
+    LevelDev& L = h->lev[l];
+    LevelDev& C = h->lev[l + 1];
     unsigned g = grid_for(L.n, 256);
-    hipLaunchKernelGGL(k_jacobi_first, dim3(g), dim3(256), 0, s, L, om, h->done);
-    for (int k = 1; k < kNu; ++k) {
-      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, om, h->done);
+    if (L.n <= kFuseBelow) {
+      hipLaunchKernelGGL(k_post1, dim3(g), dim3(256), 0, s, L, C, dn);
+      std::swap(L.x, L.t);
+    } else {
+      hipLaunchKernelGGL(k_prolong, dim3(g), dim3(256), 0, s, L, C, dn);
+      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, dn);
       std::swap(L.x, L.t);
     }
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, L, h->done);
-    LevelDev& C = h->lev[l + 1];
-    hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.n, 256)), dim3(256), 0, s, L, C, h->done);
-  }
-  hipLaunchKernelGGL(k_vcycle_tail, dim3(1), dim3(1024), 0, s, (const TailArgs*)h->tail_dev, h->done);
-  for (int l = tf - 1; l >= 0; --l) {
-    LevelDev& L = h->lev[l];
-    LevelDev& C = h->lev[l + 1];
-    float om = L.st ? kOmegaC : kOmega0;
-    unsigned g = grid_for(L.n, 256);
-    hipLaunchKernelGGL(k_prolong, dim3(g), dim3(256), 0, s, L, C, h->done);
-    for (int k = 0; k < kNu; ++k) {
-      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, om, h->done);
+    if (h->post_sweeps > 1) {
+      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, dn);
       std::swap(L.x, L.t);
     }
   }
 }
-
-// The level-0 float32 V-cycle output (z) after vcycle(); the ping-pong swaps
-// are even per level, so it is always lev[0].x.
-static inline float* vc_out(bemcs_poisson_s* h) { return h->lev[0].x; }
 
 static unsigned red_grid(int n) {
   unsigned g = grid_for(n, kRedBlock);
   return g > (unsigned)kMaxRedBlocks ? (unsigned)kMaxRedBlocks : g;
 }
 
+static CGArgs cg_args(bemcs_poisson_s* h) {
+  CGArgs g;
+  g.x = h->x;
+  g.r = h->r;
+  g.rold = h->rold;
+  g.p = h->p;
+  g.Ap = h->Ap;
+  g.bred = h->bred;
+  g.b0 = h->lev[0].b;
+  g.z = h->lev[0].x;
+  g.code = h->lev[0].code;
+  g.nx = h->nx;
+  g.ny = h->ny;
+  g.per = h->periodic;
+  g.n = h->nx * h->ny;
+  g.part = h->part;
+  g.part2 = h->part2;
+  g.scal = h->scal;
+  g.done = h->done;
+  return g;
+}
+
+
+static void enqueue_init(bemcs_poisson_s* h, const CGArgs& g, hipStream_t s) {
+  const unsigned rg = red_grid(g.n);
+  hipLaunchKernelGGL(k_cg_init, dim3(rg), dim3(kRedBlock), 0, s, g);
+  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
+                     (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_INIT);
+  vcycle(h, s);
+  hipLaunchKernelGGL(k_cg_dotz, dim3(rg), dim3(kRedBlock), 0, s, g);
+  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
+                     (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_BETA0);
+}
+
+static void enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, hipStream_t s) {
+  const unsigned rg = red_grid(g.n);
+  {
+    hipLaunchKernelGGL(k_cg_dir, dim3(rg), dim3(kRedBlock), 0, s, g);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
+                       (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_ALPHA);
+    hipLaunchKernelGGL(k_cg_step, dim3(rg), dim3(kRedBlock), 0, s, g);
+    vcycle(h, s);
+    hipLaunchKernelGGL(k_cg_dotz, dim3(rg), dim3(kRedBlock), 0, s, g);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
+                       (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_BETA);
+  }
+}
+
 // PCG on the reduced system already in h->bred with initial guess h->x.
 static int pcg(bemcs_poisson_s* h, double rtol, int max_iter, int* iters, double* relres,
                hipStream_t s) {
-  const int n = h->nx * h->ny;
-  const unsigned rg = red_grid(n);
-  const unsigned g = grid_for(n, 256);
-  LevelDev& L0 = h->lev[0];
-  BEMCS_CHECK_HIP(hipMemsetAsync(h->done, 0, sizeof(int32_t), s));
-  hipLaunchKernelGGL(k_cg_init_residual, dim3(rg), dim3(kRedBlock), 0, s, h->bred, h->x, h->r,
-                     L0.code, h->nx, h->ny, h->periodic, n, h->part);
-  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, h->part, (int)rg, h->scal, h->done, 0,
-                     rtol);
-  hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, s, h->r, L0.b, n, h->done);
-  vcycle(h, s);
-  hipLaunchKernelGGL(k_cg_p_init, dim3(rg), dim3(kRedBlock), 0, s, h->r, vc_out(h), h->p, n, h->part);
-  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, h->part, (int)rg, h->scal, h->done, 1,
-                     rtol);
+  const CGArgs g = cg_args(h);
+  hipLaunchKernelGGL(k_set_rtol, dim3(1), dim3(1), 0, s, h->scal, rtol);
+  enqueue_init(h, g, s);
   BEMCS_CHECK_LAUNCH();
-  int it = 0;
-  const int group = 4;
-  while (it < max_iter) {
-    int ng = std::min(group, max_iter - it);
-    for (int k = 0; k < ng; ++k) {
-      hipLaunchKernelGGL(k_cg_apply, dim3(rg), dim3(kRedBlock), 0, s, h->p, h->Ap, L0.code, h->nx,
-                         h->ny, h->periodic, n, h->part, h->done);
-      hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, h->part, (int)rg, h->scal, h->done,
-                         2, rtol);
-      hipLaunchKernelGGL(k_cg_update, dim3(rg), dim3(kRedBlock), 0, s, h->x, h->r, h->rold, h->p,
-                         h->Ap, L0.b, n, h->scal, h->part, h->done);
-      hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, h->part, (int)rg, h->scal, h->done,
-                         3, rtol);
-      vcycle(h, s);
-      hipLaunchKernelGGL(k_cg_dot_z, dim3(rg), dim3(kRedBlock), 0, s, h->r, h->rold, vc_out(h), n,
-                         h->part, h->done);
-      hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, h->part, (int)rg, h->scal, h->done,
-                         4, rtol);
-      hipLaunchKernelGGL(k_cg_p_update, dim3(g), dim3(256), 0, s, h->p, vc_out(h), n, h->scal,
-                         h->done);
-    }
+  // launch as many iterations as the previous solve needed, then check
+  int launched = 0;
+  int batch = std::max(1, h->pred_iters);
+  for (;;) {
+    for (int k = 0; k < batch && launched < max_iter; ++k, ++launched) enqueue_iter(h, g, s);
     BEMCS_CHECK_LAUNCH();
-    it += ng;
     BEMCS_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, sizeof(double) * S_COUNT,
                                    hipMemcpyDeviceToHost, s));
     BEMCS_CHECK_HIP(hipStreamSynchronize(s));
     double rr = h->host_scal[S_RR], tol2 = h->host_scal[S_TOL2];
-    if (rr <= tol2 || !(rr == rr)) break;
+    if (rr <= tol2 || !(rr == rr) || launched >= max_iter) break;
+    batch = 1;
   }
-  BEMCS_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, sizeof(double) * S_COUNT,
-                                 hipMemcpyDeviceToHost, s));
-  BEMCS_CHECK_HIP(hipStreamSynchronize(s));
   double rr = h->host_scal[S_RR], bb = h->host_scal[S_BB];
-  if (iters) *iters = (int)h->host_scal[S_ITERS];
+  int it = (int)h->host_scal[S_ITERS];
+  h->pred_iters = it;
+  if (iters) *iters = it;
   double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
   if (relres) *relres = rel;
   if (!(rr <= h->host_scal[S_TOL2])) {
     set_error("Poisson MG-PCG: relative residual %.3e above tolerance %.3e after %d iterations",
-              rel, rtol, (int)h->host_scal[S_ITERS]);
+              rel, rtol, it);
     return BEMCS_ENOCONV;
   }
   return BEMCS_OK;
@@ -923,7 +1071,7 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     if (ncx < 3 || ncy < (per ? 4 : 3)) break;
     H.push_back(coarsen(F));
   }
-  BEMCS_REQUIRE(H.back().nx * H.back().ny <= 4096,
+  BEMCS_REQUIRE(H.back().nx * H.back().ny <= kDenseMax,
                 "bemcs_poisson_create: coarsest level too large for the dense solve");
 
   bemcs_poisson_s* h = new bemcs_poisson_s();
@@ -933,6 +1081,9 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
   h->nlev = (int)H.size();
   h->host_scal = nullptr;
   h->have_guess = false;
+  h->pred_iters = 8;
+  const char* ps = getenv("BEMCS_MG_POST_SWEEPS");
+  h->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
   bool ok = true;
   auto A = [&](size_t bytes) {
     void* p = dalloc(h, bytes);
@@ -971,13 +1122,11 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     D.x = (float*)A(sizeof(float) * D.n);
     D.t = (float*)A(sizeof(float) * D.n);
     D.b = (float*)A(sizeof(float) * D.n);
-    D.r = (float*)A(sizeof(float) * D.n);
     if (ok) ok = hipMemset(D.x, 0, sizeof(float) * D.n) == hipSuccess;
+    if (ok) ok = hipMemset(D.t, 0, sizeof(float) * D.n) == hipSuccess;
+    if (ok) ok = hipMemset(D.b, 0, sizeof(float) * D.n) == hipSuccess;
   }
   if (ok) {
-    int tf = h->nlev - 1;
-    while (tf > 0 && h->lev[tf - 1].n <= kTailNodes) --tf;
-    h->tail_first = tf;
     std::vector<float> cinv = dense_inverse(H.back());
     h->nc = H.back().nx * H.back().ny;
     h->cinv = (float*)A(sizeof(float) * cinv.size());
@@ -998,19 +1147,14 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     h->part = (double*)A(sizeof(double) * 2 * kMaxRedBlocks);
     h->scal = (double*)A(sizeof(double) * S_COUNT);
     h->done = (int32_t*)A(sizeof(int32_t) * 4);
-    h->tail_dev = (TailArgs*)A(sizeof(TailArgs));
+    h->rscr = (float*)A(sizeof(float) * n);
+    h->part2 = (double*)A(sizeof(double) * 2 * kMaxRedBlocks);
+    if (ok) ok = hipMemset(h->done, 0, sizeof(int32_t) * 4) == hipSuccess;
+    if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
+    if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->x, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->scal, 0, sizeof(double) * S_COUNT) == hipSuccess;
     if (ok) ok = hipHostMalloc((void**)&h->host_scal, sizeof(double) * S_COUNT, 0) == hipSuccess;
-    if (ok) {
-      TailArgs ta;
-      for (int l = 0; l < h->nlev; ++l) ta.lev[l] = h->lev[l];
-      ta.first = h->tail_first;
-      ta.last = h->nlev - 1;
-      ta.cinv = h->cinv;
-      ta.nc = h->nc;
-      ok = upload(h->tail_dev, &ta, sizeof(TailArgs)) == 0;
-    }
   }
   if (!ok) {
     destroy(h);

@@ -52,7 +52,7 @@ class DigitalTwinSimulator:
 
     kernel_backend = "hip"
 
-    def __init__(self, device=None, sort_interval=20, poisson_rtol=1e-13, poisson_max_iter=200):
+    def __init__(self, device=None, sort_interval=20, poisson_rtol=1e-10, poisson_max_iter=200):
         require_gpu()
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.lib = _lib.load()
@@ -97,6 +97,7 @@ class DigitalTwinSimulator:
         self.last_poisson_relres = 0.0
         # optional {kernel name: [(start_event, end_event), ...]} filled per step
         self.kernel_timing = None
+        self.poisson_iter_log = []
 
     # ----------------------------------------------------------------- misc
     def set_material(self, name=None, props=None):
@@ -356,6 +357,7 @@ class DigitalTwinSimulator:
                                             stream_ptr()), "poisson_picard")
         self.last_poisson_iters = its.value
         self.last_poisson_relres = rel.value
+        self.poisson_iter_log.append(its.value)
 
     # ------------------------------------------------------------ particles
     @property
