@@ -85,6 +85,19 @@ double bemcs_det_exp(double x);
 double bemcs_det_log(double x);
 void bemcs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Cell-index division.  The push divides positions by float32(dx) and
+ * float32(dy) (IEEE, as NumPy).  For a given divisor the library uses a
+ * reciprocal-multiply-plus-remainder form instead, after checking on the
+ * device that it equals IEEE division for all 2^32 float32 dividends; the
+ * first use of a new divisor runs that check (about 2 ms) and synchronises the
+ * stream.  BEMCS_FASTDIV=0 in the environment keeps IEEE division.  No
+ * reference counterpart (diagnostics of this library).
+ *   bemcs_fastdiv_check: number of dividends where the fast form with
+ *     reciprocal r differs from IEEE a / b (synchronous).
+ *   bemcs_fastdiv_enabled: 1 if the push uses the fast form for b. */
+int bemcs_fastdiv_check(float b, float r, uint64_t* mismatches, bemcs_stream_t stream);
+int bemcs_fastdiv_enabled(float b, bemcs_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Device-resident time step.
  * ---------------------------------------------------------------------- */

@@ -74,6 +74,52 @@ def test_push_bit_exact(lib, n_rep, with_b):
         assert np.array_equal(g.view(np.uint32), a.view(np.uint32)), np.max(np.abs(g - a))
 
 
+@pytest.mark.parametrize("b", [0.0605, 0.0579, 6.1925 / 1023, 2.3184 / 383, 0.05, 1.0, 3.0])
+def test_fastdiv_certified_divisors(lib, b):
+    """The reciprocal division equals IEEE division for all 2^32 dividends."""
+    print(f"[gpu] fastdiv b={b}", flush=True)
+    b32 = np.float32(b)
+    r32 = np.float32(1.0) / b32
+    bad = C.c_uint64(99)
+    assert lib.bemcs_fastdiv_check(float(b32), float(r32), C.byref(bad), _stream()) == 0
+    assert bad.value == 0
+    assert lib.bemcs_fastdiv_enabled(float(b32), _stream()) == 1
+
+
+def test_fastdiv_check_detects_wrong_reciprocal(lib):
+    """The exhaustive check is live: a reciprocal 2^-10 off fails it."""
+    print("[gpu] fastdiv planted error", flush=True)
+    b32 = np.float32(0.0605)
+    r32 = np.float32(1.0) / b32 * np.float32(1.0 + 2.0 ** -10)
+    bad = C.c_uint64(0)
+    assert lib.bemcs_fastdiv_check(float(b32), float(r32), C.byref(bad), _stream()) == 0
+    assert bad.value > 1_000_000
+    assert lib.bemcs_fastdiv_enabled(0.0, _stream()) == 0
+
+
+def test_push_ieee_division_path_equal(lib, monkeypatch):
+    """BEMCS_FASTDIV=0 (IEEE division) and the default give the same bits."""
+    print("[gpu] push fastdiv vs ieee", flush=True)
+    nx, ny, dx, dy = 97, 41, 0.0605, 0.0579
+    r = np.random.default_rng(5)
+    Ex = _dev(r.normal(0, 2e5, (ny, nx)).astype(np.float32))
+    Ey = _dev(r.normal(0, 2e5, (ny, nx)).astype(np.float32))
+    st = _random_state(4, 30001, nx, ny, dx, dy, pad=2.0)
+    out = []
+    for env in (None, "0"):
+        if env is None:
+            monkeypatch.delenv("BEMCS_FASTDIV", raising=False)
+        else:
+            monkeypatch.setenv("BEMCS_FASTDIV", env)
+        d = [_dev(a) for a in st]
+        rc = lib.bemcs_push_boris(*[_p(t) for t in d], len(st[0]), _p(Ex), _p(Ey), None, None, None,
+                                  nx, ny, dx, dy, 1e-10, -9.6e9, 5, _stream())
+        assert rc == 0
+        out.append([t.cpu().numpy().view(np.uint32) for t in d])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_drop_in_push_numpy_arrays(lib):
     from thruster_cx_code_amd.kernels import push_particles_boris_hip
     nx, ny, dx, dy = 33, 17, 0.05, 0.05

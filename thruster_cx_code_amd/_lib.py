@@ -81,6 +81,8 @@ _SIGS = {
     "bemcs_det_exp": (f64, [f64]),
     "bemcs_det_log": (f64, [f64]),
     "bemcs_philox4x32_10": (None, [vp, vp, vp]),
+    "bemcs_fastdiv_check": (i32, [C.c_float, C.c_float, vp, vp]),
+    "bemcs_fastdiv_enabled": (i32, [C.c_float, vp]),
     "bemcs_ion_step": (i32, [vp, vp, vp, vp, i64, vp]),
     "bemcs_electron_step": (i32, [vp, vp, vp, i64, vp]),
     "bemcs_inject": (i32, [vp, vp, i64, i64, vp]),

@@ -47,6 +47,39 @@ inline unsigned grid_for(int64_t n, int block) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
+// ---------------------------------------------------------------- division
+// Correctly rounded float32 quotient a / b by a fixed divisor.  The FAST form
+// is Markstein's reciprocal iteration: q0 = a*r with r = RN(1/b), the
+// remainder a - b*q0 taken exactly by an FMA, and one correction
+// q = q0 + (a - b*q0)*r.  It costs a multiply and two FMAs instead of the
+// scaled Newton sequence of IEEE division.  The library enables it for a
+// divisor only after comparing it with IEEE division on every one of the 2^32
+// float32 dividends on the device (fastdiv_ok), so the quotients, and every
+// gather index and weight derived from them, are bit-identical.
+struct Divisor {
+  float b;  // divisor
+  float r;  // RN(1 / b)
+};
+
+template <bool FAST>
+__device__ __forceinline__ float div_rn(float a, Divisor d) {
+  // Zeros (signed-zero quotient), subnormal remainders and a*r overflow are
+  // left to IEEE division; positions never reach those magnitudes, so the
+  // branch costs a compare.
+  if (FAST && fabsf(a) >= 0x1p-100f && fabsf(a) <= 0x1p100f) {
+    float q0 = a * d.r;
+    float e = fmaf(-q0, d.b, a);
+    return fmaf(e, d.r, q0);
+  }
+  return a / d.b;
+}
+
+// True when div_rn<true> equals IEEE division for divisor b on all float32
+// dividends (checked once per divisor and cached; the first check of a new
+// divisor synchronises `s`).  BEMCS_FASTDIV=0 in the environment disables it.
+bool fastdiv_ok(float b, hipStream_t s);
+inline Divisor make_divisor(float b) { return Divisor{b, 1.0f / b}; }
+
 // ----------------------------------------------------------------- indices
 // NumPy cast semantics on x86: a float->int conversion that is out of range
 // or NaN yields the most negative integer, which the reference's np.clip then

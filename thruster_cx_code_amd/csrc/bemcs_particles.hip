@@ -12,6 +12,12 @@
 // :188-254): float32 positions and velocities, float64 interpolation weights
 // and kicks (float32 - int32 promotes to float64 there), float32 drift.  With
 // -ffp-contract=off every result is bit-identical to it on the same inputs.
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_map>
+
 #include "bemcs_internal.h"
 
 namespace bemcs {
@@ -35,14 +41,15 @@ struct EField {
   }
 };
 
-// One gather + Boris + drift substep.  h = 0.5*q_m*dt (f64), dt32 = f32(dt).
-template <bool INTERLEAVED, bool HAS_B>
+// One gather + Boris + drift substep.  h = 0.5*q_m*dt (f64), dt32 = f32(dt);
+// dx, dy are the float32 cell sizes.
+template <bool INTERLEAVED, bool HAS_B, bool FAST>
 __device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, float& vy, float& vz,
                                               const EField<INTERLEAVED>& E, const float* Bx,
                                               const float* By, const float* Bz, int nx, int ny,
-                                              float dx32, float dy32, double h, float dt32) {
-  float gx = x / dx32;
-  float gy = y / dy32;
+                                              Divisor dx, Divisor dy, double h, float dt32) {
+  float gx = div_rn<FAST>(x, dx);
+  float gy = div_rn<FAST>(y, dy);
   int i0 = floor_i32_np(gx, nx - 2);
   int j0 = floor_i32_np(gy, ny - 2);
   double fx = (double)gx - (double)i0;
@@ -97,20 +104,20 @@ __device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, flo
   y = y + (vy * dt32) * 1000.0f;
 }
 
-template <bool HAS_B>
+template <bool HAS_B, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_push_boris(float* __restrict__ x, float* __restrict__ y,
                                                         float* __restrict__ vx, float* __restrict__ vy,
                                                         float* __restrict__ vz, int64_t n,
                                                         const float* Ex, const float* Ey,
                                                         const float* Bx, const float* By,
-                                                        const float* Bz, int nx, int ny, float dx32,
-                                                        float dy32, double h, float dt32, int n_rep) {
+                                                        const float* Bz, int nx, int ny, Divisor dx,
+                                                        Divisor dy, double h, float dt32, int n_rep) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float px = x[i], py = y[i], pvx = vx[i], pvy = vy[i], pvz = vz[i];
   EField<false> E{Ex, Ey};
   for (int r = 0; r < n_rep; ++r)
-    boris_substep<false, HAS_B>(px, py, pvx, pvy, pvz, E, Bx, By, Bz, nx, ny, dx32, dy32, h, dt32);
+    boris_substep<false, HAS_B, FAST>(px, py, pvx, pvy, pvz, E, Bx, By, Bz, nx, ny, dx, dy, h, dt32);
   x[i] = px;
   y[i] = py;
   vx[i] = pvx;
@@ -211,9 +218,10 @@ __global__ void k_substep_finalize(const unsigned long long* ds_bits, const int3
 }
 
 // ---------------------------------------------------------- fused ion step
-template <bool PERIODIC>
+template <bool PERIODIC, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_species_t out,
-                                                      bemcs_species_t eout, bemcs_ion_step_t p) {
+                                                      bemcs_species_t eout, bemcs_ion_step_t p,
+                                                      Divisor ddx, Divisor ddy) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t cnt = *in.count;
   const int64_t n = cnt < in.cap ? cnt : in.cap;
@@ -242,24 +250,24 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
     EField<true> E{p.Exy, p.Exy + 1};
     if (active)
       for (int s = 0; s < nsub; ++s)
-        boris_substep<true, false>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, p.dx32,
-                                   p.dy32, h, dts32);
+        boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
+                                         ddx, ddy, h, dts32);
   }
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
 
   // ---- D. hits, OOB, diagnostics
   bool hit = false, oob = false;
   if (active) {
-    int ix = round_i64_np(x / p.dx32, 0, nx - 1);
-    int iy = round_i64_np(y / p.dy32, 0, ny - 1);
+    int ix = round_i64_np(div_rn<FAST>(x, ddx), 0, nx - 1);
+    int iy = round_i64_np(div_rn<FAST>(y, ddy), 0, ny - 1);
     hit = bound[iy * nx + ix] != 0;
     if (!hit) {
       for (int k = 1; k <= 8; ++k) {
         float f = (float)k / 8.0f;
         float xi = x0 + (x - x0) * f;
         float yi = y0 + (y - y0) * f;
-        int cx = round_i64_np(xi / p.dx32, 0, nx - 1);
-        int cy = round_i64_np(yi / p.dy32, 0, ny - 1);
+        int cx = round_i64_np(div_rn<FAST>(xi, ddx), 0, nx - 1);
+        int cy = round_i64_np(div_rn<FAST>(yi, ddy), 0, ny - 1);
         if (bound[cy * nx + cx]) {
           hit = true;
           break;
@@ -300,14 +308,14 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
     bool keep_e = false;
     int ekey = -1;
     if (spawn) {
-      int eix = round_i64_np(ex / p.dx32, 0, nx - 1);
-      int eiy = round_i64_np(ey / p.dy32, 0, ny - 1);
+      int eix = round_i64_np(div_rn<FAST>(ex, ddx), 0, nx - 1);
+      int eiy = round_i64_np(div_rn<FAST>(ey, ddy), 0, ny - 1);
       bool ehit = bound[eiy * nx + eix] != 0;
       bool eout = (ex < 0.0f) || (ex > p.Lx32) || (ey < 0.0f) || (ey > p.Ly32) || isnan(ex);
       keep_e = !(ehit || eout);
       if (keep_e) {
-        int dix = round_i32_np(ex / p.dx32, 1, nx - 2);
-        int diy = round_i32_np(ey / p.dy32, 1, ny - 2);
+        int dix = round_i32_np(div_rn<FAST>(ex, ddx), 1, nx - 2);
+        int diy = round_i32_np(div_rn<FAST>(ey, ddy), 1, ny - 2);
         ekey = diy * nx + dix;
       }
     }
@@ -434,11 +442,11 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
       out.vy[slot] = vy;
       out.vz[slot] = vz;
       out.tag[slot] = tag;
-      int dix = round_i32_np(x / p.dx32, 1, nx - 2);
-      int diy = round_i32_np(y / p.dy32, 1, ny - 2);
+      int dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
+      int diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
       dkey = diy * nx + dix;
-      int pix = trunc_i64_np(x / p.dx32, nx - 1);
-      int piy = trunc_i64_np(y / p.dy32, ny - 1);
+      int pix = trunc_i64_np(div_rn<FAST>(x, ddx), nx - 1);
+      int piy = trunc_i64_np(div_rn<FAST>(y, ddy), ny - 1);
       pkey = piy * nx + pix;
     } else {
       atomicOr(&p.diag->overflow, 2u);
@@ -449,9 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
 }
 
 // ------------------------------------------------------ fused electron step
-template <bool PERIODIC>
+template <bool PERIODIC, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, bemcs_species_t out,
-                                                           bemcs_electron_step_t p) {
+                                                           bemcs_electron_step_t p, Divisor ddx,
+                                                           Divisor ddy) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t cnt = *in.count;
   const int64_t n = cnt < in.cap ? cnt : in.cap;
@@ -474,14 +483,14 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, be
     EField<true> E{p.Exy, p.Exy + 1};
     if (active)
       for (int s = 0; s < nsub; ++s)
-        boris_substep<true, false>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, p.dx32,
-                                   p.dy32, h, dts32);
+        boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
+                                         ddx, ddy, h, dts32);
   }
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
   bool keep = false;
   if (active) {
-    int ix = round_i64_np(x / p.dx32, 0, nx - 1);
-    int iy = round_i64_np(y / p.dy32, 0, ny - 1);
+    int ix = round_i64_np(div_rn<FAST>(x, ddx), 0, nx - 1);
+    int iy = round_i64_np(div_rn<FAST>(y, ddy), 0, ny - 1);
     bool hit = p.is_bound[iy * nx + ix] != 0;
     bool outb = (x < 0.0f) || (x > p.Lx32) || (y < 0.0f) || (y > p.Ly32) || isnan(x);
     keep = !(hit || outb);
@@ -495,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, be
       out.vx[slot] = vx;
       out.vy[slot] = vy;
       out.vz[slot] = vz;
-      int dix = round_i32_np(x / p.dx32, 1, nx - 2);
-      int diy = round_i32_np(y / p.dy32, 1, ny - 2);
+      int dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
+      int diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
       dkey = diy * nx + dix;
     } else {
       atomicOr(p.overflow, 4u);
@@ -677,11 +686,74 @@ int scan_exclusive(uint32_t* data, int64_t n, uint32_t* sums, hipStream_t s) {
   return BEMCS_OK;
 }
 
+// ------------------------------------------------- division certification
+// Counts the float32 dividends a (all 2^32 bit patterns) for which
+// div_rn<true>(a, {b, r}) differs from IEEE a / b; two NaNs count as equal.
+__global__ __launch_bounds__(256) void k_check_div(float b, float r,
+                                                   unsigned long long* mismatches) {
+  const Divisor d{b, r};
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  unsigned long long bad = 0;
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < (1ull << 32); k += stride) {
+    float a = __uint_as_float((uint32_t)k);
+    float f = div_rn<true>(a, d);
+    float e = div_rn<false>(a, d);
+    bool same = __float_as_uint(f) == __float_as_uint(e) || (isnan(f) && isnan(e));
+    bad += same ? 0 : 1;
+  }
+  if (bad) atomicAdd(mismatches, bad);
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+static std::mutex g_div_mu;
+static std::unordered_map<uint32_t, bool> g_div_ok;
+static unsigned long long* g_div_count = nullptr;
+
+// Runs k_check_div on stream s and waits; returns BEMCS_OK and the count.
+static int check_divisor(float b, float r, hipStream_t s, unsigned long long* out) {
+  if (!g_div_count) BEMCS_CHECK_HIP(hipMalloc(&g_div_count, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, g_div_count, 0ull);
+  hipLaunchKernelGGL(k_check_div, dim3(8192), dim3(256), 0, s, b, r, g_div_count);
+  BEMCS_CHECK_LAUNCH();
+  BEMCS_CHECK_HIP(hipMemcpyAsync(out, g_div_count, sizeof(*out), hipMemcpyDeviceToHost, s));
+  BEMCS_CHECK_HIP(hipStreamSynchronize(s));
+  return BEMCS_OK;
+}
+
+bool fastdiv_ok(float b, hipStream_t s) {
+  const char* env = getenv("BEMCS_FASTDIV");
+  if (env && env[0] == '0') return false;
+  if (!(fabsf(b) >= 1e-30f && fabsf(b) <= 1e30f)) return false;
+  uint32_t key;
+  memcpy(&key, &b, sizeof(key));
+  std::lock_guard<std::mutex> lock(g_div_mu);
+  auto it = g_div_ok.find(key);
+  if (it != g_div_ok.end()) return it->second;
+  unsigned long long bad = 1;
+  bool ok = check_divisor(b, 1.0f / b, s, &bad) == BEMCS_OK && bad == 0;
+  g_div_ok[key] = ok;
+  return ok;
+}
+
 }  // namespace bemcs
 
 using namespace bemcs;
 
 extern "C" {
+
+int bemcs_fastdiv_check(float b, float r, uint64_t* mismatches, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(mismatches, "bemcs_fastdiv_check: null argument");
+  unsigned long long bad = 0;
+  std::lock_guard<std::mutex> lock(g_div_mu);
+  int rc = check_divisor(b, r, as_stream(stream), &bad);
+  *mismatches = bad;
+  return rc;
+}
+
+int bemcs_fastdiv_enabled(float b, bemcs_stream_t stream) {
+  return fastdiv_ok(b, as_stream(stream)) ? 1 : 0;
+}
 
 int bemcs_push_boris(float* x, float* y, float* vx, float* vy, float* vz, int64_t n,
                      const float* Ex, const float* Ey, const float* Bx, const float* By,
@@ -695,12 +767,12 @@ int bemcs_push_boris(float* x, float* y, float* vx, float* vy, float* vz, int64_
   double h = 0.5 * q_m * dt;
   float dt32 = (float)dt;
   hipStream_t s = as_stream(stream);
-  if (has_b)
-    hipLaunchKernelGGL(k_push_boris<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, x, y, vx,
-                       vy, vz, n, Ex, Ey, Bx, By, Bz, nx, ny, (float)dx, (float)dy, h, dt32, n_rep);
-  else
-    hipLaunchKernelGGL(k_push_boris<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, x, y, vx,
-                       vy, vz, n, Ex, Ey, Bx, By, Bz, nx, ny, (float)dx, (float)dy, h, dt32, n_rep);
+  const Divisor ddx = make_divisor((float)dx), ddy = make_divisor((float)dy);
+  const bool fast = fastdiv_ok(ddx.b, s) && fastdiv_ok(ddy.b, s);
+  auto kern = has_b ? (fast ? k_push_boris<true, true> : k_push_boris<true, false>)
+                    : (fast ? k_push_boris<false, true> : k_push_boris<false, false>);
+  hipLaunchKernelGGL(kern, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, x, y, vx, vy, vz, n, Ex,
+                     Ey, Bx, By, Bz, nx, ny, ddx, ddy, h, dt32, n_rep);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -773,12 +845,12 @@ int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(n_bound <= in->cap && out->cap >= n_bound, "bemcs_ion_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
-  if (p->periodic_y)
-    hipLaunchKernelGGL(k_ion_step<true>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in,
-                       *out, *e_out, *p);
-  else
-    hipLaunchKernelGGL(k_ion_step<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in,
-                       *out, *e_out, *p);
+  const Divisor ddx = make_divisor(p->dx32), ddy = make_divisor(p->dy32);
+  const bool fast = fastdiv_ok(ddx.b, s) && fastdiv_ok(ddy.b, s);
+  auto kern = p->periodic_y ? (fast ? k_ion_step<true, true> : k_ion_step<true, false>)
+                            : (fast ? k_ion_step<false, true> : k_ion_step<false, false>);
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out, *e_out,
+                     *p, ddx, ddy);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -792,12 +864,12 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(n_bound <= in->cap, "bemcs_electron_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
-  if (p->periodic_y)
-    hipLaunchKernelGGL(k_electron_step<true>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s,
-                       *in, *out, *p);
-  else
-    hipLaunchKernelGGL(k_electron_step<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s,
-                       *in, *out, *p);
+  const Divisor ddx = make_divisor(p->dx32), ddy = make_divisor(p->dy32);
+  const bool fast = fastdiv_ok(ddx.b, s) && fastdiv_ok(ddy.b, s);
+  auto kern = p->periodic_y ? (fast ? k_electron_step<true, true> : k_electron_step<true, false>)
+                            : (fast ? k_electron_step<false, true> : k_electron_step<false, false>);
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out, *p, ddx,
+                     ddy);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

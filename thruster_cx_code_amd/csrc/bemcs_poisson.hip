@@ -28,9 +28,6 @@ namespace bemcs {
 constexpr int kMaxLevels = 20;
 constexpr int kCoarseMax = 512;  // coarsening stops at or below this many nodes
 constexpr int kDenseMax = 4096;  // largest coarsest level the dense solve accepts
-constexpr int kFuseBelow = 32768;  // levels this small fuse residual+restrict, prolong+sweep
-constexpr int kRedBlock = 256;
-constexpr int kMaxRedBlocks = 1024;
 constexpr float kOmega0 = 0.8f;  // Jacobi damping, 5-point fine level
 constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels
 constexpr int kNu = 2;           // pre- and post-smoothing sweeps
@@ -59,21 +56,22 @@ struct bemcs_poisson_s {
   uint8_t* kind;
   int32_t* mirror;   // final mirror (unknown or Dirichlet) of Neumann nodes, -1 otherwise
   int32_t* mirror1;  // first hop of Neumann nodes, -1 otherwise
-  double *x, *r, *p, *Ap, *rold, *bred, *Vnew;
-  double* part;      // reduction partials
-  double* scal;      // device scalars, see S_* below
+  double *x, *p, *Ap, *bred, *Vnew;
+  double* R[2];       // residuals r_k, alternating
+  int nparts;         // partial pairs per reduction buffer (level-0 tiles)
+  double *pa, *pr, *pi;
+  double* pz[2];
+  double* scal;       // device scalars, see S_* below
   int32_t* done;
-  double* host_scal; // pinned copy
+  double* host_scal;  // pinned copy
   bool have_guess;
-  float* rscr;        // float32 residual scratch (largest level)
-  double* part2;     // step partials
-  int pred_iters;    // iterations the last solve needed (launch that many first)
+  int pred_iters;     // iterations the last solve needed (launch that many first)
   int post_sweeps;    // Jacobi sweeps after prolongation (1 or 2)
 };
 
 namespace bemcs {
 
-enum { S_RZ = 0, S_PAP, S_RR, S_BB, S_ALPHA, S_BETA, S_TOL2, S_ITERS, S_RTOL, S_COUNT = 16 };
+enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_COUNT = 8 };
 
 // ------------------------------------------------------------- 1D transfer
 __host__ __device__ __forceinline__ float w1d(int f, int I, int nc, bool periodic) {
@@ -104,104 +102,51 @@ __device__ __forceinline__ T apply5(const T* __restrict__ v, const uint8_t* __re
   return (T)d * v[i] - s;
 }
 
-__device__ __forceinline__ float apply9(const LevelDev& L, const float* __restrict__ v, int i) {
-  int x = i % L.nx, y = i / L.nx;
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    int dx = k % 3 - 1, dy = k / 3 - 1;
-    int xx = x + dx, yy = y + dy;
-    if (xx < 0 || xx >= L.nx) continue;
-    if (yy < 0 || yy >= L.ny) {
-      if (!L.periodic) continue;
-      yy = (yy + L.ny) % L.ny;
-    }
-    float c = L.st[(size_t)k * L.n + i];
-    s += c * v[yy * L.nx + xx];
+
+
+// ---------------------------------------------------------- V-cycle kernels
+// One launch per level on the way down and one on the way up.  A work-group
+// owns a tile and keeps what it needs, with halos, in LDS:
+//   down (fine F -> coarse C), one coarse kTC x kTC tile: two damped-Jacobi
+//     sweeps from zero on F (x1 = w D^-1 b on a halo-3 region, x2 on halo 2),
+//     the residual b - A x2 on halo 1 and its restriction (transpose of the
+//     bilinear prolongation) into C.b; x2 is stored to F.x for the owned fine
+//     nodes (the 2 kTC x 2 kTC fine nodes under the coarse tile).
+//   up (C -> F), one fine kTF x kTF tile: x + P e on a halo-`sweeps` region,
+//     then `sweeps` Jacobi sweeps; the result goes to F.t.
+// The result of a level is F.t (C.x on the coarsest level, from the dense
+// solve).  Halo values are recomputed by every tile that needs them from
+// inputs that no block of the same launch writes.
+constexpr int kTC = 16;             // coarse tile edge of a down launch
+constexpr int kTF = 2 * kTC;        // fine tile edge of an up launch
+constexpr int kDR = 2 * kTC + 5;    // x1 region edge of a down launch (halo 3)
+
+static_assert(kTC * kTC == 256, "one coarse node per thread in the restriction");
+
+__device__ __forceinline__ int gnode(const LevelDev& L, int gx, int gy) {
+  if (gx < 0 || gx >= L.nx) return -1;
+  if (gy < 0 || gy >= L.ny) {
+    if (!L.periodic) return -1;
+    gy = (gy % L.ny + L.ny) % L.ny;
   }
-  return s;
+  return gy * L.nx + gx;
 }
 
-__device__ __forceinline__ float applyL(const LevelDev& L, const float* v, int i) {
-  return L.st ? apply9(L, v, i) : apply5<float>(v, L.code, i, L.nx, L.ny, L.periodic != 0);
-}
-
-// --------------------------------------------------------- V-cycle phases
-// Device functions shared by the multi-block kernels and the single
-// work-group tail kernel.  A V-cycle does per level: two Jacobi sweeps from
-// zero (pre), residual fused with restriction, prolongation fused with the
-// first post sweep, second post sweep.  The post sweeps write t and the
-// pointers are swapped twice, so x is the output buffer again afterwards.
-
-// first Jacobi iterate from x = 0
-__device__ __forceinline__ float x1_at(const LevelDev& L, float om, int j) {
-  return om * L.inv_d[j] * L.b[j];
-}
-
-__device__ __forceinline__ float apply_x1(const LevelDev& L, float om, int i) {
-  int x = i % L.nx, y = i / L.nx;
+// A v at global node g, v held in LDS with row width W, c = &v[local node]
+__device__ __forceinline__ float lds_apply(const LevelDev& L, const float* c, int W, int g) {
   if (!L.st) {
-    int d = L.code[i];
-    int s_ = y > 0 ? i - L.nx : (L.periodic ? i + (L.ny - 1) * L.nx : i);
-    int n_ = y < L.ny - 1 ? i + L.nx : (L.periodic ? i - (L.ny - 1) * L.nx : i);
-    float sum = x1_at(L, om, i - 1) + x1_at(L, om, i + 1) + x1_at(L, om, s_) + x1_at(L, om, n_);
-    return (float)d * x1_at(L, om, i) - sum;
+    float nb = (c[-1] + c[1]) + (c[-W] + c[W]);
+    return (float)L.code[g] * c[0] - nb;
   }
-  float s = 0.0f;
+  float a = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    int xx = x + (k % 3 - 1), yy = y + (k / 3 - 1);
-    if (xx < 0 || xx >= L.nx) continue;
-    if (yy < 0 || yy >= L.ny) {
-      if (!L.periodic) continue;
-      yy = (yy + L.ny) % L.ny;
-    }
-    s += L.st[(size_t)k * L.n + i] * x1_at(L, om, yy * L.nx + xx);
-  }
-  return s;
+  for (int k = 0; k < 9; ++k) a += L.st[(size_t)k * L.n + g] * c[(k / 3 - 1) * W + (k % 3 - 1)];
+  return a;
 }
 
-__device__ __forceinline__ void ph_pre(const LevelDev& L, float om, int i) {
-  float id = L.inv_d[i];
-  if (id == 0.0f) {
-    L.x[i] = 0.0f;
-    return;
-  }
-  float x1 = om * id * L.b[i];
-  L.x[i] = x1 + om * id * (L.b[i] - apply_x1(L, om, i));
-}
+__device__ __forceinline__ float omega_of(const LevelDev& L) { return L.st ? kOmegaC : kOmega0; }
 
-__device__ __forceinline__ float resid_at(const LevelDev& F, int f) {
-  return F.inv_d[f] != 0.0f ? F.b[f] - applyL(F, F.x, f) : 0.0f;
-}
-
-__device__ __forceinline__ void ph_resid_restrict(const LevelDev& F, const LevelDev& C, int I) {
-  if (C.inv_d[I] == 0.0f) {
-    C.b[I] = 0.0f;
-    return;
-  }
-  int cx = I % C.nx, cy = I / C.nx;
-  float s = 0.0f;
-  for (int oy = -1; oy <= 1; ++oy) {
-    int fy = 2 * cy + oy;
-    if (fy < 0 || fy >= F.ny) {
-      if (!F.periodic) continue;
-      fy = (fy + F.ny) % F.ny;
-    }
-    float wy = w1d(fy, cy, C.ny, F.periodic != 0);
-    if (wy == 0.0f) continue;
-    for (int ox = -1; ox <= 1; ++ox) {
-      int fx = 2 * cx + ox;
-      if (fx < 0 || fx >= F.nx) continue;
-      float wx = w1d(fx, cx, C.nx, false);
-      if (wx == 0.0f) continue;
-      s += wx * wy * resid_at(F, fy * F.nx + fx);
-    }
-  }
-  C.b[I] = s;
-}
-
-// coarse correction P e at fine node i (0 at inactive fine nodes)
+// coarse correction P e at fine node i (0 at inactive fine nodes), e = C.x
 __device__ __forceinline__ float pe_at(const LevelDev& F, const LevelDev& C, int i) {
   if (F.inv_d[i] == 0.0f) return 0.0f;
   int fx = i % F.nx, fy = i / F.nx;
@@ -231,117 +176,276 @@ __device__ __forceinline__ float pe_at(const LevelDev& F, const LevelDev& C, int
   return s;
 }
 
-__device__ __forceinline__ float xnew_at(const LevelDev& F, const LevelDev& C, int j) {
-  return F.x[j] + pe_at(F, C, j);
+// ---------------------------------------------------- block reductions
+// Partial sums are pairs (a, b) per block.  A consumer block adds n pairs in
+// a fixed order, so every block of a launch obtains the same bits and takes
+// the same decisions without a separate scalar kernel.
+__device__ __forceinline__ void block_partials(double a, double b, double* part, int slot) {
+  __shared__ double sa[4], sb[4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane_id() == 0) {
+    sa[threadIdx.x / 64] = a;
+    sb[threadIdx.x / 64] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * slot] = (sa[0] + sa[1]) + (sa[2] + sa[3]);
+    part[2 * slot + 1] = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+  }
 }
 
-// x <- x + P e, then one Jacobi sweep; result in t
-__device__ __forceinline__ void ph_post1(const LevelDev& F, const LevelDev& C, float om, int i) {
-  float id = F.inv_d[i];
-  if (id == 0.0f) {
-    F.t[i] = 0.0f;
+template <int NT = 256>
+__device__ __forceinline__ void block_sum2(const double* __restrict__ part, int n, double& a_out,
+                                           double& b_out) {
+  __shared__ double sa[NT], sb[NT];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < n; k += NT) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o];
+      sb[threadIdx.x] += sb[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  a_out = sa[0];
+  b_out = sb[0];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------- CG state
+// Flexible (Polak-Ribiere) PCG in float64.  Iteration k:
+//   dir   : beta_k = (r_{k-1}.z_{k-1} - r_{k-2}.z_{k-1}) / r_{k-2}.z_{k-2},
+//           p = z + beta p, Ap = A z + beta Ap, partials p.Ap
+//   step  : alpha = r_{k-1}.z_{k-1} / p.Ap, x += alpha p,
+//           r_k = r_{k-1} - alpha Ap, partials r_k.r_k (fused into the first
+//           down launch, whose right-hand side is f32(r_k))
+//   check : the launch after the step stops the solve when r.r <= tol^2 b.b
+//   V-cycle z_k = M r_k, the last up launch adds partials (r_k.z_k, r_{k-1}.z_k)
+// r alternates between two vectors (r_k in R[k&1]) and the r.z partials
+// between two buffers, so no launch overwrites what another block of the same
+// launch reads.
+struct CGArgs {
+  double* x;
+  double* R[2];
+  double *p, *Ap;
+  const double* bred;
+  float* b0;       // level-0 right-hand side of the V-cycle
+  const float* z;  // level-0 V-cycle output
+  const uint8_t* code;
+  int nx, ny, per, n;
+  int nparts;      // partial pairs per buffer (one per level-0 tile)
+  double* pa;      // p.Ap
+  double* pr;      // r.r
+  double* pz[2];   // (r_k.z_k, r_{k-1}.z_k) of V-cycle k in pz[k&1]
+  double* pi;      // init: (r0.r0, b.b)
+  double* scal;
+  int32_t* done;
+};
+
+struct CheckArgs {
+  const double* pr;
+  int nparts;
+  double* scal;
+  int32_t* done;
+  int k;  // iteration whose step precedes this launch; 0 = no check
+};
+
+// Convergence test of iteration ca.k (every block computes it identically;
+// all NT threads of the block take part).
+template <int NT = 256>
+__device__ __forceinline__ bool check_converged(const CheckArgs& ca) {
+  double rr, unused;
+  block_sum2<NT>(ca.pr, ca.nparts, rr, unused);
+  const bool stop = !(rr > ca.scal[S_TOL2]);  // converged, or NaN
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    ca.scal[S_RR] = rr;
+    ca.scal[S_ITERS] = (double)ca.k;
+    if (stop) *ca.done = 1;
+  }
+  return stop;
+}
+
+// Launches that run a convergence test can see `done` change while they run
+// (block 0 sets it); a block reads it once and shares it, so all of its waves
+// take the same branch around the barriers that follow.
+__device__ __forceinline__ bool block_done(const int32_t* done) {
+  __shared__ int s;
+  if (threadIdx.x == 0) s = *(volatile const int32_t*)done;
+  __syncthreads();
+  return s != 0;
+}
+
+// ------------------------------------------------------------ down launch
+template <bool STEP>
+__global__ __launch_bounds__(256) void k_down(LevelDev F, LevelDev C, CGArgs g, int k,
+                                              CheckArgs ca) {
+  if (ca.k > 0) {
+    if (block_done(ca.done) || check_converged(ca)) return;
+  } else if (*ca.done) {
     return;
   }
-  int x = i % F.nx, y = i / F.nx;
-  float xi = xnew_at(F, C, i);
-  float ax;
-  if (!F.st) {
-    int d = F.code[i];
-    int s_ = y > 0 ? i - F.nx : (F.periodic ? i + (F.ny - 1) * F.nx : i);
-    int n_ = y < F.ny - 1 ? i + F.nx : (F.periodic ? i - (F.ny - 1) * F.nx : i);
-    float sum = xnew_at(F, C, i - 1) + xnew_at(F, C, i + 1) + xnew_at(F, C, s_) + xnew_at(F, C, n_);
-    ax = (float)d * xi - sum;
-  } else {
-    ax = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      int xx = x + (k % 3 - 1), yy = y + (k / 3 - 1);
-      if (xx < 0 || xx >= F.nx) continue;
-      if (yy < 0 || yy >= F.ny) {
-        if (!F.periodic) continue;
-        yy = (yy + F.ny) % F.ny;
+  constexpr int W1 = kDR, W2 = kDR - 2, W3 = kDR - 4;
+  __shared__ float sb[W1 * W1], sid[W1 * W1], s1[W1 * W1], s2[W2 * W2], sr[W3 * W3];
+  const float om = omega_of(F);
+  const int CX0 = blockIdx.x * kTC, CY0 = blockIdx.y * kTC;
+  const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
+  double alpha = 0.0, rr = 0.0;
+  const double* rp = g.R[(k + 1) & 1];
+  double* rn_out = g.R[k & 1];
+  if (STEP) {
+    double rz, pap, u;
+    block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
+    block_sum2(g.pa, g.nparts, pap, u);
+    alpha = pap != 0.0 ? rz / pap : 0.0;
+  }
+  for (int t = threadIdx.x; t < W1 * W1; t += 256) {
+    const int lx = t % W1, ly = t / W1;
+    const int gy = oy + ly;
+    const int gi = gnode(F, ox + lx, gy);
+    float b = 0.0f, id = 0.0f;
+    if (gi >= 0) {
+      id = F.inv_d[gi];
+      if (STEP) {
+        const double rn = rp[gi] - alpha * g.Ap[gi];
+        b = (float)rn;
+        if (lx >= 3 && lx < 3 + kTF && ly >= 3 && ly < 3 + kTF && gy < F.ny) {
+          g.x[gi] += alpha * g.p[gi];
+          rn_out[gi] = rn;
+          F.b[gi] = b;
+          rr += rn * rn;
+        }
+      } else {
+        b = F.b[gi];
       }
-      int j = yy * F.nx + xx;
-      ax += F.st[(size_t)k * F.n + i] * (j == i ? xi : xnew_at(F, C, j));
+    }
+    sb[t] = b;
+    sid[t] = id;
+    s1[t] = om * id * b;
+  }
+  if (STEP) block_partials(rr, 0.0, g.pr, blockIdx.y * gridDim.x + blockIdx.x);
+  __syncthreads();
+  for (int t = threadIdx.x; t < W2 * W2; t += 256) {
+    const int lx = t % W2 + 1, ly = t / W2 + 1;
+    const int li = ly * W1 + lx;
+    const int gy = oy + ly;
+    const int gi = gnode(F, ox + lx, gy);
+    const float id = sid[li];
+    float v = 0.0f;
+    if (id != 0.0f) v = s1[li] + om * id * (sb[li] - lds_apply(F, &s1[li], W1, gi));
+    s2[t] = v;
+    if (gi >= 0 && lx >= 3 && lx < 3 + kTF && ly >= 3 && ly < 3 + kTF && gy < F.ny) F.x[gi] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W3 * W3; t += 256) {
+    const int lx = t % W3 + 2, ly = t / W3 + 2;
+    const int li = ly * W1 + lx;
+    const float id = sid[li];
+    float r = 0.0f;
+    if (id != 0.0f) {
+      const int gi = gnode(F, ox + lx, oy + ly);
+      r = sb[li] - lds_apply(F, &s2[(ly - 1) * W2 + (lx - 1)], W2, gi);
+    }
+    sr[t] = r;
+  }
+  __syncthreads();
+  const int cx = CX0 + (int)(threadIdx.x % kTC), cy = CY0 + (int)(threadIdx.x / kTC);
+  if (cx < C.nx && cy < C.ny) {
+    const int I = cy * C.nx + cx;
+    float s = 0.0f;
+    if (C.inv_d[I] != 0.0f) {
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int fyu = 2 * cy + dy;
+        int fy = fyu;
+        if (fy < 0 || fy >= F.ny) {
+          if (!F.periodic) continue;
+          fy = (fy + F.ny) % F.ny;
+        }
+        const float wy = w1d(fy, cy, C.ny, F.periodic != 0);
+        if (wy == 0.0f) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int fx = 2 * cx + dx;
+          if (fx < 0 || fx >= F.nx) continue;
+          const float wx = w1d(fx, cx, C.nx, false);
+          if (wx == 0.0f) continue;
+          s += wx * wy * sr[(fyu - (2 * CY0 - 1)) * W3 + (fx - (2 * CX0 - 1))];
+        }
+      }
+    }
+    C.b[I] = s;
+  }
+}
+
+// -------------------------------------------------------------- up launch
+// C.x must point at the coarse result (C.t, or C.x of the coarsest level).
+template <int SWEEPS, bool DOT>
+__global__ __launch_bounds__(256) void k_up(LevelDev F, LevelDev C, CGArgs g, int k,
+                                            const int32_t* __restrict__ done) {
+  if (*done) return;
+  constexpr int W0 = kTF + 2 * SWEEPS, W1 = W0 - 2;
+  __shared__ float sx[W0 * W0], sb[W0 * W0], sid[W0 * W0], s1[W1 * W1];
+  const float om = omega_of(F);
+  const int FX0 = blockIdx.x * kTF, FY0 = blockIdx.y * kTF;
+  const int ox = FX0 - SWEEPS, oy = FY0 - SWEEPS;
+  for (int t = threadIdx.x; t < W0 * W0; t += 256) {
+    const int lx = t % W0, ly = t / W0;
+    const int gi = gnode(F, ox + lx, oy + ly);
+    float v = 0.0f, b = 0.0f, id = 0.0f;
+    if (gi >= 0) {
+      id = F.inv_d[gi];
+      b = F.b[gi];
+      if (id != 0.0f) v = F.x[gi] + pe_at(F, C, gi);
+    }
+    sx[t] = v;
+    sb[t] = b;
+    sid[t] = id;
+  }
+  __syncthreads();
+  const float* last = sx;
+  int WL = W0, off = 0;  // local coordinates of `last` are (lx - off, ly - off)
+  if (SWEEPS == 2) {
+    for (int t = threadIdx.x; t < W1 * W1; t += 256) {
+      const int lx = t % W1 + 1, ly = t / W1 + 1;
+      const int li = ly * W0 + lx;
+      const float id = sid[li];
+      float v = 0.0f;
+      if (id != 0.0f) {
+        const int gi = gnode(F, ox + lx, oy + ly);
+        v = sx[li] + om * id * (sb[li] - lds_apply(F, &sx[li], W0, gi));
+      }
+      s1[t] = v;
+    }
+    __syncthreads();
+    last = s1;
+    WL = W1;
+    off = 1;
+  }
+  double rz = 0.0, roz = 0.0;
+  for (int t = threadIdx.x; t < kTF * kTF; t += 256) {
+    const int lx = t % kTF + SWEEPS, ly = t / kTF + SWEEPS;
+    const int gx = ox + lx, gy = oy + ly;
+    if (gx >= F.nx || gy >= F.ny) continue;
+    const int gi = gy * F.nx + gx;
+    const int li = ly * W0 + lx;
+    const float id = sid[li];
+    float v = 0.0f;
+    if (id != 0.0f) {
+      const float* c = &last[(ly - off) * WL + (lx - off)];
+      v = c[0] + om * id * (sb[li] - lds_apply(F, c, WL, gi));
+    }
+    F.t[gi] = v;
+    if (DOT) {
+      rz += g.R[k & 1][gi] * (double)v;
+      roz += g.R[(k + 1) & 1][gi] * (double)v;
     }
   }
-  F.t[i] = xi + om * id * (F.b[i] - ax);
-}
-
-__device__ __forceinline__ void ph_jacobi(const LevelDev& L, float om, int i) {
-  float id = L.inv_d[i];
-  if (id == 0.0f) {
-    L.t[i] = 0.0f;
-    return;
-  }
-  L.t[i] = L.x[i] + om * id * (L.b[i] - applyL(L, L.x, i));
-}
-
-__device__ __forceinline__ float omega_of(const LevelDev& L) { return L.st ? kOmegaC : kOmega0; }
-
-__global__ __launch_bounds__(256) void k_pre(LevelDev L, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) ph_pre(L, omega_of(L), i);
-}
-__global__ __launch_bounds__(256) void k_resid_restrict(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < C.n) ph_resid_restrict(F, C, i);
-}
-__global__ __launch_bounds__(256) void k_post1(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < F.n) ph_post1(F, C, omega_of(F), i);
-}
-__global__ __launch_bounds__(256) void k_jacobi(LevelDev L, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) ph_jacobi(L, omega_of(L), i);
-}
-
-__global__ __launch_bounds__(256) void k_residual(LevelDev L, float* r, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < L.n) r[i] = resid_at(L, i);
-}
-
-// restriction of a stored residual r (fine) into C.b
-__global__ __launch_bounds__(256) void k_restrict(LevelDev F, const float* __restrict__ r, LevelDev C,
-                                                  const int32_t* __restrict__ done) {
-  if (*done) return;
-  int I = blockIdx.x * 256 + threadIdx.x;
-  if (I >= C.n) return;
-  if (C.inv_d[I] == 0.0f) {
-    C.b[I] = 0.0f;
-    return;
-  }
-  int cx = I % C.nx, cy = I / C.nx;
-  float s = 0.0f;
-  for (int oy = -1; oy <= 1; ++oy) {
-    int fy = 2 * cy + oy;
-    if (fy < 0 || fy >= F.ny) {
-      if (!F.periodic) continue;
-      fy = (fy + F.ny) % F.ny;
-    }
-    float wy = w1d(fy, cy, C.ny, F.periodic != 0);
-    if (wy == 0.0f) continue;
-    for (int ox = -1; ox <= 1; ++ox) {
-      int fx = 2 * cx + ox;
-      if (fx < 0 || fx >= F.nx) continue;
-      float wx = w1d(fx, cx, C.nx, false);
-      if (wx == 0.0f) continue;
-      s += wx * wy * r[fy * F.nx + fx];
-    }
-  }
-  C.b[I] = s;
-}
-
-// x += P e
-__global__ __launch_bounds__(256) void k_prolong(LevelDev F, LevelDev C, const int32_t* __restrict__ done) {
-  if (*done) return;
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < F.n) F.x[i] += pe_at(F, C, i);
+  if (DOT) block_partials(rz, roz, g.pz[k & 1], blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // coarsest level: x = C^-1 b with the inverse stored column-major.  A block
@@ -350,10 +454,14 @@ __global__ __launch_bounds__(256) void k_prolong(LevelDev F, LevelDev C, const i
 constexpr int kCoarseWaves = 16;
 __global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(LevelDev C,
                                                                     const float* __restrict__ cinv,
-                                                                    const int32_t* __restrict__ done) {
+                                                                    CheckArgs ca) {
   __shared__ float sb[kDenseMax];
-  if (*done) return;
   __shared__ float part[kCoarseWaves][64];
+  if (ca.k > 0) {
+    if (block_done(ca.done) || check_converged<64 * kCoarseWaves>(ca)) return;
+  } else if (*ca.done) {
+    return;
+  }
   const int nc = C.n;
   for (int j = threadIdx.x; j < nc; j += blockDim.x) sb[j] = C.b[j];
   __syncthreads();
@@ -378,186 +486,104 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(LevelDev C,
   __syncthreads();
   if (w == 0 && row < nc) {
     float t = 0.0f;
-    for (int k = 0; k < kCoarseWaves; ++k) t += part[k][lane];
+    for (int q = 0; q < kCoarseWaves; ++q) t += part[q][lane];
     C.x[row] = t;
   }
 }
 
 // ----------------------------------------------------------- CG kernels
-// Vector kernels write per-block partial sums (fixed-order wave and block
-// reductions); a one-block kernel adds the partials in a fixed order and
-// updates the scalars, so every reduction is deterministic.
-struct CGArgs {
-  double *x, *r, *rold, *p, *Ap;
-  const double* bred;
-  float* b0;      // level-0 right-hand side of the V-cycle (float32 r)
-  const float* z; // level-0 V-cycle output
-  const uint8_t* code;
-  int nx, ny, per, n;
-  double* part;   // 2 * kMaxRedBlocks: partials of dir and dotz
-  double* part2;  // 2 * kMaxRedBlocks: partials of step (r.r)
-  double* scal;
-  int32_t* done;
-};
+// grid-stride loops over g.nparts blocks, one partial pair per block
+#define GRID_LOOP(i, n) for (int i = blockIdx.x * 256 + threadIdx.x; i < (n); i += gridDim.x * 256)
 
-// grid-stride loops so the number of partials stays <= kMaxRedBlocks
-#define GRID_LOOP(i, n) for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < (n); i += gridDim.x * kRedBlock)
-
-__device__ __forceinline__ void block_partials(double a, double b, double* part) {
-  __shared__ double sa[kRedBlock / 64], sb[kRedBlock / 64];
-  a = wave_sum(a);
-  b = wave_sum(b);
-  if (lane_id() == 0) {
-    sa[threadIdx.x / 64] = a;
-    sb[threadIdx.x / 64] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double pa = 0, pb = 0;
-    for (int w = 0; w < kRedBlock / 64; ++w) {
-      pa += sa[w];
-      pb += sb[w];
-    }
-    part[2 * blockIdx.x] = pa;
-    part[2 * blockIdx.x + 1] = pb;
-  }
-}
-
-// r = b - A x, b0 = f32(r), p = Ap = 0 ; partials (r.r, b.b)
-__global__ __launch_bounds__(kRedBlock) void k_cg_init(CGArgs g) {
+// r0 = b - A x -> R[0], R[1] = 0, b0 = f32(r0); partials (r0.r0, b.b)
+__global__ __launch_bounds__(256) void k_cg_init(CGArgs g) {
   double rr = 0, bb = 0;
   GRID_LOOP(i, g.n) {
     double ri = g.code[i] ? g.bred[i] - apply5<double>(g.x, g.code, i, g.nx, g.ny, g.per != 0) : 0.0;
-    g.r[i] = ri;
+    g.R[0][i] = ri;
+    g.R[1][i] = 0.0;
     g.b0[i] = (float)ri;
-    g.p[i] = 0.0;
-    g.Ap[i] = 0.0;
     rr += ri * ri;
     bb += g.bred[i] * g.bred[i];
   }
-  block_partials(rr, bb, g.part);
+  block_partials(rr, bb, g.pi, blockIdx.x);
 }
 
-// p = z + beta p ; Ap = A z + beta Ap ; partial p.Ap
-__global__ __launch_bounds__(kRedBlock) void k_cg_dir(CGArgs g) {
+// tolerance and the initial convergence test
+__global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs g, double rtol) {
+  double rr, bb;
+  block_sum2(g.pi, g.nparts, rr, bb);
+  if (threadIdx.x != 0) return;
+  g.scal[S_RTOL] = rtol;
+  g.scal[S_RR] = rr;
+  g.scal[S_BB] = bb;
+  g.scal[S_TOL2] = rtol * rtol * bb;
+  g.scal[S_ITERS] = 0.0;
+  *g.done = (rr <= g.scal[S_TOL2]) ? 1 : 0;
+}
+
+// p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap
+__global__ __launch_bounds__(256) void k_cg_dir(CGArgs g, int k) {
   if (*g.done) return;
-  const double beta = g.scal[S_BETA];
+  double beta = 0.0;
+  if (k > 1) {
+    double rz, roz, rz_old, u;
+    block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, roz);
+    block_sum2(g.pz[k & 1], g.nparts, rz_old, u);
+    beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
+  }
   double s = 0;
   GRID_LOOP(i, g.n) {
     double az = 0.0;
-    int d = g.code[i];
+    const int d = g.code[i];
     if (d) {
-      int x = i % g.nx, y = i / g.nx;
-      int sn = y > 0 ? i - g.nx : (g.per ? i + (g.ny - 1) * g.nx : i);
-      int nn = y < g.ny - 1 ? i + g.nx : (g.per ? i - (g.ny - 1) * g.nx : i);
-      double nb = (double)g.z[i - 1] + (double)g.z[i + 1] + (double)g.z[sn] + (double)g.z[nn];
+      const int x = i % g.nx, y = i / g.nx;
+      const int sn = y > 0 ? i - g.nx : (g.per ? i + (g.ny - 1) * g.nx : i);
+      const int nn = y < g.ny - 1 ? i + g.nx : (g.per ? i - (g.ny - 1) * g.nx : i);
+      const double nb = (double)g.z[i - 1] + (double)g.z[i + 1] + (double)g.z[sn] + (double)g.z[nn];
       az = (double)d * (double)g.z[i] - nb;
       (void)x;
     }
-    double pi = (double)g.z[i] + beta * g.p[i];
-    double api = az + beta * g.Ap[i];
+    const double pi = k > 1 ? (double)g.z[i] + beta * g.p[i] : (double)g.z[i];
+    const double api = k > 1 ? az + beta * g.Ap[i] : az;
     g.p[i] = pi;
     g.Ap[i] = api;
     s += pi * api;
   }
-  block_partials(s, 0.0, g.part);
+  block_partials(s, 0.0, g.pa, blockIdx.x);
 }
 
-// x += a p ; r_old = r ; r -= a Ap ; b0 = f32(r) ; partial r.r
-__global__ __launch_bounds__(kRedBlock) void k_cg_step(CGArgs g) {
+// single-level hierarchy: the step without a fused down sweep
+__global__ __launch_bounds__(256) void k_cg_step(CGArgs g, int k) {
   if (*g.done) return;
-  const double a = g.scal[S_ALPHA];
+  double rz, pap, u;
+  block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
+  block_sum2(g.pa, g.nparts, pap, u);
+  const double a = pap != 0.0 ? rz / pap : 0.0;
+  const double* rp = g.R[(k + 1) & 1];
+  double* rn_out = g.R[k & 1];
   double rr = 0;
   GRID_LOOP(i, g.n) {
     g.x[i] += a * g.p[i];
-    double ro = g.r[i];
-    g.rold[i] = ro;
-    double rn = ro - a * g.Ap[i];
-    g.r[i] = rn;
+    const double rn = rp[i] - a * g.Ap[i];
+    rn_out[i] = rn;
     g.b0[i] = (float)rn;
     rr += rn * rn;
   }
-  block_partials(rr, 0.0, g.part2);
+  block_partials(rr, 0.0, g.pr, blockIdx.x);
 }
 
-// partials (r.z, r_old.z)
-__global__ __launch_bounds__(kRedBlock) void k_cg_dotz(CGArgs g) {
+// single-level hierarchy: partials (r_k.z, r_{k-1}.z)
+__global__ __launch_bounds__(256) void k_cg_dotz(CGArgs g, int k) {
   if (*g.done) return;
   double a = 0, b = 0;
   GRID_LOOP(i, g.n) {
-    double zi = g.z[i];
-    a += g.r[i] * zi;
-    b += g.rold[i] * zi;
+    const double zi = g.z[i];
+    a += g.R[k & 1][i] * zi;
+    b += g.R[(k + 1) & 1][i] * zi;
   }
-  block_partials(a, b, g.part);
+  block_partials(a, b, g.pz[k & 1], blockIdx.x);
 }
-
-enum { M_INIT = 0, M_ALPHA = 1, M_BETA0 = 3, M_BETA = 4 };
-
-// fixed-order sum of n partial pairs by one block
-__device__ __forceinline__ void sum_partials(const double* part, int nparts, double* a_out,
-                                             double* b_out) {
-  __shared__ double sa[256], sb[256];
-  double a = 0, b = 0;
-  for (int k = threadIdx.x; k < nparts; k += 256) {
-    a += part[2 * k];
-    b += part[2 * k + 1];
-  }
-  sa[threadIdx.x] = a;
-  sb[threadIdx.x] = b;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      sa[threadIdx.x] += sa[threadIdx.x + o];
-      sb[threadIdx.x] += sb[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  *a_out = sa[0];
-  *b_out = sb[0];
-}
-
-// scalar recurrence: INIT (tolerance, done), ALPHA = rz / p.Ap, BETA
-// (Polak-Ribiere beta, rz) together with the convergence test on the r.r
-// partials of the preceding step kernel (an iteration that converged still
-// runs its V-cycle, whose output is then unused).
-__global__ __launch_bounds__(256) void k_cg_scalar(const double* part, const double* part2, int nparts,
-                                                   double* scal, int32_t* done, int mode) {
-  if (mode != M_INIT && *done) return;
-  double a, b, rr = 0.0, unused;
-  sum_partials(part, nparts, &a, &b);
-  if (mode == M_BETA) {
-    __syncthreads();
-    sum_partials(part2, nparts, &rr, &unused);
-  }
-  if (threadIdx.x != 0) return;
-  if (mode == M_INIT) {
-    const double rt = scal[S_RTOL];
-    scal[S_RR] = a;
-    scal[S_BB] = b;
-    scal[S_TOL2] = rt * rt * b;
-    scal[S_ITERS] = 0.0;
-    scal[S_BETA] = 0.0;
-    *done = (a <= scal[S_TOL2]) ? 1 : 0;
-  } else if (mode == M_ALPHA) {
-    scal[S_PAP] = a;
-    scal[S_ALPHA] = a != 0.0 ? scal[S_RZ] / a : 0.0;
-  } else {
-    if (mode == M_BETA) {
-      scal[S_RR] = rr;
-      scal[S_ITERS] += 1.0;
-      if (rr <= scal[S_TOL2] || !(rr == rr)) {
-        *done = 1;
-        return;
-      }
-    }
-    const double rz_old = scal[S_RZ];
-    scal[S_BETA] = (mode == M_BETA0 || rz_old == 0.0) ? 0.0 : (a - b) / rz_old;
-    scal[S_RZ] = a;
-  }
-}
-
-__global__ void k_set_rtol(double* scal, double rtol) { scal[S_RTOL] = rtol; }
 
 // --------------------------------------------------- reduced system set-up
 // b_red[P] = -b[P] + sum_{non-unknown nbrs Q} b[Q] at unknown nodes, 0 else;
@@ -860,112 +886,98 @@ static void destroy(bemcs_poisson_s* h) {
 }
 
 // --------------------------------------------------------- V-cycle driver
-// Per level above the coarsest: fused pre-smoothing (2 Jacobi sweeps), then
-// residual + restriction (one fused kernel on small levels, two kernels on
-// large ones where recomputing residuals costs more than a launch), then on
-// the way up prolongation + two Jacobi sweeps (the first fused with the
-// prolongation on small levels).  The coarsest level is a dense solve.
-static void vcycle(bemcs_poisson_s* h, hipStream_t s) {
-  const int last = h->nlev - 1;
-  const int32_t* dn = h->done;
-  for (int l = 0; l < last; ++l) {
-    LevelDev& L = h->lev[l];
-    LevelDev& C = h->lev[l + 1];
-    hipLaunchKernelGGL(k_pre, dim3(grid_for(L.n, 256)), dim3(256), 0, s, L, dn);
-    if (L.n <= kFuseBelow) {
-      hipLaunchKernelGGL(k_resid_restrict, dim3(grid_for(C.n, 256)), dim3(256), 0, s, L, C, dn);
-    } else {
-      hipLaunchKernelGGL(k_residual, dim3(grid_for(L.n, 256)), dim3(256), 0, s, L, h->rscr, dn);
-      hipLaunchKernelGGL(k_restrict, dim3(grid_for(C.n, 256)), dim3(256), 0, s, L,
-                         (const float*)h->rscr, C, dn);
-    }
-  }
-  hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[last].n + 63) / 64), dim3(64 * kCoarseWaves), 0, s,
-                     h->lev[last], (const float*)h->cinv, dn);
-  for (int l = last - 1; l >= 0; --l) {
-    LevelDev& L = h->lev[l];
-    LevelDev& C = h->lev[l + 1];
-    unsigned g = grid_for(L.n, 256);
-    if (L.n <= kFuseBelow) {
-      hipLaunchKernelGGL(k_post1, dim3(g), dim3(256), 0, s, L, C, dn);
-      std::swap(L.x, L.t);
-    } else {
-      hipLaunchKernelGGL(k_prolong, dim3(g), dim3(256), 0, s, L, C, dn);
-      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, dn);
-      std::swap(L.x, L.t);
-    }
-    if (h->post_sweeps > 1) {
-      hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(256), 0, s, L, dn);
-      std::swap(L.x, L.t);
-    }
-  }
-}
-
-static unsigned red_grid(int n) {
-  unsigned g = grid_for(n, kRedBlock);
-  return g > (unsigned)kMaxRedBlocks ? (unsigned)kMaxRedBlocks : g;
-}
-
 static CGArgs cg_args(bemcs_poisson_s* h) {
   CGArgs g;
   g.x = h->x;
-  g.r = h->r;
-  g.rold = h->rold;
+  g.R[0] = h->R[0];
+  g.R[1] = h->R[1];
   g.p = h->p;
   g.Ap = h->Ap;
   g.bred = h->bred;
   g.b0 = h->lev[0].b;
-  g.z = h->lev[0].x;
+  g.z = h->nlev == 1 ? h->lev[0].x : h->lev[0].t;
   g.code = h->lev[0].code;
   g.nx = h->nx;
   g.ny = h->ny;
   g.per = h->periodic;
   g.n = h->nx * h->ny;
-  g.part = h->part;
-  g.part2 = h->part2;
+  g.nparts = h->nparts;
+  g.pa = h->pa;
+  g.pr = h->pr;
+  g.pz[0] = h->pz[0];
+  g.pz[1] = h->pz[1];
+  g.pi = h->pi;
   g.scal = h->scal;
   g.done = h->done;
   return g;
 }
 
-
-static void enqueue_init(bemcs_poisson_s* h, const CGArgs& g, hipStream_t s) {
-  const unsigned rg = red_grid(g.n);
-  hipLaunchKernelGGL(k_cg_init, dim3(rg), dim3(kRedBlock), 0, s, g);
-  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
-                     (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_INIT);
-  vcycle(h, s);
-  hipLaunchKernelGGL(k_cg_dotz, dim3(rg), dim3(kRedBlock), 0, s, g);
-  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
-                     (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_BETA0);
+// V-cycle k: k = 0 is the initial one (its right-hand side b0 was written by
+// k_cg_init); for k >= 1 the CG step of iteration k is fused into the first
+// down launch and the launch after it runs the convergence test.
+static void enqueue_vcycle(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
+  const int last = h->nlev - 1;
+  CheckArgs ck{h->pr, h->nparts, h->scal, h->done, 0};
+  if (last == 0) {
+    if (k > 0) hipLaunchKernelGGL(k_cg_step, dim3(h->nparts), dim3(256), 0, s, g, k);
+    ck.k = k;
+    hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[0].n + 63) / 64), dim3(64 * kCoarseWaves), 0, s,
+                       h->lev[0], (const float*)h->cinv, ck);
+    hipLaunchKernelGGL(k_cg_dotz, dim3(h->nparts), dim3(256), 0, s, g, k);
+    return;
+  }
+  for (int l = 0; l < last; ++l) {
+    const LevelDev& F = h->lev[l];
+    const LevelDev& C = h->lev[l + 1];
+    dim3 grid((C.nx + kTC - 1) / kTC, (C.ny + kTC - 1) / kTC);
+    CheckArgs c = ck;
+    c.k = (l == 1 && k > 0) ? k : 0;
+    if (l == 0 && k > 0)
+      hipLaunchKernelGGL(k_down<true>, grid, dim3(256), 0, s, F, C, g, k, c);
+    else
+      hipLaunchKernelGGL(k_down<false>, grid, dim3(256), 0, s, F, C, g, k, c);
+  }
+  ck.k = (last == 1 && k > 0) ? k : 0;
+  hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[last].n + 63) / 64), dim3(64 * kCoarseWaves), 0,
+                     s, h->lev[last], (const float*)h->cinv, ck);
+  for (int l = last - 1; l >= 0; --l) {
+    const LevelDev& F = h->lev[l];
+    LevelDev C = h->lev[l + 1];
+    if (l + 1 != last) C.x = C.t;  // the coarse level's result
+    dim3 grid((F.nx + kTF - 1) / kTF, (F.ny + kTF - 1) / kTF);
+    const bool two = h->post_sweeps > 1;
+    if (l == 0) {
+      if (two) hipLaunchKernelGGL((k_up<2, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      else hipLaunchKernelGGL((k_up<1, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+    } else {
+      if (two) hipLaunchKernelGGL((k_up<2, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      else hipLaunchKernelGGL((k_up<1, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+    }
+  }
 }
 
-static void enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, hipStream_t s) {
-  const unsigned rg = red_grid(g.n);
-  {
-    hipLaunchKernelGGL(k_cg_dir, dim3(rg), dim3(kRedBlock), 0, s, g);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
-                       (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_ALPHA);
-    hipLaunchKernelGGL(k_cg_step, dim3(rg), dim3(kRedBlock), 0, s, g);
-    vcycle(h, s);
-    hipLaunchKernelGGL(k_cg_dotz, dim3(rg), dim3(kRedBlock), 0, s, g);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, s, (const double*)h->part,
-                       (const double*)h->part2, (int)rg, h->scal, h->done, (int)M_BETA);
-  }
+static void enqueue_init(bemcs_poisson_s* h, const CGArgs& g, double rtol, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_init, dim3(h->nparts), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_cg_init_scalar, dim3(1), dim3(256), 0, s, g, rtol);
+  enqueue_vcycle(h, g, 0, s);
+}
+
+static void enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, s, g, k);
+  enqueue_vcycle(h, g, k, s);
 }
 
 // PCG on the reduced system already in h->bred with initial guess h->x.
 static int pcg(bemcs_poisson_s* h, double rtol, int max_iter, int* iters, double* relres,
                hipStream_t s) {
   const CGArgs g = cg_args(h);
-  hipLaunchKernelGGL(k_set_rtol, dim3(1), dim3(1), 0, s, h->scal, rtol);
-  enqueue_init(h, g, s);
+  enqueue_init(h, g, rtol, s);
   BEMCS_CHECK_LAUNCH();
   // launch as many iterations as the previous solve needed, then check
   int launched = 0;
   int batch = std::max(1, h->pred_iters);
   for (;;) {
-    for (int k = 0; k < batch && launched < max_iter; ++k, ++launched) enqueue_iter(h, g, s);
+    for (int j = 0; j < batch && launched < max_iter; ++j) enqueue_iter(h, g, ++launched, s);
     BEMCS_CHECK_LAUNCH();
     BEMCS_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, sizeof(double) * S_COUNT,
                                    hipMemcpyDeviceToHost, s));
@@ -1138,17 +1150,20 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     h->mirror1 = (int32_t*)A(sizeof(int32_t) * n);
     if (ok) ok = upload(h->mirror1, mirror1.data(), sizeof(int32_t) * n) == 0;
     h->x = (double*)A(sizeof(double) * n);
-    h->r = (double*)A(sizeof(double) * n);
+    h->R[0] = (double*)A(sizeof(double) * n);
+    h->R[1] = (double*)A(sizeof(double) * n);
     h->p = (double*)A(sizeof(double) * n);
     h->Ap = (double*)A(sizeof(double) * n);
-    h->rold = (double*)A(sizeof(double) * n);
     h->bred = (double*)A(sizeof(double) * n);
     h->Vnew = (double*)A(sizeof(double) * n);
-    h->part = (double*)A(sizeof(double) * 2 * kMaxRedBlocks);
+    h->nparts = ((nx + kTF - 1) / kTF) * ((ny + kTF - 1) / kTF);
+    h->pa = (double*)A(sizeof(double) * 2 * h->nparts);
+    h->pr = (double*)A(sizeof(double) * 2 * h->nparts);
+    h->pi = (double*)A(sizeof(double) * 2 * h->nparts);
+    h->pz[0] = (double*)A(sizeof(double) * 2 * h->nparts);
+    h->pz[1] = (double*)A(sizeof(double) * 2 * h->nparts);
     h->scal = (double*)A(sizeof(double) * S_COUNT);
     h->done = (int32_t*)A(sizeof(int32_t) * 4);
-    h->rscr = (float*)A(sizeof(float) * n);
-    h->part2 = (double*)A(sizeof(double) * 2 * kMaxRedBlocks);
     if (ok) ok = hipMemset(h->done, 0, sizeof(int32_t) * 4) == hipSuccess;
     if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;

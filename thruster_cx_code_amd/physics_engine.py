@@ -52,11 +52,16 @@ class DigitalTwinSimulator:
 
     kernel_backend = "hip"
 
-    def __init__(self, device=None, sort_interval=20, poisson_rtol=1e-10, poisson_max_iter=200):
+    def __init__(self, device=None, sort_interval=5, electron_sort_interval=1, poisson_rtol=1e-9,
+                 poisson_max_iter=200):
         require_gpu()
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.lib = _lib.load()
+        # Cell-order sorting keeps the bilinear gathers of one wavefront in a
+        # few cells.  Electrons cross many cells per step (hundreds of
+        # substeps), so their order decays within 2-3 steps; ions drift slowly.
         self.sort_interval = int(sort_interval)
+        self.electron_sort_interval = int(electron_sort_interval)
         self.poisson_rtol = float(poisson_rtol)
         self.poisson_max_iter = int(poisson_max_iter)
         # physical constants and defaults as the reference (:310-401)
@@ -246,6 +251,7 @@ class DigitalTwinSimulator:
         pre = params.get('preload')
         if pre and not preserve_state:
             self._preload(params, int(pre.get('ions', 0)), int(pre.get('electrons', 0)))
+            self._sort(ions=True, electrons=True)
 
     def _alloc_mesh(self, preserve_state):
         dev = self.device
@@ -593,8 +599,8 @@ class DigitalTwinSimulator:
         self.ring_counts[2 * slot].copy_(self.ions[0].count[0])
         self.ring_counts[2 * slot + 1].copy_(self.elec[0].count[0])
         self.ring_nsub[2 * slot:2 * slot + 2].copy_(self.n_sub_d)
-        if self.sort_interval > 0 and it % self.sort_interval == 0:
-            self._sort()
+        self._sort(ions=self.sort_interval > 0 and it % self.sort_interval == 0,
+                   electrons=self.electron_sort_interval > 0 and it % self.electron_sort_interval == 0)
         # asynchronous count readback for the host bounds
         cnts = torch.empty(2, dtype=torch.int32, pin_memory=True)
         cnts.copy_(self.ring_counts[2 * slot:2 * slot + 2], non_blocking=True)
@@ -619,9 +625,9 @@ class DigitalTwinSimulator:
         e.record()
         kt.setdefault(name, []).append((start, e))
 
-    def _sort(self):
+    def _sort(self, ions, electrons):
         s = stream_ptr()
-        for pair in (self.ions, self.elec):
+        for pair in ((self.ions,) if ions else ()) + ((self.elec,) if electrons else ()):
             bound = self.bound_i if pair is self.ions else self.bound_e
             check(self.lib.bemcs_sort_by_cell(C.byref(pair[0].struct()), C.byref(pair[1].struct()), bound,
                                               self.nx, self.ny, self.dx, self.dy, ptr(self.sort_scratch_for(bound)), s),
