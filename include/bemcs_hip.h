@@ -291,6 +291,44 @@ int bemcs_percentile(const float* angles, const uint32_t* n_dev, int64_t n_bound
 int bemcs_ppc_stats(const uint32_t* ppc, int64_t n_nodes, double* accum, uint32_t* stats,
                     bemcs_stream_t stream);
 
+/* Per-step bookkeeping of the device-resident step in one launch each (the
+ * reference keeps these as host variables of step(), :969-1784).
+ * bemcs_step_begin clears the substep reductions, the output species counts,
+ * the node arrays (next deposit counts of both species, PPC map), the ion
+ * diagnostics record and the PPC statistics, and stores the input species
+ * counts (the particles this step pushes) in nin_out[2].
+ * bemcs_step_end stores the species counts and substep counts after the step
+ * and the minimum of V over the saddle-point nodes (:1655-1676). */
+typedef struct {
+  uint64_t* ds_bits;         /* [2] */
+  int32_t* count_out_i;      /* output buffers' counts */
+  int32_t* count_out_e;
+  uint32_t* node_zero[3];    /* [n_nodes] each, may be null */
+  int64_t n_nodes;
+  uint8_t* diag;             /* bemcs_ion_diag_t record */
+  int64_t diag_bytes;
+  uint32_t* ppc_stats;       /* [2] */
+  const int32_t* count_in_i; /* input buffers' counts */
+  const int32_t* count_in_e;
+  int32_t* nin_out;          /* [2] */
+} bemcs_step_begin_t;
+
+typedef struct {
+  const int32_t* count_i;
+  const int32_t* count_e;
+  const int32_t* n_sub;      /* [2] */
+  int32_t* counts_out;       /* [2] */
+  int32_t* nsub_out;         /* [2] */
+  const double* V;
+  const int64_t* saddle_idx;
+  int32_t n_saddle;
+  int32_t pad_;
+  double* minpot_out;        /* NaN when n_saddle == 0 */
+} bemcs_step_end_t;
+
+int bemcs_step_begin(const bemcs_step_begin_t* p, bemcs_stream_t stream);
+int bemcs_step_end(const bemcs_step_end_t* p, bemcs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,44 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(_lib.Picard) == 6 * 8 + 11 * 8 + 2 * 4
 
 
+_STRUCTS = {
+    "bemcs_species_t": "Species", "bemcs_ion_diag_t": "IonDiag", "bemcs_ion_step_t": "IonStep",
+    "bemcs_electron_step_t": "ElectronStep", "bemcs_inject_t": "Inject", "bemcs_picard_t": "Picard",
+    "bemcs_step_begin_t": "StepBegin", "bemcs_step_end_t": "StepEnd",
+}
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives
+    the header's struct (compiled with gcc against include/bemcs_hip.h)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, pname in _STRUCTS.items():
+        lines.append(f'printf("{pname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in getattr(_lib, pname)._fields_:
+            lines.append(f'printf("{pname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    checked = 0
+    for ln in out:
+        if not ln:
+            continue
+        pname, what, val = ln.split()
+        cls = getattr(_lib, pname)
+        want = C.sizeof(cls) if what == "size" else getattr(cls, what).offset
+        assert int(val) == want, (pname, what, int(val), want)
+        checked += 1
+    assert checked == sum(len(getattr(_lib, p)._fields_) + 1 for p in _STRUCTS.values())
+
+
 def test_host_det_math_bit_equal(lib):
     x = np.concatenate([np.linspace(-800, 800, 8001), np.random.default_rng(0).normal(0, 30, 4000)])
     got = np.array([lib.bemcs_det_exp(float(v)) for v in x])

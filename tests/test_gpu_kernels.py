@@ -213,15 +213,19 @@ def _geometry(nx, ny, periodic):
                                             (257, 97, False), (1024, 384, False)])
 def test_poisson_solve_vs_superlu(lib, nx, ny, periodic, monkeypatch):
     """Repeated solves on one handle (with eager work and host synchronisation
-    in between) against SuperLU; two handles must give identical bits."""
+    in between) against SuperLU.  The first handle replays captured HIP graphs,
+    the second launches every kernel directly (BEMCS_POISSON_GRAPHS=0): both
+    must give identical bits on every replay."""
     print(f"[gpu] poisson {nx}x{ny} periodic={periodic}", flush=True)
     o = _geometry(nx, ny, periodic)
     ib = np.ascontiguousarray(o.isBound.astype(np.uint8))
     handles = []
-    for _ in range(2):
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("BEMCS_POISSON_GRAPHS", graphs)
         h = C.c_void_p()
         assert lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), nx, ny, int(periodic), C.byref(h)) == 0
         handles.append(h)
+    monkeypatch.delenv("BEMCS_POISSON_GRAPHS")
     try:
         r = np.random.default_rng(nx)
         for rep in range(3):

@@ -68,6 +68,17 @@ class Picard(C.Structure):
                 ("dy_m", f64), ("rtol", f64), ("iterations", i32), ("max_iter", i32)]
 
 
+class StepBegin(C.Structure):
+    _fields_ = [("ds_bits", vp), ("count_out_i", vp), ("count_out_e", vp), ("node_zero", vp * 3),
+                ("n_nodes", i64), ("diag", vp), ("diag_bytes", i64), ("ppc_stats", vp),
+                ("count_in_i", vp), ("count_in_e", vp), ("nin_out", vp)]
+
+
+class StepEnd(C.Structure):
+    _fields_ = [("count_i", vp), ("count_e", vp), ("n_sub", vp), ("counts_out", vp), ("nsub_out", vp),
+                ("V", vp), ("saddle_idx", vp), ("n_saddle", i32), ("pad_", i32), ("minpot_out", vp)]
+
+
 _SIGS = {
     "bemcs_abi_version": (i32, []),
     "bemcs_last_error": (C.c_char_p, []),
@@ -95,6 +106,8 @@ _SIGS = {
     "bemcs_efield": (i32, [vp, i32, i32, f64, f64, i32, vp, vp]),
     "bemcs_percentile": (i32, [vp, vp, i64, f64, u32, vp, vp, vp]),
     "bemcs_ppc_stats": (i32, [vp, i64, vp, vp, vp]),
+    "bemcs_step_begin": (i32, [vp, vp]),
+    "bemcs_step_end": (i32, [vp, vp]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -17,8 +17,41 @@ void set_error(const char* fmt, ...) {
 
 // k-th smallest (0-based) of n non-negative float32 values, by radix select
 // on the bit patterns (11 + 11 + 10 bits), one work-group.
+// Exclusive prefix sum over a 1024-thread block (wave shuffles, then the 16
+// wave totals); every thread also gets the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum /* LDS [16] */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t t = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      uint32_t y = __shfl_up(t, o);
+      if (lane >= o) t += y;
+    }
+    if (lane < 16) wsum[lane] = t;
+  }
+  __syncthreads();
+  const uint32_t base = w > 0 ? wsum[w - 1] : 0u;
+  __syncthreads();
+  return base + x - v;
+}
+
+// k-th smallest (0-based) of n non-negative float32 bit patterns by a 3-pass
+// MSD radix select (11/11/10 bits) in one 1024-thread block.  Each pass
+// histograms the keys that match the digits chosen so far, and a block scan
+// of the histogram finds the bucket holding the k-th key.
 __device__ uint32_t select_kth(const uint32_t* __restrict__ keys, uint32_t n, uint32_t k,
                                uint32_t* hist /* LDS, 2048 */) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t s_digit, s_k;
   uint32_t prefix = 0, mask = 0;
   const int shifts[3] = {21, 10, 0};
   const int widths[3] = {11, 11, 10};
@@ -32,15 +65,14 @@ __device__ uint32_t select_kth(const uint32_t* __restrict__ keys, uint32_t n, ui
       if ((v & mask) == prefix) atomicAdd(&hist[(v >> sh) & dm], 1u);
     }
     __syncthreads();
-    __shared__ uint32_t s_digit, s_k;
-    if (threadIdx.x == 0) {
-      uint32_t acc = 0, d = 0;
-      for (d = 0; d <= dm; ++d) {
-        if (acc + hist[d] > k) break;
-        acc += hist[d];
-      }
-      s_digit = d;
-      s_k = k - acc;
+    const uint32_t h0 = hist[2 * threadIdx.x], h1 = hist[2 * threadIdx.x + 1];
+    const uint32_t before = block_excl_scan(h0 + h1, wsum);
+    if (k >= before && k < before + h0) {
+      s_digit = 2 * threadIdx.x;
+      s_k = k - before;
+    } else if (k >= before + h0 && k < before + h0 + h1) {
+      s_digit = 2 * threadIdx.x + 1;
+      s_k = k - before - h0;
     }
     __syncthreads();
     prefix |= s_digit << sh;
@@ -80,16 +112,74 @@ __global__ __launch_bounds__(1024) void k_percentile(const float* __restrict__ a
   }
 }
 
-__global__ __launch_bounds__(256) void k_ppc_stats(const uint32_t* __restrict__ ppc, int64_t n,
-                                                   double* accum, uint32_t* stats) {
-  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  uint32_t c = 0;
-  if (i < n) {
-    c = ppc[i];
+// Occupied and low (1-2 particle) cells of the per-step PPC map, and the
+// running float64 PPC sum.  Grid-stride over at most kPpcBlocks blocks with
+// one pair of counter atomics per block.
+constexpr int kPpcBlocks = 256;
+__global__ __launch_bounds__(1024) void k_ppc_stats(const uint32_t* __restrict__ ppc, int64_t n,
+                                                    double* accum, uint32_t* stats) {
+  __shared__ uint32_t s_occ[16], s_low[16];
+  uint32_t occ = 0, low = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 1024) {
+    const uint32_t c = ppc[i];
     if (accum) accum[i] += (double)c;
+    occ += c > 0;
+    low += c > 0 && c < 3;
   }
-  wave_count(&stats[0], c > 0);
-  wave_count(&stats[1], c > 0 && c < 3);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    occ += __shfl_xor(occ, o);
+    low += __shfl_xor(low, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_occ[threadIdx.x >> 6] = occ;
+    s_low[threadIdx.x >> 6] = low;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0, b = 0;
+    for (int w = 0; w < 16; ++w) {
+      a += s_occ[w];
+      b += s_low[w];
+    }
+    if (a) atomicAdd(&stats[0], a);
+    if (b) atomicAdd(&stats[1], b);
+  }
+}
+
+// --------------------------------------------------- per-step bookkeeping
+__global__ __launch_bounds__(256) void k_step_begin(bemcs_step_begin_t p) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < p.n_nodes) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (p.node_zero[a]) p.node_zero[a][i] = 0u;
+  }
+  if (i < p.diag_bytes) p.diag[i] = 0;
+  if (i == 0) {
+    p.ds_bits[0] = 0ull;
+    p.ds_bits[1] = 0ull;
+    *p.count_out_i = 0;
+    *p.count_out_e = 0;
+    p.ppc_stats[0] = 0u;
+    p.ppc_stats[1] = 0u;
+    p.nin_out[0] = *p.count_in_i;
+    p.nin_out[1] = *p.count_in_e;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_step_end(bemcs_step_end_t p) {
+  double m = __builtin_inf();
+  for (int k = threadIdx.x; k < p.n_saddle; k += 64) m = fmin(m, p.V[p.saddle_idx[k]]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
+  if (threadIdx.x == 0) {
+    p.counts_out[0] = *p.count_i;
+    p.counts_out[1] = *p.count_e;
+    p.nsub_out[0] = p.n_sub[0];
+    p.nsub_out[1] = p.n_sub[1];
+    *p.minpot_out = p.n_saddle > 0 ? m : __builtin_nan("");
+  }
 }
 
 }  // namespace bemcs
@@ -117,8 +207,30 @@ int bemcs_percentile(const float* angles, const uint32_t* n_dev, int64_t n_bound
 int bemcs_ppc_stats(const uint32_t* ppc, int64_t n_nodes, double* accum, uint32_t* stats,
                     bemcs_stream_t stream) {
   BEMCS_REQUIRE(ppc && stats && n_nodes > 0, "bemcs_ppc_stats: bad arguments");
-  hipLaunchKernelGGL(k_ppc_stats, dim3(grid_for(n_nodes, 256)), dim3(256), 0, as_stream(stream),
-                     ppc, n_nodes, accum, stats);
+  unsigned g = grid_for(n_nodes, 1024);
+  if (g > (unsigned)kPpcBlocks) g = kPpcBlocks;
+  hipLaunchKernelGGL(k_ppc_stats, dim3(g), dim3(1024), 0, as_stream(stream), ppc, n_nodes, accum,
+                     stats);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_step_begin(const bemcs_step_begin_t* p, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(p && p->ds_bits && p->count_out_i && p->count_out_e && p->ppc_stats &&
+                    p->count_in_i && p->count_in_e && p->nin_out && (p->diag || p->diag_bytes == 0) &&
+                    p->n_nodes >= 0 && p->diag_bytes >= 0,
+                "bemcs_step_begin: bad arguments");
+  int64_t n = p->n_nodes > p->diag_bytes ? p->n_nodes : p->diag_bytes;
+  hipLaunchKernelGGL(k_step_begin, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *p);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_step_end(const bemcs_step_end_t* p, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(p && p->count_i && p->count_e && p->n_sub && p->counts_out && p->nsub_out &&
+                    p->minpot_out && p->n_saddle >= 0 && (p->n_saddle == 0 || (p->V && p->saddle_idx)),
+                "bemcs_step_end: bad arguments");
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, as_stream(stream), *p);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

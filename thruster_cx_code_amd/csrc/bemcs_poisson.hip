@@ -67,6 +67,10 @@ struct bemcs_poisson_s {
   bool have_guess;
   int pred_iters;     // iterations the last solve needed (launch that many first)
   int post_sweeps;    // Jacobi sweeps after prolongation (1 or 2)
+  bool use_graphs;    // PCG iterations replayed from captured HIP graphs
+  hipStream_t cap_stream;                  // private stream used only for capture
+  std::vector<hipGraphExec_t> iter_graph;  // [k] = iteration k (dir + V-cycle)
+  hipGraphExec_t init_graph;               // initial V-cycle
 };
 
 namespace bemcs {
@@ -117,11 +121,12 @@ __device__ __forceinline__ T apply5(const T* __restrict__ v, const uint8_t* __re
 // The result of a level is F.t (C.x on the coarsest level, from the dense
 // solve).  Halo values are recomputed by every tile that needs them from
 // inputs that no block of the same launch writes.
-constexpr int kTC = 16;             // coarse tile edge of a down launch
-constexpr int kTF = 2 * kTC;        // fine tile edge of an up launch
-constexpr int kDR = 2 * kTC + 5;    // x1 region edge of a down launch (halo 3)
+constexpr int kTC = 8;              // coarse tile edge of the level-0 down launch
+constexpr int kTF = 2 * kTC;        // fine tile edge of the level-0 up launch
+constexpr int kTCc = 8;             // the same on the coarser levels (more,
+constexpr int kTFc = 2 * kTCc;      // smaller tiles: these launches are latency bound)
 
-static_assert(kTC * kTC == 256, "one coarse node per thread in the restriction");
+static_assert(kTC * kTC <= 256 && kTCc * kTCc <= 256, "one coarse node per thread in the restriction");
 
 __device__ __forceinline__ int gnode(const LevelDev& L, int gx, int gy) {
   if (gx < 0 || gx >= L.nx) return -1;
@@ -130,50 +135,6 @@ __device__ __forceinline__ int gnode(const LevelDev& L, int gx, int gy) {
     gy = (gy % L.ny + L.ny) % L.ny;
   }
   return gy * L.nx + gx;
-}
-
-// A v at global node g, v held in LDS with row width W, c = &v[local node]
-__device__ __forceinline__ float lds_apply(const LevelDev& L, const float* c, int W, int g) {
-  if (!L.st) {
-    float nb = (c[-1] + c[1]) + (c[-W] + c[W]);
-    return (float)L.code[g] * c[0] - nb;
-  }
-  float a = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) a += L.st[(size_t)k * L.n + g] * c[(k / 3 - 1) * W + (k % 3 - 1)];
-  return a;
-}
-
-__device__ __forceinline__ float omega_of(const LevelDev& L) { return L.st ? kOmegaC : kOmega0; }
-
-// coarse correction P e at fine node i (0 at inactive fine nodes), e = C.x
-__device__ __forceinline__ float pe_at(const LevelDev& F, const LevelDev& C, int i) {
-  if (F.inv_d[i] == 0.0f) return 0.0f;
-  int fx = i % F.nx, fy = i / F.nx;
-  int ax0 = fx >> 1, ax1 = -1;
-  if (fx & 1) {
-    ax0 = (fx - 1) >> 1;
-    ax1 = (fx + 1) >> 1;
-    if (ax1 >= C.nx) ax1 = -1;
-  }
-  int ay0 = fy >> 1, ay1 = -1;
-  if (fy & 1) {
-    ay0 = (fy - 1) >> 1;
-    ay1 = (fy + 1) >> 1;
-    if (ay1 >= C.ny) ay1 = F.periodic ? ay1 - C.ny : -1;
-  }
-  float s = 0.0f;
-  int xs[2] = {ax0, ax1}, ys[2] = {ay0, ay1};
-  for (int a = 0; a < 2; ++a) {
-    if (ys[a] < 0) continue;
-    float wy = w1d(fy, ys[a], C.ny, F.periodic != 0);
-    for (int b = 0; b < 2; ++b) {
-      if (xs[b] < 0) continue;
-      float wx = w1d(fx, xs[b], C.nx, false);
-      s += wx * wy * C.x[ys[a] * C.nx + xs[b]];
-    }
-  }
-  return s;
 }
 
 // ---------------------------------------------------- block reductions
@@ -283,7 +244,24 @@ __device__ __forceinline__ bool block_done(const int32_t* done) {
 }
 
 // ------------------------------------------------------------ down launch
-template <bool STEP>
+// Every global load of a tile is issued up front (clamped addresses, values
+// selected afterwards), so a launch waits for memory about once; the
+// Jacobi sweeps, the residual and the restriction then run from LDS.  The
+// 9-point coefficients of coarse levels are staged in LDS with the values.
+template <bool IS9>
+__device__ __forceinline__ float tile_apply(const float* c, int W, const float* st, int nst, int ti,
+                                            uint8_t code) {
+  if (!IS9) {
+    const float nb = (c[-1] + c[1]) + (c[-W] + c[W]);
+    return (float)code * c[0] - nb;
+  }
+  float a = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a += st[k * nst + ti] * c[(k / 3 - 1) * W + (k % 3 - 1)];
+  return a;
+}
+
+template <int TC, bool IS9, bool STEP>
 __global__ __launch_bounds__(256) void k_down(LevelDev F, LevelDev C, CGArgs g, int k,
                                               CheckArgs ca) {
   if (ca.k > 0) {
@@ -291,153 +269,284 @@ __global__ __launch_bounds__(256) void k_down(LevelDev F, LevelDev C, CGArgs g, 
   } else if (*ca.done) {
     return;
   }
-  constexpr int W1 = kDR, W2 = kDR - 2, W3 = kDR - 4;
-  __shared__ float sb[W1 * W1], sid[W1 * W1], s1[W1 * W1], s2[W2 * W2], sr[W3 * W3];
-  const float om = omega_of(F);
-  const int CX0 = blockIdx.x * kTC, CY0 = blockIdx.y * kTC;
+  constexpr int TFd = 2 * TC;                 // owned fine nodes per edge
+  constexpr int W1 = TFd + 5, W2 = W1 - 2, W3 = W1 - 4;
+  constexpr int N1 = W1 * W1, N2 = W2 * W2, N3 = W3 * W3;
+  constexpr int IT1 = (N1 + 255) / 256, IT2 = (N2 + 255) / 256;
+  __shared__ float sb[N1], sid[N1], s1[N1], s2[N2], sr[N3];
+  __shared__ float sst[IS9 ? 9 * N2 : 1];
+  __shared__ uint8_t scode[IS9 ? 1 : N2];
+  const float om = IS9 ? kOmegaC : kOmega0;
+  const int CX0 = blockIdx.x * TC, CY0 = blockIdx.y * TC;
   const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
-  double alpha = 0.0, rr = 0.0;
   const double* rp = g.R[(k + 1) & 1];
-  double* rn_out = g.R[k & 1];
+  double alpha = 0.0;
   if (STEP) {
     double rz, pap, u;
     block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
     block_sum2(g.pa, g.nparts, pap, u);
     alpha = pap != 0.0 ? rz / pap : 0.0;
   }
-  for (int t = threadIdx.x; t < W1 * W1; t += 256) {
-    const int lx = t % W1, ly = t / W1;
-    const int gy = oy + ly;
-    const int gi = gnode(F, ox + lx, gy);
-    float b = 0.0f, id = 0.0f;
-    if (gi >= 0) {
-      id = F.inv_d[gi];
+  {
+    int vg[IT1];
+    float vid[IT1], vb[IT1];
+    double vr[STEP ? IT1 : 1], va[STEP ? IT1 : 1];
+#pragma unroll
+    for (int it = 0; it < IT1; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, N1 - 1);
+      const int gi = gnode(F, ox + t % W1, oy + t / W1);
+      const int gc = gi < 0 ? 0 : gi;
+      vg[it] = gi;
+      vid[it] = F.inv_d[gc];
       if (STEP) {
-        const double rn = rp[gi] - alpha * g.Ap[gi];
-        b = (float)rn;
-        if (lx >= 3 && lx < 3 + kTF && ly >= 3 && ly < 3 + kTF && gy < F.ny) {
-          g.x[gi] += alpha * g.p[gi];
-          rn_out[gi] = rn;
-          F.b[gi] = b;
-          rr += rn * rn;
-        }
+        vr[it] = rp[gc];
+        va[it] = g.Ap[gc];
       } else {
-        b = F.b[gi];
+        vb[it] = F.b[gc];
       }
     }
-    sb[t] = b;
-    sid[t] = id;
-    s1[t] = om * id * b;
+    float vst[IS9 ? IT2 : 1][9];
+    uint8_t vcode[IS9 ? 1 : IT2];
+    int vg2[IT2];
+#pragma unroll
+    for (int it = 0; it < IT2; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, N2 - 1);
+      const int gi = gnode(F, ox + 1 + t % W2, oy + 1 + t / W2);
+      const int gc = gi < 0 ? 0 : gi;
+      vg2[it] = gi;
+      if (IS9) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vst[it][q] = F.st[(size_t)q * F.n + gc];
+      } else {
+        vcode[it] = F.code[gc];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT1; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= N1) break;
+      const bool in = vg[it] >= 0;
+      const float id = in ? vid[it] : 0.0f;
+      float b = 0.0f;
+      if (in) b = STEP ? (float)(vr[it] - alpha * va[it]) : vb[it];
+      sb[t] = b;
+      sid[t] = id;
+      s1[t] = om * id * b;
+    }
+#pragma unroll
+    for (int it = 0; it < IT2; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= N2) break;
+      const bool in = vg2[it] >= 0;
+      if (IS9) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) sst[q * N2 + t] = in ? vst[it][q] : 0.0f;
+      } else {
+        scode[t] = in ? vcode[it] : (uint8_t)0;
+      }
+    }
   }
-  if (STEP) block_partials(rr, 0.0, g.pr, blockIdx.y * gridDim.x + blockIdx.x);
+  if (STEP) {
+    // the owned nodes take the CG step; their b is the value staged above
+    double rr = 0.0;
+    double* rn_out = g.R[k & 1];
+#pragma unroll
+    for (int it = 0; it < TFd * TFd / 256; ++it) {
+      const int t = threadIdx.x + it * 256;
+      const int gx = 2 * CX0 + t % TFd, gy = 2 * CY0 + t / TFd;
+      if (gx >= F.nx || gy >= F.ny) continue;
+      const int gi = gy * F.nx + gx;
+      const double rn = rp[gi] - alpha * g.Ap[gi];
+      g.x[gi] += alpha * g.p[gi];
+      rn_out[gi] = rn;
+      F.b[gi] = (float)rn;
+      rr += rn * rn;
+    }
+    block_partials(rr, 0.0, g.pr, blockIdx.y * gridDim.x + blockIdx.x);
+  }
   __syncthreads();
-  for (int t = threadIdx.x; t < W2 * W2; t += 256) {
+  for (int t = threadIdx.x; t < N2; t += 256) {
     const int lx = t % W2 + 1, ly = t / W2 + 1;
     const int li = ly * W1 + lx;
-    const int gy = oy + ly;
-    const int gi = gnode(F, ox + lx, gy);
     const float id = sid[li];
     float v = 0.0f;
-    if (id != 0.0f) v = s1[li] + om * id * (sb[li] - lds_apply(F, &s1[li], W1, gi));
+    if (id != 0.0f) {
+      v = s1[li] + om * id * (sb[li] - tile_apply<IS9>(&s1[li], W1, sst, N2, t, IS9 ? 0 : scode[t]));
+    }
     s2[t] = v;
-    if (gi >= 0 && lx >= 3 && lx < 3 + kTF && ly >= 3 && ly < 3 + kTF && gy < F.ny) F.x[gi] = v;
+    const int gx = ox + lx, gy = oy + ly;
+    if (lx >= 3 && lx < 3 + TFd && ly >= 3 && ly < 3 + TFd && gx < F.nx && gy < F.ny)
+      F.x[gy * F.nx + gx] = v;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < W3 * W3; t += 256) {
+  for (int t = threadIdx.x; t < N3; t += 256) {
     const int lx = t % W3 + 2, ly = t / W3 + 2;
     const int li = ly * W1 + lx;
     const float id = sid[li];
     float r = 0.0f;
     if (id != 0.0f) {
-      const int gi = gnode(F, ox + lx, oy + ly);
-      r = sb[li] - lds_apply(F, &s2[(ly - 1) * W2 + (lx - 1)], W2, gi);
+      const int t2 = (ly - 1) * W2 + (lx - 1);
+      r = sb[li] - tile_apply<IS9>(&s2[t2], W2, sst, N2, t2, IS9 ? 0 : scode[t2]);
     }
     sr[t] = r;
   }
   __syncthreads();
-  const int cx = CX0 + (int)(threadIdx.x % kTC), cy = CY0 + (int)(threadIdx.x / kTC);
-  if (cx < C.nx && cy < C.ny) {
-    const int I = cy * C.nx + cx;
-    float s = 0.0f;
-    if (C.inv_d[I] != 0.0f) {
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int fyu = 2 * cy + dy;
-        int fy = fyu;
-        if (fy < 0 || fy >= F.ny) {
-          if (!F.periodic) continue;
-          fy = (fy + F.ny) % F.ny;
-        }
-        const float wy = w1d(fy, cy, C.ny, F.periodic != 0);
-        if (wy == 0.0f) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int fx = 2 * cx + dx;
-          if (fx < 0 || fx >= F.nx) continue;
-          const float wx = w1d(fx, cx, C.nx, false);
-          if (wx == 0.0f) continue;
-          s += wx * wy * sr[(fyu - (2 * CY0 - 1)) * W3 + (fx - (2 * CX0 - 1))];
-        }
+  if (threadIdx.x >= TC * TC) return;
+  const int cx = CX0 + (int)(threadIdx.x % TC), cy = CY0 + (int)(threadIdx.x / TC);
+  if (cx >= C.nx || cy >= C.ny) return;
+  const int I = cy * C.nx + cx;
+  float s = 0.0f;
+  if (C.inv_d[I] != 0.0f) {
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int fyu = 2 * cy + dy;
+      int fy = fyu;
+      if (fy < 0 || fy >= F.ny) {
+        if (!F.periodic) continue;
+        fy = (fy + F.ny) % F.ny;
+      }
+      const float wy = w1d(fy, cy, C.ny, F.periodic != 0);
+      if (wy == 0.0f) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int fx = 2 * cx + dx;
+        if (fx < 0 || fx >= F.nx) continue;
+        const float wx = w1d(fx, cx, C.nx, false);
+        if (wx == 0.0f) continue;
+        s += wx * wy * sr[(fyu - (2 * CY0 - 1)) * W3 + (fx - (2 * CX0 - 1))];
       }
     }
-    C.b[I] = s;
   }
+  C.b[I] = s;
 }
 
 // -------------------------------------------------------------- up launch
+// The up to four (index, weight) terms of P e at fine node i; absent terms
+// have weight 0 and index 0.
+__device__ __forceinline__ void pe_terms(const LevelDev& F, const LevelDev& C, int i, int* idx,
+                                         float* w) {
+  const int fx = i % F.nx, fy = i / F.nx;
+  int xs[2] = {fx >> 1, -1}, ys[2] = {fy >> 1, -1};
+  if (fx & 1) {
+    xs[0] = (fx - 1) >> 1;
+    xs[1] = (fx + 1) >> 1;
+    if (xs[1] >= C.nx) xs[1] = -1;
+  }
+  if (fy & 1) {
+    ys[0] = (fy - 1) >> 1;
+    ys[1] = (fy + 1) >> 1;
+    if (ys[1] >= C.ny) ys[1] = F.periodic ? ys[1] - C.ny : -1;
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const bool ok = ys[a] >= 0 && xs[b] >= 0;
+      idx[2 * a + b] = ok ? ys[a] * C.nx + xs[b] : 0;
+      w[2 * a + b] = ok ? w1d(fy, ys[a], C.ny, F.periodic != 0) * w1d(fx, xs[b], C.nx, false) : 0.0f;
+    }
+}
+
 // C.x must point at the coarse result (C.t, or C.x of the coarsest level).
-template <int SWEEPS, bool DOT>
+template <int TF, int SWEEPS, bool IS9, bool DOT>
 __global__ __launch_bounds__(256) void k_up(LevelDev F, LevelDev C, CGArgs g, int k,
                                             const int32_t* __restrict__ done) {
   if (*done) return;
-  constexpr int W0 = kTF + 2 * SWEEPS, W1 = W0 - 2;
-  __shared__ float sx[W0 * W0], sb[W0 * W0], sid[W0 * W0], s1[W1 * W1];
-  const float om = omega_of(F);
-  const int FX0 = blockIdx.x * kTF, FY0 = blockIdx.y * kTF;
-  const int ox = FX0 - SWEEPS, oy = FY0 - SWEEPS;
-  for (int t = threadIdx.x; t < W0 * W0; t += 256) {
-    const int lx = t % W0, ly = t / W0;
-    const int gi = gnode(F, ox + lx, oy + ly);
-    float v = 0.0f, b = 0.0f, id = 0.0f;
-    if (gi >= 0) {
-      id = F.inv_d[gi];
-      b = F.b[gi];
-      if (id != 0.0f) v = F.x[gi] + pe_at(F, C, gi);
+  constexpr int W0 = TF + 2 * SWEEPS, W1 = W0 - 2;  // the coefficient region is W1
+  constexpr int N0 = W0 * W0, N1 = W1 * W1;
+  constexpr int IT0 = (N0 + 255) / 256, IT1 = (N1 + 255) / 256;
+  __shared__ float sx[N0], sb[N0], sid[N0], s1[SWEEPS == 2 ? N1 : 1];
+  __shared__ float sst[IS9 ? 9 * N1 : 1];
+  __shared__ uint8_t scode[IS9 ? 1 : N1];
+  const float om = IS9 ? kOmegaC : kOmega0;
+  const int ox = blockIdx.x * TF - SWEEPS, oy = blockIdx.y * TF - SWEEPS;
+  {
+    int vg[IT0];
+    float vx[IT0], vb[IT0], vid[IT0], vc[IT0][4], vw[IT0][4];
+#pragma unroll
+    for (int it = 0; it < IT0; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, N0 - 1);
+      const int gi = gnode(F, ox + t % W0, oy + t / W0);
+      const int gc = gi < 0 ? 0 : gi;
+      vg[it] = gi;
+      vid[it] = F.inv_d[gc];
+      vb[it] = F.b[gc];
+      vx[it] = F.x[gc];
+      int ci[4];
+      pe_terms(F, C, gc, ci, vw[it]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) vc[it][q] = C.x[ci[q]];
     }
-    sx[t] = v;
-    sb[t] = b;
-    sid[t] = id;
+    float vst[IS9 ? IT1 : 1][9];
+    uint8_t vcode[IS9 ? 1 : IT1];
+    int vg1[IT1];
+#pragma unroll
+    for (int it = 0; it < IT1; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, N1 - 1);
+      const int gi = gnode(F, ox + 1 + t % W1, oy + 1 + t / W1);
+      const int gc = gi < 0 ? 0 : gi;
+      vg1[it] = gi;
+      if (IS9) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vst[it][q] = F.st[(size_t)q * F.n + gc];
+      } else {
+        vcode[it] = F.code[gc];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT0; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= N0) break;
+      const bool in = vg[it] >= 0;
+      const float id = in ? vid[it] : 0.0f;
+      float v = 0.0f;
+      if (id != 0.0f) {
+        const float pe = ((vw[it][0] * vc[it][0] + vw[it][1] * vc[it][1]) +
+                          (vw[it][2] * vc[it][2] + vw[it][3] * vc[it][3]));
+        v = vx[it] + pe;
+      }
+      sx[t] = v;
+      sb[t] = in ? vb[it] : 0.0f;
+      sid[t] = id;
+    }
+#pragma unroll
+    for (int it = 0; it < IT1; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= N1) break;
+      const bool in = vg1[it] >= 0;
+      if (IS9) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) sst[q * N1 + t] = in ? vst[it][q] : 0.0f;
+      } else {
+        scode[t] = in ? vcode[it] : (uint8_t)0;
+      }
+    }
   }
   __syncthreads();
-  const float* last = sx;
-  int WL = W0, off = 0;  // local coordinates of `last` are (lx - off, ly - off)
   if (SWEEPS == 2) {
-    for (int t = threadIdx.x; t < W1 * W1; t += 256) {
+    for (int t = threadIdx.x; t < N1; t += 256) {
       const int lx = t % W1 + 1, ly = t / W1 + 1;
       const int li = ly * W0 + lx;
       const float id = sid[li];
       float v = 0.0f;
-      if (id != 0.0f) {
-        const int gi = gnode(F, ox + lx, oy + ly);
-        v = sx[li] + om * id * (sb[li] - lds_apply(F, &sx[li], W0, gi));
-      }
+      if (id != 0.0f)
+        v = sx[li] + om * id * (sb[li] - tile_apply<IS9>(&sx[li], W0, sst, N1, t, IS9 ? 0 : scode[t]));
       s1[t] = v;
     }
     __syncthreads();
-    last = s1;
-    WL = W1;
-    off = 1;
   }
   double rz = 0.0, roz = 0.0;
-  for (int t = threadIdx.x; t < kTF * kTF; t += 256) {
-    const int lx = t % kTF + SWEEPS, ly = t / kTF + SWEEPS;
+  for (int t = threadIdx.x; t < TF * TF; t += 256) {
+    const int lx = t % TF + SWEEPS, ly = t / TF + SWEEPS;
     const int gx = ox + lx, gy = oy + ly;
     if (gx >= F.nx || gy >= F.ny) continue;
     const int gi = gy * F.nx + gx;
     const int li = ly * W0 + lx;
+    const int t1 = (ly - 1) * W1 + (lx - 1);  // index in the coefficient region
     const float id = sid[li];
     float v = 0.0f;
     if (id != 0.0f) {
-      const float* c = &last[(ly - off) * WL + (lx - off)];
-      v = c[0] + om * id * (sb[li] - lds_apply(F, c, WL, gi));
+      const float* c = SWEEPS == 2 ? &s1[t1] : &sx[li];
+      const int W = SWEEPS == 2 ? W1 : W0;
+      v = c[0] + om * id * (sb[li] - tile_apply<IS9>(c, W, sst, N1, t1, IS9 ? 0 : scode[t1]));
     }
     F.t[gi] = v;
     if (DOT) {
@@ -879,7 +988,10 @@ static int upload(void* dst, const void* src, size_t bytes) {
 
 static void destroy(bemcs_poisson_s* h) {
   if (!h) return;
-
+  for (hipGraphExec_t e : h->iter_graph)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (h->init_graph) (void)hipGraphExecDestroy(h->init_graph);
+  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->host_scal) (void)hipHostFree(h->host_scal);
   delete h;
@@ -929,13 +1041,16 @@ static void enqueue_vcycle(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream
   for (int l = 0; l < last; ++l) {
     const LevelDev& F = h->lev[l];
     const LevelDev& C = h->lev[l + 1];
-    dim3 grid((C.nx + kTC - 1) / kTC, (C.ny + kTC - 1) / kTC);
     CheckArgs c = ck;
     c.k = (l == 1 && k > 0) ? k : 0;
-    if (l == 0 && k > 0)
-      hipLaunchKernelGGL(k_down<true>, grid, dim3(256), 0, s, F, C, g, k, c);
-    else
-      hipLaunchKernelGGL(k_down<false>, grid, dim3(256), 0, s, F, C, g, k, c);
+    if (l == 0) {
+      dim3 grid((C.nx + kTC - 1) / kTC, (C.ny + kTC - 1) / kTC);
+      if (k > 0) hipLaunchKernelGGL((k_down<kTC, false, true>), grid, dim3(256), 0, s, F, C, g, k, c);
+      else hipLaunchKernelGGL((k_down<kTC, false, false>), grid, dim3(256), 0, s, F, C, g, k, c);
+    } else {
+      dim3 grid((C.nx + kTCc - 1) / kTCc, (C.ny + kTCc - 1) / kTCc);
+      hipLaunchKernelGGL((k_down<kTCc, true, false>), grid, dim3(256), 0, s, F, C, g, k, c);
+    }
   }
   ck.k = (last == 1 && k > 0) ? k : 0;
   hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[last].n + 63) / 64), dim3(64 * kCoarseWaves), 0,
@@ -944,40 +1059,85 @@ static void enqueue_vcycle(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream
     const LevelDev& F = h->lev[l];
     LevelDev C = h->lev[l + 1];
     if (l + 1 != last) C.x = C.t;  // the coarse level's result
-    dim3 grid((F.nx + kTF - 1) / kTF, (F.ny + kTF - 1) / kTF);
     const bool two = h->post_sweeps > 1;
     if (l == 0) {
-      if (two) hipLaunchKernelGGL((k_up<2, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
-      else hipLaunchKernelGGL((k_up<1, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      dim3 grid((F.nx + kTF - 1) / kTF, (F.ny + kTF - 1) / kTF);
+      if (two) hipLaunchKernelGGL((k_up<kTF, 2, false, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      else hipLaunchKernelGGL((k_up<kTF, 1, false, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
     } else {
-      if (two) hipLaunchKernelGGL((k_up<2, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
-      else hipLaunchKernelGGL((k_up<1, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      dim3 grid((F.nx + kTFc - 1) / kTFc, (F.ny + kTFc - 1) / kTFc);
+      if (two) hipLaunchKernelGGL((k_up<kTFc, 2, true, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+      else hipLaunchKernelGGL((k_up<kTFc, 1, true, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
     }
   }
 }
 
-static void enqueue_init(bemcs_poisson_s* h, const CGArgs& g, double rtol, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_init, dim3(h->nparts), dim3(256), 0, s, g);
-  hipLaunchKernelGGL(k_cg_init_scalar, dim3(1), dim3(256), 0, s, g, rtol);
-  enqueue_vcycle(h, g, 0, s);
+// Captures what `enqueue` launches into an executable graph (on the private
+// capture stream: nothing runs).  The launches' arguments are fixed per
+// handle and iteration, so a graph is captured once and replayed.
+template <typename F>
+static hipError_t capture(bemcs_poisson_s* h, F enqueue, hipGraphExec_t* out) {
+  *out = nullptr;
+  hipError_t e = hipSuccess;
+  if (!h->cap_stream) e = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  e = hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  enqueue(h->cap_stream);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(h->cap_stream, &g);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return e;
 }
 
-static void enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, s, g, k);
-  enqueue_vcycle(h, g, k, s);
+static int enqueue_init(bemcs_poisson_s* h, const CGArgs& g, double rtol, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_init, dim3(h->nparts), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_cg_init_scalar, dim3(1), dim3(256), 0, s, g, rtol);
+  if (!h->use_graphs) {
+    enqueue_vcycle(h, g, 0, s);
+    return BEMCS_OK;
+  }
+  if (!h->init_graph)
+    BEMCS_CHECK_HIP(capture(h, [&](hipStream_t cs) { enqueue_vcycle(h, g, 0, cs); }, &h->init_graph));
+  BEMCS_CHECK_HIP(hipGraphLaunch(h->init_graph, s));
+  return BEMCS_OK;
+}
+
+static int enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
+  if (!h->use_graphs) {
+    hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, s, g, k);
+    enqueue_vcycle(h, g, k, s);
+    return BEMCS_OK;
+  }
+  if ((int)h->iter_graph.size() <= k) h->iter_graph.resize(k + 1, nullptr);
+  if (!h->iter_graph[k]) {
+    auto body = [&](hipStream_t cs) {
+      hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, cs, g, k);
+      enqueue_vcycle(h, g, k, cs);
+    };
+    BEMCS_CHECK_HIP(capture(h, body, &h->iter_graph[k]));
+  }
+  BEMCS_CHECK_HIP(hipGraphLaunch(h->iter_graph[k], s));
+  return BEMCS_OK;
 }
 
 // PCG on the reduced system already in h->bred with initial guess h->x.
 static int pcg(bemcs_poisson_s* h, double rtol, int max_iter, int* iters, double* relres,
                hipStream_t s) {
   const CGArgs g = cg_args(h);
-  enqueue_init(h, g, rtol, s);
+  int rc = enqueue_init(h, g, rtol, s);
+  if (rc) return rc;
   BEMCS_CHECK_LAUNCH();
   // launch as many iterations as the previous solve needed, then check
   int launched = 0;
   int batch = std::max(1, h->pred_iters);
   for (;;) {
-    for (int j = 0; j < batch && launched < max_iter; ++j) enqueue_iter(h, g, ++launched, s);
+    for (int j = 0; j < batch && launched < max_iter; ++j) {
+      rc = enqueue_iter(h, g, ++launched, s);
+      if (rc) return rc;
+    }
     BEMCS_CHECK_LAUNCH();
     BEMCS_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, sizeof(double) * S_COUNT,
                                    hipMemcpyDeviceToHost, s));
@@ -1096,6 +1256,10 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
   h->pred_iters = 8;
   const char* ps = getenv("BEMCS_MG_POST_SWEEPS");
   h->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
+  const char* pg = getenv("BEMCS_POISSON_GRAPHS");
+  h->use_graphs = !(pg && pg[0] == '0');
+  h->cap_stream = nullptr;
+  h->init_graph = nullptr;
   bool ok = true;
   auto A = [&](size_t bytes) {
     void* p = dalloc(h, bytes);

@@ -33,7 +33,7 @@ _NP_FP = np.float32
 P_INJ_COUNT = 0x20
 P_RF_COUNT = 0x23
 
-RING = 1024  # per-step diagnostic records kept on the device
+RING = 8192  # per-step diagnostic records kept on the device
 
 MATERIAL_PRESETS = {
     'Molybdenum': {'k': 138.0, 'rho': 10280.0, 'cp': 250.0, 'emissivity': 0.80,
@@ -540,12 +540,24 @@ class DigitalTwinSimulator:
             self._timed_end(kt, "poisson_picard", ev)
         # the ion step may append one SEE electron per ion
         self._ensure_capacity(self.bound_i, self.bound_e + self.bound_i)
-        # substep counts for both species (:1177-1209)
-        self.ds_bits.zero_()
         I0, I1 = self.ions
         E0, E1 = self.elec
         dsb = self.ds_bits.data_ptr()
         nsb = self.n_sub_d.data_ptr()
+        # clear this step's outputs and record the pushed populations
+        slot = it % RING
+        dsz = C.sizeof(_lib.IonDiag)
+        diag_ptr = self.ring_diag.data_ptr() + slot * dsz
+        sb = _lib.StepBegin()
+        sb.ds_bits, sb.count_out_i, sb.count_out_e = dsb, I1.count.data_ptr(), E1.count.data_ptr()
+        sb.node_zero[0], sb.node_zero[1], sb.node_zero[2] = (self.cnt_i_next.data_ptr(),
+                                                             self.cnt_e_next.data_ptr(), self.ppc_d.data_ptr())
+        sb.n_nodes, sb.diag, sb.diag_bytes = self.nx * self.ny, diag_ptr, dsz
+        sb.ppc_stats = self.ring_ppc.data_ptr() + 8 * slot
+        sb.count_in_i, sb.count_in_e = I0.count.data_ptr(), E0.count.data_ptr()
+        sb.nin_out = self.ring_nin.data_ptr() + 8 * slot
+        check(lib.bemcs_step_begin(C.byref(sb), s), "step_begin")
+        # substep counts for both species (:1177-1209)
         check(lib.bemcs_substeps(ptr(I0.x), ptr(I0.y), ptr(I0.vx), ptr(I0.vy), ptr(I0.vz), ptr(I0.count),
                                  self.bound_i, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
                                  self.q_ion / self.m_ion, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb), C.c_void_p(nsb), s),
@@ -554,23 +566,11 @@ class DigitalTwinSimulator:
                                  self.bound_e, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
                                  -self.q / self.m_e, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb + 8), C.c_void_p(nsb + 4), s),
               "substeps(electrons)")
-        # outputs
-        slot = it % RING
-        dsz = C.sizeof(_lib.IonDiag)
-        diag_ptr = self.ring_diag.data_ptr() + slot * dsz
-        self.ring_diag[slot * dsz:(slot + 1) * dsz].zero_()
-        I1.count.zero_()
-        E1.count.zero_()
-        self.cnt_i_next.zero_()
-        self.cnt_e_next.zero_()
-        self.ppc_d.zero_()
         ip = self._ion_params
         ip.Exy, ip.is_bound, ip.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb)
         ip.cnt_next, ip.ppc = ptr(self.cnt_i_next), ptr(self.ppc_d)
         ip.crossed_angle, ip.diag, ip.e_cnt_next = ptr(self.crossed_angle), C.c_void_p(diag_ptr), ptr(self.cnt_e_next)
         ip.step = it & 0xFFFFFFFF
-        self.ring_nin[2 * slot].copy_(I0.count[0])
-        self.ring_nin[2 * slot + 1].copy_(E0.count[0])
         kt = self.kernel_timing
         ev = self._timed_begin(kt)
         check(lib.bemcs_ion_step(C.byref(I0.struct()), C.byref(I1.struct()), C.byref(E1.struct()),
@@ -589,16 +589,18 @@ class DigitalTwinSimulator:
         self.cnt_i, self.cnt_i_next = self.cnt_i_next, self.cnt_i
         self.cnt_e, self.cnt_e_next = self.cnt_e_next, self.cnt_e
         # per-step records
-        self.ring_ppc[2 * slot:2 * slot + 2].zero_()
         check(lib.bemcs_ppc_stats(ptr(self.ppc_d), self.nx * self.ny, ptr(self.accum_ppc_d),
                                   C.c_void_p(self.ring_ppc.data_ptr() + 8 * slot), s), "ppc_stats")
         self.ppc_steps_count += 1
         check(lib.bemcs_percentile(ptr(self.crossed_angle), C.c_void_p(diag_ptr + 4), self.bound_i, 95.0, 5,
                                    None, C.c_void_p(self.ring_theta.data_ptr() + 8 * slot), s), "percentile")
-        self.ring_minpot[slot] = torch.min(self.V_d[self._saddle_idx])
-        self.ring_counts[2 * slot].copy_(self.ions[0].count[0])
-        self.ring_counts[2 * slot + 1].copy_(self.elec[0].count[0])
-        self.ring_nsub[2 * slot:2 * slot + 2].copy_(self.n_sub_d)
+        se = _lib.StepEnd()
+        se.count_i, se.count_e = self.ions[0].count.data_ptr(), self.elec[0].count.data_ptr()
+        se.n_sub, se.counts_out = nsb, self.ring_counts.data_ptr() + 8 * slot
+        se.nsub_out = self.ring_nsub.data_ptr() + 8 * slot
+        se.V, se.saddle_idx, se.n_saddle = self.V_d.data_ptr(), self._saddle_idx.data_ptr(), self._saddle_idx.numel()
+        se.minpot_out = self.ring_minpot.data_ptr() + 8 * slot
+        check(lib.bemcs_step_end(C.byref(se), s), "step_end")
         self._sort(ions=self.sort_interval > 0 and it % self.sort_interval == 0,
                    electrons=self.electron_sort_interval > 0 and it % self.electron_sort_interval == 0)
         # asynchronous count readback for the host bounds
