@@ -271,6 +271,24 @@ typedef struct {
 int bemcs_poisson_picard(bemcs_poisson_t h, const bemcs_picard_t* p, int32_t* iters_total,
                          double* worst_relres, bemcs_stream_t stream);
 
+/* Batched Picard: the Poisson solves of several cases (each a handle from
+ * bemcs_poisson_create, meshes of any sizes) advance together, one launch
+ * per V-cycle level serving every case (launch index z = case), so a sweep
+ * of uneven meshes costs one launch chain per PCG iteration instead of one
+ * per case.  params, iters_total and worst_relres are host arrays of n
+ * entries; iterations, rtol and max_iter must agree across cases.  Results
+ * are bit-identical to bemcs_poisson_picard on each handle.  A handle must not
+ * be used by two batches, or by a batch and its own calls, at the same time.
+ * No reference counterpart (the reference runs sweep cases in separate
+ * processes, Python/benchmarks/benchmark_perveance.py:102-113). */
+typedef struct bemcs_poisson_batch_s* bemcs_poisson_batch_t;
+int bemcs_poisson_batch_create(const bemcs_poisson_t* handles, int32_t n,
+                               bemcs_poisson_batch_t* out);
+int bemcs_poisson_batch_free(bemcs_poisson_batch_t b);
+int bemcs_poisson_batch_size(bemcs_poisson_batch_t b);
+int bemcs_poisson_batch_picard(bemcs_poisson_batch_t b, const bemcs_picard_t* params,
+                               int32_t* iters_total, double* worst_relres, bemcs_stream_t stream);
+
 /* E = -grad V only (:871-884). */
 int bemcs_efield(const double* V, int32_t nx, int32_t ny, double dx_m, double dy_m,
                  int32_t periodic_y, float* Exy, bemcs_stream_t stream);

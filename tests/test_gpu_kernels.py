@@ -294,3 +294,81 @@ def test_poisson_error_vs_tolerance(lib):
             assert errs[rt] < 5e-11
     finally:
         lib.bemcs_poisson_free(h)
+
+
+def _picard_case(lib, nx, ny, periodic, seed):
+    o = _geometry(nx, ny, periodic)
+    r = np.random.default_rng(seed)
+    n = nx * ny
+    bufs = {
+        "ci": torch.from_numpy(r.poisson(3.0, n).astype(np.int32)).cuda(),
+        "ce": torch.from_numpy(r.poisson(2.0, n).astype(np.int32)).cuda(),
+        "Vf": _dev(o.V_fixed.ravel().astype(np.float64)),
+        "V": _dev(o.V_fixed.ravel().astype(np.float64)),
+        "rho": torch.zeros(n, dtype=torch.float32, device="cuda"),
+        "E": torch.zeros(2 * n, dtype=torch.float32, device="cuda"),
+    }
+    cell_vol = (o.dx * 1e-3) * (o.dy * 1e-3) * 1e-3
+    cd = 1.602e-19 * 1000.0 / cell_vol
+    return o, bufs, cd
+
+
+def _picard_params(o, bufs, cd):
+    from thruster_cx_code_amd import _lib as L
+    p = L.Picard()
+    p.cnt_ion, p.cnt_e = _p(bufs["ci"]), _p(bufs["ce"])
+    p.V_fixed, p.V, p.rho_out, p.Exy = _p(bufs["Vf"]), _p(bufs["V"]), _p(bufs["rho"]), _p(bufs["E"])
+    p.cd_ion, p.cd_e = cd, -cd
+    p.coeff = (o.dx * 1e-3) ** 2 / 8.854187817e-12
+    p.V_plasma, p.Te_up, p.n0, p.q, p.omega = 1020.0, 3.0, 1e17, 1.602e-19, 0.2
+    p.dx_m, p.dy_m = o.dx * 1e-3, o.dy * 1e-3
+    p.rtol, p.iterations, p.max_iter = 1e-9, 5, 200
+    return p
+
+
+def test_poisson_batch_picard_matches_single(lib):
+    """Three uneven meshes (one periodic) solved as one batch give the same
+    bits as each handle's own Picard call, over two calls (warm start)."""
+    from thruster_cx_code_amd import _lib as L
+    print("[gpu] poisson batch picard", flush=True)
+    specs = [(128, 48, False, 1), (257, 97, False, 2), (130, 60, True, 3)]
+    singles, batched, hs_single, hs_batch = [], [], [], []
+    try:
+        for nx, ny, per, seed in specs:
+            o, b1, cd = _picard_case(lib, nx, ny, per, seed)
+            _, b2, _ = _picard_case(lib, nx, ny, per, seed)
+            ib = np.ascontiguousarray(o.isBound.astype(np.uint8))
+            for lst in (hs_single, hs_batch):
+                h = C.c_void_p()
+                assert lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), nx, ny, int(per), C.byref(h)) == 0
+                lst.append(h)
+            singles.append((o, b1, cd))
+            batched.append((o, b2, cd))
+        arr = (C.c_void_p * len(specs))(*[h.value for h in hs_batch])
+        batch = C.c_void_p()
+        assert lib.bemcs_poisson_batch_create(arr, len(specs), C.byref(batch)) == 0
+        assert lib.bemcs_poisson_batch_size(batch) == len(specs)
+        try:
+            for call in range(2):
+                its_single = []
+                for h, (o, b, cd) in zip(hs_single, singles):
+                    it, rel = C.c_int32(0), C.c_double(0)
+                    p = _picard_params(o, b, cd)
+                    assert lib.bemcs_poisson_picard(h, C.byref(p), C.byref(it), C.byref(rel), _stream()) == 0, \
+                        lib.bemcs_last_error()
+                    its_single.append(it.value)
+                ps = (L.Picard * len(specs))(*[_picard_params(o, b, cd) for o, b, cd in batched])
+                its = (C.c_int32 * len(specs))()
+                rels = (C.c_double * len(specs))()
+                assert lib.bemcs_poisson_batch_picard(batch, ps, its, rels, _stream()) == 0, lib.bemcs_last_error()
+                print(f"  call {call}: single iters {its_single} batch iters {list(its)}")
+                assert list(its) == its_single
+                for (_, b1, _), (_, b2, _) in zip(singles, batched):
+                    for key in ("V", "E", "rho"):
+                        assert torch.equal(b1[key].view(torch.int32) if key != "V" else b1[key].view(torch.int64),
+                                           b2[key].view(torch.int32) if key != "V" else b2[key].view(torch.int64)), key
+        finally:
+            lib.bemcs_poisson_batch_free(batch)
+    finally:
+        for h in hs_single + hs_batch:
+            lib.bemcs_poisson_free(h)

@@ -104,6 +104,10 @@ _SIGS = {
     "bemcs_poisson_solve": (i32, [vp, vp, vp, f64, i32, vp, vp, vp]),
     "bemcs_poisson_picard": (i32, [vp, vp, vp, vp, vp]),
     "bemcs_efield": (i32, [vp, i32, i32, f64, f64, i32, vp, vp]),
+    "bemcs_poisson_batch_create": (i32, [vp, i32, vp]),
+    "bemcs_poisson_batch_free": (i32, [vp]),
+    "bemcs_poisson_batch_size": (i32, [vp]),
+    "bemcs_poisson_batch_picard": (i32, [vp, vp, vp, vp, vp]),
     "bemcs_percentile": (i32, [vp, vp, i64, f64, u32, vp, vp, vp]),
     "bemcs_ppc_stats": (i32, [vp, i64, vp, vp, vp]),
     "bemcs_step_begin": (i32, [vp, vp]),

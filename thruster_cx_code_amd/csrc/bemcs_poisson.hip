@@ -18,7 +18,10 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <string.h>
+
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "bemcs_internal.h"
@@ -61,16 +64,8 @@ struct bemcs_poisson_s {
   int nparts;         // partial pairs per reduction buffer (level-0 tiles)
   double *pa, *pr, *pi;
   double* pz[2];
-  double* scal;       // device scalars, see S_* below
-  int32_t* done;
-  double* host_scal;  // pinned copy
   bool have_guess;
-  int pred_iters;     // iterations the last solve needed (launch that many first)
-  int post_sweeps;    // Jacobi sweeps after prolongation (1 or 2)
-  bool use_graphs;    // PCG iterations replayed from captured HIP graphs
-  hipStream_t cap_stream;                  // private stream used only for capture
-  std::vector<hipGraphExec_t> iter_graph;  // [k] = iteration k (dir + V-cycle)
-  hipGraphExec_t init_graph;               // initial V-cycle
+  bemcs_poisson_batch_s* own;  // solve set of this handle alone
 };
 
 namespace bemcs {
@@ -213,25 +208,42 @@ struct CGArgs {
 struct CheckArgs {
   const double* pr;
   int nparts;
+  int check;  // this launch directly follows the step: it runs the convergence test
   double* scal;
   int32_t* done;
-  int k;  // iteration whose step precedes this launch; 0 = no check
 };
 
-// Convergence test of iteration ca.k (every block computes it identically;
-// all NT threads of the block take part).
+// Convergence test of iteration k (every block of a case computes it
+// identically; all NT threads of the block take part).
 template <int NT = 256>
-__device__ __forceinline__ bool check_converged(const CheckArgs& ca) {
+__device__ __forceinline__ bool check_converged(const CheckArgs& ca, int k) {
   double rr, unused;
   block_sum2<NT>(ca.pr, ca.nparts, rr, unused);
   const bool stop = !(rr > ca.scal[S_TOL2]);  // converged, or NaN
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     ca.scal[S_RR] = rr;
-    ca.scal[S_ITERS] = (double)ca.k;
+    ca.scal[S_ITERS] = (double)k;
     if (stop) *ca.done = 1;
   }
   return stop;
 }
+
+// Arguments of one case for the V-cycle launches.  A launch serves one case
+// (arguments by value) or a batch (blockIdx.z indexes a device array).
+struct DownPack {
+  LevelDev F, C;
+  CGArgs g;
+  CheckArgs ca;
+};
+struct UpPack {
+  LevelDev F, C;  // C.x is the coarse result
+  CGArgs g;
+};
+struct CoarsePack {
+  LevelDev C;
+  const float* cinv;
+  CheckArgs ca;
+};
 
 // Launches that run a convergence test can see `done` change while they run
 // (block 0 sets it); a block reads it once and shares it, so all of its waves
@@ -261,12 +273,18 @@ __device__ __forceinline__ float tile_apply(const float* c, int W, const float* 
   return a;
 }
 
-template <int TC, bool IS9, bool STEP>
-__global__ __launch_bounds__(256) void k_down(LevelDev F, LevelDev C, CGArgs g, int k,
-                                              CheckArgs ca) {
-  if (ca.k > 0) {
-    if (block_done(ca.done) || check_converged(ca)) return;
-  } else if (*ca.done) {
+template <int TC, bool IS9, bool STEP, bool B>
+__global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __restrict__ many,
+                                              int k) {
+  const DownPack P = B ? many[blockIdx.z] : one;
+  const LevelDev& F = P.F;
+  const LevelDev& C = P.C;
+  const CGArgs& g = P.g;
+  const int tiles_x = (C.nx + TC - 1) / TC;
+  if ((int)blockIdx.x >= tiles_x || (int)blockIdx.y * TC >= C.ny) return;
+  if (k > 0 && P.ca.check) {
+    if (block_done(P.ca.done) || check_converged(P.ca, k)) return;
+  } else if (*P.ca.done) {
     return;
   }
   constexpr int TFd = 2 * TC;                 // owned fine nodes per edge
@@ -362,7 +380,7 @@ __global__ __launch_bounds__(256) void k_down(LevelDev F, LevelDev C, CGArgs g, 
       F.b[gi] = (float)rn;
       rr += rn * rn;
     }
-    block_partials(rr, 0.0, g.pr, blockIdx.y * gridDim.x + blockIdx.x);
+    block_partials(rr, 0.0, g.pr, blockIdx.y * tiles_x + blockIdx.x);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < N2; t += 256) {
@@ -446,10 +464,15 @@ __device__ __forceinline__ void pe_terms(const LevelDev& F, const LevelDev& C, i
 }
 
 // C.x must point at the coarse result (C.t, or C.x of the coarsest level).
-template <int TF, int SWEEPS, bool IS9, bool DOT>
-__global__ __launch_bounds__(256) void k_up(LevelDev F, LevelDev C, CGArgs g, int k,
-                                            const int32_t* __restrict__ done) {
-  if (*done) return;
+template <int TF, int SWEEPS, bool IS9, bool DOT, bool B>
+__global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict__ many, int k) {
+  const UpPack P = B ? many[blockIdx.z] : one;
+  const LevelDev& F = P.F;
+  const LevelDev& C = P.C;
+  const CGArgs& g = P.g;
+  const int tiles_x = (F.nx + TF - 1) / TF;
+  if ((int)blockIdx.x >= tiles_x || (int)blockIdx.y * TF >= F.ny) return;
+  if (*g.done) return;
   constexpr int W0 = TF + 2 * SWEEPS, W1 = W0 - 2;  // the coefficient region is W1
   constexpr int N0 = W0 * W0, N1 = W1 * W1;
   constexpr int IT0 = (N0 + 255) / 256, IT1 = (N1 + 255) / 256;
@@ -554,21 +577,26 @@ __global__ __launch_bounds__(256) void k_up(LevelDev F, LevelDev C, CGArgs g, in
       roz += g.R[(k + 1) & 1][gi] * (double)v;
     }
   }
-  if (DOT) block_partials(rz, roz, g.pz[k & 1], blockIdx.y * gridDim.x + blockIdx.x);
+  if (DOT) block_partials(rz, roz, g.pz[k & 1], blockIdx.y * tiles_x + blockIdx.x);
 }
 
 // coarsest level: x = C^-1 b with the inverse stored column-major.  A block
 // owns 64 rows (one per lane); its 16 waves split the columns, and the partial
 // sums are added in LDS, so each thread runs only nc/16 independent loads.
 constexpr int kCoarseWaves = 16;
-__global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(LevelDev C,
-                                                                    const float* __restrict__ cinv,
-                                                                    CheckArgs ca) {
+template <bool B>
+__global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(CoarsePack one,
+                                                                    const CoarsePack* __restrict__ many,
+                                                                    int k) {
   __shared__ float sb[kDenseMax];
   __shared__ float part[kCoarseWaves][64];
-  if (ca.k > 0) {
-    if (block_done(ca.done) || check_converged<64 * kCoarseWaves>(ca)) return;
-  } else if (*ca.done) {
+  const CoarsePack P = B ? many[blockIdx.z] : one;
+  const LevelDev& C = P.C;
+  const float* __restrict__ cinv = P.cinv;
+  if ((int)blockIdx.x * 64 >= C.n) return;
+  if (k > 0 && P.ca.check) {
+    if (block_done(P.ca.done) || check_converged<64 * kCoarseWaves>(P.ca, k)) return;
+  } else if (*P.ca.done) {
     return;
   }
   const int nc = C.n;
@@ -602,10 +630,15 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(LevelDev C,
 
 // ----------------------------------------------------------- CG kernels
 // grid-stride loops over g.nparts blocks, one partial pair per block
-#define GRID_LOOP(i, n) for (int i = blockIdx.x * 256 + threadIdx.x; i < (n); i += gridDim.x * 256)
+#define GRID_LOOP(i, n) for (int i = blockIdx.x * 256 + threadIdx.x; i < (n); i += g.nparts * 256)
+#define CG_CASE()                               \
+  const CGArgs g = B ? many[blockIdx.z] : one;  \
+  if ((int)blockIdx.x >= g.nparts) return
 
 // r0 = b - A x -> R[0], R[1] = 0, b0 = f32(r0); partials (r0.r0, b.b)
-__global__ __launch_bounds__(256) void k_cg_init(CGArgs g) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_cg_init(CGArgs one, const CGArgs* __restrict__ many) {
+  CG_CASE();
   double rr = 0, bb = 0;
   GRID_LOOP(i, g.n) {
     double ri = g.code[i] ? g.bred[i] - apply5<double>(g.x, g.code, i, g.nx, g.ny, g.per != 0) : 0.0;
@@ -619,7 +652,10 @@ __global__ __launch_bounds__(256) void k_cg_init(CGArgs g) {
 }
 
 // tolerance and the initial convergence test
-__global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs g, double rtol) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs one, const CGArgs* __restrict__ many,
+                                                        double rtol) {
+  CG_CASE();
   double rr, bb;
   block_sum2(g.pi, g.nparts, rr, bb);
   if (threadIdx.x != 0) return;
@@ -632,7 +668,9 @@ __global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs g, double rtol) {
 }
 
 // p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap
-__global__ __launch_bounds__(256) void k_cg_dir(CGArgs g, int k) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_cg_dir(CGArgs one, const CGArgs* __restrict__ many, int k) {
+  CG_CASE();
   if (*g.done) return;
   double beta = 0.0;
   if (k > 1) {
@@ -663,7 +701,9 @@ __global__ __launch_bounds__(256) void k_cg_dir(CGArgs g, int k) {
 }
 
 // single-level hierarchy: the step without a fused down sweep
-__global__ __launch_bounds__(256) void k_cg_step(CGArgs g, int k) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_cg_step(CGArgs one, const CGArgs* __restrict__ many, int k) {
+  CG_CASE();
   if (*g.done) return;
   double rz, pap, u;
   block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
@@ -683,7 +723,9 @@ __global__ __launch_bounds__(256) void k_cg_step(CGArgs g, int k) {
 }
 
 // single-level hierarchy: partials (r_k.z, r_{k-1}.z)
-__global__ __launch_bounds__(256) void k_cg_dotz(CGArgs g, int k) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_cg_dotz(CGArgs one, const CGArgs* __restrict__ many, int k) {
+  CG_CASE();
   if (*g.done) return;
   double a = 0, b = 0;
   GRID_LOOP(i, g.n) {
@@ -717,45 +759,6 @@ __global__ __launch_bounds__(256) void k_reduce_rhs(const double* __restrict__ b
   if (guess) x[i] = V[i];
 }
 
-// Picard right-hand side (:891-898) directly in reduced form:
-// rho (float32) from NGP counts, rho_total = rho + rho_e(V),
-// b_red = coeff*rho_total + sum of Dirichlet neighbour values.
-__global__ __launch_bounds__(256) void k_picard_rhs(const float* __restrict__ rho,
-                                                    const double* __restrict__ V,
-                                                    const double* __restrict__ Vf,
-                                                    const uint8_t* __restrict__ kind, double* bred,
-                                                    int nx, int ny, int per, double coeff,
-                                                    double Vp, double Te, double qn0) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nx * ny) return;
-  if (kind[i] != K_UNKNOWN) {
-    bred[i] = 0.0;
-    return;
-  }
-  double v = V[i];
-  double rho_e = qn0 * det_exp(((v < Vp ? v : Vp) - Vp) / Te);
-  double rt = (double)rho[i] + rho_e;
-  double s = -((-coeff) * rt);
-  int xx = i % nx, y = i / nx;
-  int nb[4] = {i - 1, i + 1, nbr_s(i, xx, y, nx, ny, per != 0), nbr_n(i, xx, y, nx, ny, per != 0)};
-  for (int k = 0; k < 4; ++k)
-    if (kind[nb[k]] == K_BOUND) s += Vf[nb[k]];
-  bred[i] = s;
-}
-
-// rho from NGP counts as the reference's two np.add.at deposits produce it:
-// per particle rho = f32(f64(rho) + cd), ions first, then electrons.
-__global__ __launch_bounds__(256) void k_rho_from_counts(const uint32_t* __restrict__ ci,
-                                                         const uint32_t* __restrict__ ce, int n,
-                                                         double cdi, double cde, float* rho) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float r = 0.0f;
-  for (uint32_t k = ci[i]; k > 0; --k) r = (float)((double)r + cdi);
-  for (uint32_t k = ce[i]; k > 0; --k) r = (float)((double)r + cde);
-  rho[i] = r;
-}
-
 // Full solution from the reduced one: unknown -> x, Dirichlet -> b (value),
 // Neumann -> value of its mirror plus the row's right-hand sides on the way.
 __global__ __launch_bounds__(256) void k_assemble(const double* __restrict__ x,
@@ -780,33 +783,9 @@ __global__ __launch_bounds__(256) void k_assemble(const double* __restrict__ x,
   }
 }
 
-// Picard variant: Dirichlet values from V_fixed, Neumann rows have b = 0;
-// V = (1-omega) V + omega V_new.
-__global__ __launch_bounds__(256) void k_picard_relax(const double* __restrict__ x,
-                                                      const double* __restrict__ Vf,
-                                                      const uint8_t* __restrict__ kind,
-                                                      const int32_t* __restrict__ mirror, double* V,
-                                                      double* Vnew, int n, double one_m_omega,
-                                                      double omega) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  uint8_t k = kind[i];
-  double vn;
-  if (k == K_UNKNOWN) vn = x[i];
-  else if (k == K_BOUND) vn = Vf[i];
-  else {
-    int m = mirror[i];
-    vn = kind[m] == K_UNKNOWN ? x[m] : Vf[m];
-  }
-  Vnew[i] = vn;
-  V[i] = one_m_omega * V[i] + omega * vn;
-}
-
 // E = -grad V (np.gradient, edge_order 1) with the periodic Ey fix; float32
-__global__ __launch_bounds__(256) void k_efield(const double* __restrict__ V, int nx, int ny,
-                                                double dx_m, double dy_m, int per, float* Exy) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nx * ny) return;
+__device__ __forceinline__ void efield_at(const double* __restrict__ V, int nx, int ny, double dx_m,
+                                          double dy_m, int per, float* Exy, int i) {
   int x = i % nx, y = i / nx;
   double ex, ey;
   if (x == 0) ex = (V[i] - V[i + 1]) / dx_m;
@@ -818,6 +797,80 @@ __global__ __launch_bounds__(256) void k_efield(const double* __restrict__ V, in
   else if (y == ny - 1) ey = (V[i - nx] - V[i]) / dy_m;
   else ey = (V[i - nx] - V[i + nx]) / (2.0 * dy_m);
   reinterpret_cast<float2*>(Exy)[i] = make_float2((float)ex, (float)ey);
+}
+
+__global__ __launch_bounds__(256) void k_efield(const double* __restrict__ V, int nx, int ny,
+                                                double dx_m, double dy_m, int per, float* Exy) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nx * ny) efield_at(V, nx, ny, dx_m, dy_m, per, Exy, i);
+}
+
+// ------------------------------------------------------------ Picard loop
+// Per-case arguments of the Picard loop (:886-902) and the field update.
+struct PicardPack {
+  const uint32_t* cnt_ion;
+  const uint32_t* cnt_e;
+  double cd_ion, cd_e;
+  float* rho;
+  const double* V_fixed;
+  double* V;
+  double* Vnew;
+  const uint8_t* kind;
+  const int32_t* mirror;
+  double* bred;
+  const double* x;
+  int nx, ny, per, n;
+  double coeff, Vp, Te, qn0, one_m_omega, omega, dx_m, dy_m;
+  float* Exy;
+};
+
+enum { PIC_RHO = 0, PIC_RHS = 1, PIC_RELAX = 2, PIC_EFIELD = 3 };
+
+template <int PHASE, bool B>
+__global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack* __restrict__ many) {
+  const PicardPack P = B ? many[blockIdx.z] : one;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  if (PHASE == PIC_RHO) {
+    // rho from NGP counts as the reference's two np.add.at deposits produce
+    // it: per particle rho = f32(f64(rho) + cd), ions first, then electrons
+    float r = 0.0f;
+    for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
+    for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
+    P.rho[i] = r;
+  } else if (PHASE == PIC_RHS) {
+    // right-hand side (:891-898) in reduced form: rho_total = rho + rho_e(V),
+    // b_red = coeff*rho_total + sum of Dirichlet neighbour values
+    if (P.kind[i] != K_UNKNOWN) {
+      P.bred[i] = 0.0;
+      return;
+    }
+    const double v = P.V[i];
+    const double rho_e = P.qn0 * det_exp(((v < P.Vp ? v : P.Vp) - P.Vp) / P.Te);
+    const double rt = (double)P.rho[i] + rho_e;
+    double sum = -((-P.coeff) * rt);
+    const int xx = i % P.nx, y = i / P.nx;
+    const int nb[4] = {i - 1, i + 1, nbr_s(i, xx, y, P.nx, P.ny, P.per != 0),
+                       nbr_n(i, xx, y, P.nx, P.ny, P.per != 0)};
+    for (int k = 0; k < 4; ++k)
+      if (P.kind[nb[k]] == K_BOUND) sum += P.V_fixed[nb[k]];
+    P.bred[i] = sum;
+  } else if (PHASE == PIC_RELAX) {
+    // Dirichlet values from V_fixed, Neumann rows mirror;
+    // V = (1-omega) V + omega V_new
+    const uint8_t k = P.kind[i];
+    double vn;
+    if (k == K_UNKNOWN) vn = P.x[i];
+    else if (k == K_BOUND) vn = P.V_fixed[i];
+    else {
+      const int m = P.mirror[i];
+      vn = P.kind[m] == K_UNKNOWN ? P.x[m] : P.V_fixed[m];
+    }
+    P.Vnew[i] = vn;
+    P.V[i] = P.one_m_omega * P.V[i] + P.omega * vn;
+  } else {
+    efield_at(P.V, P.nx, P.ny, P.dx_m, P.dy_m, P.per, P.Exy, i);
+  }
 }
 
 // ------------------------------------------------------------- host set-up
@@ -986,19 +1039,77 @@ static int upload(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
 }
 
+static void destroy_set(bemcs_poisson_batch_s* S);
+
 static void destroy(bemcs_poisson_s* h) {
   if (!h) return;
-  for (hipGraphExec_t e : h->iter_graph)
-    if (e) (void)hipGraphExecDestroy(e);
-  if (h->init_graph) (void)hipGraphExecDestroy(h->init_graph);
-  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+  destroy_set(h->own);
   for (void* p : h->allocs) (void)hipFree(p);
-  if (h->host_scal) (void)hipHostFree(h->host_scal);
   delete h;
 }
 
-// --------------------------------------------------------- V-cycle driver
-static CGArgs cg_args(bemcs_poisson_s* h) {
+// ------------------------------------------------------------- solve sets
+// A solve set is one or more cases whose PCG iterations advance together: a
+// launch of one V-cycle position serves every case that has that level
+// (blockIdx.z indexes the case's argument pack), so a batch of uneven meshes
+// costs one launch chain per iteration.  Each handle owns a set of one for its
+// own solves; bemcs_poisson_batch_create builds sets over several handles.
+// Per case, the set owns the PCG scalars and the stop flag; vectors and
+// partial buffers belong to the handle (a handle must not be used by two
+// sets at the same time).
+enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_COARSE, OP_UP, OP_DOTZ };
+
+struct Op {
+  OpKind kind;
+  int level;
+  dim3 grid;  // grid.z = number of cases in the op
+  void* packs_dev;
+  // the single-case path passes the (only) pack by value
+  CGArgs cg;
+  DownPack down;
+  UpPack up;
+  CoarsePack coarse;
+};
+
+}  // namespace bemcs
+
+struct bemcs_poisson_batch_s {
+  std::vector<bemcs_poisson_s*> cases;
+  int n;
+  int post_sweeps;
+  bool use_graphs;
+  std::vector<bemcs::CGArgs> cg;  // per case
+  std::vector<bemcs::Op> init_ops, iter_ops;
+  std::vector<void*> allocs;
+  double* scal;        // [n][S_COUNT]
+  int32_t* done;       // [n]
+  double* host_scal;   // pinned copy of scal
+  bemcs::PicardPack* pic_dev;
+  bemcs::PicardPack* pic_host;  // pinned staging
+  hipStream_t cap_stream;
+  std::vector<hipGraphExec_t> iter_graph;
+  hipGraphExec_t init_graph;
+  double init_rtol;
+  int pred_iters;
+};
+
+namespace bemcs {
+
+using SolveSet = bemcs_poisson_batch_s;
+
+static void destroy_set(SolveSet* S) {
+  if (!S) return;
+  for (hipGraphExec_t e : S->iter_graph)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (S->init_graph) (void)hipGraphExecDestroy(S->init_graph);
+  if (S->cap_stream) (void)hipStreamDestroy(S->cap_stream);
+  for (void* p : S->allocs) (void)hipFree(p);
+  if (S->host_scal) (void)hipHostFree(S->host_scal);
+  if (S->pic_host) (void)hipHostFree(S->pic_host);
+  delete S;
+}
+
+static CGArgs cg_args_of(const bemcs_poisson_s* h, double* scal, int32_t* done) {
   CGArgs g;
   g.x = h->x;
   g.R[0] = h->R[0];
@@ -1019,145 +1130,447 @@ static CGArgs cg_args(bemcs_poisson_s* h) {
   g.pz[0] = h->pz[0];
   g.pz[1] = h->pz[1];
   g.pi = h->pi;
-  g.scal = h->scal;
-  g.done = h->done;
+  g.scal = scal;
+  g.done = done;
   return g;
 }
 
-// V-cycle k: k = 0 is the initial one (its right-hand side b0 was written by
-// k_cg_init); for k >= 1 the CG step of iteration k is fused into the first
-// down launch and the launch after it runs the convergence test.
-static void enqueue_vcycle(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
-  const int last = h->nlev - 1;
-  CheckArgs ck{h->pr, h->nparts, h->scal, h->done, 0};
-  if (last == 0) {
-    if (k > 0) hipLaunchKernelGGL(k_cg_step, dim3(h->nparts), dim3(256), 0, s, g, k);
-    ck.k = k;
-    hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[0].n + 63) / 64), dim3(64 * kCoarseWaves), 0, s,
-                       h->lev[0], (const float*)h->cinv, ck);
-    hipLaunchKernelGGL(k_cg_dotz, dim3(h->nparts), dim3(256), 0, s, g, k);
-    return;
+static unsigned cdiv(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+// Builds the ops of one V-cycle (k = 0: the initial one; iterations add
+// OP_DIR in front) with their device packs.
+// Pack bytes to upload, and the op whose packs_dev receives the address.
+struct PackUpload {
+  std::vector<uint8_t> bytes;
+  std::vector<Op>* ops;
+  size_t index;
+};
+
+static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
+                      std::vector<PackUpload>& uploads) {
+  const int n = S->n;
+  int lmax = 0, nparts_max = 0;
+  for (auto* h : S->cases) {
+    lmax = std::max(lmax, h->nlev);
+    nparts_max = std::max(nparts_max, h->nparts);
   }
-  for (int l = 0; l < last; ++l) {
-    const LevelDev& F = h->lev[l];
-    const LevelDev& C = h->lev[l + 1];
-    CheckArgs c = ck;
-    c.k = (l == 1 && k > 0) ? k : 0;
-    if (l == 0) {
-      dim3 grid((C.nx + kTC - 1) / kTC, (C.ny + kTC - 1) / kTC);
-      if (k > 0) hipLaunchKernelGGL((k_down<kTC, false, true>), grid, dim3(256), 0, s, F, C, g, k, c);
-      else hipLaunchKernelGGL((k_down<kTC, false, false>), grid, dim3(256), 0, s, F, C, g, k, c);
-    } else {
-      dim3 grid((C.nx + kTCc - 1) / kTCc, (C.ny + kTCc - 1) / kTCc);
-      hipLaunchKernelGGL((k_down<kTCc, true, false>), grid, dim3(256), 0, s, F, C, g, k, c);
+  auto add_cg = [&](OpKind kind, std::vector<int> idx, dim3 grid) {
+    if (idx.empty()) return;
+    Op op{};
+    op.kind = kind;
+    op.grid = dim3(grid.x, grid.y, (unsigned)idx.size());
+    std::vector<uint8_t> bytes(sizeof(CGArgs) * idx.size());
+    for (size_t j = 0; j < idx.size(); ++j) memcpy(&bytes[j * sizeof(CGArgs)], &S->cg[idx[j]], sizeof(CGArgs));
+    op.cg = S->cg[idx[0]];
+    ops.push_back(op);
+    uploads.push_back({bytes, &ops, ops.size() - 1});
+  };
+  std::vector<int> all(n), plain;
+  for (int c = 0; c < n; ++c) {
+    all[c] = c;
+    if (S->cases[c]->nlev == 1) plain.push_back(c);
+  }
+  if (!iteration) {
+    add_cg(OP_INIT, all, dim3(nparts_max, 1));
+    add_cg(OP_INIT_SCALAR, all, dim3(1, 1));
+  } else {
+    add_cg(OP_DIR, all, dim3(nparts_max, 1));
+    add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
+  }
+  for (int l = 0; l + 1 < lmax; ++l) {
+    std::vector<DownPack> packs;
+    unsigned gx = 1, gy = 1;
+    const int tc = l == 0 ? kTC : kTCc;
+    for (int c = 0; c < n; ++c) {
+      bemcs_poisson_s* h = S->cases[c];
+      if (h->nlev - 1 <= l) continue;
+      DownPack P;
+      P.F = h->lev[l];
+      P.C = h->lev[l + 1];
+      P.g = S->cg[c];
+      P.ca = CheckArgs{h->pr, h->nparts, l == 1 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
+      gx = std::max(gx, cdiv(P.C.nx, tc));
+      gy = std::max(gy, cdiv(P.C.ny, tc));
+      packs.push_back(P);
     }
+    if (packs.empty()) continue;
+    Op op{};
+    op.kind = OP_DOWN;
+    op.level = l;
+    op.grid = dim3(gx, gy, (unsigned)packs.size());
+    op.down = packs[0];
+    std::vector<uint8_t> bytes(sizeof(DownPack) * packs.size());
+    memcpy(bytes.data(), packs.data(), bytes.size());
+    ops.push_back(op);
+    uploads.push_back({bytes, &ops, ops.size() - 1});
   }
-  ck.k = (last == 1 && k > 0) ? k : 0;
-  hipLaunchKernelGGL(k_coarse_solve, dim3((h->lev[last].n + 63) / 64), dim3(64 * kCoarseWaves), 0,
-                     s, h->lev[last], (const float*)h->cinv, ck);
-  for (int l = last - 1; l >= 0; --l) {
-    const LevelDev& F = h->lev[l];
-    LevelDev C = h->lev[l + 1];
-    if (l + 1 != last) C.x = C.t;  // the coarse level's result
-    const bool two = h->post_sweeps > 1;
-    if (l == 0) {
-      dim3 grid((F.nx + kTF - 1) / kTF, (F.ny + kTF - 1) / kTF);
-      if (two) hipLaunchKernelGGL((k_up<kTF, 2, false, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
-      else hipLaunchKernelGGL((k_up<kTF, 1, false, true>), grid, dim3(256), 0, s, F, C, g, k, h->done);
-    } else {
-      dim3 grid((F.nx + kTFc - 1) / kTFc, (F.ny + kTFc - 1) / kTFc);
-      if (two) hipLaunchKernelGGL((k_up<kTFc, 2, true, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
-      else hipLaunchKernelGGL((k_up<kTFc, 1, true, false>), grid, dim3(256), 0, s, F, C, g, k, h->done);
+  {
+    std::vector<CoarsePack> packs;
+    unsigned gx = 1;
+    for (int c = 0; c < n; ++c) {
+      bemcs_poisson_s* h = S->cases[c];
+      CoarsePack P;
+      P.C = h->lev[h->nlev - 1];
+      P.cinv = h->cinv;
+      P.ca = CheckArgs{h->pr, h->nparts, h->nlev <= 2 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
+      gx = std::max(gx, cdiv(P.C.n, 64));
+      packs.push_back(P);
+    }
+    Op op{};
+    op.kind = OP_COARSE;
+    op.grid = dim3(gx, 1, (unsigned)packs.size());
+    op.coarse = packs[0];
+    std::vector<uint8_t> bytes(sizeof(CoarsePack) * packs.size());
+    memcpy(bytes.data(), packs.data(), bytes.size());
+    ops.push_back(op);
+    uploads.push_back({bytes, &ops, ops.size() - 1});
+  }
+  for (int l = lmax - 2; l >= 0; --l) {
+    std::vector<UpPack> packs;
+    unsigned gx = 1, gy = 1;
+    const int tf = l == 0 ? kTF : kTFc;
+    for (int c = 0; c < n; ++c) {
+      bemcs_poisson_s* h = S->cases[c];
+      if (h->nlev - 1 <= l) continue;
+      UpPack P;
+      P.F = h->lev[l];
+      P.C = h->lev[l + 1];
+      if (l + 1 != h->nlev - 1) P.C.x = P.C.t;  // the coarse level's result
+      P.g = S->cg[c];
+      gx = std::max(gx, cdiv(P.F.nx, tf));
+      gy = std::max(gy, cdiv(P.F.ny, tf));
+      packs.push_back(P);
+    }
+    if (packs.empty()) continue;
+    Op op{};
+    op.kind = OP_UP;
+    op.level = l;
+    op.grid = dim3(gx, gy, (unsigned)packs.size());
+    op.up = packs[0];
+    std::vector<uint8_t> bytes(sizeof(UpPack) * packs.size());
+    memcpy(bytes.data(), packs.data(), bytes.size());
+    ops.push_back(op);
+    uploads.push_back({bytes, &ops, ops.size() - 1});
+  }
+  add_cg(OP_DOTZ, plain, dim3(nparts_max, 1));
+  return true;
+}
+
+static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t s) {
+  const bool B = S->n > 1;
+  const dim3 blk(256);
+  const CGArgs* cgm = (const CGArgs*)op.packs_dev;
+  const bool two = S->post_sweeps > 1;
+  switch (op.kind) {
+    case OP_INIT:
+      if (B) hipLaunchKernelGGL(k_cg_init<true>, op.grid, blk, 0, s, op.cg, cgm);
+      else hipLaunchKernelGGL(k_cg_init<false>, op.grid, blk, 0, s, op.cg, cgm);
+      break;
+    case OP_INIT_SCALAR:
+      if (B) hipLaunchKernelGGL(k_cg_init_scalar<true>, op.grid, blk, 0, s, op.cg, cgm, rtol);
+      else hipLaunchKernelGGL(k_cg_init_scalar<false>, op.grid, blk, 0, s, op.cg, cgm, rtol);
+      break;
+    case OP_DIR:
+      if (B) hipLaunchKernelGGL(k_cg_dir<true>, op.grid, blk, 0, s, op.cg, cgm, k);
+      else hipLaunchKernelGGL(k_cg_dir<false>, op.grid, blk, 0, s, op.cg, cgm, k);
+      break;
+    case OP_STEP_PLAIN:
+      if (B) hipLaunchKernelGGL(k_cg_step<true>, op.grid, blk, 0, s, op.cg, cgm, k);
+      else hipLaunchKernelGGL(k_cg_step<false>, op.grid, blk, 0, s, op.cg, cgm, k);
+      break;
+    case OP_DOTZ:
+      if (B) hipLaunchKernelGGL(k_cg_dotz<true>, op.grid, blk, 0, s, op.cg, cgm, k);
+      else hipLaunchKernelGGL(k_cg_dotz<false>, op.grid, blk, 0, s, op.cg, cgm, k);
+      break;
+    case OP_DOWN: {
+      const DownPack* pm = (const DownPack*)op.packs_dev;
+      if (op.level == 0) {
+        if (k > 0) {
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, op.down, pm, k);
+          else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, op.down, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, op.down, pm, k);
+          else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+        }
+      } else {
+        if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, op.down, pm, k);
+        else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+      }
+      break;
+    }
+    case OP_COARSE: {
+      const CoarsePack* pm = (const CoarsePack*)op.packs_dev;
+      if (B) hipLaunchKernelGGL(k_coarse_solve<true>, op.grid, dim3(64 * kCoarseWaves), 0, s, op.coarse, pm, k);
+      else hipLaunchKernelGGL(k_coarse_solve<false>, op.grid, dim3(64 * kCoarseWaves), 0, s, op.coarse, pm, k);
+      break;
+    }
+    case OP_UP: {
+      const UpPack* pm = (const UpPack*)op.packs_dev;
+      if (op.level == 0) {
+        if (two) {
+          if (B) hipLaunchKernelGGL((k_up<kTF, 2, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTF, 2, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_up<kTF, 1, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTF, 1, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
+        }
+      } else {
+        if (two) {
+          if (B) hipLaunchKernelGGL((k_up<kTFc, 2, true, false, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFc, 2, true, false, false>), op.grid, blk, 0, s, op.up, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_up<kTFc, 1, true, false, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFc, 1, true, false, false>), op.grid, blk, 0, s, op.up, pm, k);
+        }
+      }
+      break;
     }
   }
 }
 
-// Captures what `enqueue` launches into an executable graph (on the private
-// capture stream: nothing runs).  The launches' arguments are fixed per
-// handle and iteration, so a graph is captured once and replayed.
+static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
+  *out = nullptr;
+  SolveSet* S = new SolveSet();
+  S->n = n;
+  S->cases.assign(hs, hs + n);
+  const char* ps = getenv("BEMCS_MG_POST_SWEEPS");
+  S->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
+  const char* pg = getenv("BEMCS_POISSON_GRAPHS");
+  S->use_graphs = !(pg && pg[0] == '0');
+  S->pred_iters = 8;
+  S->init_rtol = -1.0;
+  bool ok = true;
+  auto A = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (ok && hipMalloc(&p, bytes) == hipSuccess) {
+      S->allocs.push_back(p);
+      return p;
+    }
+    ok = false;
+    return nullptr;
+  };
+  S->scal = (double*)A(sizeof(double) * S_COUNT * n);
+  S->done = (int32_t*)A(sizeof(int32_t) * n);
+  S->pic_dev = (PicardPack*)A(sizeof(PicardPack) * n);
+  if (ok) ok = hipMemset(S->scal, 0, sizeof(double) * S_COUNT * n) == hipSuccess;
+  if (ok) ok = hipMemset(S->done, 0, sizeof(int32_t) * n) == hipSuccess;
+  if (ok) ok = hipHostMalloc((void**)&S->host_scal, sizeof(double) * S_COUNT * n, 0) == hipSuccess;
+  if (ok) ok = hipHostMalloc((void**)&S->pic_host, sizeof(PicardPack) * n, 0) == hipSuccess;
+  if (ok) {
+    for (int c = 0; c < n; ++c) S->cg.push_back(cg_args_of(hs[c], S->scal + S_COUNT * c, S->done + c));
+    std::vector<PackUpload> uploads;
+    build_ops(S, false, S->init_ops, uploads);
+    build_ops(S, true, S->iter_ops, uploads);
+    for (auto& u : uploads) {
+      void* d = A(u.bytes.size());
+      (*u.ops)[u.index].packs_dev = d;
+      if (ok) ok = upload(d, u.bytes.data(), u.bytes.size()) == 0;
+    }
+  }
+  if (!ok) {
+    destroy_set(S);
+    set_error("Poisson solve set: device allocation or upload failed");
+    return BEMCS_EHIP;
+  }
+  *out = S;
+  return BEMCS_OK;
+}
+
+// Captures what `enqueue` launches into an executable graph (on the set's
+// private capture stream: nothing runs).  Launch arguments are fixed per set
+// and iteration number, so a graph is captured once and replayed.
 template <typename F>
-static hipError_t capture(bemcs_poisson_s* h, F enqueue, hipGraphExec_t* out) {
+static hipError_t capture(SolveSet* S, F enqueue, hipGraphExec_t* out) {
   *out = nullptr;
   hipError_t e = hipSuccess;
-  if (!h->cap_stream) e = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking);
+  if (!S->cap_stream) e = hipStreamCreateWithFlags(&S->cap_stream, hipStreamNonBlocking);
   if (e != hipSuccess) return e;
-  e = hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal);
+  e = hipStreamBeginCapture(S->cap_stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  enqueue(h->cap_stream);
+  enqueue(S->cap_stream);
   hipGraph_t g = nullptr;
-  e = hipStreamEndCapture(h->cap_stream, &g);
+  e = hipStreamEndCapture(S->cap_stream, &g);
   if (e != hipSuccess) return e;
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   return e;
 }
 
-static int enqueue_init(bemcs_poisson_s* h, const CGArgs& g, double rtol, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_init, dim3(h->nparts), dim3(256), 0, s, g);
-  hipLaunchKernelGGL(k_cg_init_scalar, dim3(1), dim3(256), 0, s, g, rtol);
-  if (!h->use_graphs) {
-    enqueue_vcycle(h, g, 0, s);
+static int enqueue_init(SolveSet* S, double rtol, hipStream_t s) {
+  auto body = [&](hipStream_t cs) {
+    for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
+  };
+  if (!S->use_graphs) {
+    body(s);
     return BEMCS_OK;
   }
-  if (!h->init_graph)
-    BEMCS_CHECK_HIP(capture(h, [&](hipStream_t cs) { enqueue_vcycle(h, g, 0, cs); }, &h->init_graph));
-  BEMCS_CHECK_HIP(hipGraphLaunch(h->init_graph, s));
+  if (!S->init_graph || S->init_rtol != rtol) {
+    if (S->init_graph) (void)hipGraphExecDestroy(S->init_graph);
+    BEMCS_CHECK_HIP(capture(S, body, &S->init_graph));
+    S->init_rtol = rtol;
+  }
+  BEMCS_CHECK_HIP(hipGraphLaunch(S->init_graph, s));
   return BEMCS_OK;
 }
 
-static int enqueue_iter(bemcs_poisson_s* h, const CGArgs& g, int k, hipStream_t s) {
-  if (!h->use_graphs) {
-    hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, s, g, k);
-    enqueue_vcycle(h, g, k, s);
+static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
+  auto body = [&](hipStream_t cs) {
+    for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+  };
+  if (!S->use_graphs) {
+    body(s);
     return BEMCS_OK;
   }
-  if ((int)h->iter_graph.size() <= k) h->iter_graph.resize(k + 1, nullptr);
-  if (!h->iter_graph[k]) {
-    auto body = [&](hipStream_t cs) {
-      hipLaunchKernelGGL(k_cg_dir, dim3(h->nparts), dim3(256), 0, cs, g, k);
-      enqueue_vcycle(h, g, k, cs);
-    };
-    BEMCS_CHECK_HIP(capture(h, body, &h->iter_graph[k]));
-  }
-  BEMCS_CHECK_HIP(hipGraphLaunch(h->iter_graph[k], s));
+  if ((int)S->iter_graph.size() <= k) S->iter_graph.resize(k + 1, nullptr);
+  if (!S->iter_graph[k]) BEMCS_CHECK_HIP(capture(S, body, &S->iter_graph[k]));
+  BEMCS_CHECK_HIP(hipGraphLaunch(S->iter_graph[k], s));
   return BEMCS_OK;
 }
 
-// PCG on the reduced system already in h->bred with initial guess h->x.
-static int pcg(bemcs_poisson_s* h, double rtol, int max_iter, int* iters, double* relres,
-               hipStream_t s) {
-  const CGArgs g = cg_args(h);
-  int rc = enqueue_init(h, g, rtol, s);
+// PCG on the reduced systems already in each case's b_red with initial guess
+// x.  Iterations advance for all cases together; converged cases skip.
+static int pcg(SolveSet* S, double rtol, int max_iter, int* iters, double* relres, hipStream_t s) {
+  int rc = enqueue_init(S, rtol, s);
   if (rc) return rc;
   BEMCS_CHECK_LAUNCH();
-  // launch as many iterations as the previous solve needed, then check
   int launched = 0;
-  int batch = std::max(1, h->pred_iters);
+  int batch = std::max(1, S->pred_iters);
+  const int n = S->n;
   for (;;) {
     for (int j = 0; j < batch && launched < max_iter; ++j) {
-      rc = enqueue_iter(h, g, ++launched, s);
+      rc = enqueue_iter(S, ++launched, s);
       if (rc) return rc;
     }
     BEMCS_CHECK_LAUNCH();
-    BEMCS_CHECK_HIP(hipMemcpyAsync(h->host_scal, h->scal, sizeof(double) * S_COUNT,
+    BEMCS_CHECK_HIP(hipMemcpyAsync(S->host_scal, S->scal, sizeof(double) * S_COUNT * n,
                                    hipMemcpyDeviceToHost, s));
     BEMCS_CHECK_HIP(hipStreamSynchronize(s));
-    double rr = h->host_scal[S_RR], tol2 = h->host_scal[S_TOL2];
-    if (rr <= tol2 || !(rr == rr) || launched >= max_iter) break;
+    bool all = true;
+    for (int c = 0; c < n && all; ++c) {
+      const double* sc = S->host_scal + S_COUNT * c;
+      const double rr = sc[S_RR];
+      if (!(rr <= sc[S_TOL2]) && rr == rr) all = false;
+    }
+    if (all || launched >= max_iter) break;
     batch = 1;
   }
-  double rr = h->host_scal[S_RR], bb = h->host_scal[S_BB];
-  int it = (int)h->host_scal[S_ITERS];
-  h->pred_iters = it;
-  if (iters) *iters = it;
-  double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
-  if (relres) *relres = rel;
-  if (!(rr <= h->host_scal[S_TOL2])) {
-    set_error("Poisson MG-PCG: relative residual %.3e above tolerance %.3e after %d iterations",
-              rel, rtol, it);
+  int worst_it = 0, bad = -1;
+  double bad_rel = 0;
+  for (int c = 0; c < n; ++c) {
+    const double* sc = S->host_scal + S_COUNT * c;
+    const double rr = sc[S_RR], bb = sc[S_BB];
+    const int it = (int)sc[S_ITERS];
+    const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    worst_it = std::max(worst_it, it);
+    if (iters) iters[c] = it;
+    if (relres) relres[c] = rel;
+    if (!(rr <= sc[S_TOL2]) && bad < 0) {
+      bad = c;
+      bad_rel = rel;
+    }
+  }
+  S->pred_iters = worst_it;
+  if (bad >= 0) {
+    set_error("Poisson MG-PCG: case %d relative residual %.3e above tolerance %.3e after %d iterations",
+              bad, bad_rel, rtol, launched);
     return BEMCS_ENOCONV;
   }
   return BEMCS_OK;
+}
+
+static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p) {
+  PicardPack P;
+  P.cnt_ion = p->cnt_ion;
+  P.cnt_e = p->cnt_e;
+  P.cd_ion = p->cd_ion;
+  P.cd_e = p->cd_e;
+  P.rho = p->rho_out;
+  P.V_fixed = p->V_fixed;
+  P.V = p->V;
+  P.Vnew = h->Vnew;
+  P.kind = h->kind;
+  P.mirror = h->mirror;
+  P.bred = h->bred;
+  P.x = h->x;
+  P.nx = h->nx;
+  P.ny = h->ny;
+  P.per = h->periodic;
+  P.n = h->nx * h->ny;
+  P.coeff = p->coeff;
+  P.Vp = p->V_plasma;
+  P.Te = p->Te_up;
+  P.qn0 = -p->q * p->n0;
+  P.one_m_omega = 1.0 - p->omega;
+  P.omega = p->omega;
+  P.dx_m = p->dx_m;
+  P.dy_m = p->dy_m;
+  P.Exy = p->Exy;
+  return P;
+}
+
+template <int PHASE>
+static void launch_picard(SolveSet* S, unsigned gx, hipStream_t s) {
+  if (S->n > 1)
+    hipLaunchKernelGGL((k_picard<PHASE, true>), dim3(gx, 1, S->n), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev);
+  else
+    hipLaunchKernelGGL((k_picard<PHASE, false>), dim3(gx, 1, 1), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev);
+}
+
+// The Picard loop of every case of the set (:886-902), then E = -grad V.
+static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, double* worst_relres,
+                  hipStream_t s) {
+  const int n = S->n;
+  const int iterations = ps[0].iterations;
+  const double rtol = ps[0].rtol;
+  const int max_iter = ps[0].max_iter;
+  unsigned gx = 1;
+  for (int c = 0; c < n; ++c) {
+    const bemcs_picard_t* p = &ps[c];
+    BEMCS_REQUIRE(p->cnt_ion && p->cnt_e && p->V_fixed && p->V && p->rho_out && p->Exy,
+                  "bemcs_poisson_picard: bad arguments");
+    BEMCS_REQUIRE(p->iterations == iterations && p->rtol == rtol && p->max_iter == max_iter,
+                  "bemcs_poisson_batch_picard: iterations, rtol and max_iter must agree across cases");
+    S->pic_host[c] = picard_pack(S->cases[c], p);
+    gx = std::max(gx, grid_for(S->pic_host[c].n, 256));
+  }
+  if (n > 1)
+    BEMCS_CHECK_HIP(hipMemcpyAsync(S->pic_dev, S->pic_host, sizeof(PicardPack) * n,
+                                   hipMemcpyHostToDevice, s));
+  launch_picard<PIC_RHO>(S, gx, s);
+  for (int c = 0; c < n; ++c) {
+    bemcs_poisson_s* h = S->cases[c];
+    if (h->have_guess) continue;
+    hipLaunchKernelGGL(k_reduce_rhs, dim3(grid_for(h->nx * h->ny, 256)), dim3(256), 0, s,
+                       ps[c].V_fixed, ps[c].V, h->kind, h->bred, h->x, h->nx, h->ny, h->periodic, 1);
+    h->have_guess = true;
+  }
+  BEMCS_CHECK_LAUNCH();
+  std::vector<int> it(n, 0), total(n, 0);
+  std::vector<double> rel(n, 0.0), worst(n, 0.0);
+  int rc = BEMCS_OK;
+  for (int k = 0; k < iterations && rc == BEMCS_OK; ++k) {
+    launch_picard<PIC_RHS>(S, gx, s);
+    BEMCS_CHECK_LAUNCH();
+    rc = pcg(S, rtol, max_iter, it.data(), rel.data(), s);
+    for (int c = 0; c < n; ++c) {
+      total[c] += it[c];
+      worst[c] = std::max(worst[c], rel[c]);
+    }
+    if (rc) break;
+    launch_picard<PIC_RELAX>(S, gx, s);
+    BEMCS_CHECK_LAUNCH();
+  }
+  if (rc == BEMCS_OK) {
+    launch_picard<PIC_EFIELD>(S, gx, s);
+    BEMCS_CHECK_LAUNCH();
+  }
+  for (int c = 0; c < n; ++c) {
+    if (iters_total) iters_total[c] = total[c];
+    if (worst_relres) worst_relres[c] = worst[c];
+  }
+  return rc;
 }
 
 }  // namespace bemcs
@@ -1251,15 +1664,8 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
   h->ny = ny;
   h->periodic = per ? 1 : 0;
   h->nlev = (int)H.size();
-  h->host_scal = nullptr;
   h->have_guess = false;
-  h->pred_iters = 8;
-  const char* ps = getenv("BEMCS_MG_POST_SWEEPS");
-  h->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
-  const char* pg = getenv("BEMCS_POISSON_GRAPHS");
-  h->use_graphs = !(pg && pg[0] == '0');
-  h->cap_stream = nullptr;
-  h->init_graph = nullptr;
+  h->own = nullptr;
   bool ok = true;
   auto A = [&](size_t bytes) {
     void* p = dalloc(h, bytes);
@@ -1326,15 +1732,12 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     h->pi = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pz[0] = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pz[1] = (double*)A(sizeof(double) * 2 * h->nparts);
-    h->scal = (double*)A(sizeof(double) * S_COUNT);
-    h->done = (int32_t*)A(sizeof(int32_t) * 4);
-    if (ok) ok = hipMemset(h->done, 0, sizeof(int32_t) * 4) == hipSuccess;
+
     if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->x, 0, sizeof(double) * n) == hipSuccess;
-    if (ok) ok = hipMemset(h->scal, 0, sizeof(double) * S_COUNT) == hipSuccess;
-    if (ok) ok = hipHostMalloc((void**)&h->host_scal, sizeof(double) * S_COUNT, 0) == hipSuccess;
   }
+  if (ok) ok = create_set(&h, 1, &h->own) == BEMCS_OK;
   if (!ok) {
     destroy(h);
     set_error("bemcs_poisson_create: device allocation or upload failed");
@@ -1361,7 +1764,7 @@ int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rt
   BEMCS_CHECK_LAUNCH();
   int it = 0;
   double rel = 0;
-  int rc = pcg(h, rtol, max_iter, &it, &rel, s);
+  int rc = pcg(h->own, rtol, max_iter, &it, &rel, s);
   if (iters) *iters = it;
   if (relres) *relres = rel;
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(n, 256)), dim3(256), 0, s, h->x, b, h->kind,
@@ -1372,47 +1775,32 @@ int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rt
 
 int bemcs_poisson_picard(bemcs_poisson_t h, const bemcs_picard_t* p, int32_t* iters_total,
                          double* worst_relres, bemcs_stream_t stream) {
-  BEMCS_REQUIRE(h && p && p->cnt_ion && p->cnt_e && p->V_fixed && p->V && p->rho_out && p->Exy,
-                "bemcs_poisson_picard: bad arguments");
-  hipStream_t s = as_stream(stream);
-  const int n = h->nx * h->ny;
-  const unsigned g = grid_for(n, 256);
-  hipLaunchKernelGGL(k_rho_from_counts, dim3(g), dim3(256), 0, s, p->cnt_ion, p->cnt_e, n,
-                     p->cd_ion, p->cd_e, p->rho_out);
-  if (!h->have_guess) {
-    hipLaunchKernelGGL(k_reduce_rhs, dim3(g), dim3(256), 0, s, p->V_fixed, p->V, h->kind, h->bred,
-                       h->x, h->nx, h->ny, h->periodic, 1);
-    h->have_guess = true;
+  BEMCS_REQUIRE(h && p, "bemcs_poisson_picard: bad arguments");
+  return picard(h->own, p, iters_total, worst_relres, as_stream(stream));
+}
+
+int bemcs_poisson_batch_create(const bemcs_poisson_t* handles, int32_t n, bemcs_poisson_batch_t* out) {
+  BEMCS_REQUIRE(handles && out && n >= 1, "bemcs_poisson_batch_create: bad arguments");
+  *out = nullptr;
+  for (int c = 0; c < n; ++c) {
+    BEMCS_REQUIRE(handles[c], "bemcs_poisson_batch_create: null handle");
+    for (int d = 0; d < c; ++d)
+      BEMCS_REQUIRE(handles[c] != handles[d], "bemcs_poisson_batch_create: a handle appears twice");
   }
-  BEMCS_CHECK_LAUNCH();
-  int total = 0;
-  double worst = 0;
-  const double qn0 = -p->q * p->n0;
-  for (int k = 0; k < p->iterations; ++k) {
-    hipLaunchKernelGGL(k_picard_rhs, dim3(g), dim3(256), 0, s, p->rho_out, p->V, p->V_fixed,
-                       h->kind, h->bred, h->nx, h->ny, h->periodic, p->coeff, p->V_plasma,
-                       p->Te_up, qn0);
-    BEMCS_CHECK_LAUNCH();
-    int it = 0;
-    double rel = 0;
-    int rc = pcg(h, p->rtol, p->max_iter, &it, &rel, s);
-    total += it;
-    worst = rel > worst ? rel : worst;
-    if (rc) {
-      if (iters_total) *iters_total = total;
-      if (worst_relres) *worst_relres = worst;
-      return rc;
-    }
-    hipLaunchKernelGGL(k_picard_relax, dim3(g), dim3(256), 0, s, h->x, p->V_fixed, h->kind,
-                       h->mirror, p->V, h->Vnew, n, 1.0 - p->omega, p->omega);
-    BEMCS_CHECK_LAUNCH();
-  }
-  hipLaunchKernelGGL(k_efield, dim3(g), dim3(256), 0, s, p->V, h->nx, h->ny, p->dx_m, p->dy_m,
-                     h->periodic, p->Exy);
-  BEMCS_CHECK_LAUNCH();
-  if (iters_total) *iters_total = total;
-  if (worst_relres) *worst_relres = worst;
+  return create_set(handles, n, out);
+}
+
+int bemcs_poisson_batch_free(bemcs_poisson_batch_t b) {
+  destroy_set(b);
   return BEMCS_OK;
+}
+
+int bemcs_poisson_batch_size(bemcs_poisson_batch_t b) { return b ? b->n : 0; }
+
+int bemcs_poisson_batch_picard(bemcs_poisson_batch_t b, const bemcs_picard_t* params,
+                               int32_t* iters_total, double* worst_relres, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(b && params, "bemcs_poisson_batch_picard: bad arguments");
+  return picard(b, params, iters_total, worst_relres, as_stream(stream));
 }
 
 int bemcs_efield(const double* V, int32_t nx, int32_t ny, double dx_m, double dy_m,
