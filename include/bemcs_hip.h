@@ -347,6 +347,92 @@ typedef struct {
 int bemcs_step_begin(const bemcs_step_begin_t* p, bemcs_stream_t stream);
 int bemcs_step_end(const bemcs_step_end_t* p, bemcs_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Batched step kernels: one launch serves n independent cases (meshes and
+ * particle counts may differ); launch index z selects the case.  `cases` is a
+ * device array of n entries (caller-owned, e.g. one upload per step of all
+ * the step's argument structs).  Each case's results are bit-identical to the
+ * single-case call with the same arguments.  No reference counterpart: the
+ * reference runs sweep cases in separate processes
+ * (Python/benchmarks/benchmark_perveance.py:102-113).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  bemcs_species_t in, out, e_out;
+  bemcs_ion_step_t p;
+} bemcs_ion_case_t;
+
+typedef struct {
+  bemcs_species_t in, out;
+  bemcs_electron_step_t p;
+} bemcs_electron_case_t;
+
+typedef struct {
+  const float* x;
+  const float* y;
+  const float* vx;
+  const float* vy;
+  const float* vz;
+  const int32_t* n_dev;
+  const float* Exy;
+  int32_t nx, ny;
+  double dx, dy, q_m, dt, dt2, frac;
+  uint64_t* ds_max_bits;
+  int32_t* n_sub_out;
+} bemcs_substeps_case_t;
+
+typedef struct {
+  bemcs_species_t sp;
+  bemcs_inject_t p;
+  int64_t n;               /* particles to inject (0 allowed) */
+  int64_t n_bound_before;  /* host bound of the count before injection */
+} bemcs_inject_case_t;
+
+typedef struct {
+  bemcs_species_t in, out;
+  int64_t n_bound;
+  int32_t nx, ny;
+  double dx, dy;
+  uint32_t* scratch;       /* as bemcs_sort_by_cell */
+} bemcs_sort_case_t;
+
+typedef struct {
+  const uint32_t* ppc;
+  int64_t n_nodes;
+  double* accum;
+  uint32_t* stats;
+} bemcs_ppc_case_t;
+
+typedef struct {
+  const float* angles;
+  const uint32_t* n_dev;
+  double q_percent;
+  uint32_t min_count;
+  uint32_t pad_;
+  double* out;
+} bemcs_percentile_case_t;
+
+/* divisors_host: host array [2n] of float32 (dx, dy) per case, for the
+ * division check of bemcs_fastdiv_enabled; max_bound: largest host count
+ * bound over the cases (sets the launch size). */
+int bemcs_ion_step_batched(const bemcs_ion_case_t* cases, int32_t n, int64_t max_bound,
+                           const float* divisors_host, bemcs_stream_t stream);
+int bemcs_electron_step_batched(const bemcs_electron_case_t* cases, int32_t n, int64_t max_bound,
+                                const float* divisors_host, bemcs_stream_t stream);
+int bemcs_substeps_batched(const bemcs_substeps_case_t* cases, int32_t n, int64_t max_bound,
+                           bemcs_stream_t stream);
+/* max_n: largest injected count over the cases */
+int bemcs_inject_batched(const bemcs_inject_case_t* cases, int32_t n, int64_t max_n,
+                         bemcs_stream_t stream);
+/* max_nodes: largest nx*ny over the cases */
+int bemcs_sort_by_cell_batched(const bemcs_sort_case_t* cases, int32_t n, int64_t max_bound,
+                               int64_t max_nodes, bemcs_stream_t stream);
+int bemcs_step_begin_batched(const bemcs_step_begin_t* cases, int32_t n, int64_t max_nodes,
+                             bemcs_stream_t stream);
+int bemcs_step_end_batched(const bemcs_step_end_t* cases, int32_t n, bemcs_stream_t stream);
+int bemcs_ppc_stats_batched(const bemcs_ppc_case_t* cases, int32_t n, bemcs_stream_t stream);
+int bemcs_percentile_batched(const bemcs_percentile_case_t* cases, int32_t n,
+                             bemcs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,10 @@ _STRUCTS = {
     "bemcs_species_t": "Species", "bemcs_ion_diag_t": "IonDiag", "bemcs_ion_step_t": "IonStep",
     "bemcs_electron_step_t": "ElectronStep", "bemcs_inject_t": "Inject", "bemcs_picard_t": "Picard",
     "bemcs_step_begin_t": "StepBegin", "bemcs_step_end_t": "StepEnd",
+    "bemcs_ion_case_t": "IonCase", "bemcs_electron_case_t": "ElectronCase",
+    "bemcs_substeps_case_t": "SubstepsCase", "bemcs_inject_case_t": "InjectCase",
+    "bemcs_sort_case_t": "SortCase", "bemcs_ppc_case_t": "PpcCase",
+    "bemcs_percentile_case_t": "PercentileCase",
 }
 
 

@@ -79,6 +79,38 @@ class StepEnd(C.Structure):
                 ("V", vp), ("saddle_idx", vp), ("n_saddle", i32), ("pad_", i32), ("minpot_out", vp)]
 
 
+class IonCase(C.Structure):
+    _fields_ = [("in", Species), ("out", Species), ("e_out", Species), ("p", IonStep)]
+
+
+class ElectronCase(C.Structure):
+    _fields_ = [("in", Species), ("out", Species), ("p", ElectronStep)]
+
+
+class SubstepsCase(C.Structure):
+    _fields_ = [("x", vp), ("y", vp), ("vx", vp), ("vy", vp), ("vz", vp), ("n_dev", vp), ("Exy", vp),
+                ("nx", i32), ("ny", i32), ("dx", f64), ("dy", f64), ("q_m", f64), ("dt", f64),
+                ("dt2", f64), ("frac", f64), ("ds_max_bits", vp), ("n_sub_out", vp)]
+
+
+class InjectCase(C.Structure):
+    _fields_ = [("sp", Species), ("p", Inject), ("n", i64), ("n_bound_before", i64)]
+
+
+class SortCase(C.Structure):
+    _fields_ = [("in", Species), ("out", Species), ("n_bound", i64), ("nx", i32), ("ny", i32),
+                ("dx", f64), ("dy", f64), ("scratch", vp)]
+
+
+class PpcCase(C.Structure):
+    _fields_ = [("ppc", vp), ("n_nodes", i64), ("accum", vp), ("stats", vp)]
+
+
+class PercentileCase(C.Structure):
+    _fields_ = [("angles", vp), ("n_dev", vp), ("q_percent", f64), ("min_count", u32), ("pad_", u32),
+                ("out", vp)]
+
+
 _SIGS = {
     "bemcs_abi_version": (i32, []),
     "bemcs_last_error": (C.c_char_p, []),
@@ -112,6 +144,15 @@ _SIGS = {
     "bemcs_ppc_stats": (i32, [vp, i64, vp, vp, vp]),
     "bemcs_step_begin": (i32, [vp, vp]),
     "bemcs_step_end": (i32, [vp, vp]),
+    "bemcs_ion_step_batched": (i32, [vp, i32, i64, vp, vp]),
+    "bemcs_electron_step_batched": (i32, [vp, i32, i64, vp, vp]),
+    "bemcs_substeps_batched": (i32, [vp, i32, i64, vp]),
+    "bemcs_inject_batched": (i32, [vp, i32, i64, vp]),
+    "bemcs_sort_by_cell_batched": (i32, [vp, i32, i64, i64, vp]),
+    "bemcs_step_begin_batched": (i32, [vp, i32, i64, vp]),
+    "bemcs_step_end_batched": (i32, [vp, i32, vp]),
+    "bemcs_ppc_stats_batched": (i32, [vp, i32, vp]),
+    "bemcs_percentile_batched": (i32, [vp, i32, vp]),
 }
 
 EXPORTED = tuple(_SIGS)

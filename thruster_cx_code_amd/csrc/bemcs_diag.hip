@@ -86,10 +86,16 @@ __device__ uint32_t select_kth(const uint32_t* __restrict__ keys, uint32_t n, ui
 // np.percentile of float32 data evaluates its index and the interpolation in
 // float32: q = f32(percent)/f32(100), vi = f32(n-1)*q, gamma = vi - floor(vi),
 // lerp a + (b-a)*gamma, or b - (b-a)*(1-gamma) when gamma >= 0.5.
-__global__ __launch_bounds__(1024) void k_percentile(const float* __restrict__ a,
-                                                     const uint32_t* n_dev, float q_percent,
-                                                     uint32_t min_count, double* out) {
+template <bool B>
+__global__ __launch_bounds__(1024) void k_percentile(bemcs_percentile_case_t one,
+                                                     const bemcs_percentile_case_t* __restrict__ many) {
   __shared__ uint32_t hist[2048];
+  const bemcs_percentile_case_t c = B ? many[blockIdx.z] : one;
+  const float* __restrict__ a = c.angles;
+  const uint32_t* n_dev = c.n_dev;
+  const float q_percent = (float)c.q_percent;
+  const uint32_t min_count = c.min_count;
+  double* out = c.out;
   const uint32_t n = *n_dev;
   if (n <= min_count || n == 0) {
     if (threadIdx.x == 0) *out = __builtin_nan("");
@@ -116,9 +122,15 @@ __global__ __launch_bounds__(1024) void k_percentile(const float* __restrict__ a
 // running float64 PPC sum.  Grid-stride over at most kPpcBlocks blocks with
 // one pair of counter atomics per block.
 constexpr int kPpcBlocks = 256;
-__global__ __launch_bounds__(1024) void k_ppc_stats(const uint32_t* __restrict__ ppc, int64_t n,
-                                                    double* accum, uint32_t* stats) {
+template <bool B>
+__global__ __launch_bounds__(1024) void k_ppc_stats(bemcs_ppc_case_t one,
+                                                    const bemcs_ppc_case_t* __restrict__ many) {
   __shared__ uint32_t s_occ[16], s_low[16];
+  const bemcs_ppc_case_t c = B ? many[blockIdx.z] : one;
+  const uint32_t* __restrict__ ppc = c.ppc;
+  const int64_t n = c.n_nodes;
+  double* accum = c.accum;
+  uint32_t* stats = c.stats;
   uint32_t occ = 0, low = 0;
   for (int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 1024) {
     const uint32_t c = ppc[i];
@@ -148,7 +160,10 @@ __global__ __launch_bounds__(1024) void k_ppc_stats(const uint32_t* __restrict__
 }
 
 // --------------------------------------------------- per-step bookkeeping
-__global__ __launch_bounds__(256) void k_step_begin(bemcs_step_begin_t p) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_step_begin(bemcs_step_begin_t one,
+                                                    const bemcs_step_begin_t* __restrict__ many) {
+  const bemcs_step_begin_t p = B ? many[blockIdx.z] : one;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < p.n_nodes) {
 #pragma unroll
@@ -168,7 +183,10 @@ __global__ __launch_bounds__(256) void k_step_begin(bemcs_step_begin_t p) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_step_end(bemcs_step_end_t p) {
+template <bool B>
+__global__ __launch_bounds__(64) void k_step_end(bemcs_step_end_t one,
+                                                 const bemcs_step_end_t* __restrict__ many) {
+  const bemcs_step_end_t p = B ? many[blockIdx.z] : one;
   double m = __builtin_inf();
   for (int k = threadIdx.x; k < p.n_saddle; k += 64) m = fmin(m, p.V[p.saddle_idx[k]]);
 #pragma unroll
@@ -198,8 +216,9 @@ int bemcs_percentile(const float* angles, const uint32_t* n_dev, int64_t n_bound
   BEMCS_REQUIRE(q_percent >= 0.0 && q_percent <= 100.0, "bemcs_percentile: q must be in [0, 100]");
   (void)scratch;
   (void)n_bound;
-  hipLaunchKernelGGL(k_percentile, dim3(1), dim3(1024), 0, as_stream(stream), angles, n_dev,
-                     (float)q_percent, min_count, out);
+  const bemcs_percentile_case_t c{angles, n_dev, q_percent, min_count, 0u, out};
+  hipLaunchKernelGGL(k_percentile<false>, dim3(1), dim3(1024), 0, as_stream(stream), c,
+                     (const bemcs_percentile_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -209,8 +228,9 @@ int bemcs_ppc_stats(const uint32_t* ppc, int64_t n_nodes, double* accum, uint32_
   BEMCS_REQUIRE(ppc && stats && n_nodes > 0, "bemcs_ppc_stats: bad arguments");
   unsigned g = grid_for(n_nodes, 1024);
   if (g > (unsigned)kPpcBlocks) g = kPpcBlocks;
-  hipLaunchKernelGGL(k_ppc_stats, dim3(g), dim3(1024), 0, as_stream(stream), ppc, n_nodes, accum,
-                     stats);
+  const bemcs_ppc_case_t c{ppc, n_nodes, accum, stats};
+  hipLaunchKernelGGL(k_ppc_stats<false>, dim3(g), dim3(1024), 0, as_stream(stream), c,
+                     (const bemcs_ppc_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -221,7 +241,8 @@ int bemcs_step_begin(const bemcs_step_begin_t* p, bemcs_stream_t stream) {
                     p->n_nodes >= 0 && p->diag_bytes >= 0,
                 "bemcs_step_begin: bad arguments");
   int64_t n = p->n_nodes > p->diag_bytes ? p->n_nodes : p->diag_bytes;
-  hipLaunchKernelGGL(k_step_begin, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *p);
+  hipLaunchKernelGGL(k_step_begin<false>, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *p,
+                     (const bemcs_step_begin_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -230,7 +251,45 @@ int bemcs_step_end(const bemcs_step_end_t* p, bemcs_stream_t stream) {
   BEMCS_REQUIRE(p && p->count_i && p->count_e && p->n_sub && p->counts_out && p->nsub_out &&
                     p->minpot_out && p->n_saddle >= 0 && (p->n_saddle == 0 || (p->V && p->saddle_idx)),
                 "bemcs_step_end: bad arguments");
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, as_stream(stream), *p);
+  hipLaunchKernelGGL(k_step_end<false>, dim3(1), dim3(64), 0, as_stream(stream), *p,
+                     (const bemcs_step_end_t*)nullptr);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_step_begin_batched(const bemcs_step_begin_t* cases, int32_t n, int64_t max_nodes,
+                             bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1 && max_nodes >= 0, "bemcs_step_begin_batched: bad arguments");
+  // the clears cover the larger of the node arrays and the diagnostics record
+  const int64_t m = max_nodes > (int64_t)sizeof(bemcs_ion_diag_t) ? max_nodes
+                                                                 : (int64_t)sizeof(bemcs_ion_diag_t);
+  hipLaunchKernelGGL(k_step_begin<true>, dim3(grid_for(m, 256), 1, n), dim3(256), 0, as_stream(stream),
+                     bemcs_step_begin_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_step_end_batched(const bemcs_step_end_t* cases, int32_t n, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_step_end_batched: bad arguments");
+  hipLaunchKernelGGL(k_step_end<true>, dim3(1, 1, n), dim3(64), 0, as_stream(stream),
+                     bemcs_step_end_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_ppc_stats_batched(const bemcs_ppc_case_t* cases, int32_t n, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_ppc_stats_batched: bad arguments");
+  hipLaunchKernelGGL(k_ppc_stats<true>, dim3(kPpcBlocks / 4, 1, n), dim3(1024), 0, as_stream(stream),
+                     bemcs_ppc_case_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_percentile_batched(const bemcs_percentile_case_t* cases, int32_t n,
+                             bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_percentile_batched: bad arguments");
+  hipLaunchKernelGGL(k_percentile<true>, dim3(1, 1, n), dim3(1024), 0, as_stream(stream),
+                     bemcs_percentile_case_t{}, cases);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

@@ -158,12 +158,22 @@ __global__ __launch_bounds__(kBlock) void k_counts_to_rho(const uint32_t* __rest
 // ds = |v| dt + 0.5 |qm E| dt^2 with the reference's mixed float32/float64
 // evaluation (:940-962); max over particles folded into *ds_bits (ds >= 0,
 // so the float64 bit pattern orders like the value).
-__global__ __launch_bounds__(kBlock) void k_substep_ds(
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ vx,
-    const float* __restrict__ vy, const float* __restrict__ vz, const int32_t* n_dev,
-    const float* __restrict__ Exy, int nx, int ny, float dx32, float dy32, double qm, float dt32,
-    double dt2, unsigned long long* ds_bits) {
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_substep_ds(bemcs_substeps_case_t one,
+                                                        const bemcs_substeps_case_t* __restrict__ many) {
   __shared__ unsigned long long smax[kBlock / 64];
+  const bemcs_substeps_case_t a = B ? many[blockIdx.z] : one;
+  const float* __restrict__ x = a.x;
+  const float* __restrict__ y = a.y;
+  const float* __restrict__ vx = a.vx;
+  const float* __restrict__ vy = a.vy;
+  const float* __restrict__ vz = a.vz;
+  const int32_t* n_dev = a.n_dev;
+  const float* __restrict__ Exy = a.Exy;
+  const int nx = a.nx, ny = a.ny;
+  const float dx32 = (float)a.dx, dy32 = (float)a.dy, dt32 = (float)a.dt;
+  const double qm = a.q_m, dt2 = a.dt2;
+  unsigned long long* ds_bits = (unsigned long long*)a.ds_max_bits;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *n_dev;
   const int64_t lim = (int64_t)gridDim.x * kBlock;
@@ -205,8 +215,14 @@ __global__ __launch_bounds__(kBlock) void k_substep_ds(
   }
 }
 
-__global__ void k_substep_finalize(const unsigned long long* ds_bits, const int32_t* n_dev,
-                                   double ds_lim, int32_t* n_sub) {
+template <bool B>
+__global__ void k_substep_finalize(bemcs_substeps_case_t one,
+                                   const bemcs_substeps_case_t* __restrict__ many) {
+  const bemcs_substeps_case_t a = B ? many[blockIdx.z] : one;
+  const unsigned long long* ds_bits = (const unsigned long long*)a.ds_max_bits;
+  const int32_t* n_dev = a.n_dev;
+  const double ds_lim = a.frac * (a.dx < a.dy ? a.dx : a.dy) * 1e-3;
+  int32_t* n_sub = a.n_sub_out;
   double ds_max = __longlong_as_double((long long)*ds_bits);
   int ns = 1;
   if (*n_dev > 0) {
@@ -218,10 +234,16 @@ __global__ void k_substep_finalize(const unsigned long long* ds_bits, const int3
 }
 
 // ---------------------------------------------------------- fused ion step
-template <bool PERIODIC, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_species_t out,
-                                                      bemcs_species_t eout, bemcs_ion_step_t p,
-                                                      Divisor ddx, Divisor ddy) {
+template <bool FAST, bool B>
+__global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
+                                                      const bemcs_ion_case_t* __restrict__ many) {
+  const bemcs_ion_case_t a = B ? many[blockIdx.z] : one;
+  const bemcs_species_t& in = a.in;
+  const bemcs_species_t& out = a.out;
+  const bemcs_species_t& eout = a.e_out;
+  const bemcs_ion_step_t& p = a.p;
+  const bool PERIODIC = p.periodic_y != 0;
+  const Divisor ddx{p.dx32, 1.0f / p.dx32}, ddy{p.dy32, 1.0f / p.dy32};
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t cnt = *in.count;
   const int64_t n = cnt < in.cap ? cnt : in.cap;
@@ -457,10 +479,15 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_species_t in, bemcs_s
 }
 
 // ------------------------------------------------------ fused electron step
-template <bool PERIODIC, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, bemcs_species_t out,
-                                                           bemcs_electron_step_t p, Divisor ddx,
-                                                           Divisor ddy) {
+template <bool FAST, bool B>
+__global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t one,
+                                                           const bemcs_electron_case_t* __restrict__ many) {
+  const bemcs_electron_case_t a = B ? many[blockIdx.z] : one;
+  const bemcs_species_t& in = a.in;
+  const bemcs_species_t& out = a.out;
+  const bemcs_electron_step_t& p = a.p;
+  const bool PERIODIC = p.periodic_y != 0;
+  const Divisor ddx{p.dx32, 1.0f / p.dx32}, ddy{p.dy32, 1.0f / p.dy32};
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t cnt = *in.count;
   const int64_t n = cnt < in.cap ? cnt : in.cap;
@@ -515,8 +542,13 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_species_t in, be
 }
 
 // --------------------------------------------------------------- injection
-__global__ __launch_bounds__(kBlock) void k_inject(bemcs_species_t sp, bemcs_inject_t p, int64_t n,
-                                                    int64_t base_max) {
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_inject(bemcs_inject_case_t one,
+                                                    const bemcs_inject_case_t* __restrict__ many) {
+  const bemcs_inject_case_t a = B ? many[blockIdx.z] : one;
+  const bemcs_species_t& sp = a.sp;
+  const bemcs_inject_t& p = a.p;
+  const int64_t n = a.n;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t base = *sp.count;
   int key = -1;
@@ -585,7 +617,12 @@ __global__ __launch_bounds__(kBlock) void k_inject(bemcs_species_t sp, bemcs_inj
   if (p.cnt) wave_hist_add(p.cnt, key);
 }
 
-__global__ void k_add_count(int32_t* count, int64_t n, int64_t cap, uint32_t* overflow) {
+template <bool B>
+__global__ void k_add_count(bemcs_inject_case_t one, const bemcs_inject_case_t* __restrict__ many) {
+  const bemcs_inject_case_t a = B ? many[blockIdx.z] : one;
+  int32_t* count = a.sp.count;
+  const int64_t n = a.n, cap = a.sp.cap;
+  uint32_t* overflow = nullptr;
   int64_t c = (int64_t)*count + n;
   if (c > cap) {
     if (overflow) atomicOr(overflow, 8u);
@@ -595,27 +632,63 @@ __global__ void k_add_count(int32_t* count, int64_t n, int64_t cap, uint32_t* ov
 }
 
 // --------------------------------------------------------------- cell sort
-__global__ __launch_bounds__(kBlock) void k_cell_hist(const float* __restrict__ x,
-                                                       const float* __restrict__ y,
-                                                       const int32_t* n_dev, int64_t n_bound,
-                                                       float dx32, float dy32, int nx, int ny,
-                                                       uint32_t* hist, uint32_t* keys) {
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  int key = -1;
-  int64_t n = *n_dev;
-  if (n > n_bound) n = n_bound;
-  if (i < n) {
-    int i0 = floor_i32_np(x[i] / dx32, nx - 2);
-    int j0 = floor_i32_np(y[i] / dy32, ny - 2);
-    key = j0 * nx + i0;
-    keys[i] = (uint32_t)key;
-  }
-  wave_hist_add(hist, key);
+// Counting sort by gather cell.  Scratch layout per case: hist [nx*ny],
+// block sums [1024 + nx*ny/1024], keys [n_bound].
+struct SortView {
+  uint32_t *hist, *sums, *keys;
+  int64_t ncell;
+};
+__device__ __forceinline__ SortView sort_view(const bemcs_sort_case_t& a) {
+  SortView v;
+  v.ncell = (int64_t)a.nx * a.ny;
+  v.hist = a.scratch;
+  v.sums = a.scratch + v.ncell;
+  v.keys = v.sums + 1024 + ((v.ncell + 1023) / 1024);
+  return v;
 }
 
-// exclusive scan in place over n entries: per-block scan of 1024 entries
-__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* data, int64_t n, uint32_t* sums) {
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_sort_prepare(bemcs_sort_case_t one,
+                                                          const bemcs_sort_case_t* __restrict__ many) {
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView v = sort_view(a);
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < v.ncell) v.hist[i] = 0u;
+  if (i == 0) {
+    const int64_t n = *a.in.count, nb = a.n_bound > 0 ? a.n_bound : 0;
+    *a.out.count = (int32_t)(n < nb ? n : nb);
+  }
+}
+
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_cell_hist(bemcs_sort_case_t one,
+                                                       const bemcs_sort_case_t* __restrict__ many) {
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView v = sort_view(a);
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int key = -1;
+  int64_t n = *a.in.count;
+  if (n > a.n_bound) n = a.n_bound;
+  if (i < n) {
+    int i0 = floor_i32_np(a.in.x[i] / (float)a.dx, a.nx - 2);
+    int j0 = floor_i32_np(a.in.y[i] / (float)a.dy, a.ny - 2);
+    key = j0 * a.nx + i0;
+    v.keys[i] = (uint32_t)key;
+  }
+  wave_hist_add(v.hist, key);
+}
+
+// exclusive scan of the histogram in place: per-block scan of 1024 entries,
+// scan of the block sums, add
+template <bool B>
+__global__ __launch_bounds__(1024) void k_scan_blocks(bemcs_sort_case_t one,
+                                                       const bemcs_sort_case_t* __restrict__ many) {
   __shared__ uint32_t s[1024];
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView sv = sort_view(a);
+  const int64_t n = sv.ncell;
+  if ((int64_t)blockIdx.x * 1024 >= n) return;
+  uint32_t* data = sv.hist;
   int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
   uint32_t v = i < n ? data[i] : 0u;
   s[threadIdx.x] = v;
@@ -627,11 +700,17 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* data, int64_t n,
     __syncthreads();
   }
   if (i < n) data[i] = s[threadIdx.x] - v;
-  if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+  if (threadIdx.x == 1023) sv.sums[blockIdx.x] = s[1023];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* sums, int nb) {
+template <bool B>
+__global__ __launch_bounds__(1024) void k_scan_sums(bemcs_sort_case_t one,
+                                                     const bemcs_sort_case_t* __restrict__ many) {
   __shared__ uint32_t s[1024];
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView sv = sort_view(a);
+  uint32_t* sums = sv.sums;
+  const int nb = (int)((sv.ncell + 1023) / 1024);
   uint32_t carry = 0;
   for (int base = 0; base < nb; base += 1024) {
     int i = base + (int)threadIdx.x;
@@ -651,19 +730,27 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* sums, int nb) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_add(uint32_t* data, int64_t n, const uint32_t* sums) {
+template <bool B>
+__global__ __launch_bounds__(1024) void k_scan_add(bemcs_sort_case_t one,
+                                                    const bemcs_sort_case_t* __restrict__ many) {
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView sv = sort_view(a);
   int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-  if (i < n) data[i] += sums[blockIdx.x];
+  if (i < sv.ncell) sv.hist[i] += sv.sums[blockIdx.x];
 }
 
-__global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_species_t in, bemcs_species_t out,
-                                                          const uint32_t* __restrict__ keys,
-                                                          uint32_t* offs, int64_t n_bound) {
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_sort_case_t one,
+                                                          const bemcs_sort_case_t* __restrict__ many) {
+  const bemcs_sort_case_t a = B ? many[blockIdx.z] : one;
+  const SortView sv = sort_view(a);
+  const bemcs_species_t& in = a.in;
+  const bemcs_species_t& out = a.out;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *in.count;
-  if (n > n_bound) n = n_bound;
+  if (n > a.n_bound) n = a.n_bound;
   if (i >= n) return;
-  uint32_t d = atomicAdd(&offs[keys[i]], 1u);
+  uint32_t d = atomicAdd(&sv.hist[sv.keys[i]], 1u);
   out.x[d] = in.x[i];
   out.y[d] = in.y[i];
   out.vx[d] = in.vx[i];
@@ -672,19 +759,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_species_t in, bem
   if (in.tag) out.tag[d] = in.tag[i];
 }
 
-__global__ void k_copy_count(const int32_t* src, int32_t* dst, int64_t n_bound) {
-  int64_t n = *src;
-  *dst = (int32_t)(n < n_bound ? n : n_bound);
-}
-
-int scan_exclusive(uint32_t* data, int64_t n, uint32_t* sums, hipStream_t s) {
-  int nb = (int)((n + 1023) / 1024);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, s, data, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, data, n, sums);
-  BEMCS_CHECK_LAUNCH();
-  return BEMCS_OK;
-}
 
 // ------------------------------------------------- division certification
 // Counts the float32 dividends a (all 2^32 bit patterns) for which
@@ -806,13 +880,13 @@ int bemcs_substeps(const float* x, const float* y, const float* vx, const float*
   BEMCS_REQUIRE(n_dev && Exy && ds_max_bits && n_sub_out && nx >= 2 && ny >= 2,
                 "bemcs_substeps: bad arguments");
   hipStream_t s = as_stream(stream);
+  const bemcs_substeps_case_t a{x, y, vx, vy, vz, n_dev, Exy, nx, ny, dx, dy, q_m, dt, dt2, frac,
+                                ds_max_bits, n_sub_out};
   if (n_bound > 0)
-    hipLaunchKernelGGL(k_substep_ds, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, x, y, vx,
-                       vy, vz, n_dev, Exy, nx, ny, (float)dx, (float)dy, q_m, (float)dt, dt2,
-                       (unsigned long long*)ds_max_bits);
-  double ds_lim = frac * (dx < dy ? dx : dy) * 1e-3;
-  hipLaunchKernelGGL(k_substep_finalize, dim3(1), dim3(1), 0, s,
-                     (const unsigned long long*)ds_max_bits, n_dev, ds_lim, n_sub_out);
+    hipLaunchKernelGGL(k_substep_ds<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
+                       (const bemcs_substeps_case_t*)nullptr);
+  hipLaunchKernelGGL(k_substep_finalize<false>, dim3(1), dim3(1), 0, s, a,
+                     (const bemcs_substeps_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -845,12 +919,11 @@ int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(n_bound <= in->cap && out->cap >= n_bound, "bemcs_ion_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
-  const Divisor ddx = make_divisor(p->dx32), ddy = make_divisor(p->dy32);
-  const bool fast = fastdiv_ok(ddx.b, s) && fastdiv_ok(ddy.b, s);
-  auto kern = p->periodic_y ? (fast ? k_ion_step<true, true> : k_ion_step<true, false>)
-                            : (fast ? k_ion_step<false, true> : k_ion_step<false, false>);
-  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out, *e_out,
-                     *p, ddx, ddy);
+  const bool fast = fastdiv_ok(p->dx32, s) && fastdiv_ok(p->dy32, s);
+  const bemcs_ion_case_t a{*in, *out, *e_out, *p};
+  auto kern = fast ? k_ion_step<true, false> : k_ion_step<false, false>;
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
+                     (const bemcs_ion_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -864,12 +937,11 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(n_bound <= in->cap, "bemcs_electron_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
-  const Divisor ddx = make_divisor(p->dx32), ddy = make_divisor(p->dy32);
-  const bool fast = fastdiv_ok(ddx.b, s) && fastdiv_ok(ddy.b, s);
-  auto kern = p->periodic_y ? (fast ? k_electron_step<true, true> : k_electron_step<true, false>)
-                            : (fast ? k_electron_step<false, true> : k_electron_step<false, false>);
-  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out, *p, ddx,
-                     ddy);
+  const bool fast = fastdiv_ok(p->dx32, s) && fastdiv_ok(p->dy32, s);
+  const bemcs_electron_case_t a{*in, *out, *p};
+  auto kern = fast ? k_electron_step<true, false> : k_electron_step<false, false>;
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
+                     (const bemcs_electron_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -883,10 +955,11 @@ int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
   BEMCS_REQUIRE(n_bound_before + n <= sp->cap, "bemcs_inject: capacity exceeded");
   if (n <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(k_inject, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, *sp, *p, n,
-                     n_bound_before);
-  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, sp->count, n, sp->cap,
-                     (uint32_t*)nullptr);
+  const bemcs_inject_case_t a{*sp, *p, n, n_bound_before};
+  hipLaunchKernelGGL(k_inject<false>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a,
+                     (const bemcs_inject_case_t*)nullptr);
+  hipLaunchKernelGGL(k_add_count<false>, dim3(1), dim3(1), 0, s, a,
+                     (const bemcs_inject_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
@@ -897,20 +970,97 @@ int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, in
   BEMCS_REQUIRE(in && out && scratch && in->count && out->count, "bemcs_sort_by_cell: null argument");
   BEMCS_REQUIRE(out->cap >= n_bound && in->cap >= n_bound, "bemcs_sort_by_cell: capacity");
   hipStream_t s = as_stream(stream);
-  int64_t ncell = (int64_t)nx * ny;
-  uint32_t* hist = scratch;
-  uint32_t* sums = scratch + ncell;
-  uint32_t* keys = sums + 1024 + ((ncell + 1023) / 1024);
-  BEMCS_CHECK_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * ncell, s));
-  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, s, in->count, out->count,
-                     n_bound > 0 ? n_bound : (int64_t)0);
-  if (n_bound <= 0) return BEMCS_OK;
-  hipLaunchKernelGGL(k_cell_hist, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, in->x, in->y,
-                     in->count, n_bound, (float)dx, (float)dy, nx, ny, hist, keys);
-  int rc = scan_exclusive(hist, ncell, sums, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *in, *out,
-                     keys, hist, n_bound);
+  const bemcs_sort_case_t a{*in, *out, n_bound, nx, ny, dx, dy, scratch};
+  const bemcs_sort_case_t* none = nullptr;
+  const int64_t ncell = (int64_t)nx * ny;
+  hipLaunchKernelGGL(k_sort_prepare<false>, dim3(grid_for(ncell, kBlock)), dim3(kBlock), 0, s, a, none);
+  if (n_bound > 0) {
+    const unsigned nb = grid_for(ncell, 1024);
+    hipLaunchKernelGGL(k_cell_hist<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a, none);
+    hipLaunchKernelGGL(k_scan_blocks<false>, dim3(nb), dim3(1024), 0, s, a, none);
+    hipLaunchKernelGGL(k_scan_sums<false>, dim3(1), dim3(1024), 0, s, a, none);
+    hipLaunchKernelGGL(k_scan_add<false>, dim3(nb), dim3(1024), 0, s, a, none);
+    hipLaunchKernelGGL(k_cell_scatter<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
+                       none);
+  }
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+// ------------------------------------------------------- batched versions
+static bool all_fast(const float* div, int32_t n, hipStream_t s) {
+  if (!div) return false;
+  for (int32_t c = 0; c < 2 * n; ++c)
+    if (!fastdiv_ok(div[c], s)) return false;
+  return true;
+}
+
+int bemcs_ion_step_batched(const bemcs_ion_case_t* cases, int32_t n, int64_t max_bound,
+                           const float* divisors_host, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_ion_step_batched: bad arguments");
+  if (max_bound <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  auto kern = all_fast(divisors_host, n, s) ? k_ion_step<true, true> : k_ion_step<false, true>;
+  hipLaunchKernelGGL(kern, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0, s,
+                     bemcs_ion_case_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_electron_step_batched(const bemcs_electron_case_t* cases, int32_t n, int64_t max_bound,
+                                const float* divisors_host, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_electron_step_batched: bad arguments");
+  if (max_bound <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  auto kern = all_fast(divisors_host, n, s) ? k_electron_step<true, true> : k_electron_step<false, true>;
+  hipLaunchKernelGGL(kern, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0, s,
+                     bemcs_electron_case_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_substeps_batched(const bemcs_substeps_case_t* cases, int32_t n, int64_t max_bound,
+                           bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_substeps_batched: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (max_bound > 0)
+    hipLaunchKernelGGL(k_substep_ds<true>, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0,
+                       s, bemcs_substeps_case_t{}, cases);
+  hipLaunchKernelGGL(k_substep_finalize<true>, dim3(1, 1, n), dim3(1), 0, s, bemcs_substeps_case_t{},
+                     cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_inject_batched(const bemcs_inject_case_t* cases, int32_t n, int64_t max_n,
+                         bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1, "bemcs_inject_batched: bad arguments");
+  if (max_n <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_inject<true>, dim3(grid_for(max_n, kBlock), 1, n), dim3(kBlock), 0, s,
+                     bemcs_inject_case_t{}, cases);
+  hipLaunchKernelGGL(k_add_count<true>, dim3(1, 1, n), dim3(1), 0, s, bemcs_inject_case_t{}, cases);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_sort_by_cell_batched(const bemcs_sort_case_t* cases, int32_t n, int64_t max_bound,
+                               int64_t max_nodes, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(cases && n >= 1 && max_nodes > 0, "bemcs_sort_by_cell_batched: bad arguments");
+  hipStream_t s = as_stream(stream);
+  const bemcs_sort_case_t z{};
+  hipLaunchKernelGGL(k_sort_prepare<true>, dim3(grid_for(max_nodes, kBlock), 1, n), dim3(kBlock), 0, s,
+                     z, cases);
+  if (max_bound > 0) {
+    const unsigned nb = grid_for(max_nodes, 1024);
+    hipLaunchKernelGGL(k_cell_hist<true>, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0, s,
+                       z, cases);
+    hipLaunchKernelGGL(k_scan_blocks<true>, dim3(nb, 1, n), dim3(1024), 0, s, z, cases);
+    hipLaunchKernelGGL(k_scan_sums<true>, dim3(1, 1, n), dim3(1024), 0, s, z, cases);
+    hipLaunchKernelGGL(k_scan_add<true>, dim3(nb, 1, n), dim3(1024), 0, s, z, cases);
+    hipLaunchKernelGGL(k_cell_scatter<true>, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0,
+                       s, z, cases);
+  }
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

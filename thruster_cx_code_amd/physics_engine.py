@@ -338,6 +338,16 @@ class DigitalTwinSimulator:
         """Picard loop with Boltzmann electrons and E field (reference :850-902)."""
         if not self._built:
             return
+        p = self._picard_params(iterations, params)
+        its = C.c_int32(0)
+        rel = C.c_double(0.0)
+        check(self.lib.bemcs_poisson_picard(self._poisson, C.byref(p), C.byref(its), C.byref(rel),
+                                            stream_ptr()), "poisson_picard")
+        self.last_poisson_iters = its.value
+        self.last_poisson_relres = rel.value
+        self.poisson_iter_log.append(its.value)
+
+    def _picard_params(self, iterations, params):
         params = params or {}
         grids = params.get('grids', [{'V': 1000}])
         cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
@@ -357,13 +367,7 @@ class DigitalTwinSimulator:
         p.rtol = self.poisson_rtol
         p.iterations = int(iterations)
         p.max_iter = self.poisson_max_iter
-        its = C.c_int32(0)
-        rel = C.c_double(0.0)
-        check(self.lib.bemcs_poisson_picard(self._poisson, C.byref(p), C.byref(its), C.byref(rel),
-                                            stream_ptr()), "poisson_picard")
-        self.last_poisson_iters = its.value
-        self.last_poisson_relres = rel.value
-        self.poisson_iter_log.append(its.value)
+        return p
 
     # ------------------------------------------------------------ particles
     @property
@@ -435,6 +439,23 @@ class DigitalTwinSimulator:
             self._launch_inject(False, p, n_e)
 
     def _inject(self, params, t_current):
+        """Injection of one step (:996-1099): plan, reserve capacity, launch."""
+        plans = self._inject_plan(params, t_current)
+        self._reserve_step(plans)
+        for is_ion, p, n in plans:
+            self._launch_inject(is_ion, p, n)
+
+    def _reserve_step(self, plans):
+        """Capacity for this step's injections and for one SEE electron per
+        ion, reserved before anything of the step is launched."""
+        ni = self.bound_i + sum(n for is_ion, _, n in plans if is_ion)
+        ne = self.bound_e + sum(n for is_ion, _, n in plans if not is_ion)
+        self._ensure_capacity(ni, ne + ni)
+
+    def _inject_plan(self, params, t_current):
+        """The step's injections as (is_ion, bemcs_inject_t, n), in launch
+        order; count samples and particle ids are drawn here."""
+        plans = []
         grids = params.get('grids', [])
         if self.injection_enabled and (self.injection_stop_time is None or t_current <= self.injection_stop_time):
             n0 = params.get('n0_plasma', 1e17)
@@ -449,7 +470,6 @@ class DigitalTwinSimulator:
             n = int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, self.iteration & 0xFFFFFFFF, lam))
             self.injected_ions_step = float(n)
             if n > 0:
-                self._ensure_capacity(self.bound_i + n, self.bound_e)
                 p = self._inject_struct(0, self.cnt_i)
                 if grids:
                     sr = grids[0]['r']
@@ -471,7 +491,7 @@ class DigitalTwinSimulator:
                 p.v_drift = float(vb)
                 p.v_spread = float(np.sqrt(self.q_ion * Ti / self.m_ion))
                 p.pid_base = self.next_ion_pid & 0xFFFFFFFF
-                self._launch_inject(True, p, n)
+                plans.append((True, p, n))
                 self.next_ion_pid += n
                 self.injected_ions += float(n)
             if params.get('rf_enable'):
@@ -483,25 +503,24 @@ class DigitalTwinSimulator:
                 if u < (nef - ne):
                     ne += 1
                 if ne > 0:
-                    self._ensure_capacity(self.bound_i, self.bound_e + ne)
                     p = self._inject_struct(2, self.cnt_e)
                     ylo = max(self.dy, 1e-9)
                     p.y_lo[0], p.y_hi[0] = ylo, max(ylo, self.Ly - ylo)
                     p.x0 = 0.1
                     p.v_drift = float(vb)
                     p.v_spread = float(vth_s)
-                    self._launch_inject(False, p, ne)
+                    plans.append((False, p, ne))
         ne = int(params.get('neut_rate', 30))
         if ne > 0:
             Te = params.get('Te', 5.0)
             nxp = float(np.clip(params.get('neut_x', self.Lx - 0.5), self.dx, self.Lx - self.dx))
             nrp = float(np.clip(params.get('neut_r', self.Ly), self.dy, self.Ly))
-            self._ensure_capacity(self.bound_i, self.bound_e + ne)
             p = self._inject_struct(1, self.cnt_e)
             p.y_lo[0], p.y_hi[0] = 0.0, nrp
             p.x0 = nxp
             p.v_spread = float(np.sqrt(2.0 * self.q * Te / self.m_e))
-            self._launch_inject(False, p, ne)
+            plans.append((False, p, ne))
+        return plans
 
     def _uniform_host(self, purpose, a, b, c):
         out = (C.c_uint32 * 4)()
@@ -538,8 +557,6 @@ class DigitalTwinSimulator:
             ev = self._timed_begin(kt)
             self.recalc_poisson(5, params)
             self._timed_end(kt, "poisson_picard", ev)
-        # the ion step may append one SEE electron per ion
-        self._ensure_capacity(self.bound_i, self.bound_e + self.bound_i)
         I0, I1 = self.ions
         E0, E1 = self.elec
         dsb = self.ds_bits.data_ptr()
