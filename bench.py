@@ -255,6 +255,76 @@ def poisson_bench(calls=10):
                       "levels": sim.lib.bemcs_poisson_levels(sim._poisson)}))
 
 
+def sweep_bench(args):
+    """Sweep throughput (cases/hr): `--sweep-cases` uneven two-grid cases
+    (thruster_cx_code_amd.sweep.uneven_sweep), `--sweep-steps` steps each,
+    advanced as one batch with the sort-interval autotuner, against the same
+    GPU running the cases one at a time (timed on every k-th case and scaled to
+    all cases).  Build (geometry, preload, initial Poisson) is included."""
+    import gc
+    import torch
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    from thruster_cx_code_amd.sweep import SortTuner, build_case, uneven_sweep
+    cases = uneven_sweep(args.sweep_cases, seed=0)
+    n = len(cases)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sims = [build_case(p) for p in cases]
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    _progress(f"built {n} cases in {t_build:.1f} s")
+    batch = BatchedSimulator(sims, cases)
+    tuner = SortTuner(batch)
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.sweep_steps:
+        k = min(100, args.sweep_steps - done)
+        tuner.advance(k)
+        done += k
+        torch.cuda.synchronize()
+        _progress(f"batch: {done} steps, {time.perf_counter() - t0:.1f} s")
+    t_batch = time.perf_counter() - t0
+    final = [(s.record()["num_p"], s.record()["num_e"]) for s in sims]
+    tuning = tuner.history
+    del tuner, batch, sims
+    gc.collect()
+    torch.cuda.empty_cache()
+    stride = max(1, n // max(1, args.sweep_seq_cases))
+    sample = list(range(0, n, stride))
+    t_seq_sample = 0.0
+    for c in sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = build_case(cases[c])
+        s.advance(cases[c], args.sweep_steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        t_seq_sample += dt
+        _progress(f"alone: case {c} ({cases[c]['mesh']['nx']}x{cases[c]['mesh']['ny']}) {dt:.1f} s")
+        del s
+        gc.collect()
+        torch.cuda.empty_cache()
+    t_seq = t_seq_sample * n / len(sample)
+    batched_hr = n / (t_build + t_batch) * 3600.0
+    seq_hr = n / t_seq * 3600.0
+    out = {
+        "metric": "sweep cases/hr (uneven two-grid cases, 256x96..1024x384)",
+        "value": batched_hr, "unit": "cases/hr", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "data": "synthetic (seeded preload, Philox streams)",
+        "config": {"workload": f"{n} cases, {args.sweep_steps} steps each: nx ~ U{{256..1024 step 64}}, "
+                               "ny = 3nx/8, Va ~ U[-400,-100] V, r_accel ~ U[0.45,0.75] mm, "
+                               "Vs ~ U[800,1600] V, particles per species log-uniform [2e5, 4e6]",
+                   "cases": n, "steps": args.sweep_steps},
+        "batched": {"cases_per_hr": batched_hr, "build_s": t_build, "steps_s": t_batch,
+                    "sort_tuning": tuning},
+        "one_at_a_time": {"cases_per_hr": seq_hr, "timed_cases": sample, "timed_s": t_seq_sample,
+                          "scaled_s": t_seq},
+        "speedup": batched_hr / seq_hr,
+        "final_counts_first_cases": final[:4],
+    }
+    print(json.dumps(out))
+
+
 def copy_calibration(mbytes=512, reps=20):
     """A plain device-to-device copy of known size, to scale the FETCH_SIZE and
     WRITE_SIZE counters (run it under rocprofv3 --pmc)."""
@@ -274,6 +344,14 @@ def main():
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--poisson-bench":
         poisson_bench()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--sweep":
+        sp = argparse.ArgumentParser()
+        sp.add_argument("--sweep", action="store_true")
+        sp.add_argument("--sweep-cases", type=int, default=64)
+        sp.add_argument("--sweep-steps", type=int, default=3000)
+        sp.add_argument("--sweep-seq-cases", type=int, default=16)
+        sweep_bench(sp.parse_args())
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
         w, ni, ne, secs, steps = sys.argv[2:7]

@@ -205,6 +205,10 @@ struct CGArgs {
   int32_t* done;
 };
 
+// a[i] for a two-entry array member without indexing memory (a select of the
+// two values keeps argument structs in registers)
+__device__ __forceinline__ double* pick2(double* const (&a)[2], int i) { return i ? a[1] : a[0]; }
+
 struct CheckArgs {
   const double* pr;
   int nparts;
@@ -297,11 +301,11 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const float om = IS9 ? kOmegaC : kOmega0;
   const int CX0 = blockIdx.x * TC, CY0 = blockIdx.y * TC;
   const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
-  const double* rp = g.R[(k + 1) & 1];
+  const double* rp = pick2(g.R, (k + 1) & 1);
   double alpha = 0.0;
   if (STEP) {
     double rz, pap, u;
-    block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
+    block_sum2(pick2(g.pz, (k + 1) & 1), g.nparts, rz, u);
     block_sum2(g.pa, g.nparts, pap, u);
     alpha = pap != 0.0 ? rz / pap : 0.0;
   }
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   if (STEP) {
     // the owned nodes take the CG step; their b is the value staged above
     double rr = 0.0;
-    double* rn_out = g.R[k & 1];
+    double* rn_out = pick2(g.R, k & 1);
 #pragma unroll
     for (int it = 0; it < TFd * TFd / 256; ++it) {
       const int t = threadIdx.x + it * 256;
@@ -573,11 +577,11 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     }
     F.t[gi] = v;
     if (DOT) {
-      rz += g.R[k & 1][gi] * (double)v;
-      roz += g.R[(k + 1) & 1][gi] * (double)v;
+      rz += pick2(g.R, k & 1)[gi] * (double)v;
+      roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
     }
   }
-  if (DOT) block_partials(rz, roz, g.pz[k & 1], blockIdx.y * tiles_x + blockIdx.x);
+  if (DOT) block_partials(rz, roz, pick2(g.pz, k & 1), blockIdx.y * tiles_x + blockIdx.x);
 }
 
 // coarsest level: x = C^-1 b with the inverse stored column-major.  A block
@@ -675,8 +679,8 @@ __global__ __launch_bounds__(256) void k_cg_dir(CGArgs one, const CGArgs* __rest
   double beta = 0.0;
   if (k > 1) {
     double rz, roz, rz_old, u;
-    block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, roz);
-    block_sum2(g.pz[k & 1], g.nparts, rz_old, u);
+    block_sum2(pick2(g.pz, (k + 1) & 1), g.nparts, rz, roz);
+    block_sum2(pick2(g.pz, k & 1), g.nparts, rz_old, u);
     beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
   }
   double s = 0;
@@ -706,11 +710,11 @@ __global__ __launch_bounds__(256) void k_cg_step(CGArgs one, const CGArgs* __res
   CG_CASE();
   if (*g.done) return;
   double rz, pap, u;
-  block_sum2(g.pz[(k + 1) & 1], g.nparts, rz, u);
+  block_sum2(pick2(g.pz, (k + 1) & 1), g.nparts, rz, u);
   block_sum2(g.pa, g.nparts, pap, u);
   const double a = pap != 0.0 ? rz / pap : 0.0;
-  const double* rp = g.R[(k + 1) & 1];
-  double* rn_out = g.R[k & 1];
+  const double* rp = pick2(g.R, (k + 1) & 1);
+  double* rn_out = pick2(g.R, k & 1);
   double rr = 0;
   GRID_LOOP(i, g.n) {
     g.x[i] += a * g.p[i];
@@ -730,10 +734,10 @@ __global__ __launch_bounds__(256) void k_cg_dotz(CGArgs one, const CGArgs* __res
   double a = 0, b = 0;
   GRID_LOOP(i, g.n) {
     const double zi = g.z[i];
-    a += g.R[k & 1][i] * zi;
-    b += g.R[(k + 1) & 1][i] * zi;
+    a += pick2(g.R, k & 1)[i] * zi;
+    b += pick2(g.R, (k + 1) & 1)[i] * zi;
   }
-  block_partials(a, b, g.pz[k & 1], blockIdx.x);
+  block_partials(a, b, pick2(g.pz, k & 1), blockIdx.x);
 }
 
 // --------------------------------------------------- reduced system set-up
