@@ -325,6 +325,37 @@ def sweep_bench(args):
     print(json.dumps(out))
 
 
+def serve_bench(args):
+    """Served sweep (continuous batching): `--serve-requests` perveance /
+    backstreaming requests with Poisson arrivals at `--serve-rate` per second,
+    mesh from {256x96, 512x192, 1024x384}, accel voltage ~ U[-400,-100] V,
+    beam current ~ U[0.5, 3] mA per aperture, each run to the steady-state rule
+    or `--serve-max-steps`; p50/p99 latency (arrival to result) and
+    throughput."""
+    import torch
+    from thruster_cx_code_amd.serve import SweepServer, latency_summary, poisson_stream
+    reqs = poisson_stream(args.serve_requests, args.serve_rate, seed=0)
+    srv = SweepServer(max_steps=args.serve_max_steps, min_steps=int(0.75 * args.serve_max_steps),
+                      max_active=args.serve_slots)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    summ = latency_summary(res, wall)
+    _progress(f"served {len(res)} requests in {wall:.1f} s")
+    print(json.dumps({
+        "metric": "served sweep p50/p99 latency (continuous batching)",
+        "value": summ["p50_s"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "vs_baseline": None, "data": "synthetic (Poisson arrivals, seeded cases)",
+        "config": {"workload": f"{args.serve_requests} requests at {args.serve_rate}/s, meshes 256x96/512x192/"
+                               f"1024x384, Va ~ U[-400,-100] V, I ~ U[0.5,3] mA, steady-state rule after "
+                               f"{int(0.75 * args.serve_max_steps)} steps, cap {args.serve_max_steps} steps",
+                   "slots": args.serve_slots},
+        "latency": summ, "wall_s": wall,
+        "max_active": max((a for _, a, _ in srv.log), default=0),
+    }))
+
+
 def copy_calibration(mbytes=512, reps=20):
     """A plain device-to-device copy of known size, to scale the FETCH_SIZE and
     WRITE_SIZE counters (run it under rocprofv3 --pmc)."""
@@ -344,6 +375,15 @@ def main():
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--poisson-bench":
         poisson_bench()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--serve":
+        sp = argparse.ArgumentParser()
+        sp.add_argument("--serve", action="store_true")
+        sp.add_argument("--serve-requests", type=int, default=64)
+        sp.add_argument("--serve-rate", type=float, default=4.0)
+        sp.add_argument("--serve-max-steps", type=int, default=1000)
+        sp.add_argument("--serve-slots", type=int, default=64)
+        serve_bench(sp.parse_args())
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--sweep":
         sp = argparse.ArgumentParser()

@@ -78,3 +78,38 @@ def test_batched_steps_match_single_case_steps():
             assert torch.equal(a.cnt_i, b.cnt_i) and torch.equal(a.cnt_e, b.cnt_e), (step, c)
     its = batch.poisson_iter_log[-1]
     assert its == [s.poisson_iter_log[-1] for s in alone]
+
+
+def _same(a, b, where):
+    ra, rb = a.record(), b.record()
+    for key in ("num_p", "num_e", "n_sub_ion", "n_sub_e", "entered", "crossed", "mcc_events"):
+        assert ra[key] == rb[key], (where, key)
+    ta, ia, ea = _state(a)
+    tb, ib, eb = _state(b)
+    assert np.array_equal(ta, tb) and np.array_equal(ia.view(np.uint32), ib.view(np.uint32)), where
+    assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32)), where
+    assert np.array_equal(a.V.view(np.uint64), b.V.view(np.uint64)), where
+
+
+def test_cases_join_and_leave_between_steps():
+    """A case admitted later (its own iteration counter, opposite Poisson
+    parity) and a case retired midway match their alone twins."""
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    params = _cases()[:2]
+    alone = _build(params)
+    together = _build(params)
+    batch = BatchedSimulator([together[0]], [params[0]])
+    active = {0}
+    for step in range(1, 10):
+        if step == 4:
+            batch.add(together[1], params[1])
+            active.add(1)
+        if step == 8:
+            batch.remove(together[0])
+            active.discard(0)
+        print(f"[gpu] dynamic batch step {step} cases {sorted(active)}", flush=True)
+        batch.advance(1)
+        for c in active:
+            alone[c].advance(params[c], 1)
+            _same(alone[c], together[c], (step, c))
+    assert together[0].iteration == 7 and together[1].iteration == 6

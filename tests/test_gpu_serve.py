@@ -1,0 +1,60 @@
+"""Served sweep: every request's result is the same alone and in the mix
+(continuous batching admits requests while others run), and the steady-state
+rule matches the reference's sweep rule on the recorded divergence history."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _requests():
+    from thruster_cx_code_amd.serve import beam_request
+    reqs = [beam_request(0, (256, 96), -200.0, 0.6, seed=0),
+            beam_request(1, (320, 120), -300.0, 1.0, seed=1),
+            beam_request(2, (256, 96), -150.0, 0.7, seed=2)]
+    for k, r in enumerate(reqs):
+        r.arrival = 0.01 * k
+    return reqs
+
+
+def _serve(reqs, **kw):
+    from thruster_cx_code_amd.serve import SweepServer
+    srv = SweepServer(max_steps=80, min_steps=40, check_every=10, chunk=5, window=20, tol_deg=5.0, **kw)
+    return {r.id: r for r in srv.run(reqs, realtime=False)}, srv
+
+
+def test_results_same_alone_and_in_mix():
+    print("[gpu] serve mixed", flush=True)
+    mixed, srv = _serve(_requests())
+    assert sorted(mixed) == [0, 1, 2]
+    assert max(n for _, n, _ in srv.log) >= 2  # requests overlapped on the device
+    for req in _requests():
+        print(f"[gpu] serve alone {req.id}", flush=True)
+        alone, _ = _serve([req])
+        a, m = alone[req.id], mixed[req.id]
+        assert (a.steps, a.converged) == (m.steps, m.converged)
+        assert np.array_equal(np.array(a.window), np.array(m.window))
+        assert (np.isnan(a.divergence) and np.isnan(m.divergence)) or a.divergence == m.divergence
+
+
+def test_invalid_request_is_rejected_not_fatal():
+    """A request whose mesh fails the reference's Debye check is rejected
+    with its error while the others are served."""
+    from thruster_cx_code_amd.serve import beam_request
+    reqs = _requests()[:2] + [beam_request(9, (256, 96), -200.0, 3.0, seed=9)]
+    print("[gpu] serve reject", flush=True)
+    res, _ = _serve(reqs)
+    assert res[9].error and "Debye" in res[9].error
+    assert not res[0].error and not res[1].error
+
+
+def test_admission_respects_memory_budget():
+    """With a budget that fits one case at a time the requests run one after
+    another and still complete."""
+    from thruster_cx_code_amd.serve import footprint_bytes
+    reqs = _requests()
+    budget = torch.cuda.memory_allocated() + max(footprint_bytes(r.params) for r in reqs) + (1 << 20)
+    print("[gpu] serve budget", flush=True)
+    res, srv = _serve(reqs, hbm_budget=budget, max_active=8)
+    assert sorted(res) == [0, 1, 2]

@@ -61,71 +61,108 @@ class _ArgRing:
 
 
 class BatchedSimulator:
-    """Lockstep time steps of several built DigitalTwinSimulator cases.
+    """Time steps of several built DigitalTwinSimulator cases, one launch per
+    phase serving all of them.
 
     sims: the cases (each already build_domain'ed, on the same device);
     params: the params dict of each case (the step reads them like step()).
-    Poisson runs on every case on the same (even) steps, as in the reference
-    schedule; all cases share the iteration counter parity because they
-    advance together.
+    Cases keep their own iteration counters, so they may join (add) and
+    leave (remove) between steps: each case runs its Poisson update on its own
+    even steps and sorts on its own schedule, as when stepped alone; the
+    Poisson solves of the cases due in a step form one batched solve.
     """
 
-    def __init__(self, sims, params):
-        if not sims:
-            raise ValueError("BatchedSimulator needs at least one case")
+    def __init__(self, sims=(), params=()):
         if len(params) != len(sims):
             raise ValueError("one params dict per case")
-        self.sims = list(sims)
-        self.params = list(params)
-        self.n = len(self.sims)
-        self.device = self.sims[0].device
+        self.sims, self.params = [], []
+        self.device = None
         self.lib = _lib.load()
-        its = {s.iteration for s in self.sims}
-        if len(its) != 1:
-            raise ValueError("cases must start at the same iteration")
-        for s, p in zip(self.sims, self.params):
-            if not s._built:
-                raise ValueError("every case must be built (build_domain) first")
-            s._prepare_step_params(p)
-        handles = (C.c_void_p * self.n)(*[s._poisson.value for s in self.sims])
-        self._batch = C.c_void_p()
-        check(self.lib.bemcs_poisson_batch_create(handles, self.n, C.byref(self._batch)),
-              "poisson_batch_create")
-        self.args = _ArgRing(self.device, nbytes=max(1 << 20, 8192 * self.n))
-        self._div = (C.c_float * (2 * self.n))()
+        self._sets = {}
+        self.args = None
+        self.poisson_iter_log = []
+        self.kernel_timing = None
+        for s, p in zip(sims, params):
+            self.add(s, p)
+
+    @property
+    def n(self):
+        return len(self.sims)
+
+    def add(self, sim, params):
+        """Admit a built case (between steps)."""
+        if not sim._built:
+            raise ValueError("every case must be built (build_domain) first")
+        if self.device is None:
+            self.device = sim.device
+        sim._prepare_step_params(params)
+        self.sims.append(sim)
+        self.params.append(params)
+        self._membership_changed()
+
+    def remove(self, sim):
+        """Retire a case (between steps); its state stays valid for reading."""
+        k = self.sims.index(sim)
+        del self.sims[k]
+        del self.params[k]
+        self._membership_changed()
+
+    def _membership_changed(self):
+        self._free_sets()
+        if self.device is None:
+            return
+        n = max(1, self.n)
+        if self.args is None or self.args.dev.numel() < 8192 * n:
+            self.args = _ArgRing(self.device, nbytes=max(1 << 20, 16384 * n))
+        self._div = (C.c_float * (2 * n))()
         for c, s in enumerate(self.sims):
             self._div[2 * c] = float(np.float32(s.dx))
             self._div[2 * c + 1] = float(np.float32(s.dy))
-        self.max_nodes = max(s.nx * s.ny for s in self.sims)
-        self.poisson_iter_log = []
-        self.kernel_timing = None
+        self.max_nodes = max((s.nx * s.ny for s in self.sims), default=1)
 
-    def __del__(self):
-        b = getattr(self, "_batch", None)
-        if b is not None and b.value:
+    def _free_sets(self):
+        for b in self._sets.values():
             try:
                 self.lib.bemcs_poisson_batch_free(b)
             except Exception:
                 pass
+        self._sets = {}
+
+    def __del__(self):
+        try:
+            self._free_sets()
+        except Exception:
+            pass
 
     # ------------------------------------------------------------ step
     def advance(self, n_steps):
         for _ in range(int(n_steps)):
             self._step()
 
-    def _picard(self):
-        ps = (_lib.Picard * self.n)(*[s._picard_params(5, p) for s, p in zip(self.sims, self.params)])
-        its = (C.c_int32 * self.n)()
-        rel = (C.c_double * self.n)()
-        check(self.lib.bemcs_poisson_batch_picard(self._batch, ps, its, rel, stream_ptr()),
-              "poisson_batch_picard")
-        for s, it, r in zip(self.sims, its, rel):
+    def _picard(self, idx):
+        """Batched Picard update of the cases idx (their own even steps)."""
+        key = tuple(self.sims[c]._poisson.value for c in idx)
+        b = self._sets.get(key)
+        if b is None:
+            handles = (C.c_void_p * len(idx))(*key)
+            b = C.c_void_p()
+            check(self.lib.bemcs_poisson_batch_create(handles, len(idx), C.byref(b)), "poisson_batch_create")
+            self._sets[key] = b
+        m = len(idx)
+        ps = (_lib.Picard * m)(*[self.sims[c]._picard_params(5, self.params[c]) for c in idx])
+        its = (C.c_int32 * m)()
+        rel = (C.c_double * m)()
+        check(self.lib.bemcs_poisson_batch_picard(b, ps, its, rel, stream_ptr()), "poisson_batch_picard")
+        for c, it, r in zip(idx, its, rel):
+            s = self.sims[c]
             s.last_poisson_iters, s.last_poisson_relres = int(it), float(r)
             s.poisson_iter_log.append(int(it))
         self.poisson_iter_log.append(list(its))
 
     def _step(self):
         sims, n, lib = self.sims, self.n, self.lib
+        if n == 0:
+            return
         s = stream_ptr()
         A = self.args
         A.begin()
@@ -160,8 +197,6 @@ class BatchedSimulator:
                 max_n = max(max_n, int(cnt))
             inj_launch.append((dptr, len(members), max_n))
         # ---- per-step argument structs of every case
-        it = sims[0].iteration
-        slot = it % RING
         dsz = C.sizeof(_lib.IonDiag)
         sb, sb_d = A.alloc(_lib.StepBegin, n)
         sub, sub_d = A.alloc(_lib.SubstepsCase, 2 * n)
@@ -170,12 +205,16 @@ class BatchedSimulator:
         pc, pc_d = A.alloc(_lib.PpcCase, n)
         qc, qc_d = A.alloc(_lib.PercentileCase, n)
         se, se_d = A.alloc(_lib.StepEnd, n)
-        sort_i = sims[0].sort_interval > 0 and it % sims[0].sort_interval == 0
-        sort_e = sims[0].electron_sort_interval > 0 and it % sims[0].electron_sort_interval == 0
-        si = A.alloc(_lib.SortCase, n) if sort_i else None
-        sx = A.alloc(_lib.SortCase, n) if sort_e else None
+        sort_i = [c for c, m in enumerate(sims) if m.sort_interval > 0 and m.iteration % m.sort_interval == 0]
+        sort_e = [c for c, m in enumerate(sims)
+                  if m.electron_sort_interval > 0 and m.iteration % m.electron_sort_interval == 0]
+        si = A.alloc(_lib.SortCase, len(sort_i)) if sort_i else None
+        sx = A.alloc(_lib.SortCase, len(sort_e)) if sort_e else None
+        poisson = [c for c, m in enumerate(sims) if m.iteration % 2 == 0]
         max_bi = max_be = 0
         for c, sim in enumerate(sims):
+            it = sim.iteration
+            slot = it % RING
             I0, I1 = sim.ions
             E0, E1 = sim.elec
             bi, be = sim.bound_i, sim.bound_e
@@ -228,28 +267,29 @@ class BatchedSimulator:
             sim._be.grow(sim.bound_i)
         post_bi = max(sim.bound_i for sim in sims)
         post_be = max(sim.bound_e for sim in sims)
-        for c, sim in enumerate(sims):
-            # sort cases read the post-step buffers (index 1 before the swap)
-            if sort_i:
-                o = si[0][c]
-                setattr(o, "in", sim.ions[1].struct())
-                o.out, o.n_bound = sim.ions[0].struct(), sim.bound_i
-                o.nx, o.ny, o.dx, o.dy = sim.nx, sim.ny, sim.dx, sim.dy
-                o.scratch = sim.sort_scratch_for(max(sim.bound_i, sim.bound_e)).data_ptr()
-            if sort_e:
-                o = sx[0][c]
-                setattr(o, "in", sim.elec[1].struct())
-                o.out, o.n_bound = sim.elec[0].struct(), sim.bound_e
-                o.nx, o.ny, o.dx, o.dy = sim.nx, sim.ny, sim.dx, sim.dy
-                o.scratch = sim.sort_scratch_for(max(sim.bound_i, sim.bound_e)).data_ptr()
+        # sort cases read the post-step buffers (index 1 before the swap)
+        for j, c in enumerate(sort_i):
+            sim = sims[c]
+            o = si[0][j]
+            setattr(o, "in", sim.ions[1].struct())
+            o.out, o.n_bound = sim.ions[0].struct(), sim.bound_i
+            o.nx, o.ny, o.dx, o.dy = sim.nx, sim.ny, sim.dx, sim.dy
+            o.scratch = sim.sort_scratch_for(max(sim.bound_i, sim.bound_e)).data_ptr()
+        for j, c in enumerate(sort_e):
+            sim = sims[c]
+            o = sx[0][j]
+            setattr(o, "in", sim.elec[1].struct())
+            o.out, o.n_bound = sim.elec[0].struct(), sim.bound_e
+            o.nx, o.ny, o.dx, o.dy = sim.nx, sim.ny, sim.dx, sim.dy
+            o.scratch = sim.sort_scratch_for(max(sim.bound_i, sim.bound_e)).data_ptr()
         A.commit()
         # ---- launches
         for dptr, cnt, max_n in inj_launch:
             check(lib.bemcs_inject_batched(dptr, cnt, max_n, s), "inject_batched")
-        if it % 2 == 0:
+        if poisson:
             kt = self.kernel_timing
             ev = DigitalTwinSimulator._timed_begin(kt)
-            self._picard()
+            self._picard(poisson)
             DigitalTwinSimulator._timed_end(kt, "poisson_picard", ev)
         check(lib.bemcs_step_begin_batched(sb_d, n, self.max_nodes, s), "step_begin_batched")
         check(lib.bemcs_substeps_batched(sub_d, 2 * n, max(max_bi, max_be), s), "substeps_batched")
@@ -270,23 +310,24 @@ class BatchedSimulator:
             sim.cnt_e, sim.cnt_e_next = sim.cnt_e_next, sim.cnt_e
             sim.ppc_steps_count += 1
         if sort_i:
-            check(lib.bemcs_sort_by_cell_batched(si[1], n, post_bi, self.max_nodes, s), "sort_batched")
-            for sim in sims:
-                sim.ions.reverse()
+            check(lib.bemcs_sort_by_cell_batched(si[1], len(sort_i), post_bi, self.max_nodes, s), "sort_batched")
+            for c in sort_i:
+                sims[c].ions.reverse()
         if sort_e:
-            check(lib.bemcs_sort_by_cell_batched(sx[1], n, post_be, self.max_nodes, s), "sort_batched")
-            for sim in sims:
-                sim.elec.reverse()
+            check(lib.bemcs_sort_by_cell_batched(sx[1], len(sort_e), post_be, self.max_nodes, s), "sort_batched")
+            for c in sort_e:
+                sims[c].elec.reverse()
         # asynchronous count readback for the host bounds (one event for all cases)
         ev = torch.cuda.Event()
         cnts = []
         for sim in sims:
+            sl = sim.iteration % RING
             h = torch.empty(2, dtype=torch.int32, pin_memory=True)
-            h.copy_(sim.ring_counts[2 * slot:2 * slot + 2], non_blocking=True)
+            h.copy_(sim.ring_counts[2 * sl:2 * sl + 2], non_blocking=True)
             cnts.append(h)
         ev.record()
         for sim, h in zip(sims, cnts):
-            sim._pending.append((ev, h, sim._bi.snapshot(), sim._be.snapshot(), it))
+            sim._pending.append((ev, h, sim._bi.snapshot(), sim._be.snapshot(), sim.iteration))
 
     # ------------------------------------------------------ diagnostics
     def records(self):
