@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 1
+#define BEMCS_ABI_VERSION 2
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -128,8 +128,11 @@ typedef struct {
   uint32_t mcc_events;     /* CEX collisions */
   uint32_t crossed_primary;
   uint32_t overflow;       /* bit flags: 1 SEE append, 2 survivor append beyond
-                              capacity, 16 count above the launch bound */
-  uint32_t pad_;
+                              capacity, 16 count above the launch bound,
+                              32 impact heat or damage out of the fixed-point
+                              range (non-finite or beyond 2^53 units) */
+  uint32_t surface_hits;   /* hit & x > 0.25 mm: ions that heat, sputter or
+                              emit SEE (:1359, :1391, :1419) */
   double sum_vx, sum_vx2, sum_v, sum_v2, sum_E;
   double sum_v_primary, sum_v_cex;
 } bemcs_ion_diag_t;
@@ -166,6 +169,32 @@ typedef struct {
   int32_t cx_model;          /* 0 = reference fit, 1 = log-linear */
   int32_t mcc_enabled;
   uint32_t seed, step;
+  /* Electrode-surface effects of impacts (:1359-1449), for every ion that
+   * hits an electrode with x > 0.25 mm, at its first impact node (the first
+   * electrode node along the 8-sample segment, else the final node), with
+   * v2 = f32(f32(vx*vx + vy*vy) + vz*vz):
+   *   heat_acc (null unless sim_mode is Thermal/Both) += rint(dT * 2^20),
+   *     dT = f32(f32(f32(f32(half_m32 * v2) * w32) / ccell32) * accel32) [K];
+   *   damage_acc (null unless Erosion/Both) += rint(damage * 2^24) when
+   *     E = f32(f32(half_m32 * v2) / q32) > eth32,
+   *     damage = f32(f32(ycoef32 * f32(E - eth32)) * w32).
+   * The accumulators are 64-bit fixed point, so their sums do not depend on
+   * the order of the ions; bemcs_thermal and bemcs_erosion consume them.
+   * see_defer = 1 appends every spawned SEE electron, unpurged, to see_stage
+   * (bemcs_see_commit purges and appends them after bemcs_erosion, so the
+   * purge sees cells eroded in the same step, as :1417-1449 and :1534-1561
+   * order it); 0 purges them here and appends them to e_out. */
+  uint64_t* heat_acc;
+  uint64_t* damage_acc;
+  bemcs_species_t see_stage;
+  float half_m32;            /* float32(0.5 * m_ion) */
+  float w32;                 /* float32(macro_weight) */
+  float ccell32;             /* float32(C_cell) */
+  float accel32;             /* float32(thermal_accel) */
+  float q32;                 /* float32(q) */
+  float eth32;               /* float32(sputter E_th) */
+  float ycoef32;             /* float32(sputter Y_coeff) */
+  int32_t see_defer;
 } bemcs_ion_step_t;
 
 int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
@@ -329,6 +358,7 @@ typedef struct {
   const int32_t* count_in_i; /* input buffers' counts */
   const int32_t* count_in_e;
   int32_t* nin_out;          /* [2] */
+  int32_t* count_out_x;      /* a third count cleared (SEE staging), may be null */
 } bemcs_step_begin_t;
 
 typedef struct {
@@ -346,6 +376,77 @@ typedef struct {
 
 int bemcs_step_begin(const bemcs_step_begin_t* p, bemcs_stream_t stream);
 int bemcs_step_end(const bemcs_step_end_t* p, bemcs_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Electrode-surface physics (sim_mode Thermal / Erosion / Both) and RF.
+ * ---------------------------------------------------------------------- */
+
+/* Sputter damage and cell failure of one step (:1419-1449), after
+ * bemcs_ion_step: skipped entirely when diag->surface_hits is 0 (the
+ * reference tests for failed cells only on steps with electrode hits);
+ * otherwise damage_map += damage_acc * 2^-24 and damage_acc = 0 on every
+ * node, and each node with damage_map > thresh that is still an electrode
+ * fails: eroded_depth += dy, is_bound = 0, damage_map = 0, *n_failed += 1
+ * (a running total: the caller rebuilds the Poisson operator when it grows,
+ * build_sparse_matrix at :1448). */
+typedef struct {
+  uint64_t* damage_acc;
+  double* damage_map;
+  double* eroded_depth;
+  uint8_t* is_bound;
+  const bemcs_ion_diag_t* diag;
+  uint32_t* n_failed;
+  int64_t n_nodes;
+  double thresh;             /* params Thresh */
+  double dy;                 /* mm */
+} bemcs_erosion_t;
+
+int bemcs_erosion(const bemcs_erosion_t* p, bemcs_stream_t stream);
+
+/* Purge and append of SEE electrons staged by bemcs_ion_step with see_defer:
+ * an electron survives unless its nearest node is an electrode (after this
+ * step's erosion) or it lies outside [0,Lx]x[0,Ly] or is NaN (:1534-1561);
+ * survivors are appended to e_out with their NGP counts in e_cnt_next and
+ * counted in diag->see_kept.  n_bound bounds the staged count. */
+int bemcs_see_commit(const bemcs_species_t* stage, const bemcs_species_t* e_out,
+                     const uint8_t* is_bound, uint32_t* e_cnt_next, bemcs_ion_diag_t* diag,
+                     int32_t nx, int32_t ny, float dx32, float dy32, float Lx32, float Ly32,
+                     int64_t n_bound, bemcs_stream_t stream);
+
+/* Thermal update of one step (:1563-1617) in one pass over the map:
+ * T = T_in + heat_acc * 2^-20 (this step's impact heat), radiative cooling
+ * of electrode nodes T -= f32(cool32 * f32(f32(T^4) - t300_4_32)), `sweeps`
+ * explicit conduction sweeps over interior electrode nodes (insulated
+ * towards non-electrode neighbours, floor 300 K, float32 in the reference
+ * loop's operation order, :1598-1613), the 300 K floor on every electrode
+ * node, written to T_out (distinct from T_in); heat_clear (the other
+ * accumulator of a ping-pong pair, may be null) is zeroed for the next step.
+ * The mean of T_out over the nodes of each grid (bit g of grid_bits) is
+ * written to T_grids[g] as float32 for grids with at least one node.
+ * workspace: bemcs_thermal_workspace_bytes(nx, ny, n_grids) bytes.
+ * sweeps <= 16, n_grids <= 8. */
+typedef struct {
+  const float* T_in;
+  float* T_out;
+  const uint64_t* heat_acc;  /* may be null (no impact heat) */
+  uint64_t* heat_clear;
+  const uint8_t* is_bound;
+  const uint8_t* grid_bits;
+  void* workspace;
+  float* T_grids;            /* [n_grids] */
+  float cool32;              /* float32(emissivity*sigma*A_cell*dt*accel/C_cell) */
+  float t300_4_32;           /* float32(300.0**4) */
+  float fo_x32, fo_y32;      /* float32 Fourier numbers */
+  int32_t nx, ny, n_grids, sweeps;
+} bemcs_thermal_t;
+
+int64_t bemcs_thermal_workspace_bytes(int32_t nx, int32_t ny, int32_t n_grids);
+int bemcs_thermal(const bemcs_thermal_t* p, bemcs_stream_t stream);
+
+/* RF modulation of a grid potential (:984-991): V_fixed[idx[k]] = vdc[k] +
+ * v_rf for k < n (idx: the RF grid's nodes, vdc: their DC potentials). */
+int bemcs_rf_set(double* V_fixed, const int32_t* idx, const double* vdc, int64_t n, double v_rf,
+                 bemcs_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Batched step kernels: one launch serves n independent cases (meshes and

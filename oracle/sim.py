@@ -11,6 +11,14 @@ that the MI355X backend shares:
   formulas of ``oracle.kernels.mcc_prob_det64`` (``mcc_math="det64"``); the
   reference's own float32 formula is available as ``mcc_math="ref32"``.
 
+Electrode-surface physics (``sim_mode`` Thermal / Erosion / Both: impact
+heating, sputter damage and cell failure, cooling, conduction, thermal bow,
+:1359-1449 and :1563-1647) and the RF modulation (:984-991) follow
+``oracle.surface``.  Heat and damage are added one ion at a time in array
+order with ``np.add.at``, as the reference does; the device sums each node's
+increments exactly (fixed point, independent of particle order) and adds the
+sum once, which differs from this only by the rounding of the per-node adds.
+
 Additive options (absent keys give the reference's behaviour):
 ``mesh = {"nx", "ny"}`` fixes the node count; ``preload = {"ions", "electrons"}``
 seeds particles uniformly over non-electrode nodes (ions in x <= x_plume, where
@@ -32,6 +40,7 @@ from .kernels import (push_boris, deposit_counts, rho_from_counts, substeps,
                       mcc_prob_ref32)
 from .poisson import SuperLUPoisson, efield
 from .dmath import det_exp
+from . import surface
 
 F32 = np.float32
 
@@ -62,6 +71,30 @@ class OracleSimulator:
         self.transmitted_ions = 0.0
         self.entered_optics = 0.0
         self.injected_ions = 0.0
+        # electrode material and thermal constants (:321-349); the cell heat
+        # capacity and radiating area use dx, dy at set_material time (:450-452)
+        self.sb_sigma = 5.67e-8
+        self.thermal_accel = 1e7
+        self.set_material('Molybdenum')
+        self.T_grids = []
+        self.mask_grids = []
+        self.grid_deflections = []
+        self.Tmap = None
+
+    MATERIAL_PRESETS = {
+        'Molybdenum': {'k': 138.0, 'rho': 10280.0, 'cp': 250.0, 'emissivity': 0.80,
+                       'alpha': 4.8e-6, 'E_mod': 329e9, 'Y_coeff': 1.05e-4, 'E_th': 30.0},
+    }
+
+    def set_material(self, name=None, props=None):
+        mat = self.MATERIAL_PRESETS.get(name) if name else props
+        if mat is None:
+            return
+        self.mat_k, self.mat_rho, self.mat_cp = mat['k'], mat['rho'], mat['cp']
+        self.emissivity, self.alpha_thermal = mat['emissivity'], mat['alpha']
+        self.sputter_Y_coeff, self.sputter_E_th = mat['Y_coeff'], mat['E_th']
+        self.C_cell = self.mat_rho * (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3 * self.mat_cp
+        self.A_cell = 2 * (self.dx * 1e-3) * 1e-3
 
     # ------------------------------------------------------------------
     def _fdt(self):
@@ -163,6 +196,11 @@ class OracleSimulator:
             self.next_ion_pid = 0
             self.injected_ions = self.transmitted_ions = self.entered_optics = 0.0
             self.V = np.zeros((self.ny, self.nx))
+            self.rho = np.zeros((self.ny, self.nx), F32)
+            self.Tmap = np.full((self.ny, self.nx), 300.0, dtype=F32)
+            self.damage_map = np.zeros((self.ny, self.nx))
+            self.eroded_depth = np.zeros((self.ny, self.nx))
+        if len(self.grid_deflections) != len(grids):
             self.grid_deflections = [0.0] * len(grids)
         inj_time = params.get("inj_time", 0.0)
         self.injection_stop_time = inj_time if inj_time > 0.0 else None
@@ -172,6 +210,7 @@ class OracleSimulator:
         self.isBound = np.zeros((self.ny, self.nx), bool)
         self.V_fixed = np.zeros((self.ny, self.nx))
         self.mask_grids = []
+        self.T_grids = []
         if grids:
             sr = grids[0]['r']
             if geometry == 'two_holes':
@@ -209,13 +248,15 @@ class OracleSimulator:
             self.isBound[mask] = True
             self.V_fixed[mask] = g["V"]
             self.mask_grids.append(mask)
+            self.T_grids.append(float(np.mean(self.Tmap[mask])) if np.any(mask) else 300.0)
             cur = ge + g["gap"]
         self.Vdc = self.V_fixed.copy()
         vpb = (grids[0]["V"] if grids else 1000.0) + params.get("V_plasma_offset", 20.0)
         self.V_fixed[:, 0] = vpb
         self.isBound[:, 0] = True
+        if not preserve_state:
+            self.Tmap[self.isBound] = 300.0
         self.poisson = SuperLUPoisson(self.isBound, self.periodic_y)
-        self.rho = np.zeros((self.ny, self.nx), F32)
         self.recalc_poisson(30 if not preserve_state else 10, params)
         pre = params.get('preload')
         if pre and not preserve_state:
@@ -373,6 +414,18 @@ class OracleSimulator:
         self.entered_optics_step = 0.0
         self.injected_ions_step = 0.0
         self.lost_to_grid_step = 0.0
+        self.remeshed = False
+        sim_mode = params.get('sim_mode', 'Both')
+
+        # RF co-extraction (:984-991): the RF grid follows Vdc + A sin(2 pi f t)
+        if params.get('rf_enable') and grids:
+            rf_idx = params.get('rf_grid_idx', 0)
+            if rf_idx < len(grids):
+                v_rf = surface.rf_potential(params.get('rf_amp', 100.0), params.get('rf_freq', 13.56),
+                                            t_current)
+                m = self.mask_grids[rf_idx]
+                self.V_fixed[m] = self.Vdc[m] + v_rf
+                self.recalc_poisson(2, params)
 
         # A. injection
         self._inject(params, t_current)
@@ -405,16 +458,69 @@ class OracleSimulator:
                 self.p_y = np.mod(self.p_y, self.Ly)
             if self.num_e:
                 self.e_y = np.mod(self.e_y, self.Ly)
-        self._ion_diagnostics(params, grids, x_old, y_old)
+        self._ion_diagnostics(params, grids, x_old, y_old, sim_mode)
         self._electron_purge()
+        if sim_mode in ('Thermal', 'Both'):
+            self._thermal(params, grids)
         min_pot = self._saddle(grids)
         self._mcc(params)
         self._ppc()
         trans = (self.transmitted3_step / self.entered_optics_step) if self.entered_optics_step > 0 else 0.0
-        return False, min_pot, self.current_div, [], trans
+        return self.remeshed, min_pot, self.current_div, self.T_grids, trans
 
     # ------------------------------------------------------------------
-    def _ion_diagnostics(self, params, grids, x_old, y_old):
+    def _surface(self, params, sim_mode, hit, p_x, p_vx, p_vy, p_vz):
+        """Impact heating (:1359-1388) and sputter damage with cell failure
+        (:1419-1449) of the ions that hit an electrode with x > 0.25 mm."""
+        surf = hit & (p_x > 0.25)
+        self.surface_hits = int(np.count_nonzero(surf))
+        if not np.any(surf):
+            return
+        iy, ix = self.impact_iy[surf], self.impact_ix[surf]
+        vx, vy, vz = p_vx[surf], p_vy[surf], p_vz[surf]
+        if sim_mode in ('Thermal', 'Both'):
+            dT = surface.impact_heat(vx, vy, vz, self.m_ion, self.macro_weight, self.C_cell,
+                                     self.thermal_accel)
+            np.add.at(self.Tmap, (iy, ix), dT)
+        if sim_mode in ('Erosion', 'Both'):
+            valid, dmg = surface.sputter_damage(vx, vy, vz, self.m_ion, self.q, self.sputter_E_th,
+                                                self.sputter_Y_coeff, self.macro_weight)
+            if np.any(valid):
+                np.add.at(self.damage_map, (iy[valid], ix[valid]), dmg)
+            broken = (self.damage_map > params.get('Thresh', 1e5)) & self.isBound
+            if np.any(broken):
+                self.eroded_depth[broken] += self.dy
+                self.isBound[broken] = False
+                self.damage_map[broken] = 0.0
+                self.poisson = SuperLUPoisson(self.isBound, self.periodic_y)
+                self.remeshed = True
+
+    def _thermal(self, params, grids):
+        """Radiative cooling, 10 conduction sweeps, 300 K floor, grid means and
+        thermal bow with remesh (:1563-1647)."""
+        cf = surface.cooling_factor(self.emissivity, self.sb_sigma, self.A_cell, self.dt,
+                                    self.thermal_accel, self.C_cell)
+        surface.radiative_cooling(self.Tmap, self.isBound, cf)
+        Fo_x, Fo_y = surface.fourier_numbers(self.mat_k, self.mat_rho, self.mat_cp, self.dt,
+                                             self.thermal_accel, self.dx, self.dy)
+        self.Tmap = surface.conduction(self.Tmap, self.isBound, Fo_x, Fo_y, 10)
+        surface.clamp_bound(self.Tmap, self.isBound)
+        self.T_grids = surface.grid_means(self.Tmap, self.mask_grids, self.T_grids)
+        dc = params.get("discharge_chamber")
+        pitch = params.get("pitch_mm", dc.get("pitch_mm", 3.0) if isinstance(dc, dict) else 3.0)
+        remesh = False
+        for i, g in enumerate(grids):
+            d = surface.bow_deflection(self.T_grids[i], g, self.geometry, self.Ly, pitch,
+                                       self.alpha_thermal)
+            if abs(d - self.grid_deflections[i]) > 0.005:
+                self.grid_deflections[i] = d
+                remesh = True
+        if remesh:
+            self.build_domain(params, preserve_state=True)
+            self.remeshed = True
+
+    # ------------------------------------------------------------------
+    def _ion_diagnostics(self, params, grids, x_old, y_old, sim_mode='Trajectories'):
         p_x, p_y = self.p_x, self.p_y
         p_vx, p_vy, p_vz = self.p_vx, self.p_vy, self.p_vz
         if grids:
@@ -466,6 +572,7 @@ class OracleSimulator:
                 self._add_electrons(ex, ey, (z0 * vt).astype(F32), (z1 * vt).astype(F32),
                                     (z2 * vt).astype(F32))
                 self.see_count = ns
+        self._surface(params, sim_mode, hit, p_x, p_vx, p_vy, p_vz)
         exited = (p_x > x_exit) & (p_vx > 0.0)
         crossed = (x_old <= x_exit) & (p_x > x_exit) & (p_vx > 0.0)
         self.crossed_mask = crossed

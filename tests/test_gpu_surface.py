@@ -1,0 +1,336 @@
+"""GPU parity of the electrode-surface physics (sim_mode Thermal / Erosion /
+Both) and of the RF modulation against the oracle.
+
+Kernel tests call the C ABI on identical inputs.  Step tests feed the oracle
+the GPU's state before each step (particles, V, E, temperature, damage and
+erosion maps, electrode map) and compare after it: particles bit-exact on
+steps without a Poisson solve, electrode map and erosion depth exactly,
+temperatures within 1e-6 relative (the device sums each node's impact heat
+exactly and adds it once; the reference adds ion by ion in float32), damage
+within 1e-9 relative + 1e-6 absolute (2^-24 fixed-point increments).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import c1_params
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from oracle import surface as S  # noqa: E402
+from oracle.sim import OracleSimulator  # noqa: E402
+from oracle.poisson import SuperLUPoisson  # noqa: E402
+
+F32 = np.float32
+
+
+def _dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+# --------------------------------------------------------------- kernels
+def _thermal_case(nx, ny, seed):
+    r = np.random.default_rng(seed)
+    B = np.zeros((ny, nx), bool)
+    B[:, 0] = True
+    B[:, nx // 3: nx // 3 + 7] = True
+    B[ny // 4: ny // 4 + 5, nx // 2: nx - 3] = True
+    B[1:-1, 1:-1] |= r.random((ny - 2, nx - 2)) < 0.05
+    T = r.uniform(250.0, 3000.0, (ny, nx)).astype(F32)
+    heat = np.zeros((ny, nx), np.uint64)
+    hit = B & (r.random((ny, nx)) < 0.3)
+    dT = r.uniform(0.0, 5000.0, hit.sum()).astype(F32)
+    dT[: len(dT) // 4] = r.uniform(0.0, 8.0, len(dT) // 4).astype(F32)   # below 8 K: rounded
+    heat[hit] = S.fixed_point(dT, S.HEAT_FRACTION_BITS)
+    g0 = np.zeros((ny, nx), bool)
+    g0[:, nx // 3: nx // 3 + 7] = True
+    g1 = np.zeros((ny, nx), bool)
+    g1[ny // 4: ny // 4 + 5, nx // 2: nx - 3] = True
+    return T, B, heat, [g0, g1]
+
+
+def _thermal_oracle(T, B, heat, cf, fx, fy, sweeps=10):
+    """The kernel's contract restated: exact fixed-point heat added once,
+    then the reference's cooling, conduction and floor (oracle.surface)."""
+    T = T.copy()
+    h = heat != 0
+    T[h] = (T[h].astype(np.float64) + heat[h].astype(np.float64) * 2.0 ** -20).astype(F32)
+    S.radiative_cooling(T, B, cf)
+    T = S.conduction(T, B, fx, fy, sweeps)
+    return S.clamp_bound(T, B)
+
+
+def _run_thermal(lib, T, B, heat, masks, cf, fx, fy, sweeps=10):
+    from thruster_cx_code_amd import _lib
+    ny, nx = T.shape
+    bits = np.zeros((ny, nx), np.uint8)
+    for g, m in enumerate(masks):
+        bits |= (m.astype(np.uint8) << g)
+    Ti, To = _dev(T.ravel()), torch.zeros(nx * ny, dtype=torch.float32, device="cuda")
+    hd = _dev(heat.ravel().view(np.int64))
+    clear = torch.full((nx * ny,), 77, dtype=torch.int64, device="cuda")
+    bd, gd = _dev(B.astype(np.uint8).ravel()), _dev(bits.ravel())
+    ws = torch.zeros(int(lib.bemcs_thermal_workspace_bytes(nx, ny, len(masks))), dtype=torch.uint8,
+                     device="cuda")
+    tg = torch.full((len(masks),), -1.0, dtype=torch.float32, device="cuda")
+    p = _lib.Thermal()
+    p.T_in, p.T_out, p.heat_acc, p.heat_clear = _ptr(Ti), _ptr(To), _ptr(hd), _ptr(clear)
+    p.is_bound, p.grid_bits, p.workspace, p.T_grids = _ptr(bd), _ptr(gd), _ptr(ws), _ptr(tg)
+    p.cool32, p.t300_4_32 = float(F32(cf)), float(F32(300.0 ** 4))
+    p.fo_x32, p.fo_y32 = float(F32(fx)), float(F32(fy))
+    p.nx, p.ny, p.n_grids, p.sweeps = nx, ny, len(masks), sweeps
+    assert lib.bemcs_thermal(C.byref(p), _stream()) == 0
+    torch.cuda.synchronize()
+    assert int(clear.abs().sum().item()) == 0
+    return To.cpu().numpy().reshape(ny, nx), tg.cpu().numpy()
+
+
+@pytest.mark.parametrize("nx,ny,seed", [(150, 70, 0), (64, 32, 1), (37, 11, 2), (300, 130, 3)])
+def test_thermal_kernel_conduction_bit_exact(hip_lib, nx, ny, seed):
+    """Heat + conduction + floor with the cooling switched off: bit-identical
+    to the oracle on maps that are not multiples of the 64 x 32 tile."""
+    print(f"[gpu] thermal kernel {nx}x{ny} seed {seed}", flush=True)
+    T, B, heat, masks = _thermal_case(nx, ny, seed)
+    got, tg = _run_thermal(hip_lib, T, B, heat, masks, 0.0, 0.2, 0.137)
+    want = _thermal_oracle(T, B, heat, 0.0, 0.2, 0.137)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    for g, m in enumerate(masks):
+        mean = want[m].astype(np.float64).mean()
+        assert abs(tg[g] - mean) <= 1e-6 * mean
+
+
+def test_thermal_kernel_with_cooling(hip_lib):
+    """With radiative cooling the device evaluates T^4 in float64 (one
+    rounding to float32); NumPy's float32 power may differ by an ulp."""
+    T, B, heat, masks = _thermal_case(160, 72, 5)
+    C_cell = 10280.0 * (0.02e-3) ** 2 * 1e-3 * 250.0
+    cf = S.cooling_factor(0.8, 5.67e-8, 2 * 0.02e-3 * 1e-3, 5e-10, 1e7, C_cell)
+    got, _ = _run_thermal(hip_lib, T, B, heat, masks, cf, 0.2, 0.2)
+    want = _thermal_oracle(T, B, heat, cf, 0.2, 0.2)
+    rel = np.abs(got.astype(np.float64) - want) / np.abs(want.astype(np.float64))
+    print(f"  max rel diff {rel.max():.2e}; changed by cooling: {np.count_nonzero(got != T)}")
+    assert rel.max() < 1e-6
+    assert np.array_equal(got[~B], want[~B])
+
+
+def test_erosion_kernel(hip_lib):
+    from thruster_cx_code_amd import _lib
+    r = np.random.default_rng(7)
+    n = 5000
+    dmg = r.uniform(0.0, 120.0, n)
+    acc = np.where(r.random(n) < 0.3, r.integers(1, 2 ** 34, n), 0).astype(np.uint64)
+    bound = (r.random(n) < 0.6).astype(np.uint8)
+    depth = r.uniform(0.0, 0.1, n)
+    diag = torch.zeros(C.sizeof(_lib.IonDiag), dtype=torch.uint8, device="cuda")
+    nf = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_dmg, d_acc, d_b, d_dep = _dev(dmg), _dev(acc.view(np.int64)), _dev(bound), _dev(depth)
+    p = _lib.Erosion()
+    p.damage_acc, p.damage_map, p.eroded_depth, p.is_bound = _ptr(d_acc), _ptr(d_dmg), _ptr(d_dep), _ptr(d_b)
+    p.diag, p.n_failed, p.n_nodes, p.thresh, p.dy = _ptr(diag), _ptr(nf), n, 100.0, 0.0123
+    # no electrode hit this step: nothing changes
+    assert hip_lib.bemcs_erosion(C.byref(p), _stream()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(d_dmg.cpu().numpy(), dmg) and int(nf.item()) == 0
+    # with hits
+    off = _lib.IonDiag.surface_hits.offset
+    diag[off] = 3
+    assert hip_lib.bemcs_erosion(C.byref(p), _stream()) == 0
+    torch.cuda.synchronize()
+    want = dmg + acc.astype(np.float64) * 2.0 ** -24
+    fail = (want > 100.0) & (bound != 0)
+    want_depth = depth.copy()
+    want_depth[fail] += 0.0123
+    want[fail] = 0.0
+    assert np.array_equal(d_dmg.cpu().numpy(), want)
+    assert np.array_equal(d_dep.cpu().numpy(), want_depth)
+    assert np.array_equal(d_b.cpu().numpy(), np.where(fail, 0, bound))
+    assert int(nf.item()) == int(fail.sum()) > 0
+    assert int(d_acc.abs().sum().item()) == 0
+
+
+def test_rf_set(hip_lib):
+    V = torch.zeros(100, dtype=torch.float64, device="cuda")
+    idx = _dev(np.array([3, 50, 99], np.int32))
+    vdc = _dev(np.array([-200.0, -200.0, 10.0]))
+    assert hip_lib.bemcs_rf_set(_ptr(V), _ptr(idx), _ptr(vdc), 3, 12.5, _stream()) == 0
+    v = V.cpu().numpy()
+    assert v[3] == -187.5 and v[50] == -187.5 and v[99] == 22.5 and np.count_nonzero(v) == 3
+
+
+# ----------------------------------------------------------------- steps
+def _mk(params, setup=None):
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    g = DigitalTwinSimulator()
+    o = OracleSimulator()
+    for s in (g, o):
+        s.m_e = s.m_ion / 200.0
+        if setup:
+            setup(s)
+    g.build_domain(params)
+    o.build_domain(params)
+    return g, o
+
+
+def _sync(o, g):
+    tag = g._ion_host("tag").view(np.uint32)
+    o.p_x, o.p_y, o.p_vx, o.p_vy, o.p_vz = g.p_x, g.p_y, g.p_vx, g.p_vy, g.p_vz
+    o.p_pid = tag & np.uint32(0x7FFFFFFF)
+    o.p_isCEX = (tag >> 31).astype(bool)
+    o.e_x, o.e_y, o.e_vx, o.e_vy, o.e_vz = g.e_x, g.e_y, g.e_vx, g.e_vy, g.e_vz
+    o.V = g.V.copy()
+    o.Ex, o.Ey = g.Ex.copy(), g.Ey.copy()
+    o.iteration = g.iteration
+    o.next_ion_pid = g.next_ion_pid
+    o.Tmap = g.Tmap.copy()
+    o.damage_map = g.damage_map.copy()
+    o.eroded_depth = g.eroded_depth.copy()
+    o.T_grids = list(g.T_grids)
+    o.grid_deflections = list(g.grid_deflections)
+    if not np.array_equal(o.isBound, g.isBound):
+        o.isBound = g.isBound.copy()
+        o.poisson = SuperLUPoisson(o.isBound, o.periodic_y)
+
+
+def _ions(s):
+    order = np.argsort(s.p_pid, kind="stable")
+    return [a[order] for a in (s.p_x, s.p_y, s.p_vx, s.p_vy, s.p_vz, s.p_pid, s.p_isCEX)]
+
+
+def _electrons(s):
+    a = np.stack([s.e_x, s.e_y, s.e_vx, s.e_vy, s.e_vz]).astype(F32)
+    return a[:, np.lexsort(a[::-1])]
+
+
+def _compare_surface(g, o, what):
+    assert np.array_equal(g.isBound, o.isBound), what
+    assert np.array_equal(g.eroded_depth, o.eroded_depth), what
+    Tg, To = g.Tmap.astype(np.float64), o.Tmap.astype(np.float64)
+    assert np.max(np.abs(Tg - To) / np.abs(To)) < 1e-6, what
+    Dg, Do = g.damage_map, o.damage_map
+    assert np.all(np.abs(Dg - Do) <= 1e-9 * np.abs(Do) + 1e-6), what
+
+
+@pytest.mark.parametrize("mode", ["Thermal", "Erosion", "Both"])
+def test_surface_modes_step_parity(mode):
+    """Impact heat, sputter damage with cell failures, SEE purge after
+    erosion, cooling and conduction against the oracle, step by step."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=20000)
+    p["sim_mode"] = mode
+    p["Thresh"] = 1e-3         # any sputtering ion fails its cell
+
+    def low_threshold(s):      # preloaded ions hit with a few eV in the first steps
+        s.sputter_E_th = 1.0
+
+    g, o = _mk(p, low_threshold)
+    failed = hits = 0
+    for k in range(5):
+        _sync(o, g)
+        print(f"[gpu] {mode} step {g.iteration + 1}", flush=True)
+        rg = g.step(p)
+        ro = o.step(p)
+        rec = g.record()
+        even = g.iteration % 2 == 0
+        assert rec["surface_hits"] == o.surface_hits
+        hits += o.surface_hits
+        assert rg[0] == ro[0]
+        _compare_surface(g, o, (mode, k))
+        failed = int(np.count_nonzero(o.eroded_depth))
+        if not even:
+            gi, oi = _ions(g), _ions(o)
+            for a, b in zip(gi, oi):
+                assert np.array_equal(a, b)
+            assert np.array_equal(_electrons(g), _electrons(o))
+        if mode != "Erosion":
+            for a, b in zip(g.T_grids, o.T_grids):
+                assert abs(float(a) - float(b)) <= 1e-6 * float(b)
+    print(f"  electrode hits {hits}, failed cells {failed}")
+    assert hits > 0
+    if mode != "Thermal":
+        assert failed > 0
+
+
+def test_thermal_bow_remesh_parity():
+    """Grids preheated so that the first thermal update bows them past the
+    5 um remesh threshold: both sides rebuild the domain with the same
+    deflections and electrode map and re-solve the potential."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    p = c1_params(nx=128, ny=48, ions=5000, electrons=5000)
+    p["sim_mode"] = "Thermal"
+    g, o = _mk(p)
+    hot = o.isBound.copy()
+    hot[:, 0] = False
+    T0 = np.where(hot, F32(760.0), F32(300.0)).astype(F32)
+    g.Tmap_d[g._tcur].copy_(torch.from_numpy(T0.ravel()))
+    _sync(o, g)
+    rg = g.step(p)
+    ro = o.step(p)
+    print(f"[gpu] bow: deflections gpu {[float(d) for d in g.grid_deflections]} "
+          f"oracle {[float(d) for d in o.grid_deflections]}", flush=True)
+    assert rg[0] and ro[0]
+    for a, b in zip(g.grid_deflections, o.grid_deflections):
+        assert abs(float(a) - float(b)) <= 1e-6 * abs(float(b)) + 1e-9
+    assert np.array_equal(g.isBound, o.isBound)
+    err = np.max(np.abs(g.V - o.V)) / np.max(np.abs(o.V))
+    assert err < 1e-9, err
+    assert abs(rg[1] - ro[1]) <= 1e-9 * abs(ro[1])
+    _compare_surface(g, o, "bow")
+
+
+def test_rf_step_parity():
+    """RF grid potential Vdc + A sin(2 pi f t) and its two Picard iterations
+    on the previous step's deposit."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    p = c1_params(nx=128, ny=48, ions=8000, electrons=8000)
+    p.update(rf_enable=True, rf_grid_idx=1, rf_amp=150.0, rf_freq=13.56)
+    g, o = _mk(p)
+    for k in range(3):
+        _sync(o, g)
+        o.V_fixed = g.V_fixed.copy()
+        print(f"[gpu] rf step {g.iteration + 1}", flush=True)
+        g.step(p)
+        o.step(p)
+        assert np.array_equal(g.V_fixed, o.V_fixed)
+        err = np.max(np.abs(g.V - o.V)) / np.max(np.abs(o.V))
+        assert err < 1e-9, err
+        gi, oi = _ions(g), _ions(o)
+        assert np.array_equal(gi[5], oi[5])
+        for a, b in zip(gi[:5], oi[:5]):
+            assert np.max(np.abs(a - b)) <= 1e-6 * max(np.max(np.abs(b)), 1.0)
+
+
+def test_surface_modes_deterministic():
+    """Two runs with every surface effect and RF give identical bits
+    (fixed-point heat and damage sums do not depend on atomic order)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    p = c1_params(nx=128, ny=48, ions=8000, electrons=8000)
+    p.update(sim_mode="Both", Thresh=20.0, rf_enable=True, rf_grid_idx=1)
+    out = []
+    for _ in range(2):
+        g = DigitalTwinSimulator()
+        g.m_e = g.m_ion / 200
+        g.build_domain(p)
+        g.advance(p, 8)
+        out.append((g.V.copy(), g.Tmap.copy(), g.damage_map.copy(), g.isBound.copy(), _ions(g)))
+    for a, b in zip(out[0][:4], out[1][:4]):
+        assert np.array_equal(a, b)
+    for a, b in zip(out[0][4], out[1][4]):
+        assert np.array_equal(a, b)

@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -29,7 +29,7 @@ class Species(C.Structure):
 class IonDiag(C.Structure):
     _fields_ = [("entered", u32), ("crossed", u32), ("exited", u32), ("lost_to_grid", u32),
                 ("see_spawned", u32), ("see_kept", u32), ("mcc_events", u32),
-                ("crossed_primary", u32), ("overflow", u32), ("pad_", u32),
+                ("crossed_primary", u32), ("overflow", u32), ("surface_hits", u32),
                 ("sum_vx", f64), ("sum_vx2", f64), ("sum_v", f64), ("sum_v2", f64),
                 ("sum_E", f64), ("sum_v_primary", f64), ("sum_v_cex", f64)]
 
@@ -43,7 +43,10 @@ class IonStep(C.Structure):
                 ("dx32", f32), ("dy32", f32), ("x_entry32", f32), ("x_exit32", f32),
                 ("x_plume32", f32), ("Lx32", f32), ("Ly32", f32), ("dt32", f32),
                 ("nx", i32), ("ny", i32), ("periodic_y", i32), ("neutral_model", i32),
-                ("cx_model", i32), ("mcc_enabled", i32), ("seed", u32), ("step", u32)]
+                ("cx_model", i32), ("mcc_enabled", i32), ("seed", u32), ("step", u32),
+                ("heat_acc", vp), ("damage_acc", vp), ("see_stage", Species),
+                ("half_m32", f32), ("w32", f32), ("ccell32", f32), ("accel32", f32), ("q32", f32),
+                ("eth32", f32), ("ycoef32", f32), ("see_defer", i32)]
 
 
 class ElectronStep(C.Structure):
@@ -71,7 +74,7 @@ class Picard(C.Structure):
 class StepBegin(C.Structure):
     _fields_ = [("ds_bits", vp), ("count_out_i", vp), ("count_out_e", vp), ("node_zero", vp * 3),
                 ("n_nodes", i64), ("diag", vp), ("diag_bytes", i64), ("ppc_stats", vp),
-                ("count_in_i", vp), ("count_in_e", vp), ("nin_out", vp)]
+                ("count_in_i", vp), ("count_in_e", vp), ("nin_out", vp), ("count_out_x", vp)]
 
 
 class StepEnd(C.Structure):
@@ -104,6 +107,18 @@ class SortCase(C.Structure):
 
 class PpcCase(C.Structure):
     _fields_ = [("ppc", vp), ("n_nodes", i64), ("accum", vp), ("stats", vp)]
+
+
+class Erosion(C.Structure):
+    _fields_ = [("damage_acc", vp), ("damage_map", vp), ("eroded_depth", vp), ("is_bound", vp),
+                ("diag", vp), ("n_failed", vp), ("n_nodes", i64), ("thresh", f64), ("dy", f64)]
+
+
+class Thermal(C.Structure):
+    _fields_ = [("T_in", vp), ("T_out", vp), ("heat_acc", vp), ("heat_clear", vp), ("is_bound", vp),
+                ("grid_bits", vp), ("workspace", vp), ("T_grids", vp), ("cool32", f32),
+                ("t300_4_32", f32), ("fo_x32", f32), ("fo_y32", f32), ("nx", i32), ("ny", i32),
+                ("n_grids", i32), ("sweeps", i32)]
 
 
 class PercentileCase(C.Structure):
@@ -144,6 +159,12 @@ _SIGS = {
     "bemcs_ppc_stats": (i32, [vp, i64, vp, vp, vp]),
     "bemcs_step_begin": (i32, [vp, vp]),
     "bemcs_step_end": (i32, [vp, vp]),
+    "bemcs_erosion": (i32, [vp, vp]),
+    "bemcs_see_commit": (i32, [vp, vp, vp, vp, vp, i32, i32, C.c_float, C.c_float, C.c_float,
+                               C.c_float, i64, vp]),
+    "bemcs_thermal_workspace_bytes": (i64, [i32, i32, i32]),
+    "bemcs_thermal": (i32, [vp, vp]),
+    "bemcs_rf_set": (i32, [vp, vp, vp, i64, f64, vp]),
     "bemcs_ion_step_batched": (i32, [vp, i32, i64, vp, vp]),
     "bemcs_electron_step_batched": (i32, [vp, i32, i64, vp, vp]),
     "bemcs_substeps_batched": (i32, [vp, i32, i64, vp]),

@@ -49,7 +49,9 @@ class _ArgRing:
         if off + size > self.host[0].numel():
             raise RuntimeError("batch argument buffer too small")
         self.off = off + size
-        arr = (ctype * n).from_address(self.host[self.slot].data_ptr() + off)
+        addr = self.host[self.slot].data_ptr() + off
+        C.memset(addr, 0, size)  # fields a caller does not set are zero (null pointers)
+        arr = (ctype * n).from_address(addr)
         return arr, C.c_void_p(self.dev.data_ptr() + off)
 
     def commit(self):
@@ -93,6 +95,10 @@ class BatchedSimulator:
         """Admit a built case (between steps)."""
         if not sim._built:
             raise ValueError("every case must be built (build_domain) first")
+        if params.get('sim_mode', 'Both') != 'Trajectories' or params.get('rf_enable'):
+            raise ValueError("batched cases run sim_mode 'Trajectories' without RF; step thermal, "
+                             "erosion and RF cases with DigitalTwinSimulator.step/advance")
+        sim._resolve_surface()
         if self.device is None:
             self.device = sim.device
         sim._prepare_step_params(params)
@@ -230,6 +236,7 @@ class BatchedSimulator:
             b.ppc_stats = sim.ring_ppc.data_ptr() + 8 * slot
             b.count_in_i, b.count_in_e = I0.count.data_ptr(), E0.count.data_ptr()
             b.nin_out = sim.ring_nin.data_ptr() + 8 * slot
+            b.count_out_x = None
             for k, (sp, qm, off) in enumerate(((I0, sim.q_ion / sim.m_ion, 0), (E0, -sim.q / sim.m_e, 8))):
                 u = sub[2 * c + k]
                 u.x, u.y, u.vx, u.vy, u.vz = (sp.x.data_ptr(), sp.y.data_ptr(), sp.vx.data_ptr(),

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libbemcs_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["bemcs_particles.hip", "bemcs_poisson.hip", "bemcs_diag.hip"]
+SOURCES = ["bemcs_particles.hip", "bemcs_poisson.hip", "bemcs_diag.hip", "bemcs_surface.hip"]
 ARCH = os.environ.get("BEMCS_OFFLOAD_ARCH", "gfx950")
 
 

@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void k_step_begin(bemcs_step_begin_t one,
     p.ppc_stats[1] = 0u;
     p.nin_out[0] = *p.count_in_i;
     p.nin_out[1] = *p.count_in_e;
+    if (p.count_out_x) *p.count_out_x = 0;
   }
 }
 

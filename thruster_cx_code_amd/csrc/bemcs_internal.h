@@ -174,6 +174,20 @@ __device__ __forceinline__ void wave_hist_add(uint32_t* hist, int key) {
   }
 }
 
+// Adds a non-negative float32 value to a 64-bit fixed-point accumulator as
+// rint(v * scale).  Integer sums do not depend on the order of the adds, so
+// per-node totals are deterministic.  Values that are not finite, negative or
+// beyond 2^53 units set bit 32 of *flags instead of being added.
+__device__ __forceinline__ void fixed_add(uint64_t* acc, float v, double scale, uint32_t* flags) {
+  double s = (double)v * scale;
+  if (!(s >= 0.0 && s <= 9007199254740992.0)) {
+    atomicOr(flags, 32u);
+    return;
+  }
+  unsigned long long q = (unsigned long long)rint(s);
+  if (q) atomicAdd((unsigned long long*)acc, q);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;

@@ -278,12 +278,18 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
 
   // ---- D. hits, OOB, diagnostics
+  // The impact node is the first electrode node along the segment, else the
+  // final node (:1329-1335); the segment is walked for a final-node hit only
+  // when the impact node is needed (heat or damage).
+  const bool SURF = p.heat_acc != nullptr || p.damage_acc != nullptr;
   bool hit = false, oob = false;
+  int impact = 0;
   if (active) {
     int ix = round_i64_np(div_rn<FAST>(x, ddx), 0, nx - 1);
     int iy = round_i64_np(div_rn<FAST>(y, ddy), 0, ny - 1);
-    hit = bound[iy * nx + ix] != 0;
-    if (!hit) {
+    const bool hit_final = bound[iy * nx + ix] != 0;
+    impact = iy * nx + ix;
+    if (!hit_final || SURF) {
       for (int k = 1; k <= 8; ++k) {
         float f = (float)k / 8.0f;
         float xi = x0 + (x - x0) * f;
@@ -292,10 +298,12 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
         int cy = round_i64_np(div_rn<FAST>(yi, ddy), 0, ny - 1);
         if (bound[cy * nx + cx]) {
           hit = true;
+          impact = cy * nx + cx;
           break;
         }
       }
     }
+    hit = hit || hit_final;
     oob = (x < 0.0f) || (x > p.x_plume32) || isnan(x);
     if (!PERIODIC) oob = oob || (y < 0.0f) || (y > p.Ly32);
   }
@@ -303,11 +311,32 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
   wave_count(&p.diag->entered, entered);
   wave_count(&p.diag->lost_to_grid, active && hit && !oob);
 
+  const bool surf = active && hit && x > 0.25f;
+  wave_count(&p.diag->surface_hits, surf);
+  // impact heating (:1359-1388) and sputter damage (:1419-1438), float32 as
+  // the reference evaluates them, summed per node in 64-bit fixed point
+  if (surf && SURF) {
+    float v2 = vx * vx + vy * vy;
+    v2 = v2 + vz * vz;
+    if (p.heat_acc) {
+      float ej = (p.half_m32 * v2) * p.w32;
+      float dT = (ej / p.ccell32) * p.accel32;
+      fixed_add(p.heat_acc + impact, dT, 1048576.0, &p.diag->overflow);
+    }
+    if (p.damage_acc) {
+      float E = (p.half_m32 * v2) / p.q32;
+      if (E > p.eth32) {
+        float dmg = (p.ycoef32 * (E - p.eth32)) * p.w32;
+        fixed_add(p.damage_acc + impact, dmg, 16777216.0, &p.diag->overflow);
+      }
+    }
+  }
+
   // SEE: electrons spawned by grid hits (:1390-1417), purge-checked (:1541-1551)
   {
     bool spawn = false;
     float ex = 0.f, ey = 0.f, evx = 0.f, evy = 0.f, evz = 0.f;
-    if (active && hit && x > 0.25f) {
+    if (surf) {
       double a = vx, b = vy, c = vz;
       double v2 = a * a + b * b + c * c;
       double E_eV = (0.5 * p.m_ion * v2) / p.q;
@@ -327,6 +356,23 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
       }
     }
     wave_count(&p.diag->see_spawned, spawn);
+    if (p.see_defer) {
+      // staged unpurged; bemcs_see_commit purges them after this step's erosion
+      const bemcs_species_t& st = p.see_stage;
+      int64_t slot = wave_append(st.count, spawn);
+      if (spawn) {
+        if (slot < st.cap) {
+          st.x[slot] = ex;
+          st.y[slot] = ey;
+          st.vx[slot] = evx;
+          st.vy[slot] = evy;
+          st.vz[slot] = evz;
+        } else {
+          atomicOr(&p.diag->overflow, 1u);
+        }
+      }
+      spawn = false;
+    }
     bool keep_e = false;
     int ekey = -1;
     if (spawn) {

@@ -78,7 +78,6 @@ class DigitalTwinSimulator:
         self.m_e = self.m_XE / 100.0
         self.user_cs = {}
         self.MATERIAL_PRESETS = MATERIAL_PRESETS
-        self.set_material('Molybdenum')
         self.macro_weight = 3e5
         self.sb_sigma = 5.67e-8
         self.thermal_accel = 1e7
@@ -88,6 +87,9 @@ class DigitalTwinSimulator:
         self.ny = int(self.Ly / self.dy) + 1
         self.dx = self.Lx / (self.nx - 1)
         self.dy = self.Ly / (self.ny - 1)
+        # the cell heat capacity and radiating area use dx, dy at the time
+        # set_material runs (:450-452, :473), not the built mesh's
+        self.set_material('Molybdenum')
         self.T_grids = []
         self.mask_grids = []
         self.iteration = 0
@@ -116,7 +118,8 @@ class DigitalTwinSimulator:
         self.emissivity, self.alpha_thermal = mat['emissivity'], mat['alpha']
         self.E_modulus = mat['E_mod']
         self.sputter_Y_coeff, self.sputter_E_th = mat['Y_coeff'], mat['E_th']
-        self.C_cell = self.mat_rho * (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3 * self.mat_cp if hasattr(self, 'dx') else None
+        self.C_cell = self.mat_rho * (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3 * self.mat_cp
+        self.A_cell = 2 * (self.dx * 1e-3) * 1e-3
 
     def __del__(self):
         try:
@@ -181,14 +184,10 @@ class DigitalTwinSimulator:
                 f"Grid spacing and time step too large for velocity resolution: "
                 f"dx/dt={self.dx * 1e-3 / self.dt:.2e} m/s, dy/dt={self.dy * 1e-3 / self.dt:.2e} m/s, "
                 f"vmax={vmax:.2e} m/s.")
-        self.set_material(props={'k': self.mat_k, 'rho': self.mat_rho, 'cp': self.mat_cp,
-                                 'emissivity': self.emissivity, 'alpha': self.alpha_thermal,
-                                 'E_mod': self.E_modulus, 'Y_coeff': self.sputter_Y_coeff,
-                                 'E_th': self.sputter_E_th})
         self.xpts = np.linspace(0, self.Lx, self.nx)
         self.ypts = np.linspace(0, self.Ly, self.ny)
         X, Y = np.meshgrid(self.xpts, self.ypts)
-        if not preserve_state or not hasattr(self, 'grid_deflections'):
+        if len(getattr(self, 'grid_deflections', [])) != len(grids):
             self.grid_deflections = [0.0] * len(grids)
         inj_time = params.get("inj_time", 0.0)
         self.injection_stop_time = inj_time if inj_time > 0.0 else None
@@ -233,30 +232,97 @@ class DigitalTwinSimulator:
             isBound[mask] = True
             V_fixed[mask] = g["V"]
             self.mask_grids.append(mask)
-            self.T_grids.append(300.0)
             cur = ge + g["gap"]
         self.Vdc = V_fixed.copy()
         vpb = (grids[0]["V"] if grids else 1000.0) + params.get("V_plasma_offset", 20.0)
         V_fixed[:, 0] = vpb
         isBound[:, 0] = True
+        if len(grids) > 8 and params.get('sim_mode', 'Both') != 'Trajectories':
+            raise ValueError("thermal and erosion modes support at most 8 grids")
         self._isBound_host = isBound
         self._V_fixed_host = V_fixed
         self._grids = grids
+        self._rf_nodes = None
         self._alloc_mesh(preserve_state)
         if not preserve_state:
             self._alloc_particles(params)
+        # grid temperatures over the (new) masks (:830-833)
+        Tm = self.Tmap
+        self.T_grids = [float(np.mean(Tm[m])) if np.any(m) else 300.0 for m in self.mask_grids]
+        if self.mask_grids:
+            self.T_grids_d.copy_(torch.tensor(self.T_grids, dtype=torch.float32))
         self._set_geometry_constants(params)
         self._built = True
-        self.recalc_poisson(30 if not preserve_state else 10, params)
+        # the last step's deposit (none before the first step)
+        counts = (self.cnt_i_next, self.cnt_e_next) if preserve_state else None
+        self.recalc_poisson(30 if not preserve_state else 10, params, counts=counts)
         pre = params.get('preload')
         if pre and not preserve_state:
             self._preload(params, int(pre.get('ions', 0)), int(pre.get('electrons', 0)))
             self._sort(ions=True, electrons=True)
 
     def _alloc_mesh(self, preserve_state):
+        """Device mesh state.  A rebuild with preserve_state (thermal bow,
+        :1645) keeps the potential, deposit counts, temperature, damage and
+        erosion maps and the per-step records; a fresh build clears them as
+        reset_arrays does (:404-448)."""
         dev = self.device
         nn = self.nx * self.ny
+        keep = preserve_state and getattr(self, '_nn', None) == nn
+        self._rebuild_operator()
+        self.is_bound_d = torch.from_numpy(self._isBound_host.astype(np.uint8).ravel()).to(dev)
+        self.V_fixed_d = torch.from_numpy(self._V_fixed_host.ravel().copy()).to(dev)
+        bits = np.zeros(nn, np.uint8)
+        for g, m in enumerate(self.mask_grids[:8]):
+            bits |= (m.ravel().astype(np.uint8) << g)
+        self.grid_bits_d = torch.from_numpy(bits).to(dev)
+        ng = len(self.mask_grids)
+        self.T_grids_d = torch.full((max(ng, 1),), 300.0, dtype=torch.float32, device=dev)
+        wsb = int(self.lib.bemcs_thermal_workspace_bytes(self.nx, self.ny, min(ng, 8)))
+        self.thermal_ws = zeros(max(wsb, 8), torch.uint8, dev)
+        if not keep:
+            self.V_d = zeros(nn, torch.float64, dev)
+            self.cnt_i = zeros(nn, torch.int32, dev)
+            self.cnt_e = zeros(nn, torch.int32, dev)
+            self.cnt_i_next = zeros(nn, torch.int32, dev)
+            self.cnt_e_next = zeros(nn, torch.int32, dev)
+            self.ppc_d = zeros(nn, torch.int32, dev)
+            self.accum_ppc_d = zeros(nn, torch.float64, dev)
+            # electrode temperature (ping-pong), impact heat (ping-pong, 2^-20 K
+            # fixed point), sputter damage (2^-24 fixed point), maps (:383-386, :428-429)
+            self.Tmap_d = [torch.full((nn,), 300.0, dtype=torch.float32, device=dev) for _ in range(2)]
+            self.heat_acc = [zeros(nn, torch.int64, dev) for _ in range(2)]
+            self.damage_acc = zeros(nn, torch.int64, dev)
+            self.damage_map_d = zeros(nn, torch.float64, dev)
+            self.eroded_depth_d = zeros(nn, torch.float64, dev)
+            self.n_failed_d = zeros(1, torch.int32, dev)
+            self._n_failed_seen = 0
+            self._last_thermal_it = None
+            self._tcur = 0
+        self.Exy = zeros(2 * nn, torch.float32, dev)
+        self.rho_d = zeros(nn, torch.float32, dev)
+        self.sort_scratch = zeros(nn + (nn + 1023) // 1024 + 1024, torch.int32, dev)
+        if not (preserve_state and hasattr(self, 'ring_diag')):
+            self.ds_bits = zeros(2, torch.int64, dev)
+            self.n_sub_d = torch.ones(2, dtype=torch.int32, device=dev)
+            self.overflow_d = zeros(1, torch.int32, dev)
+            sz = C.sizeof(_lib.IonDiag)
+            self.ring_diag = zeros(RING * sz, torch.uint8, dev)
+            self.ring_theta = zeros(RING, torch.float64, dev)
+            self.ring_minpot = zeros(RING, torch.float64, dev)
+            self.ring_ppc = zeros(2 * RING, torch.int32, dev)
+            self.ring_counts = zeros(2 * RING, torch.int32, dev)
+            self.ring_nsub = zeros(2 * RING, torch.int32, dev)
+            self.ring_nin = zeros(2 * RING, torch.int32, dev)
+            self.ring_remeshed = np.zeros(RING, bool)
+            self._surface_pending = None
+        self._nn = nn
+
+    def _rebuild_operator(self):
+        """Poisson operator of the current electrode map (build_sparse_matrix,
+        :525-626): a new MG hierarchy."""
         if self._poisson:
+            torch.cuda.current_stream().synchronize()
             self.lib.bemcs_poisson_free(self._poisson)
             self._poisson = None
         h = C.c_void_p()
@@ -264,32 +330,6 @@ class DigitalTwinSimulator:
         check(self.lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
                                             int(self.periodic_y), C.byref(h)), "poisson_create")
         self._poisson = h
-        self.is_bound_d = torch.from_numpy(ib.ravel()).to(dev)
-        self.V_fixed_d = torch.from_numpy(self._V_fixed_host.ravel().copy()).to(dev)
-        keep_V = preserve_state and hasattr(self, 'V_d') and self.V_d.numel() == nn
-        if not keep_V:
-            self.V_d = zeros(nn, torch.float64, dev)
-        self.Exy = zeros(2 * nn, torch.float32, dev)
-        self.rho_d = zeros(nn, torch.float32, dev)
-        if not preserve_state or not hasattr(self, 'cnt_i') or self.cnt_i.numel() != nn:
-            self.cnt_i = zeros(nn, torch.int32, dev)
-            self.cnt_e = zeros(nn, torch.int32, dev)
-            self.cnt_i_next = zeros(nn, torch.int32, dev)
-            self.cnt_e_next = zeros(nn, torch.int32, dev)
-            self.ppc_d = zeros(nn, torch.int32, dev)
-            self.accum_ppc_d = zeros(nn, torch.float64, dev)
-        self.sort_scratch = zeros(nn + (nn + 1023) // 1024 + 1024, torch.int32, dev)
-        self.ds_bits = zeros(2, torch.int64, dev)
-        self.n_sub_d = torch.ones(2, dtype=torch.int32, device=dev)
-        self.overflow_d = zeros(1, torch.int32, dev)
-        sz = C.sizeof(_lib.IonDiag)
-        self.ring_diag = zeros(RING * sz, torch.uint8, dev)
-        self.ring_theta = zeros(RING, torch.float64, dev)
-        self.ring_minpot = zeros(RING, torch.float64, dev)
-        self.ring_ppc = zeros(2 * RING, torch.int32, dev)
-        self.ring_counts = zeros(2 * RING, torch.int32, dev)
-        self.ring_nsub = zeros(2 * RING, torch.int32, dev)
-        self.ring_nin = zeros(2 * RING, torch.int32, dev)
 
     def _alloc_particles(self, params):
         pre = params.get('preload') or {}
@@ -334,11 +374,13 @@ class DigitalTwinSimulator:
         self._saddle_idx = torch.tensor(idx, dtype=torch.int64, device=self.device)
 
     # ------------------------------------------------------------ Poisson
-    def recalc_poisson(self, iterations=5, params=None):
-        """Picard loop with Boltzmann electrons and E field (reference :850-902)."""
+    def recalc_poisson(self, iterations=5, params=None, counts=None):
+        """Picard loop with Boltzmann electrons and E field (reference :850-902).
+        rho comes from the NGP counts `counts` (ions, electrons), by default
+        the current deposit."""
         if not self._built:
             return
-        p = self._picard_params(iterations, params)
+        p = self._picard_params(iterations, params, counts)
         its = C.c_int32(0)
         rel = C.c_double(0.0)
         check(self.lib.bemcs_poisson_picard(self._poisson, C.byref(p), C.byref(its), C.byref(rel),
@@ -347,13 +389,14 @@ class DigitalTwinSimulator:
         self.last_poisson_relres = rel.value
         self.poisson_iter_log.append(its.value)
 
-    def _picard_params(self, iterations, params):
+    def _picard_params(self, iterations, params, counts=None):
         params = params or {}
         grids = params.get('grids', [{'V': 1000}])
         cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
         cd = self.q * self.macro_weight / cell_vol
         p = _lib.Picard()
-        p.cnt_ion, p.cnt_e = ptr(self.cnt_i), ptr(self.cnt_e)
+        ci, ce = counts if counts is not None else (self.cnt_i, self.cnt_e)
+        p.cnt_ion, p.cnt_e = ptr(ci), ptr(ce)
         p.V_fixed, p.V = ptr(self.V_fixed_d), ptr(self.V_d)
         p.rho_out, p.Exy = ptr(self.rho_d), ptr(self.Exy)
         p.cd_ion, p.cd_e = cd, -cd
@@ -545,12 +588,21 @@ class DigitalTwinSimulator:
             self._be.refresh(int(cnts[1]), ge)
 
     def _one_step(self, params):
+        self._resolve_surface()
         self.iteration += 1
         it = self.iteration
         t_current = it * self.dt
         s = stream_ptr()
         lib = self.lib
+        grids = params.get('grids', [])
+        mode = params.get('sim_mode', 'Both')
+        thermal = mode in ('Thermal', 'Both')
+        erosion = mode in ('Erosion', 'Both')
         self._refresh_bounds()
+        # RF co-extraction (:984-991): the RF grid at Vdc + A sin(2 pi f t), then
+        # two Picard iterations on the previous step's deposit
+        if params.get('rf_enable') and grids and params.get('rf_grid_idx', 0) < len(grids):
+            self._rf_update(params, t_current)
         self._inject(params, t_current)
         if it % 2 == 0:
             kt = self.kernel_timing
@@ -563,8 +615,11 @@ class DigitalTwinSimulator:
         nsb = self.n_sub_d.data_ptr()
         # clear this step's outputs and record the pushed populations
         slot = it % RING
+        self.ring_remeshed[slot] = False
         dsz = C.sizeof(_lib.IonDiag)
         diag_ptr = self.ring_diag.data_ptr() + slot * dsz
+        if erosion:
+            self._ensure_see_stage()
         sb = _lib.StepBegin()
         sb.ds_bits, sb.count_out_i, sb.count_out_e = dsb, I1.count.data_ptr(), E1.count.data_ptr()
         sb.node_zero[0], sb.node_zero[1], sb.node_zero[2] = (self.cnt_i_next.data_ptr(),
@@ -573,6 +628,7 @@ class DigitalTwinSimulator:
         sb.ppc_stats = self.ring_ppc.data_ptr() + 8 * slot
         sb.count_in_i, sb.count_in_e = I0.count.data_ptr(), E0.count.data_ptr()
         sb.nin_out = self.ring_nin.data_ptr() + 8 * slot
+        sb.count_out_x = self.see_stage.count.data_ptr() if erosion else None
         check(lib.bemcs_step_begin(C.byref(sb), s), "step_begin")
         # substep counts for both species (:1177-1209)
         check(lib.bemcs_substeps(ptr(I0.x), ptr(I0.y), ptr(I0.vx), ptr(I0.vy), ptr(I0.vz), ptr(I0.count),
@@ -588,11 +644,14 @@ class DigitalTwinSimulator:
         ip.cnt_next, ip.ppc = ptr(self.cnt_i_next), ptr(self.ppc_d)
         ip.crossed_angle, ip.diag, ip.e_cnt_next = ptr(self.crossed_angle), C.c_void_p(diag_ptr), ptr(self.cnt_e_next)
         ip.step = it & 0xFFFFFFFF
+        self._surface_params(ip, it, thermal, erosion)
         kt = self.kernel_timing
         ev = self._timed_begin(kt)
         check(lib.bemcs_ion_step(C.byref(I0.struct()), C.byref(I1.struct()), C.byref(E1.struct()),
                                  C.byref(ip), self.bound_i, s), "ion_step")
         self._timed_end(kt, "ion_step", ev)
+        if erosion:
+            self._erosion(params, diag_ptr, E1)
         ep = self._elec_params
         ep.Exy, ep.is_bound, ep.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb + 4)
         ep.cnt_next, ep.overflow = ptr(self.cnt_e_next), ptr(self.overflow_d)
@@ -605,19 +664,25 @@ class DigitalTwinSimulator:
         self.elec.reverse()
         self.cnt_i, self.cnt_i_next = self.cnt_i_next, self.cnt_i
         self.cnt_e, self.cnt_e_next = self.cnt_e_next, self.cnt_e
+        if thermal:
+            self._thermal(it)
+        if thermal or erosion:
+            # failed-cell total and grid temperatures, read back for the
+            # remesh decisions (resolved before the next step)
+            hb = torch.empty(1 + len(self.mask_grids), dtype=torch.float32, pin_memory=True)
+            hb[:1].view(torch.int32).copy_(self.n_failed_d, non_blocking=True)
+            if self.mask_grids:
+                hb[1:].copy_(self.T_grids_d[:len(self.mask_grids)], non_blocking=True)
+            sev = torch.cuda.Event()
+            sev.record()
+            self._surface_pending = (sev, hb, slot, params, thermal, erosion)
         # per-step records
         check(lib.bemcs_ppc_stats(ptr(self.ppc_d), self.nx * self.ny, ptr(self.accum_ppc_d),
                                   C.c_void_p(self.ring_ppc.data_ptr() + 8 * slot), s), "ppc_stats")
         self.ppc_steps_count += 1
         check(lib.bemcs_percentile(ptr(self.crossed_angle), C.c_void_p(diag_ptr + 4), self.bound_i, 95.0, 5,
                                    None, C.c_void_p(self.ring_theta.data_ptr() + 8 * slot), s), "percentile")
-        se = _lib.StepEnd()
-        se.count_i, se.count_e = self.ions[0].count.data_ptr(), self.elec[0].count.data_ptr()
-        se.n_sub, se.counts_out = nsb, self.ring_counts.data_ptr() + 8 * slot
-        se.nsub_out = self.ring_nsub.data_ptr() + 8 * slot
-        se.V, se.saddle_idx, se.n_saddle = self.V_d.data_ptr(), self._saddle_idx.data_ptr(), self._saddle_idx.numel()
-        se.minpot_out = self.ring_minpot.data_ptr() + 8 * slot
-        check(lib.bemcs_step_end(C.byref(se), s), "step_end")
+        self._step_end(slot)
         self._sort(ions=self.sort_interval > 0 and it % self.sort_interval == 0,
                    electrons=self.electron_sort_interval > 0 and it % self.electron_sort_interval == 0)
         # asynchronous count readback for the host bounds
@@ -627,6 +692,152 @@ class DigitalTwinSimulator:
         ev.record()
         self._pending.append((ev, cnts, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
+
+    def _step_end(self, slot):
+        se = _lib.StepEnd()
+        se.count_i, se.count_e = self.ions[0].count.data_ptr(), self.elec[0].count.data_ptr()
+        se.n_sub, se.counts_out = self.n_sub_d.data_ptr(), self.ring_counts.data_ptr() + 8 * slot
+        se.nsub_out = self.ring_nsub.data_ptr() + 8 * slot
+        se.V, se.saddle_idx, se.n_saddle = self.V_d.data_ptr(), self._saddle_idx.data_ptr(), self._saddle_idx.numel()
+        se.minpot_out = self.ring_minpot.data_ptr() + 8 * slot
+        check(self.lib.bemcs_step_end(C.byref(se), stream_ptr()), "step_end")
+
+    # --------------------------------------------- electrode-surface physics
+    def _surface_params(self, ip, it, thermal, erosion):
+        """Impact heat / damage accumulators and SEE staging of the ion step."""
+        f = lambda v: float(np.float32(v))
+        ip.heat_acc = None
+        ip.damage_acc = None
+        ip.see_defer = 0
+        if thermal:
+            if self._last_thermal_it != it - 1:
+                # accumulators may hold heat of a step whose thermal update was skipped
+                self.heat_acc[0].zero_()
+                self.heat_acc[1].zero_()
+            ip.heat_acc = ptr(self.heat_acc[it & 1])
+        if erosion:
+            ip.damage_acc = ptr(self.damage_acc)
+            ip.see_defer = 1
+            ip.see_stage = self.see_stage.struct()
+        ip.half_m32, ip.w32, ip.q32 = f(0.5 * self.m_ion), f(self.macro_weight), f(self.q)
+        ip.ccell32, ip.accel32 = f(self.C_cell), f(self.thermal_accel)
+        ip.eth32, ip.ycoef32 = f(self.sputter_E_th), f(self.sputter_Y_coeff)
+
+    def _ensure_see_stage(self):
+        cap = self.ions[0].cap
+        st = getattr(self, 'see_stage', None)
+        if st is None or st.cap < cap:
+            self.see_stage = SpeciesBuffers(cap, self.device, False)
+
+    def _erosion(self, params, diag_ptr, E1):
+        """Sputter damage and cell failure (:1419-1449), then the purge of this
+        step's SEE electrons against the eroded map (:1534-1561)."""
+        e = _lib.Erosion()
+        e.damage_acc, e.damage_map = ptr(self.damage_acc), ptr(self.damage_map_d)
+        e.eroded_depth, e.is_bound = ptr(self.eroded_depth_d), ptr(self.is_bound_d)
+        e.diag, e.n_failed = C.c_void_p(diag_ptr), ptr(self.n_failed_d)
+        e.n_nodes = self.nx * self.ny
+        e.thresh = float(params.get('Thresh', 1e5))
+        e.dy = self.dy
+        s = stream_ptr()
+        check(self.lib.bemcs_erosion(C.byref(e), s), "erosion")
+        f = lambda v: float(np.float32(v))
+        check(self.lib.bemcs_see_commit(C.byref(self.see_stage.struct()), C.byref(E1.struct()),
+                                        ptr(self.is_bound_d), ptr(self.cnt_e_next), C.c_void_p(diag_ptr),
+                                        self.nx, self.ny, f(self.dx), f(self.dy), f(self.Lx), f(self.Ly),
+                                        self.bound_i, s), "see_commit")
+
+    def _thermal(self, it):
+        """Impact heat, radiative cooling, 10 conduction sweeps, 300 K floor and
+        grid means (:1563-1617) in one pass (bemcs_thermal)."""
+        cf = (self.emissivity * self.sb_sigma * self.A_cell * self.dt * self.thermal_accel) / self.C_cell
+        alpha = self.mat_k / (self.mat_rho * self.mat_cp)
+        dt_th = self.dt * self.thermal_accel
+        fo_x = alpha * dt_th / (self.dx * 1e-3) ** 2
+        fo_y = alpha * dt_th / (self.dy * 1e-3) ** 2
+        if fo_x > 0.2 or fo_y > 0.2:
+            sc = 0.2 / max(fo_x, fo_y)
+            fo_x *= sc
+            fo_y *= sc
+        f = lambda v: float(np.float32(v))
+        t = _lib.Thermal()
+        t.T_in, t.T_out = ptr(self.Tmap_d[self._tcur]), ptr(self.Tmap_d[1 - self._tcur])
+        t.heat_acc, t.heat_clear = ptr(self.heat_acc[it & 1]), ptr(self.heat_acc[(it + 1) & 1])
+        t.is_bound, t.grid_bits = ptr(self.is_bound_d), ptr(self.grid_bits_d)
+        t.workspace, t.T_grids = ptr(self.thermal_ws), ptr(self.T_grids_d)
+        t.cool32, t.t300_4_32 = f(cf), f(300.0 ** 4)
+        t.fo_x32, t.fo_y32 = f(fo_x), f(fo_y)
+        t.nx, t.ny, t.n_grids, t.sweeps = self.nx, self.ny, min(len(self.mask_grids), 8), 10
+        check(self.lib.bemcs_thermal(C.byref(t), stream_ptr()), "thermal")
+        self._tcur = 1 - self._tcur
+        self._last_thermal_it = it
+
+    def _rf_update(self, params, t_current):
+        rf_idx = params.get('rf_grid_idx', 0)
+        if self._rf_nodes is None or self._rf_nodes[0] != rf_idx:
+            idx = np.flatnonzero(self.mask_grids[rf_idx].ravel()).astype(np.int32)
+            vdc = self.Vdc.ravel()[idx].astype(np.float64)
+            self._rf_nodes = (rf_idx, torch.from_numpy(idx).to(self.device),
+                              torch.from_numpy(vdc).to(self.device))
+        _, idx_d, vdc_d = self._rf_nodes
+        v_rf = params.get('rf_amp', 100.0) * np.sin(2.0 * np.pi * (params.get('rf_freq', 13.56) * 1e6) * t_current)
+        check(self.lib.bemcs_rf_set(ptr(self.V_fixed_d), ptr(idx_d), ptr(vdc_d), int(idx_d.numel()),
+                                    float(v_rf), stream_ptr()), "rf_set")
+        self.recalc_poisson(2, params, counts=(self.cnt_i_next, self.cnt_e_next))
+
+    def _resolve_surface(self):
+        """Remesh decisions of the last step with surface physics: rebuild the
+        Poisson operator when cells failed (:1443-1449); update the grid
+        temperatures and, when a grid's thermal bow changed by more than 5 um,
+        rebuild the domain keeping the state (:1617-1647) and redo that step's
+        saddle-point record with the new potential."""
+        pend = self._surface_pending
+        if pend is None:
+            return
+        self._surface_pending = None
+        sev, hb, slot, params, thermal, erosion = pend
+        sev.synchronize()
+        n_failed = int(hb[:1].view(torch.int32).item())
+        remeshed = False
+        if erosion and n_failed != self._n_failed_seen:
+            self._n_failed_seen = n_failed
+            self._isBound_host = self.is_bound_d.view(self.ny, self.nx).cpu().numpy().astype(bool)
+            self._rebuild_operator()
+            remeshed = True
+        if thermal:
+            tg = hb[1:].numpy()
+            for i, m in enumerate(self.mask_grids):
+                if np.any(m):
+                    self.T_grids[i] = np.float32(tg[i])
+            grids = params.get('grids', [])
+            dc = params.get("discharge_chamber")
+            pitch = params.get("pitch_mm", dc.get("pitch_mm", 3.0) if isinstance(dc, dict) else 3.0)
+            bow = False
+            for i, g in enumerate(grids):
+                d = self._bow_deflection(self.T_grids[i], g, pitch)
+                if abs(d - self.grid_deflections[i]) > 0.005:
+                    self.grid_deflections[i] = d
+                    bow = True
+            if bow:
+                self.build_domain(params, preserve_state=True)
+                self._step_end(slot)
+                remeshed = True
+        self.ring_remeshed[slot] = remeshed
+
+    def _bow_deflection(self, T_grid, grid, pitch):
+        """Cantilever thermal bow delta = alpha dT L^2 / (2 t) in mm (:1621-1638);
+        float32 when T_grid is a float32 mean, as in the reference."""
+        if self.geometry == 'two_holes':
+            L = max(0.2, pitch - 2.0 * grid['r'])
+        elif self.geometry == 'one_hole':
+            L = max(0.2, 0.5 * (self.Ly - 2.0 * grid['r']))
+        else:
+            L = max(0.2, self.Ly - grid['r'])
+        L_m, t_m = L * 1e-3, grid['t'] * 1e-3
+        if t_m > 0 and L_m > 0:
+            dT = T_grid - 300.0
+            return self.alpha_thermal * dT * L_m ** 2 / (2.0 * t_m) * 1e3
+        return 0.0
 
     @staticmethod
     def _timed_begin(kt):
@@ -692,8 +903,6 @@ class DigitalTwinSimulator:
         ep.Lx32, ep.Ly32 = f(self.Lx), f(self.Ly)
         ep.nx, ep.ny, ep.periodic_y = self.nx, self.ny, int(self.periodic_y)
         self._elec_params = ep
-        if grids and params.get('sim_mode', 'Both') != 'Trajectories' and not getattr(self, '_warned_mode', False):
-            self._warned_mode = True
 
     def advance(self, params, n_steps):
         """Run n_steps time steps without synchronising the host."""
@@ -713,17 +922,21 @@ class DigitalTwinSimulator:
         rec = self.record(slot)
         self.transmitted_ions += rec['transmitted']
         self.entered_optics += rec['entered']
-        return False, rec['min_pot'], rec['current_div'], self.T_grids, rec['trans_last_frame']
+        return rec['remeshed'], rec['min_pot'], rec['current_div'], self.T_grids, rec['trans_last_frame']
 
     # ------------------------------------------------------- diagnostics
     def record(self, slot=None):
         """Diagnostics of one step (default: the last), synchronising."""
         if slot is None:
             slot = self.iteration % RING
+        self._resolve_surface()
         torch.cuda.current_stream().synchronize()
         dsz = C.sizeof(_lib.IonDiag)
         raw = self.ring_diag[slot * dsz:(slot + 1) * dsz].cpu().numpy().tobytes()
         d = _lib.IonDiag.from_buffer_copy(raw)
+        if d.overflow & 32:
+            raise _lib.BemcsError(_lib.EOVERFLOW, "impact heat or sputter damage outside the fixed-point "
+                                  "accumulator range (non-finite or beyond 2^53 units)")
         if d.overflow:
             raise _lib.BemcsError(_lib.EOVERFLOW, f"particle buffer overflow (flags {d.overflow})")
         nc = d.crossed
@@ -733,6 +946,8 @@ class DigitalTwinSimulator:
         rec['current_div'] = float(self.ring_theta[slot].item()) if nc > 5 else float('nan')
         rec['min_pot'] = float(self.ring_minpot[slot].item())
         rec['trans_last_frame'] = (rec['transmitted'] / rec['entered']) if rec['entered'] > 0 else 0.0
+        rec['remeshed'] = bool(self.ring_remeshed[slot])
+        rec['surface_hits'] = d.surface_hits
         if nc > 0:
             rec['exit_vx_mean'] = d.sum_vx / nc
             rec['exit_v_mean'] = d.sum_v / nc
@@ -814,11 +1029,26 @@ class DigitalTwinSimulator:
 
     @property
     def isBound(self):
+        self._resolve_surface()
         return self._isBound_host
 
     @property
     def V_fixed(self):
-        return self._V_fixed_host
+        return self.V_fixed_d.view(self.ny, self.nx).cpu().numpy()
+
+    @property
+    def Tmap(self):
+        return self.Tmap_d[self._tcur].view(self.ny, self.nx).cpu().numpy()
+
+    T_map = Tmap
+
+    @property
+    def damage_map(self):
+        return self.damage_map_d.view(self.ny, self.nx).cpu().numpy()
+
+    @property
+    def eroded_depth(self):
+        return self.eroded_depth_d.view(self.ny, self.nx).cpu().numpy()
 
     @property
     def current_ppc_map(self):
