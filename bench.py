@@ -32,6 +32,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def workload_params(name, ions=None, electrons=None):
+    if name == "c5":
+        return c5_params(ions, electrons)
     mesh = {"c2": (1024, 384, 8_000_000), "c1": (256, 96, 100_000)}[name]
     nx, ny, npre = mesh
     return {
@@ -43,6 +45,27 @@ def workload_params(name, ions=None, electrons=None):
         "neut_rate": 30, "sim_mode": "Trajectories", "mesh": {"nx": nx, "ny": ny},
         "preload": {"ions": npre if ions is None else ions,
                     "electrons": npre if electrons is None else electrons},
+        "seed": 0,
+    }
+
+
+def c5_params(ions=None, electrons=None):
+    """Three-grid optics with sputter erosion, thermal conduction and bow, and
+    RF co-extraction: screen +1200 V / accel -250 V / decel 0 V, each 0.5 mm
+    thick with 0.6 mm gaps (apertures 0.95 / 0.55 / 0.8 mm), 50 V at 13.56 MHz
+    on the screen, 2048 x 768 nodes, 32M + 32M preloaded."""
+    n = 32_000_000
+    return {
+        "grids": [{"V": 1200.0, "t": 0.5, "gap": 0.6, "r": 0.95, "cham": 0.0},
+                  {"V": -250.0, "t": 0.5, "gap": 0.6, "r": 0.55, "cham": 0.0},
+                  {"V": 0.0, "t": 0.5, "gap": 0.6, "r": 0.8, "cham": 0.0}],
+        "pitch_mm": 6.1447, "geometry": "half_hole", "V_plasma_offset": 20.0,
+        "n0_plasma": 1e17, "Te_up": 3.0, "Ti": 0.2, "Te": 5.0, "Tn": 300.0,
+        "n0": 1e18, "neutral_model": "uniform", "cx_model": "loglinear",
+        "neut_rate": 30, "sim_mode": "Both", "Thresh": 1e5,
+        "rf_enable": True, "rf_grid_idx": 0, "rf_amp": 50.0, "rf_freq": 13.56,
+        "mesh": {"nx": 2048, "ny": 768},
+        "preload": {"ions": n if ions is None else ions, "electrons": n if electrons is None else electrons},
         "seed": 0,
     }
 
@@ -178,11 +201,15 @@ def run(args):
             kstats[name]["particles_range"] = [int(nin[:, col].min()), int(nin[:, col].max())]
             byts = sum(algorithmic_bytes(name, int(r[0]), int(r[1]), n_nodes) for r in nin)
             kstats[name]["algorithmic_bytes_per_launch"] = byts / kstats[name]["launches"]
-    if "poisson_picard" in kstats:
-        its = np.asarray(sim.poisson_iter_log[-kstats["poisson_picard"]["launches"]:])
-        kstats["poisson_picard"]["pcg_iterations_per_call"] = {
-            "min": int(its.min()), "median": float(np.median(its)), "max": int(its.max()),
-            "total": int(its.sum())}
+    n_calls = sum(kstats[k]["launches"] for k in ("poisson_picard", "poisson_rf") if k in kstats)
+    if n_calls:
+        # every Picard call of the timed steps (RF calls interleave with the
+        # every-2nd-step calls); geometry rebuilds add their own calls
+        its = np.asarray(sim.poisson_iter_log[-n_calls:])
+        kstats["pcg_iterations_per_picard_call"] = {
+            "calls": n_calls, "min": int(its.min()), "median": float(np.median(its)),
+            "max": int(its.max()), "total": int(its.sum())}
+    remeshes = int(sim.ring_remeshed[slots].sum())
     # dominant single kernel: the fused particle steps (one launch each); the
     # Poisson call is a sequence of ~30 launches per PCG iteration
     dom = max(("ion_step", "electron_step"), key=lambda k: kstats[k]["total_ms"])
@@ -193,8 +220,19 @@ def run(args):
                          for r in nin))
     step_achieved = step_bytes / (wall_ms * 1e-3) / 1e9
     rec = sim.record()
+    labels = {
+        "c2": ("particle-steps/s incl. Poisson at 1024x384",
+               "c2: two-grid Xe+ extraction, 1024x384 nodes, 8M ions + 8M electrons "
+               f"preloaded, steps {first}-{first + args.steps - 1}, dt 5e-10 s, "
+               f"MG-PCG to {sim.poisson_rtol:g}, 5 Picard iterations every 2nd step"),
+        "c1": ("particle-steps/s incl. Poisson at 256x96", "c1: 256x96 nodes, 1e5 + 1e5 preloaded"),
+        "c5": ("particle-steps/s incl. Poisson, thermal, erosion and RF at 2048x768",
+               "c5: three-grid (+1200/-250/0 V) with sputter erosion, thermal conduction and bow, "
+               "RF 50 V at 13.56 MHz on the screen (2 Picard iterations every step plus 5 every 2nd), "
+               f"2048x768 nodes, 32M ions + 32M electrons preloaded, steps {first}-{first + args.steps - 1}"),
+    }
     out = {
-        "metric": "particle-steps/s incl. Poisson at 1024x384",
+        "metric": labels[args.workload][0],
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -206,15 +244,13 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f32 state, f64 gather/kick, f64 CG + f32 multigrid",
         "data": "synthetic (seeded preload, Philox streams)",
-        "config": {"workload": ("c2: two-grid Xe+ extraction, 1024x384 nodes, 8M ions + 8M electrons "
-                                f"preloaded, steps {first}-{first + args.steps - 1}, dt 5e-10 s, "
-                                f"MG-PCG to {sim.poisson_rtol:g}, 5 Picard iterations every 2nd step")
-                   if args.workload == "c2" else "c1: 256x96 nodes, 1e5 + 1e5 preloaded",
+        "config": {"workload": labels[args.workload][1],
                    "nodes": n_nodes, "particles_pushed": pushed, "build_s": t_build,
                    "ions_range": kstats["ion_step"]["particles_range"],
                    "electrons_range": kstats["electron_step"]["particles_range"],
                    "n_sub_last": [rec["n_sub_ion"], rec["n_sub_e"]],
-                   "num_p_last": rec["num_p"], "num_e_last": rec["num_e"]},
+                   "num_p_last": rec["num_p"], "num_e_last": rec["num_e"],
+                   "remeshes": remeshes},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "step_roofline": {"bytes_per_step": step_bytes / args.steps, "achieved": step_achieved,
@@ -401,7 +437,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()

@@ -268,6 +268,16 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny,
 int bemcs_poisson_free(bemcs_poisson_t h);
 int bemcs_poisson_levels(bemcs_poisson_t h);
 
+/* Rebuilds the operator of handle h for a new electrode map of the same mesh
+ * (device uint8 [ny][nx]) in place, on the device: the reference re-runs
+ * build_sparse_matrix and factorises again after erosion (:1448) and after a
+ * thermal-bow rebuild (:1646).  The multigrid hierarchy and the coarsest-level
+ * inverse are recomputed into the handle's existing arrays (bemcs_poisson_create
+ * runs the same set-up), so results equal those of a handle created from the
+ * new map; the next Picard call starts its solve from the current V.  Waits
+ * for the set-up on `stream` to report an invalid map. */
+int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_stream_t stream);
+
 /* Solves the full reference system A V = b (laplacian_lu(b), :900) for
  * device f64 vectors; V holds the initial guess on entry.  Stops when
  * ||r||_2 <= rtol * ||b_reduced||_2 on the eliminated system. */
@@ -443,10 +453,50 @@ typedef struct {
 int64_t bemcs_thermal_workspace_bytes(int32_t nx, int32_t ny, int32_t n_grids);
 int bemcs_thermal(const bemcs_thermal_t* p, bemcs_stream_t stream);
 
-/* RF modulation of a grid potential (:984-991): V_fixed[idx[k]] = vdc[k] +
- * v_rf for k < n (idx: the RF grid's nodes, vdc: their DC potentials). */
-int bemcs_rf_set(double* V_fixed, const int32_t* idx, const double* vdc, int64_t n, double v_rf,
-                 bemcs_stream_t stream);
+/* RF modulation of a grid potential (:984-991): on every node of grid rf_grid
+ * (bit rf_grid of grid_bits), V_fixed = Vdc + v_rf, Vdc being the potential
+ * of the last grid covering the node (grid_V: host array of n_grids values). */
+int bemcs_rf_set(double* V_fixed, const uint8_t* grid_bits, int64_t n_nodes, int32_t n_grids,
+                 const double* grid_V, int32_t rf_grid, double v_rf, bemcs_stream_t stream);
+
+/* Electrode rasterisation of build_domain (:791-843) on the device: node
+ * coordinates as np.linspace(0, L, n), for each grid in order the slab
+ * gstart + dxbow <= x <= gend + dxbow (dxbow = delta * eta^2, eta per
+ * geometry, when |delta| > 1e-6) minus the apertures |y - yc| <= r - max(0,
+ * x - gstart - dxbow) tan(chamfer) (clipped at 0); is_bound = 1 and V_fixed
+ * = V on the grid's nodes (later grids overwrite), bit g of grid_bits set;
+ * column 0 is Dirichlet at V_plasma.  Float64 in the reference's operation
+ * order: the map equals NumPy's. */
+typedef struct {
+  double gstart, gend;       /* x extent of the grid (mm) */
+  double V;                  /* grid potential */
+  double r;                  /* aperture radius (mm) */
+  double tan_cham;           /* np.tan(np.radians(chamfer)) */
+  double delta;              /* thermal-bow deflection (mm) */
+} bemcs_grid_geom_t;
+
+typedef struct {
+  uint8_t* is_bound;         /* [ny][nx] out */
+  double* V_fixed;           /* [ny][nx] out */
+  uint8_t* grid_bits;        /* [ny][nx] out */
+  bemcs_grid_geom_t grids[8];
+  double hole_y[2];          /* aperture centres (mm) */
+  double Lx, Ly;             /* mm */
+  double screen_r, pitch;    /* mm (web position of the bow profiles) */
+  double V_plasma;           /* column-0 Dirichlet value */
+  int32_t n_grids, n_holes;
+  int32_t nx, ny;
+  int32_t geometry;          /* 0 half_hole, 1 one_hole, 2 two_holes */
+  int32_t pad_;
+} bemcs_electrodes_t;
+
+int bemcs_electrodes(const bemcs_electrodes_t* p, bemcs_stream_t stream);
+
+/* Mean of T over the nodes of each grid (bit g of grid_bits), written to
+ * T_grids[g] as float32 for grids with at least one node (others are left
+ * unchanged); workspace as bemcs_thermal_workspace_bytes(nx, ny, n_grids). */
+int bemcs_grid_means(const float* T, const uint8_t* grid_bits, int32_t nx, int32_t ny, int32_t n_grids,
+                     void* workspace, float* T_grids, bemcs_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Batched step kernels: one launch serves n independent cases (meshes and

@@ -60,7 +60,7 @@ _STRUCTS = {
     "bemcs_substeps_case_t": "SubstepsCase", "bemcs_inject_case_t": "InjectCase",
     "bemcs_sort_case_t": "SortCase", "bemcs_ppc_case_t": "PpcCase",
     "bemcs_percentile_case_t": "PercentileCase", "bemcs_erosion_t": "Erosion",
-    "bemcs_thermal_t": "Thermal",
+    "bemcs_thermal_t": "Thermal", "bemcs_grid_geom_t": "GridGeom", "bemcs_electrodes_t": "Electrodes",
 }
 
 

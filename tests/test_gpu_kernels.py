@@ -372,3 +372,60 @@ def test_poisson_batch_picard_matches_single(lib):
     finally:
         for h in hs_single + hs_batch:
             lib.bemcs_poisson_free(h)
+
+
+def test_poisson_update_equals_create(lib):
+    """An operator updated in place to a new electrode map (eroded cells,
+    moved grid faces) solves bit-identically to a handle created from that
+    map, and both agree with SuperLU."""
+    from oracle.poisson import SuperLUPoisson
+    nx, ny = 257, 97
+    o = _geometry(nx, ny, False)
+    A = o.isBound.copy()
+    B = A.copy()
+    r = np.random.default_rng(5)
+    cand = np.argwhere(A[1:-1, 1:-1]) + 1
+    for iy, ix in cand[r.choice(len(cand), 40, replace=False)]:
+        B[iy, ix] = False                          # eroded cells
+    cols = np.where(A.any(axis=0))[0]
+    B[: ny // 2, cols[cols > 2].min() - 3] = True  # a bowed face
+    ha, hb = C.c_void_p(), C.c_void_p()
+    ia = np.ascontiguousarray(A.astype(np.uint8))
+    ibm = np.ascontiguousarray(B.astype(np.uint8))
+    assert lib.bemcs_poisson_create(ia.ctypes.data_as(C.c_void_p), nx, ny, 0, C.byref(ha)) == 0
+    assert lib.bemcs_poisson_create(ibm.ctypes.data_as(C.c_void_p), nx, ny, 0, C.byref(hb)) == 0
+    try:
+        bd_map = _dev(ibm.ravel())
+        assert lib.bemcs_poisson_update(ha, _p(bd_map), _stream()) == 0, lib.bemcs_last_error()
+        sp = SuperLUPoisson(B, False)
+        b = sp.rhs_vector(r.normal(0, 1.0, nx * ny) * 1e-3, o.V_fixed.ravel(), 1.0)
+        want = sp.solve(b)
+        bd = _dev(b)
+        outs = []
+        for h in (ha, hb):
+            V = torch.zeros(nx * ny, dtype=torch.float64, device="cuda")
+            its, rel = C.c_int32(0), C.c_double(0)
+            assert lib.bemcs_poisson_solve(h, _p(bd), _p(V), 1e-12, 300, C.byref(its), C.byref(rel), _stream()) == 0
+            outs.append(V.cpu().numpy())
+            err = np.max(np.abs(outs[-1] - want)) / np.max(np.abs(want))
+            print(f"[gpu] poisson update: iters {its.value} err {err:.2e}", flush=True)
+            assert err < 1e-10
+        assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+    finally:
+        lib.bemcs_poisson_free(ha)
+        lib.bemcs_poisson_free(hb)
+
+
+def test_poisson_update_rejects_invalid_map(lib):
+    nx, ny = 128, 48
+    o = _geometry(nx, ny, False)
+    ia = np.ascontiguousarray(o.isBound.astype(np.uint8))
+    h = C.c_void_p()
+    assert lib.bemcs_poisson_create(ia.ctypes.data_as(C.c_void_p), nx, ny, 0, C.byref(h)) == 0
+    try:
+        bad = ia.copy()
+        bad[:, 0] = 0                               # no plasma boundary: unknowns on the x edge
+        assert lib.bemcs_poisson_update(h, _p(_dev(bad.ravel())), _stream()) != 0
+        assert b"edge" in lib.bemcs_last_error()
+    finally:
+        lib.bemcs_poisson_free(h)

@@ -162,12 +162,44 @@ def test_erosion_kernel(hip_lib):
 
 
 def test_rf_set(hip_lib):
+    """RF nodes take Vdc + v_rf, Vdc from the last grid covering them."""
     V = torch.zeros(100, dtype=torch.float64, device="cuda")
-    idx = _dev(np.array([3, 50, 99], np.int32))
-    vdc = _dev(np.array([-200.0, -200.0, 10.0]))
-    assert hip_lib.bemcs_rf_set(_ptr(V), _ptr(idx), _ptr(vdc), 3, 12.5, _stream()) == 0
+    bits = np.zeros(100, np.uint8)
+    bits[[3, 50]] = 0b010        # grid 1 only
+    bits[99] = 0b110             # grids 1 and 2: grid 2 wrote last
+    bits[7] = 0b001              # grid 0: not the RF grid
+    gv = (C.c_double * 3)(1000.0, -200.0, 10.0)
+    assert hip_lib.bemcs_rf_set(_ptr(V), _ptr(_dev(bits)), 100, 3, gv, 1, 12.5, _stream()) == 0
     v = V.cpu().numpy()
     assert v[3] == -187.5 and v[50] == -187.5 and v[99] == 22.5 and np.count_nonzero(v) == 3
+
+
+@pytest.mark.parametrize("geometry,cham,defl", [
+    ("half_hole", 0.0, None), ("half_hole", 15.0, (0.013, -0.0071)), ("one_hole", 10.0, (0.02, 0.004)),
+    ("two_holes", 0.0, (-0.009, 0.0166))])
+def test_electrode_rasterisation_matches_numpy(geometry, cham, defl):
+    """Device electrode map, Dirichlet values and grid masks equal the
+    oracle's NumPy rasterisation, with chamfers and thermal bow (float32
+    deflections as the bow produces them)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    ny = 200 if geometry == "two_holes" else 72   # two_holes spans a full pitch in y
+    p = c1_params(nx=160, ny=ny, ions=0, electrons=0, geometry=geometry)
+    for g in p["grids"]:
+        g["cham"] = cham
+
+    def bow(s):
+        if defl is not None:
+            s.grid_deflections = [np.float32(d) for d in defl]
+
+    g, o = _mk(p, bow)
+    print(f"[gpu] electrodes {geometry} cham {cham} bow {defl}", flush=True)
+    assert np.array_equal(g.isBound, o.isBound)
+    assert np.array_equal(g.V_fixed, o.V_fixed)
+    assert len(g.mask_grids) == len(o.mask_grids)
+    for a, b in zip(g.mask_grids, o.mask_grids):
+        assert np.array_equal(a, b)
+    assert np.array_equal(g.Vdc, o.Vdc)
 
 
 # ----------------------------------------------------------------- steps

@@ -121,6 +121,18 @@ class Thermal(C.Structure):
                 ("n_grids", i32), ("sweeps", i32)]
 
 
+class GridGeom(C.Structure):
+    _fields_ = [("gstart", f64), ("gend", f64), ("V", f64), ("r", f64), ("tan_cham", f64),
+                ("delta", f64)]
+
+
+class Electrodes(C.Structure):
+    _fields_ = [("is_bound", vp), ("V_fixed", vp), ("grid_bits", vp), ("grids", GridGeom * 8),
+                ("hole_y", f64 * 2), ("Lx", f64), ("Ly", f64), ("screen_r", f64), ("pitch", f64),
+                ("V_plasma", f64), ("n_grids", i32), ("n_holes", i32), ("nx", i32), ("ny", i32),
+                ("geometry", i32), ("pad_", i32)]
+
+
 class PercentileCase(C.Structure):
     _fields_ = [("angles", vp), ("n_dev", vp), ("q_percent", f64), ("min_count", u32), ("pad_", u32),
                 ("out", vp)]
@@ -148,6 +160,7 @@ _SIGS = {
     "bemcs_poisson_create": (i32, [vp, i32, i32, i32, vp]),
     "bemcs_poisson_free": (i32, [vp]),
     "bemcs_poisson_levels": (i32, [vp]),
+    "bemcs_poisson_update": (i32, [vp, vp, vp]),
     "bemcs_poisson_solve": (i32, [vp, vp, vp, f64, i32, vp, vp, vp]),
     "bemcs_poisson_picard": (i32, [vp, vp, vp, vp, vp]),
     "bemcs_efield": (i32, [vp, i32, i32, f64, f64, i32, vp, vp]),
@@ -164,7 +177,9 @@ _SIGS = {
                                C.c_float, i64, vp]),
     "bemcs_thermal_workspace_bytes": (i64, [i32, i32, i32]),
     "bemcs_thermal": (i32, [vp, vp]),
-    "bemcs_rf_set": (i32, [vp, vp, vp, i64, f64, vp]),
+    "bemcs_rf_set": (i32, [vp, vp, i64, i32, vp, i32, f64, vp]),
+    "bemcs_electrodes": (i32, [vp, vp]),
+    "bemcs_grid_means": (i32, [vp, vp, i32, i32, i32, vp, vp, vp]),
     "bemcs_ion_step_batched": (i32, [vp, i32, i64, vp, vp]),
     "bemcs_electron_step_batched": (i32, [vp, i32, i64, vp, vp]),
     "bemcs_substeps_batched": (i32, [vp, i32, i64, vp]),

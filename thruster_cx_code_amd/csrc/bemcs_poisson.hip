@@ -66,6 +66,12 @@ struct bemcs_poisson_s {
   double* pz[2];
   bool have_guess;
   bemcs_poisson_batch_s* own;  // solve set of this handle alone
+  // operator set-up workspace (in-place updates)
+  double* su_st[2];   // float64 9-point stencils of two consecutive coarse levels
+  uint8_t* su_act[2];  // their active flags
+  double* su_A;       // dense coarsest operator, then its inverse
+  double *su_C, *su_R, *su_T;
+  int* su_err;
 };
 
 namespace bemcs {
@@ -877,13 +883,336 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
   }
 }
 
-// ------------------------------------------------------------- host set-up
-struct HostLevel {
+// ------------------------------------------------------ operator set-up
+// The reduced operator of build_sparse_matrix (:525-626) and its multigrid
+// hierarchy are built on the device from the electrode map, into the
+// handle's existing arrays: level shapes depend only on (nx, ny), so a new
+// map (erosion, thermal bow) updates a handle in place and its captured
+// launch graphs stay valid.
+//   k_setup_kind     node kinds (Dirichlet, right / top / bottom Neumann, unknown)
+//   k_setup_mirror   Neumann mirrors (first hop and final node)
+//   k_setup_level0   level-0 diagonal 4 - #Neumann neighbours, 1/diagonal
+//   k_galerkin       coarse 9-point operator P^T A P (bilinear P), one thread
+//                    per coarse node gathering its fine contributions in a
+//                    fixed order (deterministic)
+//   k_galerkin_finish  drop couplings of inactive nodes, 1/diagonal, float32 copy
+//   k_dense_*, k_gj_*  inverse of the coarsest operator by blocked in-place
+//                    Gauss-Jordan (SPD, no pivoting), 16-wide panels
+constexpr int kGJPanel = 16;
+enum : int { SETUP_ERR_CHAIN = 1, SETUP_ERR_MIRROR = 2, SETUP_ERR_EDGE = 4 };
+
+__device__ __forceinline__ uint8_t node_kind(const uint8_t* is_bound, int i, int nx, int ny, bool per) {
+  const int x = i % nx, y = i / nx;
+  if (is_bound[i]) return K_BOUND;
+  if (x == nx - 1) return K_RIGHT;
+  if (!per && y == ny - 1) return K_TOP;
+  if (!per && y == 0) return K_BOTTOM;
+  return K_UNKNOWN;
+}
+
+__device__ __forceinline__ int mirror1_of(const uint8_t* kind, int i, int nx, int ny) {
+  const int x = i % nx, y = i / nx;
+  const uint8_t k = kind[i];
+  if (k == K_RIGHT) return y * nx + (nx - 2);
+  if (k == K_TOP) return (ny - 2) * nx + x;
+  if (k == K_BOTTOM) return nx + x;
+  return -1;
+}
+
+__global__ __launch_bounds__(256) void k_setup_kind(const uint8_t* __restrict__ is_bound, uint8_t* kind,
+                                                    int nx, int ny, int per) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nx * ny) kind[i] = node_kind(is_bound, i, nx, ny, per != 0);
+}
+
+__global__ __launch_bounds__(256) void k_setup_mirror(const uint8_t* __restrict__ kind, int32_t* mirror,
+                                                      int32_t* mirror1, int nx, int ny, int* err) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nx * ny) return;
+  int m = mirror1_of(kind, i, nx, ny);
+  mirror1[i] = m;
+  if (m < 0) {
+    mirror[i] = -1;
+    return;
+  }
+  int hops = 0;
+  while (kind[m] != K_UNKNOWN && kind[m] != K_BOUND && hops < 4) {
+    m = mirror1_of(kind, m, nx, ny);
+    ++hops;
+  }
+  if (hops >= 2) atomicOr(err, SETUP_ERR_CHAIN);
+  mirror[i] = m;
+}
+
+__global__ __launch_bounds__(256) void k_setup_level0(const uint8_t* __restrict__ kind,
+                                                      const int32_t* __restrict__ mirror1, uint8_t* code,
+                                                      float* inv0, int nx, int ny, int per, int* err) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nx * ny) return;
+  if (kind[i] != K_UNKNOWN) {
+    code[i] = 0;
+    inv0[i] = 0.0f;
+    return;
+  }
+  const int x = i % nx, y = i / nx;
+  const int s = y > 0 ? i - nx : (per ? i + (ny - 1) * nx : -1);
+  const int nn = y < ny - 1 ? i + nx : (per ? i - (ny - 1) * nx : -1);
+  if (x == 0 || x == nx - 1 || s < 0 || nn < 0) {
+    atomicOr(err, SETUP_ERR_EDGE);
+    code[i] = 0;
+    inv0[i] = 0.0f;
+    return;
+  }
+  const int nb[4] = {i - 1, i + 1, s, nn};
+  int d = 4;
+  for (int k = 0; k < 4; ++k) {
+    const uint8_t t = kind[nb[k]];
+    if (t == K_RIGHT || t == K_TOP || t == K_BOTTOM) {
+      if (mirror1[nb[k]] != i) atomicOr(err, SETUP_ERR_MIRROR);
+      --d;
+    }
+  }
+  code[i] = (uint8_t)d;
+  inv0[i] = (float)(1.0 / (double)d);
+}
+
+// 1D bilinear prolongation parents of fine index f (coarse nc), weights 1 or 1/2.
+__device__ __forceinline__ int parents1d(int f, int nc, bool per, int* idx, double* w) {
+  if ((f & 1) == 0) {
+    idx[0] = f >> 1;
+    w[0] = 1.0;
+    return 1;
+  }
+  int a = (f - 1) >> 1, b = (f + 1) >> 1;
+  if (b >= nc) {
+    if (per) b -= nc;
+    else {
+      idx[0] = a;
+      w[0] = 1.0;
+      return 1;
+    }
+  }
+  if (a == b) {
+    idx[0] = a;
+    w[0] = 1.0;
+    return 1;
+  }
+  idx[0] = a;
+  w[0] = 0.5;
+  idx[1] = b;
+  w[1] = 0.5;
+  return 2;
+}
+
+// Fine level read by the Galerkin product: level 0 from kinds and codes
+// (5-point, -1 to unknown neighbours), coarser levels from their float64
+// 9-point stencils.
+struct FineOp {
   int nx, ny;
-  bool periodic;
-  std::vector<double> st;  // [9][n]
-  std::vector<uint8_t> active;
+  const uint8_t* kind;  // level 0
+  const uint8_t* code;  // level 0
+  const double* st;     // level >= 1: [9][n]
+  const uint8_t* act;   // level >= 1
 };
+
+template <bool L0>
+__device__ __forceinline__ bool fine_active(const FineOp& F, int f) {
+  return L0 ? F.kind[f] == K_UNKNOWN : F.act[f] != 0;
+}
+
+template <bool L0>
+__device__ __forceinline__ double fine_coef(const FineOp& F, int k, int f, int g) {
+  if (!L0) return F.st[(size_t)k * F.nx * F.ny + f];
+  if (k == 4) return (double)F.code[f];
+  if (k == 1 || k == 3 || k == 5 || k == 7) return F.kind[g] == K_UNKNOWN ? -1.0 : 0.0;
+  return 0.0;
+}
+
+template <bool L0>
+__global__ __launch_bounds__(256) void k_galerkin(FineOp F, double* stC, uint8_t* actC, int ncx,
+                                                  int ncy, int per) {
+  const int I = blockIdx.x * 256 + threadIdx.x;
+  const int nC = ncx * ncy;
+  if (I >= nC) return;
+  const int cx = I % ncx, cy = I / ncx;
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (int oy = -1; oy <= 1; ++oy) {
+    int fy = 2 * cy + oy;
+    if (fy < 0 || fy >= F.ny) {
+      if (!per) continue;
+      fy = (fy + F.ny) % F.ny;
+    }
+    int pyi[2];
+    double pyw[2];
+    const int npy = parents1d(fy, ncy, per != 0, pyi, pyw);
+    double wy = 0.0;
+    for (int a = 0; a < npy; ++a)
+      if (pyi[a] == cy) wy = pyw[a];
+    if (wy == 0.0) continue;
+    for (int ox = -1; ox <= 1; ++ox) {
+      const int fx = 2 * cx + ox;
+      if (fx < 0 || fx >= F.nx) continue;
+      int pxi[2];
+      double pxw[2];
+      const int npx = parents1d(fx, ncx, false, pxi, pxw);
+      double wx = 0.0;
+      for (int a = 0; a < npx; ++a)
+        if (pxi[a] == cx) wx = pxw[a];
+      if (wx == 0.0) continue;
+      const int f = fy * F.nx + fx;
+      if (!fine_active<L0>(F, f)) continue;
+      for (int k = 0; k < 9; ++k) {
+        int gx = fx + (k % 3 - 1), gy = fy + (k / 3 - 1);
+        if (gx < 0 || gx >= F.nx) continue;
+        if (gy < 0 || gy >= F.ny) {
+          if (!per) continue;
+          gy = (gy + F.ny) % F.ny;
+        }
+        const int g = gy * F.nx + gx;
+        const double a = fine_coef<L0>(F, k, f, g);
+        if (a == 0.0 || !fine_active<L0>(F, g)) continue;
+        const double w1 = wx * wy * a;
+        int qxi[2], qyi[2];
+        double qxw[2], qyw[2];
+        const int nqx = parents1d(gx, ncx, false, qxi, qxw);
+        const int nqy = parents1d(gy, ncy, per != 0, qyi, qyw);
+        for (int by = 0; by < nqy; ++by)
+          for (int bx = 0; bx < nqx; ++bx) {
+            const int ddx = qxi[bx] - cx;
+            int ddy = qyi[by] - cy;
+            if (per) {
+              if (ddy > 1) ddy -= ncy;
+              if (ddy < -1) ddy += ncy;
+            }
+            acc[(ddy + 1) * 3 + (ddx + 1)] += w1 * qxw[bx] * qyw[by];
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) stC[(size_t)k * nC + I] = acc[k];
+  actC[I] = acc[4] > 1e-12 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_galerkin_finish(double* stC, const uint8_t* __restrict__ actC,
+                                                         float* st32, float* inv, int ncx, int ncy,
+                                                         int per) {
+  const int I = blockIdx.x * 256 + threadIdx.x;
+  const int nC = ncx * ncy;
+  if (I >= nC) return;
+  const int cx = I % ncx, cy = I / ncx;
+  const bool aI = actC[I] != 0;
+  for (int k = 0; k < 9; ++k) {
+    int gx = cx + (k % 3 - 1), gy = cy + (k / 3 - 1);
+    bool ok = gx >= 0 && gx < ncx;
+    if (gy < 0 || gy >= ncy) {
+      if (per) gy = (gy + ncy) % ncy;
+      else ok = false;
+    }
+    double v = stC[(size_t)k * nC + I];
+    if (!ok || !aI || !actC[gy * ncx + gx]) v = 0.0;
+    stC[(size_t)k * nC + I] = v;
+    st32[(size_t)k * nC + I] = (float)v;
+  }
+  inv[I] = aI ? (float)(1.0 / stC[(size_t)4 * nC + I]) : 0.0f;
+}
+
+// Dense matrix of the coarsest operator, row-major; inactive rows are the
+// identity (the active block is decoupled from them).
+template <bool L0>
+__global__ __launch_bounds__(256) void k_dense_assemble(FineOp L, double* A, int per) {
+  const int n = L.nx * L.ny;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  double v = 0.0;
+  if (!fine_active<L0>(L, i)) {
+    v = (i == j) ? 1.0 : 0.0;
+  } else {
+    const int cx = i % L.nx, cy = i / L.nx;
+    for (int k = 0; k < 9; ++k) {
+      int gx = cx + (k % 3 - 1), gy = cy + (k / 3 - 1);
+      if (gx < 0 || gx >= L.nx) continue;
+      if (gy < 0 || gy >= L.ny) {
+        if (!per) continue;
+        gy = (gy + L.ny) % L.ny;
+      }
+      const int g = gy * L.nx + gx;
+      if (g != j || !fine_active<L0>(L, g)) continue;
+      v += fine_coef<L0>(L, k, i, g);
+    }
+  }
+  A[e] = v;
+}
+
+// Panel step of the blocked in-place Gauss-Jordan inversion, part 1: every
+// work-group inverts the p x p diagonal block T = A[K,K]^-1 in LDS (scalar
+// in-place Gauss-Jordan), then C[i,:] = A[i,K] T for its rows and
+// R[:,j] = A[K,j] for its columns; work-group 0 stores T.
+__global__ __launch_bounds__(256) void k_gj_panel(const double* __restrict__ A, int n, int K0, int p,
+                                                  double* C, double* R, double* Tg) {
+  __shared__ double L[kGJPanel][kGJPanel];
+  const int t = threadIdx.x;
+  const int ti = t / kGJPanel, tj = t % kGJPanel;
+  if (ti < p && tj < p) L[ti][tj] = A[(size_t)(K0 + ti) * n + K0 + tj];
+  for (int k = 0; k < p; ++k) {
+    __syncthreads();
+    double piv = L[k][k], rk = 0.0, fi = 0.0, cur = 0.0;
+    if (ti < p && tj < p) {
+      rk = ((tj == k) ? 1.0 : L[k][tj]) / piv;
+      fi = L[ti][k];
+      cur = (tj == k) ? 0.0 : L[ti][tj];
+    }
+    __syncthreads();
+    if (ti < p && tj < p) L[ti][tj] = (ti == k) ? rk : cur - fi * rk;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && ti < p && tj < p) Tg[ti * kGJPanel + tj] = L[ti][tj];
+  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < (int64_t)n * p; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e / p), a = (int)(e % p);
+    double c = 0.0;
+    for (int q = 0; q < p; ++q) c += A[(size_t)i * n + K0 + q] * L[q][a];
+    C[(size_t)i * kGJPanel + a] = c;
+    const int j = i;  // reuse the index space for R (p rows x n columns)
+    R[(size_t)a * n + j] = A[(size_t)(K0 + a) * n + j];
+  }
+}
+
+// Part 2: the panel update of every entry from C, R and T only.
+__global__ __launch_bounds__(256) void k_gj_update(double* A, int n, int K0, int p,
+                                                   const double* __restrict__ C,
+                                                   const double* __restrict__ R,
+                                                   const double* __restrict__ Tg) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const bool ki = i >= K0 && i < K0 + p, kj = j >= K0 && j < K0 + p;
+  double v;
+  if (ki && kj) {
+    v = Tg[(i - K0) * kGJPanel + (j - K0)];
+  } else if (ki) {
+    v = 0.0;
+    for (int a = 0; a < p; ++a) v += Tg[(i - K0) * kGJPanel + a] * R[(size_t)a * n + j];
+  } else if (kj) {
+    v = -C[(size_t)i * kGJPanel + (j - K0)];
+  } else {
+    v = A[e];
+    for (int a = 0; a < p; ++a) v -= C[(size_t)i * kGJPanel + a] * R[(size_t)a * n + j];
+  }
+  A[e] = v;
+}
+
+template <bool L0>
+__global__ __launch_bounds__(256) void k_dense_store(FineOp L, const double* __restrict__ A, float* cinv) {
+  const int n = L.nx * L.ny;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const bool on = fine_active<L0>(L, i) && fine_active<L0>(L, j);
+  cinv[(size_t)j * n + i] = on ? (float)A[e] : 0.0f;  // column-major
+}
 
 static void* dalloc(bemcs_poisson_s* h, size_t bytes) {
   void* p = nullptr;
@@ -892,151 +1221,83 @@ static void* dalloc(bemcs_poisson_s* h, size_t bytes) {
   return p;
 }
 
-// Galerkin coarse operator P^T A P for bilinear P (w1d tensor product),
-// P rows zero at inactive fine nodes.
-static HostLevel coarsen(const HostLevel& F) {
-  HostLevel C;
-  C.nx = (F.nx + 1) / 2;
-  C.ny = (F.ny + 1) / 2;
-  C.periodic = F.periodic;
-  const int nF = F.nx * F.ny, nC = C.nx * C.ny;
-  C.st.assign((size_t)9 * nC, 0.0);
-  C.active.assign(nC, 0);
-  auto parents = [](int f, int nf, int nc, bool per, int* idx, double* w) -> int {
-    if ((f & 1) == 0) {
-      idx[0] = f >> 1;
-      w[0] = 1.0;
-      return 1;
-    }
-    int a = (f - 1) >> 1, b = (f + 1) >> 1;
-    if (b >= nc) {
-      if (per) b -= nc;
-      else {
-        idx[0] = a;
-        w[0] = 1.0;
-        return 1;
-      }
-    }
-    if (a == b) {
-      idx[0] = a;
-      w[0] = 1.0;
-      return 1;
-    }
-    idx[0] = a;
-    w[0] = 0.5;
-    idx[1] = b;
-    w[1] = 0.5;
-    return 2;
-  };
-  for (int f = 0; f < nF; ++f) {
-    if (!F.active[f]) continue;
-    int fx = f % F.nx, fy = f / F.nx;
-    int pxi[2], pyi[2];
-    double pxw[2], pyw[2];
-    int npx = parents(fx, F.nx, C.nx, false, pxi, pxw);
-    int npy = parents(fy, F.ny, C.ny, F.periodic, pyi, pyw);
-    for (int k = 0; k < 9; ++k) {
-      double a = F.st[(size_t)k * nF + f];
-      if (a == 0.0) continue;
-      int gx = fx + (k % 3 - 1), gy = fy + (k / 3 - 1);
-      if (gx < 0 || gx >= F.nx) continue;
-      if (gy < 0 || gy >= F.ny) {
-        if (!F.periodic) continue;
-        gy = (gy + F.ny) % F.ny;
-      }
-      int g = gy * F.nx + gx;
-      if (!F.active[g]) continue;
-      int qxi[2], qyi[2];
-      double qxw[2], qyw[2];
-      int nqx = parents(gx, F.nx, C.nx, false, qxi, qxw);
-      int nqy = parents(gy, F.ny, C.ny, F.periodic, qyi, qyw);
-      for (int ay = 0; ay < npy; ++ay)
-        for (int ax = 0; ax < npx; ++ax) {
-          int I = pyi[ay] * C.nx + pxi[ax];
-          double w1 = pxw[ax] * pyw[ay] * a;
-          for (int by = 0; by < nqy; ++by)
-            for (int bx = 0; bx < nqx; ++bx) {
-              int ddx = qxi[bx] - pxi[ax];
-              int ddy = qyi[by] - pyi[ay];
-              if (C.periodic) {
-                if (ddy > 1) ddy -= C.ny;
-                if (ddy < -1) ddy += C.ny;
-              }
-              int kk = (ddy + 1) * 3 + (ddx + 1);
-              C.st[(size_t)kk * nC + I] += w1 * qxw[bx] * qyw[by];
-            }
-        }
-    }
+static FineOp fine_op(const bemcs_poisson_s* h, int l) {
+  FineOp F{};
+  F.nx = h->lev[l].nx;
+  F.ny = h->lev[l].ny;
+  if (l == 0) {
+    F.kind = h->kind;
+    F.code = h->lev[0].code;
+  } else {
+    F.st = h->su_st[(l - 1) & 1];
+    F.act = h->su_act[(l - 1) & 1];
   }
-  for (int I = 0; I < nC; ++I) C.active[I] = C.st[(size_t)4 * nC + I] > 1e-12 ? 1 : 0;
-  // drop couplings to inactive coarse nodes and from inactive rows
-  for (int I = 0; I < nC; ++I) {
-    int cx = I % C.nx, cy = I / C.nx;
-    for (int k = 0; k < 9; ++k) {
-      int gx = cx + (k % 3 - 1), gy = cy + (k / 3 - 1);
-      bool ok = gx >= 0 && gx < C.nx;
-      if (gy < 0 || gy >= C.ny) {
-        if (C.periodic) gy = (gy + C.ny) % C.ny;
-        else ok = false;
-      }
-      if (!ok || !C.active[I] || !C.active[gy * C.nx + gx]) C.st[(size_t)k * nC + I] = 0.0;
-    }
-  }
-  return C;
+  return F;
 }
 
-// Dense inverse of the active block (Gauss-Jordan with partial pivoting),
-// returned column-major: element (i, j) at j * n + i.
-static std::vector<float> dense_inverse(const HostLevel& L) {
-  const int n = L.nx * L.ny;
-  std::vector<int> act;
-  for (int i = 0; i < n; ++i)
-    if (L.active[i]) act.push_back(i);
-  const int m = (int)act.size();
-  std::vector<int> pos(n, -1);
-  for (int k = 0; k < m; ++k) pos[act[k]] = k;
-  std::vector<double> A((size_t)m * m, 0.0), Inv((size_t)m * m, 0.0);
-  for (int k = 0; k < m; ++k) {
-    int I = act[k], cx = I % L.nx, cy = I / L.nx;
-    for (int s = 0; s < 9; ++s) {
-      double c = L.st[(size_t)s * n + I];
-      if (c == 0.0) continue;
-      int gx = cx + (s % 3 - 1), gy = cy + (s / 3 - 1);
-      if (gy < 0 || gy >= L.ny) gy = (gy + L.ny) % L.ny;
-      int j = pos[gy * L.nx + gx];
-      if (j >= 0) A[(size_t)k * m + j] += c;
-    }
-    Inv[(size_t)k * m + k] = 1.0;
+// Builds the operator of electrode map is_bound (device) into h; asynchronous
+// on s; validation failures are OR-ed into h->su_err.
+static int setup_operator(bemcs_poisson_s* h, const uint8_t* is_bound, hipStream_t s) {
+  const int nx = h->nx, ny = h->ny, n = nx * ny, per = h->periodic;
+  BEMCS_CHECK_HIP(hipMemsetAsync(h->su_err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_setup_kind, dim3(grid_for(n, 256)), dim3(256), 0, s, is_bound, h->kind, nx, ny, per);
+  hipLaunchKernelGGL(k_setup_mirror, dim3(grid_for(n, 256)), dim3(256), 0, s, h->kind, h->mirror,
+                     h->mirror1, nx, ny, h->su_err);
+  hipLaunchKernelGGL(k_setup_level0, dim3(grid_for(n, 256)), dim3(256), 0, s, h->kind, h->mirror1,
+                     const_cast<uint8_t*>(h->lev[0].code), const_cast<float*>(h->lev[0].inv_d), nx, ny,
+                     per, h->su_err);
+  for (int l = 1; l < h->nlev; ++l) {
+    const LevelDev& Cl = h->lev[l];
+    FineOp F = fine_op(h, l - 1);
+    double* st = h->su_st[(l - 1) & 1];
+    uint8_t* act = h->su_act[(l - 1) & 1];
+    if (l == 1)
+      hipLaunchKernelGGL(k_galerkin<true>, dim3(grid_for(Cl.n, 256)), dim3(256), 0, s, F, st, act, Cl.nx,
+                         Cl.ny, per);
+    else
+      hipLaunchKernelGGL(k_galerkin<false>, dim3(grid_for(Cl.n, 256)), dim3(256), 0, s, F, st, act, Cl.nx,
+                         Cl.ny, per);
+    hipLaunchKernelGGL(k_galerkin_finish, dim3(grid_for(Cl.n, 256)), dim3(256), 0, s, st, act,
+                       const_cast<float*>(Cl.st), const_cast<float*>(Cl.inv_d), Cl.nx, Cl.ny, per);
   }
-  for (int c = 0; c < m; ++c) {
-    int piv = c;
-    for (int r = c + 1; r < m; ++r)
-      if (fabs(A[(size_t)r * m + c]) > fabs(A[(size_t)piv * m + c])) piv = r;
-    if (piv != c)
-      for (int j = 0; j < m; ++j) {
-        std::swap(A[(size_t)c * m + j], A[(size_t)piv * m + j]);
-        std::swap(Inv[(size_t)c * m + j], Inv[(size_t)piv * m + j]);
-      }
-    double d = A[(size_t)c * m + c];
-    if (fabs(d) < 1e-300) continue;
-    for (int j = 0; j < m; ++j) {
-      A[(size_t)c * m + j] /= d;
-      Inv[(size_t)c * m + j] /= d;
-    }
-    for (int r = 0; r < m; ++r) {
-      if (r == c) continue;
-      double f = A[(size_t)r * m + c];
-      if (f == 0.0) continue;
-      for (int j = 0; j < m; ++j) {
-        A[(size_t)r * m + j] -= f * A[(size_t)c * m + j];
-        Inv[(size_t)r * m + j] -= f * Inv[(size_t)c * m + j];
-      }
-    }
+  const int lc = h->nlev - 1;
+  const FineOp Lc = fine_op(h, lc);
+  const int nc = h->nc;
+  const int64_t nn = (int64_t)nc * nc;
+  const unsigned gd = grid_for(nn, 256);
+  if (lc == 0) hipLaunchKernelGGL(k_dense_assemble<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, per);
+  else hipLaunchKernelGGL(k_dense_assemble<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, per);
+  for (int K0 = 0; K0 < nc; K0 += kGJPanel) {
+    const int p = std::min(kGJPanel, nc - K0);
+    const unsigned gp = std::min<unsigned>(grid_for((int64_t)nc * p, 256), 1024u);
+    hipLaunchKernelGGL(k_gj_panel, dim3(gp), dim3(256), 0, s, h->su_A, nc, K0, p, h->su_C, h->su_R, h->su_T);
+    hipLaunchKernelGGL(k_gj_update, dim3(gd), dim3(256), 0, s, h->su_A, nc, K0, p, h->su_C, h->su_R,
+                       h->su_T);
   }
-  std::vector<float> out((size_t)n * n, 0.0f);
-  for (int a = 0; a < m; ++a)
-    for (int b = 0; b < m; ++b) out[(size_t)act[b] * n + act[a]] = (float)Inv[(size_t)a * m + b];
-  return out;
+  if (lc == 0) hipLaunchKernelGGL(k_dense_store<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->cinv);
+  else hipLaunchKernelGGL(k_dense_store<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->cinv);
+  BEMCS_CHECK_LAUNCH();
+  h->have_guess = false;  // the next Picard call restarts from V on the new unknowns
+  return BEMCS_OK;
+}
+
+static int setup_status(bemcs_poisson_s* h, hipStream_t s) {
+  int err = 0;
+  BEMCS_CHECK_HIP(hipMemcpyAsync(&err, h->su_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  BEMCS_CHECK_HIP(hipStreamSynchronize(s));
+  if (err & SETUP_ERR_CHAIN) {
+    set_error("bemcs_poisson: Neumann chain longer than two nodes");
+    return BEMCS_EINVAL;
+  }
+  if (err & SETUP_ERR_MIRROR) {
+    set_error("bemcs_poisson: Neumann neighbour mirrors another node");
+    return BEMCS_EINVAL;
+  }
+  if (err & SETUP_ERR_EDGE) {
+    set_error("bemcs_poisson: unknown node on the domain edge (column 0 must be Dirichlet)");
+    return BEMCS_EINVAL;
+  }
+  return BEMCS_OK;
 }
 
 static int upload(void* dst, const void* src, size_t bytes) {
@@ -1589,120 +1850,49 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
   *out = nullptr;
   const int n = nx * ny;
   const bool per = periodic_y != 0;
-  // node kinds, as build_sparse_matrix assigns them
-  std::vector<uint8_t> kind(n, K_UNKNOWN);
-  for (int y = 0; y < ny; ++y)
-    for (int x = 0; x < nx; ++x) {
-      int i = y * nx + x;
-      bool b = is_bound_host[i] != 0;
-      if (b) kind[i] = K_BOUND;
-      else if (x == nx - 1) kind[i] = K_RIGHT;
-      else if (!per && y == ny - 1) kind[i] = K_TOP;
-      else if (!per && y == 0) kind[i] = K_BOTTOM;
-    }
-  auto mirror1_of = [&](int i) -> int {
-    int x = i % nx, y = i / nx;
-    if (kind[i] == K_RIGHT) return y * nx + (nx - 2);
-    if (kind[i] == K_TOP) return (ny - 2) * nx + x;
-    if (kind[i] == K_BOTTOM) return nx + x;
-    return -1;
-  };
-  std::vector<int32_t> mirror(n, -1), mirror1(n, -1);
-  for (int i = 0; i < n; ++i) {
-    int m = mirror1_of(i);
-    if (m < 0) continue;
-    mirror1[i] = m;
-    int hops = 0;
-    while (kind[m] != K_UNKNOWN && kind[m] != K_BOUND && hops < 4) {
-      m = mirror1_of(m);
-      ++hops;
-    }
-    BEMCS_REQUIRE(hops < 2, "bemcs_poisson_create: Neumann chain longer than two nodes");
-    mirror[i] = m;
-  }
-  // fine operator: diagonal 4 - #Neumann neighbours at unknown nodes
-  HostLevel L0;
-  L0.nx = nx;
-  L0.ny = ny;
-  L0.periodic = per;
-  L0.st.assign((size_t)9 * n, 0.0);
-  L0.active.assign(n, 0);
-  std::vector<uint8_t> code(n, 0);
-  for (int i = 0; i < n; ++i) {
-    if (kind[i] != K_UNKNOWN) continue;
-    int x = i % nx, y = i / nx;
-    BEMCS_REQUIRE(x > 0 && x < nx - 1, "bemcs_poisson_create: unknown node on x edge");
-    int s = y > 0 ? i - nx : (per ? i + (ny - 1) * nx : -1);
-    int nn = y < ny - 1 ? i + nx : (per ? i - (ny - 1) * nx : -1);
-    BEMCS_REQUIRE(s >= 0 && nn >= 0, "bemcs_poisson_create: unknown node on y edge");
-    int nb[4] = {i - 1, i + 1, s, nn};
-    int kk[4] = {3, 5, 1, 7};
-    int d = 4;
-    for (int k = 0; k < 4; ++k) {
-      uint8_t t = kind[nb[k]];
-      if (t == K_RIGHT || t == K_TOP || t == K_BOTTOM) {
-        BEMCS_REQUIRE(mirror1[nb[k]] == i, "bemcs_poisson_create: Neumann neighbour mirrors another node");
-        --d;
-      } else if (t == K_UNKNOWN) {
-        L0.st[(size_t)kk[k] * n + i] = -1.0;
-      }
-    }
-    code[i] = (uint8_t)d;
-    L0.st[(size_t)4 * n + i] = d;
-    L0.active[i] = 1;
-  }
-  std::vector<HostLevel> H;
-  H.push_back(std::move(L0));
-  while ((int)H.size() < kMaxLevels) {
-    const HostLevel& F = H.back();
-    int ncx = (F.nx + 1) / 2, ncy = (F.ny + 1) / 2;
-    if (F.nx * F.ny <= kCoarseMax) break;
+  // level shapes (they depend only on the mesh): halve until the coarsest
+  // level has at most kCoarseMax nodes
+  int lnx[kMaxLevels], lny[kMaxLevels], nlev = 1;
+  lnx[0] = nx;
+  lny[0] = ny;
+  while (nlev < kMaxLevels) {
+    const int fx = lnx[nlev - 1], fy = lny[nlev - 1];
+    if (fx * fy <= kCoarseMax) break;
+    const int ncx = (fx + 1) / 2, ncy = (fy + 1) / 2;
     if (ncx < 3 || ncy < (per ? 4 : 3)) break;
-    H.push_back(coarsen(F));
+    lnx[nlev] = ncx;
+    lny[nlev] = ncy;
+    ++nlev;
   }
-  BEMCS_REQUIRE(H.back().nx * H.back().ny <= kDenseMax,
-                "bemcs_poisson_create: coarsest level too large for the dense solve");
+  const int nc = lnx[nlev - 1] * lny[nlev - 1];
+  BEMCS_REQUIRE(nc <= kDenseMax, "bemcs_poisson_create: coarsest level too large for the dense solve");
 
   bemcs_poisson_s* h = new bemcs_poisson_s();
   h->nx = nx;
   h->ny = ny;
   h->periodic = per ? 1 : 0;
-  h->nlev = (int)H.size();
+  h->nlev = nlev;
   h->have_guess = false;
   h->own = nullptr;
+  h->nc = nc;
   bool ok = true;
   auto A = [&](size_t bytes) {
     void* p = dalloc(h, bytes);
     if (!p) ok = false;
     return p;
   };
-  for (int l = 0; l < h->nlev && ok; ++l) {
-    const HostLevel& HL = H[l];
+  for (int l = 0; l < nlev && ok; ++l) {
     LevelDev& D = h->lev[l];
-    D.nx = HL.nx;
-    D.ny = HL.ny;
-    D.n = HL.nx * HL.ny;
+    D.nx = lnx[l];
+    D.ny = lny[l];
+    D.n = D.nx * D.ny;
     D.periodic = per ? 1 : 0;
-    std::vector<float> inv(D.n, 0.0f);
-    for (int i = 0; i < D.n; ++i)
-      if (HL.active[i]) inv[i] = (float)(1.0 / HL.st[(size_t)4 * D.n + i]);
-    float* dinv = (float*)A(sizeof(float) * D.n);
-    if (!ok) break;
-    ok = upload(dinv, inv.data(), sizeof(float) * D.n) == 0;
-    D.inv_d = dinv;
+    D.inv_d = (float*)A(sizeof(float) * D.n);
     if (l == 0) {
-      uint8_t* dc = (uint8_t*)A(D.n);
-      if (!ok) break;
-      ok = upload(dc, code.data(), D.n) == 0;
-      D.code = dc;
+      D.code = (uint8_t*)A(D.n);
       D.st = nullptr;
     } else {
-      std::vector<float> st((size_t)9 * D.n);
-      for (size_t k = 0; k < st.size(); ++k) st[k] = (float)HL.st[k];
-      float* ds = (float*)A(sizeof(float) * st.size());
-      if (!ok) break;
-      ok = upload(ds, st.data(), sizeof(float) * st.size()) == 0;
-      D.st = ds;
+      D.st = (float*)A(sizeof(float) * 9 * D.n);
       D.code = nullptr;
     }
     D.x = (float*)A(sizeof(float) * D.n);
@@ -1713,16 +1903,10 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     if (ok) ok = hipMemset(D.b, 0, sizeof(float) * D.n) == hipSuccess;
   }
   if (ok) {
-    std::vector<float> cinv = dense_inverse(H.back());
-    h->nc = H.back().nx * H.back().ny;
-    h->cinv = (float*)A(sizeof(float) * cinv.size());
-    if (ok) ok = upload(h->cinv, cinv.data(), sizeof(float) * cinv.size()) == 0;
+    h->cinv = (float*)A(sizeof(float) * (size_t)nc * nc);
     h->kind = (uint8_t*)A(n);
-    if (ok) ok = upload(h->kind, kind.data(), n) == 0;
     h->mirror = (int32_t*)A(sizeof(int32_t) * n);
-    if (ok) ok = upload(h->mirror, mirror.data(), sizeof(int32_t) * n) == 0;
     h->mirror1 = (int32_t*)A(sizeof(int32_t) * n);
-    if (ok) ok = upload(h->mirror1, mirror1.data(), sizeof(int32_t) * n) == 0;
     h->x = (double*)A(sizeof(double) * n);
     h->R[0] = (double*)A(sizeof(double) * n);
     h->R[1] = (double*)A(sizeof(double) * n);
@@ -1736,19 +1920,49 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     h->pi = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pz[0] = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pz[1] = (double*)A(sizeof(double) * 2 * h->nparts);
-
+    // set-up workspace, kept for in-place updates
+    const int n1 = nlev > 1 ? h->lev[1].n : 1;
+    h->su_st[0] = (double*)A(sizeof(double) * 9 * n1);
+    h->su_st[1] = (double*)A(sizeof(double) * 9 * n1);
+    h->su_act[0] = (uint8_t*)A(n1);
+    h->su_act[1] = (uint8_t*)A(n1);
+    h->su_A = (double*)A(sizeof(double) * (size_t)nc * nc);
+    h->su_C = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
+    h->su_R = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
+    h->su_T = (double*)A(sizeof(double) * kGJPanel * kGJPanel);
+    h->su_err = (int*)A(sizeof(int));
     if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->x, 0, sizeof(double) * n) == hipSuccess;
   }
-  if (ok) ok = create_set(&h, 1, &h->own) == BEMCS_OK;
-  if (!ok) {
+  uint8_t* ib = nullptr;
+  if (ok) ok = hipMalloc(&ib, n) == hipSuccess;
+  if (ok) ok = upload(ib, is_bound_host, n) == 0;
+  int rc = BEMCS_OK;
+  if (ok) {
+    rc = setup_operator(h, ib, nullptr);
+    if (rc == BEMCS_OK) rc = setup_status(h, nullptr);
+  }
+  if (ib) (void)hipFree(ib);
+  if (ok && rc == BEMCS_OK) ok = create_set(&h, 1, &h->own) == BEMCS_OK;
+  if (!ok || rc != BEMCS_OK) {
     destroy(h);
-    set_error("bemcs_poisson_create: device allocation or upload failed");
-    return BEMCS_EHIP;
+    if (rc == BEMCS_OK) {
+      set_error("bemcs_poisson_create: device allocation or upload failed");
+      rc = BEMCS_EHIP;
+    }
+    return rc;
   }
   *out = h;
   return BEMCS_OK;
+}
+
+int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(h && is_bound, "bemcs_poisson_update: bad arguments");
+  hipStream_t s = as_stream(stream);
+  int rc = setup_operator(h, is_bound, s);
+  if (rc != BEMCS_OK) return rc;
+  return setup_status(h, s);
 }
 
 int bemcs_poisson_free(bemcs_poisson_t h) {

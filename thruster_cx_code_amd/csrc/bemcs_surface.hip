@@ -9,6 +9,9 @@
 //                 (:1359-1388, :1563-1617)
 //   k_grid_means  per-grid mean temperatures from the partial sums
 //   k_rf_set      V_fixed of the RF grid = Vdc + A sin(2 pi f t) (:984-991)
+//   k_electrodes  electrode map, Dirichlet values and grid masks of the grids,
+//                 bow included (:791-843)
+//   k_grid_sums   per-grid temperature sums (grid means at a domain build)
 #include "bemcs_internal.h"
 
 namespace bemcs {
@@ -270,11 +273,98 @@ __global__ __launch_bounds__(kBlock) void k_grid_means(const double* __restrict_
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rf_set(double* V_fixed, const int32_t* __restrict__ idx,
-                                                   const double* __restrict__ vdc, int64_t n,
-                                                   double v_rf) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k < n) V_fixed[idx[k]] = vdc[k] + v_rf;
+// RF grid nodes: their DC potential is that of the last grid covering them
+// (grids are rasterised in order, later ones overwrite, :826-827).
+struct GridV {
+  double v[kMaxGrids];
+};
+
+__global__ __launch_bounds__(kBlock) void k_rf_set(double* V_fixed, const uint8_t* __restrict__ bits,
+                                                   int64_t n, int n_grids, GridV gv, int rf, double v_rf) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const unsigned b = bits[i] & ((1u << n_grids) - 1u);
+  if (!((b >> rf) & 1u)) return;
+  const int last = 31 - __clz(b);
+  V_fixed[i] = gv.v[last] + v_rf;
+}
+
+// Electrode map of the grids (:791-843): node coordinates as np.linspace,
+// bowed grid faces x + delta eta^2, apertures with chamfer, in grid order;
+// column 0 is the plasma boundary (Dirichlet).
+__global__ __launch_bounds__(kBlock) void k_electrodes(bemcs_electrodes_t a) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nx = a.nx, ny = a.ny;
+  if (i >= (int64_t)nx * ny) return;
+  const int ix = (int)(i % nx), iy = (int)(i / nx);
+  const double X = ix == nx - 1 ? a.Lx : (double)ix * (a.Lx / (double)(nx - 1));
+  const double Y = iy == ny - 1 ? a.Ly : (double)iy * (a.Ly / (double)(ny - 1));
+  uint8_t bound = 0, bits = 0;
+  double vf = 0.0;
+  for (int g = 0; g < a.n_grids; ++g) {
+    const bemcs_grid_geom_t& G = a.grids[g];
+    double dxb = 0.0;
+    if (fabs(G.delta) > 1e-6) {
+      double eta;
+      if (a.geometry == 2) {
+        const double yw = 1.5 * a.screen_r + 0.5 * a.pitch;
+        const double m = a.pitch * 0.5 > 1e-6 ? a.pitch * 0.5 : 1e-6;
+        eta = fmin(fmax(1.0 - fabs(Y - yw) / m, 0.0), 1.0);
+        dxb = G.delta * (eta * eta);
+      } else if (a.geometry == 1) {
+        const double ym = 1.5 * a.screen_r;
+        const double m = a.Ly * 0.5 > 1e-6 ? a.Ly * 0.5 : 1e-6;
+        eta = fmin(fmax(1.0 - fabs(Y - ym) / m, 0.0), 1.0);
+        dxb = G.delta * (eta * eta);
+      } else {
+        const double Lc = a.Ly - G.r;
+        if (Lc > 0) {
+          eta = fmin(fmax((a.Ly - Y) / Lc, 0.0), 1.0);
+          dxb = G.delta * (eta * eta);
+        }
+      }
+    }
+    const bool inx = (X >= G.gstart + dxb) && (X <= G.gend + dxb);
+    bool mask = inx;
+    for (int h = 0; h < a.n_holes; ++h) {
+      double lr = G.r - fmax(0.0, X - G.gstart - dxb) * G.tan_cham;
+      lr = fmax(lr, 0.0);
+      if (inx && fabs(Y - a.hole_y[h]) <= lr) mask = false;
+    }
+    if (mask) {
+      bound = 1;
+      vf = G.V;
+      bits |= (uint8_t)(1u << g);
+    }
+  }
+  if (ix == 0) {
+    bound = 1;
+    vf = a.V_plasma;
+  }
+  a.is_bound[i] = bound;
+  a.V_fixed[i] = vf;
+  a.grid_bits[i] = bits;
+}
+
+// Per-grid sums of T over a node range (grid stride), partials per work-group.
+__global__ __launch_bounds__(kBlock) void k_grid_sums(const float* __restrict__ T,
+                                                      const uint8_t* __restrict__ bits, int64_t n,
+                                                      int n_grids, double* part) {
+  __shared__ double red[2 * (kBlock / 64)];
+  double gsum[kMaxGrids], gcnt[kMaxGrids];
+#pragma unroll
+  for (int g = 0; g < kMaxGrids; ++g) gsum[g] = gcnt[g] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const uint8_t b = bits[i];
+    if (!b) continue;
+    const double v = (double)T[i];
+    for (int g = 0; g < n_grids; ++g)
+      if (b & (1u << g)) {
+        gsum[g] += v;
+        gcnt[g] += 1.0;
+      }
+  }
+  grid_partials(gsum, gcnt, n_grids, part + (size_t)blockIdx.x * 2 * n_grids, red);
 }
 
 inline int thermal_blocks(int nx, int ny, dim3* grid) {
@@ -342,12 +432,41 @@ int bemcs_thermal(const bemcs_thermal_t* p, bemcs_stream_t stream) {
   return BEMCS_OK;
 }
 
-int bemcs_rf_set(double* V_fixed, const int32_t* idx, const double* vdc, int64_t n, double v_rf,
-                 bemcs_stream_t stream) {
-  BEMCS_REQUIRE(n >= 0 && (n == 0 || (V_fixed && idx && vdc)), "bemcs_rf_set: bad arguments");
-  if (n == 0) return BEMCS_OK;
-  hipLaunchKernelGGL(k_rf_set, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), V_fixed,
-                     idx, vdc, n, v_rf);
+int bemcs_rf_set(double* V_fixed, const uint8_t* grid_bits, int64_t n_nodes, int32_t n_grids,
+                 const double* grid_V, int32_t rf_grid, double v_rf, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(V_fixed && grid_bits && grid_V && n_nodes > 0 && n_grids >= 1 && n_grids <= kMaxGrids &&
+                    rf_grid >= 0 && rf_grid < n_grids,
+                "bemcs_rf_set: bad arguments (1 <= n_grids <= 8, 0 <= rf_grid < n_grids)");
+  GridV gv{};
+  for (int g = 0; g < n_grids; ++g) gv.v[g] = grid_V[g];
+  hipLaunchKernelGGL(k_rf_set, dim3(grid_for(n_nodes, kBlock)), dim3(kBlock), 0, as_stream(stream),
+                     V_fixed, grid_bits, n_nodes, (int)n_grids, gv, (int)rf_grid, v_rf);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_electrodes(const bemcs_electrodes_t* p, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(p && p->is_bound && p->V_fixed && p->grid_bits && p->nx >= 3 && p->ny >= 3 &&
+                    p->n_grids >= 0 && p->n_grids <= kMaxGrids && p->n_holes >= 0 && p->n_holes <= 2,
+                "bemcs_electrodes: bad arguments (n_grids <= 8, n_holes <= 2)");
+  const int64_t n = (int64_t)p->nx * p->ny;
+  hipLaunchKernelGGL(k_electrodes, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *p);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_grid_means(const float* T, const uint8_t* grid_bits, int32_t nx, int32_t ny, int32_t n_grids,
+                     void* workspace, float* T_grids, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(T && grid_bits && workspace && T_grids && nx >= 1 && ny >= 1 && n_grids >= 1 &&
+                    n_grids <= kMaxGrids,
+                "bemcs_grid_means: bad arguments");
+  dim3 g;
+  const int nb = thermal_blocks(nx, ny, &g);
+  hipLaunchKernelGGL(k_grid_sums, dim3(nb), dim3(kBlock), 0, as_stream(stream), T, grid_bits,
+                     (int64_t)nx * ny, (int)n_grids, reinterpret_cast<double*>(workspace));
+  BEMCS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_grid_means, dim3(1), dim3(kBlock), 0, as_stream(stream),
+                     reinterpret_cast<const double*>(workspace), nb, (int)n_grids, T_grids);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

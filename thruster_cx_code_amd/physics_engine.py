@@ -91,7 +91,8 @@ class DigitalTwinSimulator:
         # set_material runs (:450-452, :473), not the built mesh's
         self.set_material('Molybdenum')
         self.T_grids = []
-        self.mask_grids = []
+        self.n_grids = 0
+        self._host_maps = {}
         self.iteration = 0
         self.seed = 0
         self._poisson = None
@@ -186,7 +187,8 @@ class DigitalTwinSimulator:
                 f"vmax={vmax:.2e} m/s.")
         self.xpts = np.linspace(0, self.Lx, self.nx)
         self.ypts = np.linspace(0, self.Ly, self.ny)
-        X, Y = np.meshgrid(self.xpts, self.ypts)
+        if len(grids) > 8:
+            raise ValueError("at most 8 grids are supported")
         if len(getattr(self, 'grid_deflections', [])) != len(grids):
             self.grid_deflections = [0.0] * len(grids)
         inj_time = params.get("inj_time", 0.0)
@@ -195,11 +197,7 @@ class DigitalTwinSimulator:
         n0 = params.get('n0_plasma', 1e17)
         cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
         self.macro_weight = max(n0 * cell_vol / 80.0, 1e3)
-
-        isBound = np.zeros((self.ny, self.nx), bool)
-        V_fixed = np.zeros((self.ny, self.nx))
-        self.mask_grids = []
-        self.T_grids = []
+        # electrode geometry (:775-843), rasterised on the device
         if grids:
             sr = grids[0]['r']
             centers = {'two_holes': [1.5 * sr, 1.5 * sr + pitch], 'one_hole': [1.5 * sr]}.get(geometry, [0.0])
@@ -207,50 +205,39 @@ class DigitalTwinSimulator:
             centers = []
         self.hole_centers = centers
         self.grid_x_starts, self.grid_x_ends = [], []
+        el = _lib.Electrodes()
         cur = self.upstream_gap_mm
         for i, g in enumerate(grids):
             gs, ge = cur, cur + g["t"]
             self.grid_x_starts.append(gs)
             self.grid_x_ends.append(ge)
-            d = self.grid_deflections[i] if i < len(self.grid_deflections) else 0.0
-            dxb = 0.0
-            if abs(d) > 1e-6:
-                if geometry == 'two_holes':
-                    eta = np.clip(1.0 - np.abs(Y - (1.5 * sr + 0.5 * pitch)) / max(pitch * 0.5, 1e-6), 0.0, 1.0)
-                    dxb = d * eta ** 2
-                elif geometry == 'one_hole':
-                    eta = np.clip(1.0 - np.abs(Y - 1.5 * sr) / max(self.Ly * 0.5, 1e-6), 0.0, 1.0)
-                    dxb = d * eta ** 2
-                else:
-                    Lc = self.Ly - g["r"]
-                    dxb = d * np.clip((self.Ly - Y) / Lc, 0.0, 1.0) ** 2 if Lc > 0 else 0.0
-            inx = (X >= gs + dxb) & (X <= ge + dxb)
-            mask = inx.copy()
-            for yc in centers:
-                lr = np.maximum(g["r"] - np.maximum(0.0, X - gs - dxb) * np.tan(np.radians(g["cham"])), 0.0)
-                mask &= ~(inx & (np.abs(Y - yc) <= lr))
-            isBound[mask] = True
-            V_fixed[mask] = g["V"]
-            self.mask_grids.append(mask)
+            G = el.grids[i]
+            G.gstart, G.gend, G.V, G.r = gs, ge, float(g["V"]), float(g["r"])
+            G.tan_cham = float(np.tan(np.radians(g["cham"])))
+            G.delta = float(self.grid_deflections[i])
             cur = ge + g["gap"]
-        self.Vdc = V_fixed.copy()
-        vpb = (grids[0]["V"] if grids else 1000.0) + params.get("V_plasma_offset", 20.0)
-        V_fixed[:, 0] = vpb
-        isBound[:, 0] = True
-        if len(grids) > 8 and params.get('sim_mode', 'Both') != 'Trajectories':
-            raise ValueError("thermal and erosion modes support at most 8 grids")
-        self._isBound_host = isBound
-        self._V_fixed_host = V_fixed
+        for h, yc in enumerate(centers):
+            el.hole_y[h] = yc
+        el.Lx, el.Ly = self.Lx, self.Ly
+        el.screen_r = grids[0]['r'] if grids else 0.0
+        el.pitch = pitch
+        el.V_plasma = (grids[0]["V"] if grids else 1000.0) + params.get("V_plasma_offset", 20.0)
+        el.n_grids, el.n_holes, el.nx, el.ny = len(grids), len(centers), self.nx, self.ny
+        el.geometry = {'one_hole': 1, 'two_holes': 2}.get(geometry, 0)
+        self._electrodes = el
         self._grids = grids
-        self._rf_nodes = None
+        self.n_grids = len(grids)
+        self._grid_V = (C.c_double * max(len(grids), 1))(*[float(g["V"]) for g in grids])
         self._alloc_mesh(preserve_state)
         if not preserve_state:
             self._alloc_particles(params)
-        # grid temperatures over the (new) masks (:830-833)
-        Tm = self.Tmap
-        self.T_grids = [float(np.mean(Tm[m])) if np.any(m) else 300.0 for m in self.mask_grids]
-        if self.mask_grids:
-            self.T_grids_d.copy_(torch.tensor(self.T_grids, dtype=torch.float32))
+        # grid temperatures over the (new) masks (:830-833); 300 K for an empty mask
+        self.T_grids_d.fill_(300.0)
+        if self.n_grids:
+            check(self.lib.bemcs_grid_means(ptr(self.Tmap_d[self._tcur]), ptr(self.grid_bits_d), self.nx, self.ny,
+                                            self.n_grids, ptr(self.thermal_ws), ptr(self.T_grids_d), stream_ptr()),
+                  "grid_means")
+        self.T_grids = [float(v) for v in self.T_grids_d[:self.n_grids].cpu().numpy()]
         self._set_geometry_constants(params)
         self._built = True
         # the last step's deposit (none before the first step)
@@ -264,23 +251,19 @@ class DigitalTwinSimulator:
     def _alloc_mesh(self, preserve_state):
         """Device mesh state.  A rebuild with preserve_state (thermal bow,
         :1645) keeps the potential, deposit counts, temperature, damage and
-        erosion maps and the per-step records; a fresh build clears them as
-        reset_arrays does (:404-448)."""
+        erosion maps and the per-step records, rasterises the electrodes into
+        the existing arrays and updates the Poisson operator in place; a fresh
+        build clears them as reset_arrays does (:404-448)."""
         dev = self.device
         nn = self.nx * self.ny
         keep = preserve_state and getattr(self, '_nn', None) == nn
-        self._rebuild_operator()
-        self.is_bound_d = torch.from_numpy(self._isBound_host.astype(np.uint8).ravel()).to(dev)
-        self.V_fixed_d = torch.from_numpy(self._V_fixed_host.ravel().copy()).to(dev)
-        bits = np.zeros(nn, np.uint8)
-        for g, m in enumerate(self.mask_grids[:8]):
-            bits |= (m.ravel().astype(np.uint8) << g)
-        self.grid_bits_d = torch.from_numpy(bits).to(dev)
-        ng = len(self.mask_grids)
-        self.T_grids_d = torch.full((max(ng, 1),), 300.0, dtype=torch.float32, device=dev)
-        wsb = int(self.lib.bemcs_thermal_workspace_bytes(self.nx, self.ny, min(ng, 8)))
-        self.thermal_ws = zeros(max(wsb, 8), torch.uint8, dev)
         if not keep:
+            self.is_bound_d = zeros(nn, torch.uint8, dev)
+            self.V_fixed_d = zeros(nn, torch.float64, dev)
+            self.grid_bits_d = zeros(nn, torch.uint8, dev)
+            self.T_grids_d = torch.full((8,), 300.0, dtype=torch.float32, device=dev)
+            wsb = int(self.lib.bemcs_thermal_workspace_bytes(self.nx, self.ny, 8))
+            self.thermal_ws = zeros(max(wsb, 8), torch.uint8, dev)
             self.V_d = zeros(nn, torch.float64, dev)
             self.cnt_i = zeros(nn, torch.int32, dev)
             self.cnt_e = zeros(nn, torch.int32, dev)
@@ -299,9 +282,21 @@ class DigitalTwinSimulator:
             self._n_failed_seen = 0
             self._last_thermal_it = None
             self._tcur = 0
-        self.Exy = zeros(2 * nn, torch.float32, dev)
-        self.rho_d = zeros(nn, torch.float32, dev)
-        self.sort_scratch = zeros(nn + (nn + 1023) // 1024 + 1024, torch.int32, dev)
+            self.Exy = zeros(2 * nn, torch.float32, dev)
+            self.rho_d = zeros(nn, torch.float32, dev)
+            self.sort_scratch = zeros(nn + (nn + 1023) // 1024 + 1024, torch.int32, dev)
+        el = self._electrodes
+        el.is_bound, el.V_fixed, el.grid_bits = ptr(self.is_bound_d), ptr(self.V_fixed_d), ptr(self.grid_bits_d)
+        check(self.lib.bemcs_electrodes(C.byref(el), stream_ptr()), "electrodes")
+        self._host_maps = {}
+        if not preserve_state:
+            # a fresh domain starts at 300 K on the electrodes (:846)
+            self.Tmap_d[self._tcur].fill_(300.0)
+        if keep and self._poisson:
+            check(self.lib.bemcs_poisson_update(self._poisson, ptr(self.is_bound_d), stream_ptr()),
+                  "poisson_update")
+        else:
+            self._rebuild_operator()
         if not (preserve_state and hasattr(self, 'ring_diag')):
             self.ds_bits = zeros(2, torch.int64, dev)
             self.n_sub_d = torch.ones(2, dtype=torch.int32, device=dev)
@@ -319,14 +314,14 @@ class DigitalTwinSimulator:
         self._nn = nn
 
     def _rebuild_operator(self):
-        """Poisson operator of the current electrode map (build_sparse_matrix,
-        :525-626): a new MG hierarchy."""
+        """New Poisson operator and multigrid hierarchy of the current
+        electrode map (build_sparse_matrix, :525-626)."""
         if self._poisson:
             torch.cuda.current_stream().synchronize()
             self.lib.bemcs_poisson_free(self._poisson)
             self._poisson = None
         h = C.c_void_p()
-        ib = np.ascontiguousarray(self._isBound_host.astype(np.uint8))
+        ib = np.ascontiguousarray(self.is_bound_d.cpu().numpy())
         check(self.lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
                                             int(self.periodic_y), C.byref(h)), "poisson_create")
         self._poisson = h
@@ -602,7 +597,9 @@ class DigitalTwinSimulator:
         # RF co-extraction (:984-991): the RF grid at Vdc + A sin(2 pi f t), then
         # two Picard iterations on the previous step's deposit
         if params.get('rf_enable') and grids and params.get('rf_grid_idx', 0) < len(grids):
+            ev = self._timed_begin(self.kernel_timing)
             self._rf_update(params, t_current)
+            self._timed_end(self.kernel_timing, "poisson_rf", ev)
         self._inject(params, t_current)
         if it % 2 == 0:
             kt = self.kernel_timing
@@ -651,7 +648,9 @@ class DigitalTwinSimulator:
                                  C.byref(ip), self.bound_i, s), "ion_step")
         self._timed_end(kt, "ion_step", ev)
         if erosion:
+            ev = self._timed_begin(kt)
             self._erosion(params, diag_ptr, E1)
+            self._timed_end(kt, "erosion", ev)
         ep = self._elec_params
         ep.Exy, ep.is_bound, ep.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb + 4)
         ep.cnt_next, ep.overflow = ptr(self.cnt_e_next), ptr(self.overflow_d)
@@ -665,14 +664,16 @@ class DigitalTwinSimulator:
         self.cnt_i, self.cnt_i_next = self.cnt_i_next, self.cnt_i
         self.cnt_e, self.cnt_e_next = self.cnt_e_next, self.cnt_e
         if thermal:
+            ev = self._timed_begin(kt)
             self._thermal(it)
+            self._timed_end(kt, "thermal", ev)
         if thermal or erosion:
             # failed-cell total and grid temperatures, read back for the
             # remesh decisions (resolved before the next step)
-            hb = torch.empty(1 + len(self.mask_grids), dtype=torch.float32, pin_memory=True)
+            hb = torch.empty(1 + self.n_grids, dtype=torch.float32, pin_memory=True)
             hb[:1].view(torch.int32).copy_(self.n_failed_d, non_blocking=True)
-            if self.mask_grids:
-                hb[1:].copy_(self.T_grids_d[:len(self.mask_grids)], non_blocking=True)
+            if self.n_grids:
+                hb[1:].copy_(self.T_grids_d[:self.n_grids], non_blocking=True)
             sev = torch.cuda.Event()
             sev.record()
             self._surface_pending = (sev, hb, slot, params, thermal, erosion)
@@ -767,22 +768,16 @@ class DigitalTwinSimulator:
         t.workspace, t.T_grids = ptr(self.thermal_ws), ptr(self.T_grids_d)
         t.cool32, t.t300_4_32 = f(cf), f(300.0 ** 4)
         t.fo_x32, t.fo_y32 = f(fo_x), f(fo_y)
-        t.nx, t.ny, t.n_grids, t.sweeps = self.nx, self.ny, min(len(self.mask_grids), 8), 10
+        t.nx, t.ny, t.n_grids, t.sweeps = self.nx, self.ny, self.n_grids, 10
         check(self.lib.bemcs_thermal(C.byref(t), stream_ptr()), "thermal")
         self._tcur = 1 - self._tcur
         self._last_thermal_it = it
 
     def _rf_update(self, params, t_current):
-        rf_idx = params.get('rf_grid_idx', 0)
-        if self._rf_nodes is None or self._rf_nodes[0] != rf_idx:
-            idx = np.flatnonzero(self.mask_grids[rf_idx].ravel()).astype(np.int32)
-            vdc = self.Vdc.ravel()[idx].astype(np.float64)
-            self._rf_nodes = (rf_idx, torch.from_numpy(idx).to(self.device),
-                              torch.from_numpy(vdc).to(self.device))
-        _, idx_d, vdc_d = self._rf_nodes
+        rf_idx = int(params.get('rf_grid_idx', 0))
         v_rf = params.get('rf_amp', 100.0) * np.sin(2.0 * np.pi * (params.get('rf_freq', 13.56) * 1e6) * t_current)
-        check(self.lib.bemcs_rf_set(ptr(self.V_fixed_d), ptr(idx_d), ptr(vdc_d), int(idx_d.numel()),
-                                    float(v_rf), stream_ptr()), "rf_set")
+        check(self.lib.bemcs_rf_set(ptr(self.V_fixed_d), ptr(self.grid_bits_d), self.nx * self.ny, self.n_grids,
+                                    self._grid_V, rf_idx, float(v_rf), stream_ptr()), "rf_set")
         self.recalc_poisson(2, params, counts=(self.cnt_i_next, self.cnt_e_next))
 
     def _resolve_surface(self):
@@ -801,14 +796,15 @@ class DigitalTwinSimulator:
         remeshed = False
         if erosion and n_failed != self._n_failed_seen:
             self._n_failed_seen = n_failed
-            self._isBound_host = self.is_bound_d.view(self.ny, self.nx).cpu().numpy().astype(bool)
-            self._rebuild_operator()
+            self._host_maps = {}
+            check(self.lib.bemcs_poisson_update(self._poisson, ptr(self.is_bound_d), stream_ptr()),
+                  "poisson_update")
             remeshed = True
         if thermal:
+            # grids without nodes keep their value on the device (:1615-1617)
             tg = hb[1:].numpy()
-            for i, m in enumerate(self.mask_grids):
-                if np.any(m):
-                    self.T_grids[i] = np.float32(tg[i])
+            for i in range(self.n_grids):
+                self.T_grids[i] = np.float32(tg[i])
             grids = params.get('grids', [])
             dc = params.get("discharge_chamber")
             pitch = params.get("pitch_mm", dc.get("pitch_mm", 3.0) if isinstance(dc, dict) else 3.0)
@@ -1027,10 +1023,36 @@ class DigitalTwinSimulator:
     def rho(self):
         return self.rho_d.view(self.ny, self.nx).cpu().numpy()
 
+    def _host_map(self, name, make):
+        m = self._host_maps.get(name)
+        if m is None:
+            m = self._host_maps[name] = make()
+        return m
+
     @property
     def isBound(self):
         self._resolve_surface()
-        return self._isBound_host
+        return self._host_map('isBound', lambda: self.is_bound_d.view(self.ny, self.nx).cpu().numpy().astype(bool))
+
+    @property
+    def mask_grids(self):
+        if not self._built:
+            return []
+
+        def make():
+            bits = self.grid_bits_d.view(self.ny, self.nx).cpu().numpy()
+            return [(bits >> g) & 1 == 1 for g in range(self.n_grids)]
+        return self._host_map('mask_grids', make)
+
+    @property
+    def Vdc(self):
+        """Grid potentials before the plasma boundary and RF (:836)."""
+        def make():
+            out = np.zeros((self.ny, self.nx))
+            for g, m in enumerate(self.mask_grids):
+                out[m] = self._grid_V[g]
+            return out
+        return self._host_map('Vdc', make)
 
     @property
     def V_fixed(self):
