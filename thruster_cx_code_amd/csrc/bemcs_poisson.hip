@@ -124,6 +124,9 @@ __device__ __forceinline__ T apply5(const T* __restrict__ v, const uint8_t* __re
 // inputs that no block of the same launch writes.
 constexpr int kTC = 8;              // coarse tile edge of the level-0 down launch
 constexpr int kTF = 2 * kTC;        // fine tile edge of the level-0 up launch
+constexpr int kTCL = 16;            // the same for large meshes: 4x fewer partial sums,
+constexpr int kTFL = 2 * kTCL;      // which every consumer block adds up
+constexpr int kLargeMesh = 600000;  // nodes above which a solve set uses the large tiles
 constexpr int kTCc = 8;             // the same on the coarser levels (more,
 constexpr int kTFc = 2 * kTCc;      // smaller tiles: these launches are latency bound)
 
@@ -1356,6 +1359,7 @@ struct bemcs_poisson_batch_s {
   hipGraphExec_t init_graph;
   double init_rtol;
   int pred_iters;
+  int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
 };
 
 namespace bemcs {
@@ -1374,7 +1378,7 @@ static void destroy_set(SolveSet* S) {
   delete S;
 }
 
-static CGArgs cg_args_of(const bemcs_poisson_s* h, double* scal, int32_t* done) {
+static CGArgs cg_args_of(const bemcs_poisson_s* h, double* scal, int32_t* done, int tile0) {
   CGArgs g;
   g.x = h->x;
   g.R[0] = h->R[0];
@@ -1389,7 +1393,8 @@ static CGArgs cg_args_of(const bemcs_poisson_s* h, double* scal, int32_t* done) 
   g.ny = h->ny;
   g.per = h->periodic;
   g.n = h->nx * h->ny;
-  g.nparts = h->nparts;
+  const int tf = 2 * tile0;
+  g.nparts = ((h->nx + tf - 1) / tf) * ((h->ny + tf - 1) / tf);
   g.pa = h->pa;
   g.pr = h->pr;
   g.pz[0] = h->pz[0];
@@ -1415,9 +1420,9 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
                       std::vector<PackUpload>& uploads) {
   const int n = S->n;
   int lmax = 0, nparts_max = 0;
-  for (auto* h : S->cases) {
-    lmax = std::max(lmax, h->nlev);
-    nparts_max = std::max(nparts_max, h->nparts);
+  for (int c = 0; c < n; ++c) {
+    lmax = std::max(lmax, S->cases[c]->nlev);
+    nparts_max = std::max(nparts_max, S->cg[c].nparts);
   }
   auto add_cg = [&](OpKind kind, std::vector<int> idx, dim3 grid) {
     if (idx.empty()) return;
@@ -1445,7 +1450,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
   for (int l = 0; l + 1 < lmax; ++l) {
     std::vector<DownPack> packs;
     unsigned gx = 1, gy = 1;
-    const int tc = l == 0 ? kTC : kTCc;
+    const int tc = l == 0 ? S->tile0 : kTCc;
     for (int c = 0; c < n; ++c) {
       bemcs_poisson_s* h = S->cases[c];
       if (h->nlev - 1 <= l) continue;
@@ -1453,7 +1458,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
       P.F = h->lev[l];
       P.C = h->lev[l + 1];
       P.g = S->cg[c];
-      P.ca = CheckArgs{h->pr, h->nparts, l == 1 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
+      P.ca = CheckArgs{h->pr, S->cg[c].nparts, l == 1 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
       gx = std::max(gx, cdiv(P.C.nx, tc));
       gy = std::max(gy, cdiv(P.C.ny, tc));
       packs.push_back(P);
@@ -1477,7 +1482,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
       CoarsePack P;
       P.C = h->lev[h->nlev - 1];
       P.cinv = h->cinv;
-      P.ca = CheckArgs{h->pr, h->nparts, h->nlev <= 2 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
+      P.ca = CheckArgs{h->pr, S->cg[c].nparts, h->nlev <= 2 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
       gx = std::max(gx, cdiv(P.C.n, 64));
       packs.push_back(P);
     }
@@ -1493,7 +1498,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
   for (int l = lmax - 2; l >= 0; --l) {
     std::vector<UpPack> packs;
     unsigned gx = 1, gy = 1;
-    const int tf = l == 0 ? kTF : kTFc;
+    const int tf = l == 0 ? 2 * S->tile0 : kTFc;
     for (int c = 0; c < n; ++c) {
       bemcs_poisson_s* h = S->cases[c];
       if (h->nlev - 1 <= l) continue;
@@ -1549,7 +1554,15 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
       break;
     case OP_DOWN: {
       const DownPack* pm = (const DownPack*)op.packs_dev;
-      if (op.level == 0) {
+      if (op.level == 0 && S->tile0 == kTCL) {
+        if (k > 0) {
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, true, true>), op.grid, blk, 0, s, op.down, pm, k);
+          else hipLaunchKernelGGL((k_down<kTCL, false, true, false>), op.grid, blk, 0, s, op.down, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, false, true>), op.grid, blk, 0, s, op.down, pm, k);
+          else hipLaunchKernelGGL((k_down<kTCL, false, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+        }
+      } else if (op.level == 0) {
         if (k > 0) {
           if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, op.down, pm, k);
           else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, op.down, pm, k);
@@ -1571,7 +1584,15 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
     }
     case OP_UP: {
       const UpPack* pm = (const UpPack*)op.packs_dev;
-      if (op.level == 0) {
+      if (op.level == 0 && S->tile0 == kTCL) {
+        if (two) {
+          if (B) hipLaunchKernelGGL((k_up<kTFL, 2, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFL, 2, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_up<kTFL, 1, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFL, 1, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
+        }
+      } else if (op.level == 0) {
         if (two) {
           if (B) hipLaunchKernelGGL((k_up<kTF, 2, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
           else hipLaunchKernelGGL((k_up<kTF, 2, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
@@ -1604,6 +1625,11 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   S->use_graphs = !(pg && pg[0] == '0');
   S->pred_iters = 8;
   S->init_rtol = -1.0;
+  S->tile0 = kTC;
+  for (int c = 0; c < n; ++c)
+    if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
+  const char* tl = getenv("BEMCS_MG_TILE0");
+  if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
   bool ok = true;
   auto A = [&](size_t bytes) -> void* {
     void* p = nullptr;
@@ -1622,7 +1648,8 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   if (ok) ok = hipHostMalloc((void**)&S->host_scal, sizeof(double) * S_COUNT * n, 0) == hipSuccess;
   if (ok) ok = hipHostMalloc((void**)&S->pic_host, sizeof(PicardPack) * n, 0) == hipSuccess;
   if (ok) {
-    for (int c = 0; c < n; ++c) S->cg.push_back(cg_args_of(hs[c], S->scal + S_COUNT * c, S->done + c));
+    for (int c = 0; c < n; ++c)
+      S->cg.push_back(cg_args_of(hs[c], S->scal + S_COUNT * c, S->done + c, S->tile0));
     std::vector<PackUpload> uploads;
     build_ops(S, false, S->init_ops, uploads);
     build_ops(S, true, S->iter_ops, uploads);
