@@ -265,6 +265,13 @@ typedef struct bemcs_poisson_s* bemcs_poisson_t;
  * mode, plus its multigrid hierarchy; allocates device memory. */
 int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny,
                          int32_t periodic_y, bemcs_poisson_t* out);
+/* The same with the size of the coarsest multigrid level chosen: levels halve
+ * until one has at most coarse_max nodes (16..4096), whose operator is
+ * inverted densely.  bemcs_poisson_create uses 2048 (fewest launches per
+ * V-cycle); 512 suits operators rebuilt every few steps (bemcs_poisson_update
+ * inverts the coarsest level again, at cost growing as its size cubed). */
+int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t ny, int32_t periodic_y,
+                             int32_t coarse_max, bemcs_poisson_t* out);
 int bemcs_poisson_free(bemcs_poisson_t h);
 int bemcs_poisson_levels(bemcs_poisson_t h);
 

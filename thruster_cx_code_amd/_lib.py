@@ -158,6 +158,7 @@ _SIGS = {
     "bemcs_inject": (i32, [vp, vp, i64, i64, vp]),
     "bemcs_sort_by_cell": (i32, [vp, vp, i64, i32, i32, f64, f64, vp, vp]),
     "bemcs_poisson_create": (i32, [vp, i32, i32, i32, vp]),
+    "bemcs_poisson_create_opt": (i32, [vp, i32, i32, i32, i32, vp]),
     "bemcs_poisson_free": (i32, [vp]),
     "bemcs_poisson_levels": (i32, [vp]),
     "bemcs_poisson_update": (i32, [vp, vp, vp]),

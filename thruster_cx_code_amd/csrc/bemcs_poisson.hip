@@ -29,7 +29,7 @@
 namespace bemcs {
 
 constexpr int kMaxLevels = 20;
-constexpr int kCoarseMax = 512;  // coarsening stops at or below this many nodes
+constexpr int kCoarseMax = 2048;  // default: coarsening stops at or below this many nodes
 constexpr int kDenseMax = 4096;  // largest coarsest level the dense solve accepts
 constexpr float kOmega0 = 0.8f;  // Jacobi damping, 5-point fine level
 constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels
@@ -71,6 +71,8 @@ struct bemcs_poisson_s {
   uint8_t* su_act[2];  // their active flags
   double* su_A;       // dense coarsest operator, then its inverse
   double *su_C, *su_R, *su_T;
+  double* su_d0;      // its original diagonal
+  uint8_t* su_swept;  // pivots taken (0: vanished, generalised inverse)
   int* su_err;
 };
 
@@ -837,7 +839,9 @@ struct PicardPack {
   float* Exy;
 };
 
-enum { PIC_RHO = 0, PIC_RHS = 1, PIC_RELAX = 2, PIC_EFIELD = 3 };
+// Picard phases launched on their own (the right-hand side is part of
+// k_picard_start)
+enum { PIC_RHO = 0, PIC_RELAX = 2, PIC_EFIELD = 3 };
 
 template <int PHASE, bool B>
 __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack* __restrict__ many) {
@@ -851,23 +855,6 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
     for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
     for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
     P.rho[i] = r;
-  } else if (PHASE == PIC_RHS) {
-    // right-hand side (:891-898) in reduced form: rho_total = rho + rho_e(V),
-    // b_red = coeff*rho_total + sum of Dirichlet neighbour values
-    if (P.kind[i] != K_UNKNOWN) {
-      P.bred[i] = 0.0;
-      return;
-    }
-    const double v = P.V[i];
-    const double rho_e = P.qn0 * det_exp(((v < P.Vp ? v : P.Vp) - P.Vp) / P.Te);
-    const double rt = (double)P.rho[i] + rho_e;
-    double sum = -((-P.coeff) * rt);
-    const int xx = i % P.nx, y = i / P.nx;
-    const int nb[4] = {i - 1, i + 1, nbr_s(i, xx, y, P.nx, P.ny, P.per != 0),
-                       nbr_n(i, xx, y, P.nx, P.ny, P.per != 0)};
-    for (int k = 0; k < 4; ++k)
-      if (P.kind[nb[k]] == K_BOUND) sum += P.V_fixed[nb[k]];
-    P.bred[i] = sum;
   } else if (PHASE == PIC_RELAX) {
     // Dirichlet values from V_fixed, Neumann rows mirror;
     // V = (1-omega) V + omega V_new
@@ -886,6 +873,55 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
   }
 }
 
+// One launch for the start of Picard iteration j: the relaxation of
+// iteration j-1 (RELAX), the right-hand side of iteration j and the initial
+// PCG residual r0 = b - A x with its partials (r0.r0, b.b).  Each node needs
+// only its own V (relaxed here) and neighbouring x and V_fixed, which no
+// thread writes, so the three steps fuse exactly.
+template <bool RELAX, bool B>
+__global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const PicardPack* __restrict__ pmany,
+                                                      CGArgs one, const CGArgs* __restrict__ many) {
+  const CGArgs g = B ? many[blockIdx.z] : one;
+  if ((int)blockIdx.x >= g.nparts) return;
+  const PicardPack P = B ? pmany[blockIdx.z] : pone;
+  double rr = 0.0, bb = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < P.n; i += g.nparts * 256) {
+    const uint8_t kd = P.kind[i];
+    double v = P.V[i];
+    if (RELAX) {
+      double vn;
+      if (kd == K_UNKNOWN) vn = P.x[i];
+      else if (kd == K_BOUND) vn = P.V_fixed[i];
+      else {
+        const int m = P.mirror[i];
+        vn = P.kind[m] == K_UNKNOWN ? P.x[m] : P.V_fixed[m];
+      }
+      P.Vnew[i] = vn;
+      v = P.one_m_omega * v + P.omega * vn;
+      P.V[i] = v;
+    }
+    double b = 0.0;
+    if (kd == K_UNKNOWN) {
+      const double rho_e = P.qn0 * det_exp(((v < P.Vp ? v : P.Vp) - P.Vp) / P.Te);
+      const double rt = (double)P.rho[i] + rho_e;
+      b = -((-P.coeff) * rt);
+      const int xx = i % P.nx, y = i / P.nx;
+      const int nb[4] = {i - 1, i + 1, nbr_s(i, xx, y, P.nx, P.ny, P.per != 0),
+                         nbr_n(i, xx, y, P.nx, P.ny, P.per != 0)};
+      for (int k = 0; k < 4; ++k)
+        if (P.kind[nb[k]] == K_BOUND) b += P.V_fixed[nb[k]];
+    }
+    P.bred[i] = b;
+    const double ri = g.code[i] ? b - apply5<double>(g.x, g.code, i, g.nx, g.ny, g.per != 0) : 0.0;
+    g.R[0][i] = ri;
+    g.R[1][i] = 0.0;
+    g.b0[i] = (float)ri;
+    rr += ri * ri;
+    bb += b * b;
+  }
+  block_partials(rr, bb, g.pi, blockIdx.x);
+}
+
 // ------------------------------------------------------ operator set-up
 // The reduced operator of build_sparse_matrix (:525-626) and its multigrid
 // hierarchy are built on the device from the electrode map, into the
@@ -900,7 +936,13 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
 //                    fixed order (deterministic)
 //   k_galerkin_finish  drop couplings of inactive nodes, 1/diagonal, float32 copy
 //   k_dense_*, k_gj_*  inverse of the coarsest operator by blocked in-place
-//                    Gauss-Jordan (SPD, no pivoting), 16-wide panels
+//                    Gauss-Jordan, 16-wide panels.  The coarse operator is
+//                    positive semidefinite: fine nodes enclosed by electrodes
+//                    (eroded cells) can make the prolongation columns of their
+//                    coarse parents dependent.  A pivot that has fallen below
+//                    1e-10 of its original diagonal is skipped and its row and
+//                    column are zeroed at the end, which gives a generalised
+//                    inverse that solves every consistent coarse system.
 constexpr int kGJPanel = 16;
 enum : int { SETUP_ERR_CHAIN = 1, SETUP_ERR_MIRROR = 2, SETUP_ERR_EDGE = 4 };
 
@@ -1125,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_galerkin_finish(double* stC, const uint
 // Dense matrix of the coarsest operator, row-major; inactive rows are the
 // identity (the active block is decoupled from them).
 template <bool L0>
-__global__ __launch_bounds__(256) void k_dense_assemble(FineOp L, double* A, int per) {
+__global__ __launch_bounds__(256) void k_dense_assemble(FineOp L, double* A, double* d0, int per) {
   const int n = L.nx * L.ny;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)n * n) return;
@@ -1148,50 +1190,77 @@ __global__ __launch_bounds__(256) void k_dense_assemble(FineOp L, double* A, int
     }
   }
   A[e] = v;
+  if (i == j) d0[i] = v;
 }
 
 // Panel step of the blocked in-place Gauss-Jordan inversion, part 1: every
 // work-group inverts the p x p diagonal block T = A[K,K]^-1 in LDS (scalar
-// in-place Gauss-Jordan), then C[i,:] = A[i,K] T for its rows and
-// R[:,j] = A[K,j] for its columns; work-group 0 stores T.
-__global__ __launch_bounds__(256) void k_gj_panel(const double* __restrict__ A, int n, int K0, int p,
-                                                  double* C, double* R, double* Tg) {
+// in-place Gauss-Jordan over the pivots that have not vanished; the others
+// keep zero rows and columns in T), then C[i,:] = A[i,K] T for its rows and
+// R[:,j] = A[K,j] for its columns; work-group 0 stores T and the panel's
+// pivot flags.
+__global__ __launch_bounds__(256) void k_gj_panel(const double* __restrict__ A, const double* __restrict__ d0,
+                                                  int n, int K0, int p, double* C, double* R, double* Tg,
+                                                  uint8_t* swept) {
   __shared__ double L[kGJPanel][kGJPanel];
+  __shared__ double rowk[kGJPanel], colk[kGJPanel];
+  __shared__ uint8_t ok[kGJPanel];
   const int t = threadIdx.x;
-  const int ti = t / kGJPanel, tj = t % kGJPanel;
-  if (ti < p && tj < p) L[ti][tj] = A[(size_t)(K0 + ti) * n + K0 + tj];
+  for (int e = t; e < kGJPanel * kGJPanel; e += 256) {
+    const int ti = e / kGJPanel, tj = e % kGJPanel;
+    L[ti][tj] = (ti < p && tj < p) ? A[(size_t)(K0 + ti) * n + K0 + tj] : 0.0;
+  }
+  if (t < kGJPanel) ok[t] = 0;
   for (int k = 0; k < p; ++k) {
     __syncthreads();
-    double piv = L[k][k], rk = 0.0, fi = 0.0, cur = 0.0;
-    if (ti < p && tj < p) {
-      rk = ((tj == k) ? 1.0 : L[k][tj]) / piv;
-      fi = L[ti][k];
-      cur = (tj == k) ? 0.0 : L[ti][tj];
+    const double piv = L[k][k];
+    if (!(fabs(piv) > 1e-10 * fabs(d0[K0 + k]))) continue;  // vanished pivot (same in every thread)
+    if (t < p) {
+      rowk[t] = ((t == k) ? 1.0 : L[k][t]) / piv;  // new row k
+      colk[t] = L[t][k];                            // multipliers
     }
+    if (t == 0) ok[k] = 1;
     __syncthreads();
-    if (ti < p && tj < p) L[ti][tj] = (ti == k) ? rk : cur - fi * rk;
+    for (int e = t; e < p * kGJPanel; e += 256) {
+      const int ti = e / kGJPanel, tj = e % kGJPanel;
+      if (tj >= p) continue;
+      if (ti == k) L[ti][tj] = rowk[tj];
+      else L[ti][tj] = ((tj == k) ? 0.0 : L[ti][tj]) - colk[ti] * rowk[tj];
+    }
   }
   __syncthreads();
-  if (blockIdx.x == 0 && ti < p && tj < p) Tg[ti * kGJPanel + tj] = L[ti][tj];
+  for (int e = t; e < p * kGJPanel; e += 256) {
+    const int ti = e / kGJPanel, tj = e % kGJPanel;
+    if (tj < p && !(ok[ti] && ok[tj])) L[ti][tj] = 0.0;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int e = t; e < p * kGJPanel; e += 256) {
+      const int ti = e / kGJPanel, tj = e % kGJPanel;
+      if (tj < p) Tg[ti * kGJPanel + tj] = L[ti][tj];
+    }
+    if (t < p) swept[K0 + t] = ok[t];
+  }
   for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < (int64_t)n * p; e += (int64_t)gridDim.x * 256) {
     const int i = (int)(e / p), a = (int)(e % p);
     double c = 0.0;
     for (int q = 0; q < p; ++q) c += A[(size_t)i * n + K0 + q] * L[q][a];
     C[(size_t)i * kGJPanel + a] = c;
-    const int j = i;  // reuse the index space for R (p rows x n columns)
-    R[(size_t)a * n + j] = A[(size_t)(K0 + a) * n + j];
+    R[(size_t)a * n + i] = A[(size_t)(K0 + a) * n + i];  // p rows x n columns, same index space
   }
 }
 
-// Part 2: the panel update of every entry from C, R and T only.
+// Part 2: the panel update of every entry from C, R and T only (the panel's
+// vanished pivots count as rows and columns outside it).
 __global__ __launch_bounds__(256) void k_gj_update(double* A, int n, int K0, int p,
                                                    const double* __restrict__ C,
                                                    const double* __restrict__ R,
-                                                   const double* __restrict__ Tg) {
+                                                   const double* __restrict__ Tg,
+                                                   const uint8_t* __restrict__ swept) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e / n), j = (int)(e % n);
-  const bool ki = i >= K0 && i < K0 + p, kj = j >= K0 && j < K0 + p;
+  const bool ki = i >= K0 && i < K0 + p && swept[i], kj = j >= K0 && j < K0 + p && swept[j];
   double v;
   if (ki && kj) {
     v = Tg[(i - K0) * kGJPanel + (j - K0)];
@@ -1208,12 +1277,13 @@ __global__ __launch_bounds__(256) void k_gj_update(double* A, int n, int K0, int
 }
 
 template <bool L0>
-__global__ __launch_bounds__(256) void k_dense_store(FineOp L, const double* __restrict__ A, float* cinv) {
+__global__ __launch_bounds__(256) void k_dense_store(FineOp L, const double* __restrict__ A,
+                                                     const uint8_t* __restrict__ swept, float* cinv) {
   const int n = L.nx * L.ny;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e / n), j = (int)(e % n);
-  const bool on = fine_active<L0>(L, i) && fine_active<L0>(L, j);
+  const bool on = fine_active<L0>(L, i) && fine_active<L0>(L, j) && swept[i] && swept[j];
   cinv[(size_t)j * n + i] = on ? (float)A[e] : 0.0f;  // column-major
 }
 
@@ -1268,17 +1338,20 @@ static int setup_operator(bemcs_poisson_s* h, const uint8_t* is_bound, hipStream
   const int nc = h->nc;
   const int64_t nn = (int64_t)nc * nc;
   const unsigned gd = grid_for(nn, 256);
-  if (lc == 0) hipLaunchKernelGGL(k_dense_assemble<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, per);
-  else hipLaunchKernelGGL(k_dense_assemble<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, per);
+  if (lc == 0) hipLaunchKernelGGL(k_dense_assemble<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->su_d0, per);
+  else hipLaunchKernelGGL(k_dense_assemble<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->su_d0, per);
   for (int K0 = 0; K0 < nc; K0 += kGJPanel) {
     const int p = std::min(kGJPanel, nc - K0);
     const unsigned gp = std::min<unsigned>(grid_for((int64_t)nc * p, 256), 1024u);
-    hipLaunchKernelGGL(k_gj_panel, dim3(gp), dim3(256), 0, s, h->su_A, nc, K0, p, h->su_C, h->su_R, h->su_T);
+    hipLaunchKernelGGL(k_gj_panel, dim3(gp), dim3(256), 0, s, h->su_A, h->su_d0, nc, K0, p, h->su_C, h->su_R,
+                       h->su_T, h->su_swept);
     hipLaunchKernelGGL(k_gj_update, dim3(gd), dim3(256), 0, s, h->su_A, nc, K0, p, h->su_C, h->su_R,
-                       h->su_T);
+                       h->su_T, h->su_swept);
   }
-  if (lc == 0) hipLaunchKernelGGL(k_dense_store<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->cinv);
-  else hipLaunchKernelGGL(k_dense_store<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->cinv);
+  if (lc == 0)
+    hipLaunchKernelGGL(k_dense_store<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->su_swept, h->cinv);
+  else
+    hipLaunchKernelGGL(k_dense_store<false>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->su_swept, h->cinv);
   BEMCS_CHECK_LAUNCH();
   h->have_guess = false;  // the next Picard call restarts from V on the new unknowns
   return BEMCS_OK;
@@ -1347,6 +1420,7 @@ struct bemcs_poisson_batch_s {
   int post_sweeps;
   bool use_graphs;
   std::vector<bemcs::CGArgs> cg;  // per case
+  void* cg_dev;                   // the same on the device
   std::vector<bemcs::Op> init_ops, iter_ops;
   std::vector<void*> allocs;
   double* scal;        // [n][S_COUNT]
@@ -1356,10 +1430,10 @@ struct bemcs_poisson_batch_s {
   bemcs::PicardPack* pic_host;  // pinned staging
   hipStream_t cap_stream;
   std::vector<hipGraphExec_t> iter_graph;
-  hipGraphExec_t init_graph;
-  double init_rtol;
   int pred_iters;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
+  std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
+  std::vector<double> pcg_rtol;
 };
 
 namespace bemcs {
@@ -1370,7 +1444,8 @@ static void destroy_set(SolveSet* S) {
   if (!S) return;
   for (hipGraphExec_t e : S->iter_graph)
     if (e) (void)hipGraphExecDestroy(e);
-  if (S->init_graph) (void)hipGraphExecDestroy(S->init_graph);
+  for (hipGraphExec_t e : S->pcg_graph)
+    if (e) (void)hipGraphExecDestroy(e);
   if (S->cap_stream) (void)hipStreamDestroy(S->cap_stream);
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->host_scal) (void)hipHostFree(S->host_scal);
@@ -1441,8 +1516,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     if (S->cases[c]->nlev == 1) plain.push_back(c);
   }
   if (!iteration) {
-    add_cg(OP_INIT, all, dim3(nparts_max, 1));
-    add_cg(OP_INIT_SCALAR, all, dim3(1, 1));
+    add_cg(OP_INIT_SCALAR, all, dim3(1, 1));  // the caller has launched the r0 kernel
   } else {
     add_cg(OP_DIR, all, dim3(nparts_max, 1));
     add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
@@ -1624,7 +1698,6 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   const char* pg = getenv("BEMCS_POISSON_GRAPHS");
   S->use_graphs = !(pg && pg[0] == '0');
   S->pred_iters = 8;
-  S->init_rtol = -1.0;
   S->tile0 = kTC;
   for (int c = 0; c < n; ++c)
     if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
@@ -1653,6 +1726,8 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
     std::vector<PackUpload> uploads;
     build_ops(S, false, S->init_ops, uploads);
     build_ops(S, true, S->iter_ops, uploads);
+    S->cg_dev = A(sizeof(CGArgs) * n);
+    if (ok) ok = upload(S->cg_dev, S->cg.data(), sizeof(CGArgs) * n) == 0;
     for (auto& u : uploads) {
       void* d = A(u.bytes.size());
       (*u.ops)[u.index].packs_dev = d;
@@ -1688,23 +1763,6 @@ static hipError_t capture(SolveSet* S, F enqueue, hipGraphExec_t* out) {
   return e;
 }
 
-static int enqueue_init(SolveSet* S, double rtol, hipStream_t s) {
-  auto body = [&](hipStream_t cs) {
-    for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
-  };
-  if (!S->use_graphs) {
-    body(s);
-    return BEMCS_OK;
-  }
-  if (!S->init_graph || S->init_rtol != rtol) {
-    if (S->init_graph) (void)hipGraphExecDestroy(S->init_graph);
-    BEMCS_CHECK_HIP(capture(S, body, &S->init_graph));
-    S->init_rtol = rtol;
-  }
-  BEMCS_CHECK_HIP(hipGraphLaunch(S->init_graph, s));
-  return BEMCS_OK;
-}
-
 static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
     for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
@@ -1719,14 +1777,41 @@ static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   return BEMCS_OK;
 }
 
+// The initial V-cycle and iterations 1..N as one graph (one host launch).
+static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
+  auto body = [&](hipStream_t cs) {
+    for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
+    for (int k = 1; k <= N; ++k)
+      for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+  };
+  if (!S->use_graphs) {
+    body(s);
+    return BEMCS_OK;
+  }
+  if ((int)S->pcg_graph.size() <= N) {
+    S->pcg_graph.resize(N + 1, nullptr);
+    S->pcg_rtol.resize(N + 1, -1.0);
+  }
+  if (!S->pcg_graph[N] || S->pcg_rtol[N] != rtol) {
+    if (S->pcg_graph[N]) (void)hipGraphExecDestroy(S->pcg_graph[N]);
+    S->pcg_graph[N] = nullptr;
+    BEMCS_CHECK_HIP(capture(S, body, &S->pcg_graph[N]));
+    S->pcg_rtol[N] = rtol;
+  }
+  BEMCS_CHECK_HIP(hipGraphLaunch(S->pcg_graph[N], s));
+  return BEMCS_OK;
+}
+
 // PCG on the reduced systems already in each case's b_red with initial guess
-// x.  Iterations advance for all cases together; converged cases skip.
+// x.  Iterations advance for all cases together; converged cases skip.  The
+// initial V-cycle and the predicted number of iterations go out as one graph;
+// further iterations, if needed, one graph each.
 static int pcg(SolveSet* S, double rtol, int max_iter, int* iters, double* relres, hipStream_t s) {
-  int rc = enqueue_init(S, rtol, s);
+  int launched = std::min(std::max(1, S->pred_iters), max_iter);
+  int rc = enqueue_pcg_head(S, launched, rtol, s);
   if (rc) return rc;
   BEMCS_CHECK_LAUNCH();
-  int launched = 0;
-  int batch = std::max(1, S->pred_iters);
+  int batch = 0;
   const int n = S->n;
   for (;;) {
     for (int j = 0; j < batch && launched < max_iter; ++j) {
@@ -1800,6 +1885,28 @@ static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p) {
   return P;
 }
 
+template <bool RELAX>
+static void launch_picard_start(SolveSet* S, hipStream_t s) {
+  unsigned gx = 1;
+  for (int c = 0; c < S->n; ++c) gx = std::max(gx, (unsigned)S->cg[c].nparts);
+  const CGArgs* cgm = (const CGArgs*)S->cg_dev;
+  if (S->n > 1)
+    hipLaunchKernelGGL((k_picard_start<RELAX, true>), dim3(gx, 1, S->n), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev, S->cg[0], cgm);
+  else
+    hipLaunchKernelGGL((k_picard_start<RELAX, false>), dim3(gx, 1, 1), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev, S->cg[0], cgm);
+}
+
+// r0 = b - A x of a solve without a Picard right-hand side
+static void launch_cg_init(SolveSet* S, hipStream_t s) {
+  unsigned gx = 1;
+  for (int c = 0; c < S->n; ++c) gx = std::max(gx, (unsigned)S->cg[c].nparts);
+  const CGArgs* cgm = (const CGArgs*)S->cg_dev;
+  if (S->n > 1) hipLaunchKernelGGL(k_cg_init<true>, dim3(gx, 1, S->n), dim3(256), 0, s, S->cg[0], cgm);
+  else hipLaunchKernelGGL(k_cg_init<false>, dim3(gx, 1, 1), dim3(256), 0, s, S->cg[0], cgm);
+}
+
 template <int PHASE>
 static void launch_picard(SolveSet* S, unsigned gx, hipStream_t s) {
   if (S->n > 1)
@@ -1843,14 +1950,17 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
   std::vector<double> rel(n, 0.0), worst(n, 0.0);
   int rc = BEMCS_OK;
   for (int k = 0; k < iterations && rc == BEMCS_OK; ++k) {
-    launch_picard<PIC_RHS>(S, gx, s);
+    // relaxation of the previous iteration + this right-hand side + r0
+    if (k == 0) launch_picard_start<false>(S, s);
+    else launch_picard_start<true>(S, s);
     BEMCS_CHECK_LAUNCH();
     rc = pcg(S, rtol, max_iter, it.data(), rel.data(), s);
     for (int c = 0; c < n; ++c) {
       total[c] += it[c];
       worst[c] = std::max(worst[c], rel[c]);
     }
-    if (rc) break;
+  }
+  if (rc == BEMCS_OK) {
     launch_picard<PIC_RELAX>(S, gx, s);
     BEMCS_CHECK_LAUNCH();
   }
@@ -1873,7 +1983,13 @@ extern "C" {
 
 int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, int32_t periodic_y,
                          bemcs_poisson_t* out) {
-  BEMCS_REQUIRE(is_bound_host && out && nx >= 3 && ny >= 3, "bemcs_poisson_create: bad arguments");
+  return bemcs_poisson_create_opt(is_bound_host, nx, ny, periodic_y, kCoarseMax, out);
+}
+
+int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t ny, int32_t periodic_y,
+                             int32_t coarse_max, bemcs_poisson_t* out) {
+  BEMCS_REQUIRE(is_bound_host && out && nx >= 3 && ny >= 3 && coarse_max >= 16 && coarse_max <= kDenseMax,
+                "bemcs_poisson_create: bad arguments (16 <= coarse_max <= 4096)");
   *out = nullptr;
   const int n = nx * ny;
   const bool per = periodic_y != 0;
@@ -1884,7 +2000,7 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
   lny[0] = ny;
   while (nlev < kMaxLevels) {
     const int fx = lnx[nlev - 1], fy = lny[nlev - 1];
-    if (fx * fy <= kCoarseMax) break;
+    if (fx * fy <= coarse_max) break;
     const int ncx = (fx + 1) / 2, ncy = (fy + 1) / 2;
     if (ncx < 3 || ncy < (per ? 4 : 3)) break;
     lnx[nlev] = ncx;
@@ -1957,6 +2073,8 @@ int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny, i
     h->su_C = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
     h->su_R = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
     h->su_T = (double*)A(sizeof(double) * kGJPanel * kGJPanel);
+    h->su_d0 = (double*)A(sizeof(double) * nc);
+    h->su_swept = (uint8_t*)A(nc);
     h->su_err = (int*)A(sizeof(int));
     if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;
@@ -2009,6 +2127,7 @@ int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rt
   BEMCS_CHECK_LAUNCH();
   int it = 0;
   double rel = 0;
+  launch_cg_init(h->own, s);
   int rc = pcg(h->own, rtol, max_iter, &it, &rel, s);
   if (iters) *iters = it;
   if (relres) *relres = rel;

@@ -226,6 +226,8 @@ class DigitalTwinSimulator:
         el.geometry = {'one_hole': 1, 'two_holes': 2}.get(geometry, 0)
         self._electrodes = el
         self._grids = grids
+        self._rebuilds_expected = params.get('sim_mode', 'Both') != 'Trajectories'
+        self._rebuilds_expected = self._rebuilds_expected and bool(grids)
         self.n_grids = len(grids)
         self._grid_V = (C.c_double * max(len(grids), 1))(*[float(g["V"]) for g in grids])
         self._alloc_mesh(preserve_state)
@@ -322,8 +324,11 @@ class DigitalTwinSimulator:
             self._poisson = None
         h = C.c_void_p()
         ib = np.ascontiguousarray(self.is_bound_d.cpu().numpy())
-        check(self.lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
-                                            int(self.periodic_y), C.byref(h)), "poisson_create")
+        # surface modes rebuild the operator every few steps: a smaller
+        # coarsest level makes each rebuild cheaper
+        coarse = 512 if self._rebuilds_expected else 2048
+        check(self.lib.bemcs_poisson_create_opt(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
+                                                int(self.periodic_y), coarse, C.byref(h)), "poisson_create")
         self._poisson = h
 
     def _alloc_particles(self, params):
