@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 2
+#define BEMCS_ABI_VERSION 4
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -64,6 +64,26 @@ int bemcs_deposit_ngp_counts(const float* x, const float* y, int64_t n, double d
  * added value of one deposit is the same. */
 int bemcs_counts_to_rho(const uint32_t* counts, int64_t n_nodes, double charge_density,
                         float* rho, bemcs_stream_t stream);
+
+/* Bilinear (cloud-in-cell) deposit of the model manual
+ * (Python/docs/PY-BEMCS_Manual.tex:215-230), the deposit the reference's
+ * documentation describes beside the NGP deposit its code uses (:178-185).
+ * Adds the four corner weights of the first min(n, *n_dev) particles (n_dev
+ * may be null) to weights[] as signed 64-bit fixed point, 2^32 units per unit
+ * weight.  Index and fractions are the field gather's: g = x/f32(dx),
+ * i = clip(floor(g), 0, nx-2), h = f64(g) - i, weights (1-hx)(1-hy),
+ * hx(1-hy), (1-hx)hy, hx hy at [iy][ix], [iy][ix+1], [iy+1][ix],
+ * [iy+1][ix+1], each rounded to the nearest unit.  Particles with a
+ * non-finite coordinate or g outside [-1, n] deposit nothing.  Totals are
+ * exact integer sums, independent of particle order.  weights is not
+ * cleared. */
+int bemcs_deposit_cic(const float* x, const float* y, int64_t n, const int32_t* n_dev, double dx,
+                      double dy, int32_t nx, int32_t ny, uint64_t* weights, bemcs_stream_t stream);
+
+/* rho[c] = f32(f64(rho[c]) + charge_density * W[c] 2^-32) for the signed
+ * fixed-point weight sums of bemcs_deposit_cic. */
+int bemcs_weights_to_rho(const uint64_t* weights, int64_t n_nodes, double charge_density,
+                         float* rho, bemcs_stream_t stream);
 
 /* compute_particle_substeps (:928-966): folds max(ds) of the first *n_dev
  * particles into *ds_max_bits (the bit pattern of a non-negative double; zero
@@ -195,6 +215,18 @@ typedef struct {
   float eth32;               /* float32(sputter E_th) */
   float ycoef32;             /* float32(sputter Y_coeff) */
   int32_t see_defer;
+  /* User tables (lookup_user_cs, :475-484, used at :1398 for the SEE yield
+   * and :1714 for the CEX cross-section): cubic B-splines of log10(value) in
+   * log10(E / eV), knots t[n] and coefficients c as SciPy's UnivariateSpline
+   * holds them, evaluated as FITPACK's splev with log10(E) clipped to
+   * [lo, hi] and 10**y = exp(y ln 10) from the deterministic exp.  A null t
+   * selects the built-in yield 0.05 + 1e-4 E or cross-section. */
+  const double* cx_t;
+  const double* cx_c;
+  const double* see_t;
+  const double* see_c;
+  double cx_lo, cx_hi, see_lo, see_hi;
+  int32_t cx_n, see_n;
 } bemcs_ion_step_t;
 
 int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
@@ -296,7 +328,10 @@ int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rt
  * electrons, rho = ion/electron NGP counts converted as bemcs_counts_to_rho
  * (cd_ion first, then cd_e), SuperLU replaced by MG-PCG, then E = -grad V
  * (np.gradient, periodic fix) written interleaved to Exy (float32).  V is
- * f64 [ny][nx] in/out; V_fixed f64; rho_out (float32) receives rho. */
+ * f64 [ny][nx] in/out; V_fixed f64; rho_out (float32) receives rho.  When
+ * w_ion is non-null, rho comes instead from the bilinear weight sums w_ion and
+ * w_e (bemcs_deposit_cic) converted as bemcs_weights_to_rho on a zero rho,
+ * ions first; the count pointers are then ignored. */
 typedef struct {
   const uint32_t* cnt_ion;
   const uint32_t* cnt_e;
@@ -312,6 +347,8 @@ typedef struct {
   double rtol;
   int32_t iterations;
   int32_t max_iter;
+  const uint64_t* w_ion;  /* bilinear deposit (null: NGP counts) */
+  const uint64_t* w_e;
 } bemcs_picard_t;
 
 int bemcs_poisson_picard(bemcs_poisson_t h, const bemcs_picard_t* p, int32_t* iters_total,

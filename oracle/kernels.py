@@ -18,12 +18,20 @@ nearest_index       the ``np.round(.../d).astype(int)`` + clip idiom used at
                     :183-184, :1326-1327, :1541-1542, :1883-1891.
 deposit_counts      :178-185 (NGP).  Pinned by brute-force loops and by
 rho_from_counts     bit equality with ``np.add.at`` (:185).
+deposit_cic         the bilinear deposit of the model manual
+rho_from_weights    (Python/docs/PY-BEMCS_Manual.tex:215-230) with the field
+                    gather's index and fractions, weights in 2^-32 fixed point.
+                    Pinned by a literal per-particle loop in Python floats, by
+                    charge conservation and exact first-moment conservation
+                    (sum_nodes W x_node = sum_p x_p), by particles on nodes, and
+                    by the float64 formula to 1e-6 of max |rho|.
 substeps            :928-966.  Pinned by a per-particle loop.
 segment_hits        :1880-1901.  Pinned by a per-particle loop.
 ppc_map             :1749-1758.  Pinned by ``np.bincount`` on brute-force indices.
 mcc_prob_ref32      :1690-1719 evaluated with the reference's float32 math.
 mcc_prob_det64      the same quantities in float64 with oracle.dmath; agrees
-                    with mcc_prob_ref32 to float32 rounding (tested).
+                    with mcc_prob_ref32 to float32 rounding (tested); user
+                    cross-section tables through oracle.spline.
 """
 import numpy as np
 
@@ -156,6 +164,50 @@ def rho_from_counts(counts_first, cd_first, counts_second=None, cd_second=None,
     return rho
 
 
+CIC_SCALE = 2.0 ** 32   # fixed-point units per unit weight of the bilinear deposit
+
+
+def cic_weights(x, y, dx, dy, nx, ny):
+    """(base node int64[n] or -1, fixed-point corner weights int64[n, 4]) of
+    the bilinear deposit: g = x/f32(dx), i = clip(floor(g), 0, nx-2),
+    h = f64(g) - i; corners [i,j], [i+1,j], [i,j+1], [i+1,j+1]; particles
+    with g outside [-1, n] (or NaN) deposit nothing."""
+    gx = np.asarray(x, F32) / F32(dx)
+    gy = np.asarray(y, F32) / F32(dy)
+    ok = (gx >= -1.0) & (gx <= nx) & (gy >= -1.0) & (gy <= ny)
+    ix = np.clip(_floor_i32(gx), 0, nx - 2).astype(np.int64)
+    iy = np.clip(_floor_i32(gy), 0, ny - 2).astype(np.int64)
+    hx = gx.astype(np.float64) - ix
+    hy = gy.astype(np.float64) - iy
+    ax, ay = 1.0 - hx, 1.0 - hy
+    w = np.stack([np.rint((ax * ay) * CIC_SCALE), np.rint((hx * ay) * CIC_SCALE),
+                  np.rint((ax * hy) * CIC_SCALE), np.rint((hx * hy) * CIC_SCALE)], axis=1)
+    w = np.where(ok[:, None], w, 0.0).astype(np.int64)
+    return np.where(ok, iy * nx + ix, -1), w
+
+
+def deposit_cic(x, y, dx, dy, nx, ny):
+    """Per-node sums (int64[ny*nx]) of the fixed-point bilinear weights."""
+    base, w = cic_weights(x, y, dx, dy, nx, ny)
+    ok = base >= 0
+    b, w = base[ok], w[ok]
+    W = np.zeros(nx * ny, np.int64)
+    for c, off in enumerate((0, 1, nx, nx + 1)):
+        np.add.at(W, b + off, w[:, c])
+    return W
+
+
+def rho_from_weights(w_first, cd_first, w_second=None, cd_second=None, rho0=None):
+    """rho = f32(f64(rho) + cd * W 2^-32), first species then second."""
+    rho = (np.zeros(np.shape(w_first), F32) if rho0 is None else np.array(rho0, F32, copy=True))
+    for W, cd in ((w_first, cd_first), (w_second, cd_second)):
+        if W is None:
+            continue
+        s = np.asarray(W, np.int64).astype(np.float64) * (1.0 / CIC_SCALE)
+        rho = (rho.astype(np.float64) + float(cd) * s).astype(F32)
+    return rho
+
+
 def substeps(x, y, vx, vy, vz, Ex, Ey, dx, dy, nx, ny, qm, dt, frac=0.25):
     """(n_sub, ds_max, ds_lim) of compute_particle_substeps."""
     if len(x) == 0:
@@ -249,16 +301,22 @@ def _sigma_det(g, m_ion, q, model):
 
 
 def mcc_prob_det64(x, y, vx, vy, vz, Ly, n0, dt, m_ion, q,
-                   neutral_model="roy", cx_model="reference"):
+                   neutral_model="roy", cx_model="reference", cx_table=None):
     """CEX probability in float64 from float32 state (bit-exact twin of the
-    HIP kernel)."""
+    HIP kernel).  cx_table: a user cross-section table (oracle.spline), which
+    takes precedence over the built-in formulas as in the reference
+    (Python/physics_engine.py:1714-1716)."""
+    from .spline import table_value
     x64, y64 = np.asarray(x, np.float64), np.asarray(y, np.float64)
     a = np.asarray(vx, np.float64)
     b = np.asarray(vy, np.float64)
     c = np.asarray(vz, np.float64)
     n = _neutral_density(x64, y64, Ly, n0, neutral_model)
     g = np.maximum(np.sqrt(a * a + b * b + c * c), 1.0)
-    sigma = _sigma_det(g, m_ion, q, cx_model)
+    if cx_table is not None:
+        sigma = table_value(cx_table, (0.5 * m_ion * (g * g)) / q)
+    else:
+        sigma = _sigma_det(g, m_ion, q, cx_model)
     return 1.0 - det_exp(-(((n * sigma) * g) * dt))
 
 

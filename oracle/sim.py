@@ -35,12 +35,13 @@ import math
 import numpy as np
 
 from . import rng
-from .kernels import (push_boris, deposit_counts, rho_from_counts, substeps,
+from .kernels import (push_boris, deposit_counts, rho_from_counts, deposit_cic, rho_from_weights, substeps,
                       segment_hits, nearest_index, ppc_map, mcc_prob_det64,
                       mcc_prob_ref32)
 from .poisson import SuperLUPoisson, efield
 from .dmath import det_exp
 from . import surface
+from . import spline
 
 F32 = np.float32
 
@@ -75,6 +76,8 @@ class OracleSimulator:
         # capacity and radiating area use dx, dy at set_material time (:450-452)
         self.sb_sigma = 5.67e-8
         self.thermal_accel = 1e7
+        self.user_cs = {}
+        self.deposit_shape = 'ngp'   # 'cic': the model manual's bilinear deposit
         self.set_material('Molybdenum')
         self.T_grids = []
         self.mask_grids = []
@@ -145,6 +148,7 @@ class OracleSimulator:
         n0_n = params.get("n0_plasma", 1e17)
         bulk = params.get("entire_bulk_plasma", False)
         self.seed = int(params.get("seed", 0))
+        self.deposit_shape = params.get("deposit_shape", "ngp")
         debye = np.sqrt(self.eps0 * Te_up / (self.q * n0_n * 0.61))
         if bulk:
             n_debye = 80 if n0_n <= 1e17 else (40 if n0_n <= 4e17 else 30)
@@ -399,6 +403,12 @@ class OracleSimulator:
     def deposit(self):
         cell_vol = (self.dx * 1e-3) * (self.dy * 1e-3) * 1e-3
         cd = self.q * self.macro_weight / cell_vol
+        if self.deposit_shape == 'cic':
+            # bilinear deposit of the model manual (PY-BEMCS_Manual.tex:215-230)
+            self.w_i = deposit_cic(self.p_x, self.p_y, self.dx, self.dy, self.nx, self.ny)
+            self.w_e = deposit_cic(self.e_x, self.e_y, self.dx, self.dy, self.nx, self.ny)
+            self.rho = rho_from_weights(self.w_i, cd, self.w_e, -cd).reshape(self.ny, self.nx)
+            return
         self.cnt_i = deposit_counts(self.p_x, self.p_y, self.dx, self.dy, self.nx, self.ny)
         self.cnt_e = deposit_counts(self.e_x, self.e_y, self.dx, self.dy, self.nx, self.ny)
         self.rho = rho_from_counts(self.cnt_i, cd, self.cnt_e, -cd).reshape(self.ny, self.nx)
@@ -556,7 +566,11 @@ class OracleSimulator:
                 v2 = (svx.astype(np.float64) ** 2 + svy.astype(np.float64) ** 2
                       + svz.astype(np.float64) ** 2)
                 E_eV = (0.5 * self.m_ion * v2) / self.q
-                gamma = np.clip(0.05 + 1e-4 * E_eV, 0.0, 1.0)
+                see_table = spline.lookup(self.user_cs, 'SEE')
+                if see_table is not None:                  # user yield table (:1398-1399)
+                    gamma = np.clip(spline.table_value(see_table, E_eV), 0.0, 1.0)
+                else:
+                    gamma = np.clip(0.05 + 1e-4 * E_eV, 0.0, 1.0)
             else:
                 E_eV = (0.5 * self.m_ion * (svx ** 2 + svy ** 2 + svz ** 2)) / self.q
                 gamma = np.clip(0.05 + 1e-4 * E_eV, 0.0, 1.0)
@@ -642,7 +656,8 @@ class OracleSimulator:
         xs, ys = self.p_x[prim], self.p_y[prim]
         vx, vy, vz = self.p_vx[prim], self.p_vy[prim], self.p_vz[prim]
         if self.mcc_math == "det64":
-            P = mcc_prob_det64(xs, ys, vx, vy, vz, self.Ly, n0, self.dt, self.m_ion, self.q, nm, cm)
+            P = mcc_prob_det64(xs, ys, vx, vy, vz, self.Ly, n0, self.dt, self.m_ion, self.q, nm, cm,
+                               spline.lookup(self.user_cs, 'CX'))
         else:
             P = mcc_prob_ref32(xs, ys, vx, vy, vz, self.Ly, n0, self.dt, self.m_ion, self.q)
         pid = self.p_pid[prim].astype(np.uint64)

@@ -30,6 +30,8 @@ bow_deflection     :1619-1643.  Closed form of the cantilever formula
                    delta = alpha dT L^2 / (2 t) per geometry, float32 as the
                    reference evaluates it.
 rf_potential       :984-991.  Closed form A sin(2 pi f t).
+groove_profile     :1903-1932 (per-row depth of a grid face).  Pinned by a
+                   literal per-row scan on small maps for every face.
 """
 import numpy as np
 
@@ -171,3 +173,28 @@ def fixed_point(values, fraction_bits):
     point (value * 2^fraction_bits, round half to even); returns uint64."""
     s = np.asarray(values, np.float64) * float(2 ** fraction_bits)
     return np.rint(s).astype(np.uint64)
+
+
+def groove_profile(mask, eroded_depth, damage_map, dy, thresh=None, accumulate_subcell=True,
+                   face='upstream'):
+    """(y_mm, depth_um) of one grid's face per mesh row (:1903-1932)."""
+    mask = np.asarray(mask, bool)
+    if not np.any(mask):
+        return np.array([]), np.array([])
+    ny, nx = mask.shape
+    depth = np.asarray(eroded_depth, np.float64)
+    if accumulate_subcell and thresh and thresh > 0:
+        depth = depth + (np.asarray(damage_map, np.float64) / thresh) * dy
+    per_y = np.zeros(ny, np.float64)
+    rows = mask.any(axis=1)
+    if face == 'any':
+        cols = mask.any(axis=0)
+        per_y[rows] = depth[rows][:, cols].max(axis=1)
+    elif face == 'upstream':
+        per_y[rows] = depth[np.nonzero(rows)[0], np.argmax(mask, axis=1)[rows]]
+    elif face == 'downstream':
+        last = nx - 1 - np.argmax(mask[:, ::-1], axis=1)
+        per_y[rows] = depth[np.nonzero(rows)[0], last[rows]]
+    else:
+        raise ValueError(f"face must be 'upstream','downstream', or 'any' (got {face!r})")
+    return np.arange(ny) * dy, per_y * 1000.0

@@ -1,0 +1,144 @@
+"""GPU parity of the bilinear (cloud-in-cell) deposit
+(Python/docs/PY-BEMCS_Manual.tex:215-230): fixed-point weight sums bit-exact
+against the oracle for cell-sorted input (LDS window path), random input
+(direct path), edge positions, a device particle count, the drop-in
+function, and the time step with deposit_shape 'cic' (rho bit-exact, V within
+1e-9 of SuperLU)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import c1_params
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from oracle import kernels as K  # noqa: E402
+
+F32 = np.float32
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _cloud(n, nx, ny, dx, dy, seed, sort):
+    r = np.random.default_rng(seed)
+    x = (r.random(n) * (nx - 1) * dx).astype(F32)
+    y = (r.random(n) * (ny - 1) * dy).astype(F32)
+    x[:4] = np.array([0.0, (nx - 1) * dx, -0.5 * dx, np.nan], F32)
+    y[:4] = np.array([(ny - 1) * dy, 0.0, 0.5 * dy, 0.1], F32)
+    if sort:
+        key = np.floor(y / F32(dy)).astype(np.int64) * nx + np.floor(x / F32(dx)).astype(np.int64)
+        order = np.argsort(np.where(np.isfinite(x), key, -1), kind="stable")
+        x, y = x[order], y[order]
+    return x, y
+
+
+@pytest.mark.parametrize("nx,ny,n,sort", [(257, 97, 300000, True), (257, 97, 300000, False),
+                                          (1024, 384, 2000000, True), (33, 9, 777, False),
+                                          (3000, 40, 100000, True)])
+def test_deposit_cic_bit_exact(hip_lib, nx, ny, n, sort):
+    """(3000, 40): a cell-sorted chunk spans more than the LDS window
+    (row length 3000), so every work-group takes the direct path."""
+    print(f"[gpu] cic deposit {nx}x{ny} n={n} sorted={sort}", flush=True)
+    dx, dy = 0.02, 0.0231
+    x, y = _cloud(n, nx, ny, dx, dy, nx + n, sort)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    W = torch.zeros(nx * ny, dtype=torch.int64, device="cuda")
+    assert hip_lib.bemcs_deposit_cic(_ptr(xd), _ptr(yd), n, None, dx, dy, nx, ny, _ptr(W), _stream()) == 0
+    torch.cuda.synchronize()
+    want = K.deposit_cic(x, y, dx, dy, nx, ny)
+    assert np.array_equal(W.cpu().numpy(), want)
+    # device count: only the first m particles
+    m = n // 3 + 5
+    cnt = torch.tensor([m], dtype=torch.int32, device="cuda")
+    W.zero_()
+    assert hip_lib.bemcs_deposit_cic(_ptr(xd), _ptr(yd), n, _ptr(cnt), dx, dy, nx, ny, _ptr(W), _stream()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(W.cpu().numpy(), K.deposit_cic(x[:m], y[:m], dx, dy, nx, ny))
+
+
+def test_weights_to_rho_and_drop_in(hip_lib):
+    from thruster_cx_code_amd import accumulate_rho_cic_hip
+    nx, ny, dx, dy = 130, 50, 0.02, 0.02
+    x, y = _cloud(50000, nx, ny, dx, dy, 9, True)
+    rho0 = np.random.default_rng(1).normal(0.0, 1e-3, (ny, nx)).astype(F32)
+    rho = rho0.copy()
+    cd = 1.602e-19 * 1e3 / (dx * dy * 1e-9)
+    accumulate_rho_cic_hip(x, y, rho, len(x), dx, dy, nx, ny, cd)
+    want = K.rho_from_weights(K.deposit_cic(x, y, dx, dy, nx, ny), cd, rho0=rho0.ravel()).reshape(ny, nx)
+    assert np.array_equal(rho.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def cic_pair():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    from oracle.sim import OracleSimulator
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=20000)
+    p["deposit_shape"] = "cic"
+    g, o = DigitalTwinSimulator(), OracleSimulator()
+    for s in (g, o):
+        s.m_e = s.m_ion / 200.0
+        s.deposit_shape = "cic"
+    g.build_domain(p)
+    o.build_domain(p)
+    return p, g, o
+
+
+def _sync(o, g):
+    tag = g._ion_host("tag").view(np.uint32)
+    o.p_x, o.p_y, o.p_vx, o.p_vy, o.p_vz = g.p_x, g.p_y, g.p_vx, g.p_vy, g.p_vz
+    o.p_pid = tag & np.uint32(0x7FFFFFFF)
+    o.p_isCEX = (tag >> 31).astype(bool)
+    o.e_x, o.e_y, o.e_vx, o.e_vy, o.e_vz = g.e_x, g.e_y, g.e_vx, g.e_vy, g.e_vz
+    o.V = g.V.copy()
+    o.Ex, o.Ey = g.Ex.copy(), g.Ey.copy()
+    o.iteration = g.iteration
+    o.next_ion_pid = g.next_ion_pid
+
+
+def test_step_with_bilinear_deposit(cic_pair):
+    p, g, o = cic_pair
+    for k in range(4):
+        _sync(o, g)
+        print(f"[gpu] cic step {g.iteration + 1}", flush=True)
+        g.step(p)
+        o.step(p)
+        if g.iteration % 2 == 0:
+            assert np.array_equal(g.w_i.cpu().numpy(), o.w_i)
+            assert np.array_equal(g.w_e.cpu().numpy(), o.w_e)
+            assert np.array_equal(g.rho.view(np.uint32), o.rho.view(np.uint32))
+            err = np.max(np.abs(g.V - o.V)) / np.max(np.abs(o.V))
+            print(f"  Poisson rel err {err:.2e}")
+            assert err < 1e-9
+    assert g.deposit_shape == "cic"
+
+
+def test_bilinear_differs_from_ngp_and_is_rejected_by_batches():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    p = c1_params(nx=128, ny=48, ions=5000, electrons=5000)
+    out = {}
+    for shape in ("ngp", "cic"):
+        q = dict(p, deposit_shape=shape)
+        g = DigitalTwinSimulator()
+        g.m_e = g.m_ion / 200.0
+        g.build_domain(q)
+        g.advance(q, 2)
+        out[shape] = g.rho.copy()
+        if shape == "cic":
+            with pytest.raises(ValueError):
+                BatchedSimulator().add(g, q)
+    assert not np.array_equal(out["ngp"], out["cic"])
+    with pytest.raises(ValueError):
+        DigitalTwinSimulator().build_domain(dict(p, deposit_shape="tsc"))

@@ -294,6 +294,14 @@ def test_surface_modes_step_parity(mode):
     assert hits > 0
     if mode != "Thermal":
         assert failed > 0
+        for gi_ in range(len(p["grids"])):
+            for face in ("upstream", "downstream", "any"):
+                for thresh in (None, 37.5):
+                    yg, dg = g.get_groove_profile(gi_, thresh, True, face)
+                    yo, do = S.groove_profile(g.mask_grids[gi_], g.eroded_depth, g.damage_map, g.dy,
+                                              thresh, True, face)
+                    assert np.array_equal(yg, yo)
+                    assert np.allclose(dg, do, rtol=1e-14, atol=0.0), (gi_, face, thresh)
 
 
 def test_thermal_bow_remesh_parity():
@@ -322,6 +330,58 @@ def test_thermal_bow_remesh_parity():
     assert err < 1e-9, err
     assert abs(rg[1] - ro[1]) <= 1e-9 * abs(ro[1])
     _compare_surface(g, o, "bow")
+
+
+def _table_file(path, E, values):
+    with open(path, "w") as f:
+        for e, v in zip(E, values):
+            f.write(f"{float(e)!r} {float(v)!r}\n")
+    return str(path)
+
+
+def test_user_cross_section_tables_step_parity(tmp_path):
+    """User CX cross-section and SEE yield tables (cubic splines of log10
+    value in log10 E, clipped to the table range) drive the collision and
+    secondary-emission draws: event counts, spawned electrons and particles
+    bit-exact against the oracle's FITPACK restatement."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from thruster_cx_code_amd.config import load_cross_section
+    E = np.geomspace(0.5, 3000.0, 23)
+    cx = 4e-19 * (1.0 + 0.3 * np.sin(np.log(E)))
+    see = np.clip(0.15 + 0.25 * np.log10(E), 0.05, 2.0)     # > 1 at high energy: clipped to 1
+    tables = {"CX": load_cross_section(_table_file(tmp_path / "cx.txt", E, cx), 0.0, "CX"),
+              "SEE": load_cross_section(_table_file(tmp_path / "see.txt", E, see), 0.0, "SEE")}
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=4000, neut_rate=0)
+    p["n0"] = 3e21
+
+    def with_tables(s):
+        s.user_cs = dict(tables)
+
+    g, o = _mk(p, with_tables)
+    events = spawned = 0
+    for k in range(3):
+        _sync(o, g)
+        print(f"[gpu] user tables step {g.iteration + 1}", flush=True)
+        g.step(p)
+        o.step(p)
+        rec = g.record()
+        assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
+        assert rec["see_spawned"] == o.see_count
+        events += rec["mcc_events"]
+        spawned += rec["see_spawned"]
+        gi, oi = _ions(g), _ions(o)
+        assert np.array_equal(gi[5], oi[5]) and np.array_equal(gi[6], oi[6])
+        exact = g.iteration % 2 == 1
+        for a, b in zip(gi[:5], oi[:5]):
+            if exact:
+                assert np.array_equal(a, b)
+            else:
+                assert np.max(np.abs(a - b)) <= 1e-6 * max(np.max(np.abs(b)), 1.0)
+        if exact:
+            assert np.array_equal(_electrons(g), _electrons(o))
+    print(f"  CEX events {events}, SEE electrons {spawned}")
+    assert events > 50 and spawned > 0
 
 
 def test_rf_step_parity():

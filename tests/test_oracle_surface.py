@@ -149,3 +149,37 @@ def test_fixed_point_exact_above_8():
     v = np.array([8.0, 1234.5678, 3.3e7, 16777215.0], F32)
     q = S.fixed_point(v, S.HEAT_FRACTION_BITS)
     assert np.array_equal(q.astype(np.float64) / 2 ** S.HEAT_FRACTION_BITS, v.astype(np.float64))
+
+
+@pytest.mark.parametrize("face", ["upstream", "downstream", "any"])
+def test_groove_profile_equals_row_scan(face):
+    r = np.random.default_rng(7)
+    ny, nx, dy, thresh = 11, 17, 0.02, 1e4
+    mask = np.zeros((ny, nx), bool)
+    mask[2:9, 5:9] = True
+    mask[4, 5] = False                          # a notch: the first node of row 4 moves right
+    depth = r.random((ny, nx)) * 0.05
+    damage = r.random((ny, nx)) * 3e3
+    y, d = S.groove_profile(mask, depth, damage, dy, thresh, True, face)
+    want = np.zeros(ny)
+    for iy in range(ny):
+        xs = [ix for ix in range(nx) if mask[iy, ix]]
+        if not xs:
+            continue
+        tot = [depth[iy, ix] + damage[iy, ix] / thresh * dy for ix in range(nx)]
+        if face == 'upstream':
+            want[iy] = tot[xs[0]]
+        elif face == 'downstream':
+            want[iy] = tot[xs[-1]]
+        else:
+            cols = [ix for ix in range(nx) if mask[:, ix].any()]
+            want[iy] = max(tot[ix] for ix in cols)
+    assert np.allclose(d, want * 1000.0, rtol=1e-15, atol=0.0)
+    assert np.array_equal(y, np.arange(ny) * dy)
+    _, d0 = S.groove_profile(mask, depth, damage, dy, None, True, face)
+    assert np.allclose(d0[mask.any(axis=1)], 1000.0 * np.array(
+        [depth[iy, [ix for ix in range(nx) if mask[iy, ix]][0 if face == 'upstream' else -1]]
+         if face != 'any' else depth[iy, 5:9].max() for iy in range(ny) if mask[iy].any()]), rtol=1e-15)
+    assert S.groove_profile(np.zeros((3, 3), bool), depth[:3, :3], damage[:3, :3], dy)[0].size == 0
+    with pytest.raises(ValueError):
+        S.groove_profile(mask, depth, damage, dy, face='side')

@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 4
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -46,7 +46,10 @@ class IonStep(C.Structure):
                 ("cx_model", i32), ("mcc_enabled", i32), ("seed", u32), ("step", u32),
                 ("heat_acc", vp), ("damage_acc", vp), ("see_stage", Species),
                 ("half_m32", f32), ("w32", f32), ("ccell32", f32), ("accel32", f32), ("q32", f32),
-                ("eth32", f32), ("ycoef32", f32), ("see_defer", i32)]
+                ("eth32", f32), ("ycoef32", f32), ("see_defer", i32),
+                ("cx_t", vp), ("cx_c", vp), ("see_t", vp), ("see_c", vp),
+                ("cx_lo", f64), ("cx_hi", f64), ("see_lo", f64), ("see_hi", f64),
+                ("cx_n", i32), ("see_n", i32)]
 
 
 class ElectronStep(C.Structure):
@@ -68,7 +71,8 @@ class Picard(C.Structure):
     _fields_ = [("cnt_ion", vp), ("cnt_e", vp), ("V_fixed", vp), ("V", vp), ("rho_out", vp),
                 ("Exy", vp), ("cd_ion", f64), ("cd_e", f64), ("coeff", f64), ("V_plasma", f64),
                 ("Te_up", f64), ("n0", f64), ("q", f64), ("omega", f64), ("dx_m", f64),
-                ("dy_m", f64), ("rtol", f64), ("iterations", i32), ("max_iter", i32)]
+                ("dy_m", f64), ("rtol", f64), ("iterations", i32), ("max_iter", i32),
+                ("w_ion", vp), ("w_e", vp)]
 
 
 class StepBegin(C.Structure):
@@ -145,6 +149,8 @@ _SIGS = {
                                f64, f64, i32, vp]),
     "bemcs_deposit_ngp_counts": (i32, [vp, vp, i64, f64, f64, i32, i32, vp, vp]),
     "bemcs_counts_to_rho": (i32, [vp, i64, f64, vp, vp]),
+    "bemcs_deposit_cic": (i32, [vp, vp, i64, vp, f64, f64, i32, i32, vp, vp]),
+    "bemcs_weights_to_rho": (i32, [vp, i64, f64, vp, vp]),
     "bemcs_substeps": (i32, [vp, vp, vp, vp, vp, vp, i64, vp, i32, i32, f64, f64, f64, f64, f64,
                              f64, vp, vp, vp]),
     "bemcs_poisson_count": (i64, [u32, u32, u32, f64]),

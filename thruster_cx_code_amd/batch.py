@@ -98,6 +98,9 @@ class BatchedSimulator:
         if params.get('sim_mode', 'Both') != 'Trajectories' or params.get('rf_enable'):
             raise ValueError("batched cases run sim_mode 'Trajectories' without RF; step thermal, "
                              "erosion and RF cases with DigitalTwinSimulator.step/advance")
+        if sim.deposit_shape != 'ngp':
+            raise ValueError("batched cases use the NGP deposit; step bilinear-deposit cases with "
+                             "DigitalTwinSimulator.step/advance")
         sim._resolve_surface()
         if self.device is None:
             self.device = sim.device

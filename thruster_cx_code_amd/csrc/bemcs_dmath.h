@@ -26,6 +26,7 @@ constexpr double kLn2Lo = 1.9082149292705877e-10;  // ln2 - kLn2Hi
 constexpr double kSqrtHalf = 0.7071067811865476;
 constexpr double kHalfPi = 1.5707963267948966;
 constexpr double kInvLn10 = 0.43429448190325176;
+constexpr double kLn10 = 2.302585092994045684;
 
 BEMCS_HD double pow2i(int k) {  // 2^k for -1022 <= k <= 1023, exact
   uint64_t bits = (uint64_t)(k + 1023) << 52;

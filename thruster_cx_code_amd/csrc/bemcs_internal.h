@@ -188,6 +188,50 @@ __device__ __forceinline__ void fixed_add(uint64_t* acc, float v, double scale, 
   if (q) atomicAdd((unsigned long long*)acc, q);
 }
 
+// Fixed point of the bilinear deposit's weight sums: 2^32 units per unit
+// weight (signed 64-bit, stored as uint64 two's complement).
+constexpr double kCicScale = 4294967296.0;
+constexpr double kCicUnit = 1.0 / 4294967296.0;
+
+// Cubic B-spline (knots t[n], coefficients c[n-4]) at x, as FITPACK's splev
+// evaluates it: the knot interval l with t[l] <= x < t[l+1] (the last one for
+// x at the right end) and fpbspl's Cox-de Boor recursion in its operation
+// order.  x must lie in [t[3], t[n-4]].
+__device__ __forceinline__ double bspline3(const double* __restrict__ t, const double* __restrict__ c,
+                                           int n, double x) {
+  int lo = 3, hi = n - 5;  // largest l in [3, n-5] with t[l] <= x
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  const int l = lo;
+  double h[4] = {1.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 1; j <= 3; ++j) {
+    double hh[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+      if (i > j) break;
+      const double tl = t[l + i], tj = t[l + i - j];
+      if (tl != tj) {
+        const double f = hh[i - 1] / (tl - tj);
+        h[i - 1] = h[i - 1] + f * (tl - x);
+        h[i] = f * (x - tj);
+      } else {
+        h[i] = 0.0;
+      }
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) s = s + c[l - 3 + j] * h[j];
+  return s;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;

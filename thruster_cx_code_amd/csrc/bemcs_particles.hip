@@ -12,6 +12,7 @@
 // :188-254): float32 positions and velocities, float64 interpolation weights
 // and kicks (float32 - int32 promotes to float64 there), float32 drift.  With
 // -ffp-contract=off every result is bit-identical to it on the same inputs.
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -142,6 +143,136 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const float* __restrict__ x,
   wave_hist_add(counts, key);
 }
 
+// ------------------------------------------------------ bilinear deposit
+// Bilinear (cloud-in-cell) deposit of the model manual
+// (Python/docs/PY-BEMCS_Manual.tex:215-230).  Index and fractions are those
+// of the field gather: g = x / f32(dx), i = clip(floor(g), 0, nx-2),
+// h = f64(g) - i, so the deposit is the adjoint of the gather.  Weights are
+// rounded to 64-bit fixed point (2^-32 units, two's complement for the small
+// negative weights of a node one rounding step past the last cell) and
+// summed with integer atomics, so per-node totals are exact and independent
+// of the order of the adds.
+//
+// A work-group takes kCicChunk consecutive particles.  When the nodes they
+// touch fit a window of kCicWin nodes (cell-sorted particles: a few rows),
+// it accumulates in LDS and adds each touched node to HBM once; otherwise
+// it adds to HBM directly.  Within a wavefront, runs of lanes on the same
+// cell are first summed by a segmented shuffle reduction, so a run costs one
+// atomic per corner.
+constexpr int kCicPer = 8;
+constexpr int kCicChunk = kBlock * kCicPer;
+constexpr int kCicWin = 6144;  // uint64 nodes: 48 KiB of LDS
+
+__device__ __forceinline__ int cic_weights(float px, float py, float dx32, float dy32, int nx, int ny,
+                                           long long w[4]) {
+  const float gx = px / dx32, gy = py / dy32;
+  // non-finite or far outside the mesh: no deposit
+  if (!(gx >= -1.0f && gx <= (float)nx && gy >= -1.0f && gy <= (float)ny)) return -1;
+  const int ix = floor_i32_np(gx, nx - 2), iy = floor_i32_np(gy, ny - 2);
+  const double hx = (double)gx - (double)ix, hy = (double)gy - (double)iy;
+  const double ax = 1.0 - hx, ay = 1.0 - hy;
+  w[0] = llrint((ax * ay) * kCicScale);
+  w[1] = llrint((hx * ay) * kCicScale);
+  w[2] = llrint((ax * hy) * kCicScale);
+  w[3] = llrint((hx * hy) * kCicScale);
+  return iy * nx + ix;
+}
+
+// Adds the four corner weights of each lane's cell `key` (< 0: none) to
+// dst[key], dst[key+1], dst[key+nx], dst[key+nx+1], one atomic per run of
+// equal keys in consecutive lanes.
+__device__ __forceinline__ void wave_cic_add(unsigned long long* dst, int key, long long w[4], int nx) {
+  const int lane = lane_id();
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || key != prev;
+  const uint64_t heads = __ballot(head);
+  if (heads != ~0ull) {
+    // last lane of this lane's run
+    const uint64_t later = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int end = later ? lane + __ffsll((unsigned long long)later) - 1 : 63;
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const long long o = __shfl_down(w[c], off);
+        if (lane + off <= end) w[c] += o;
+      }
+    }
+  }
+  if (head && key >= 0) {
+    atomicAdd(dst + key, (unsigned long long)w[0]);
+    atomicAdd(dst + key + 1, (unsigned long long)w[1]);
+    atomicAdd(dst + key + nx, (unsigned long long)w[2]);
+    atomicAdd(dst + key + nx + 1, (unsigned long long)w[3]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_deposit_cic(const float* __restrict__ x,
+                                                         const float* __restrict__ y, int64_t n,
+                                                         const int32_t* n_dev, float dx32, float dy32,
+                                                         int nx, int ny, unsigned long long* W) {
+  __shared__ unsigned long long win[kCicWin];
+  __shared__ int red[2][kBlock / 64];
+  int64_t nn = n_dev ? (int64_t)*n_dev : n;
+  if (nn > n) nn = n;
+  const int64_t start = (int64_t)blockIdx.x * kCicChunk;
+  if (start >= nn) return;  // whole work-group
+  int key[kCicPer];
+  long long w[kCicPer][4];
+  int lo = INT_MAX, hi = -1;
+#pragma unroll
+  for (int k = 0; k < kCicPer; ++k) {
+    const int64_t i = start + (int64_t)k * kBlock + threadIdx.x;
+    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
+    key[k] = i < nn ? cic_weights(x[i], y[i], dx32, dy32, nx, ny, w[k]) : -1;
+    if (key[k] >= 0) {
+      lo = min(lo, key[k]);
+      hi = max(hi, key[k]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+  }
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    red[0][wid] = lo;
+    red[1][wid] = hi;
+  }
+  __syncthreads();
+  lo = red[0][0];
+  hi = red[1][0];
+#pragma unroll
+  for (int v = 1; v < kBlock / 64; ++v) {
+    lo = min(lo, red[0][v]);
+    hi = max(hi, red[1][v]);
+  }
+  if (hi < 0) return;  // no particle of the work-group deposits
+  const int span = hi - lo + nx + 2;
+  if (span <= kCicWin) {
+    for (int e = threadIdx.x; e < span; e += kBlock) win[e] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kCicPer; ++k) wave_cic_add(win, key[k] >= 0 ? key[k] - lo : -1, w[k], nx);
+    __syncthreads();
+    for (int e = threadIdx.x; e < span; e += kBlock) {
+      const unsigned long long v = win[e];
+      if (v) atomicAdd(W + lo + e, v);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kCicPer; ++k) wave_cic_add(W, key[k], w[k], nx);
+  }
+}
+
+// rho[c] = f32(f64(rho[c]) + cd * W[c] 2^-32) for fixed-point weight sums W
+__global__ __launch_bounds__(kBlock) void k_weights_to_rho(const unsigned long long* __restrict__ W,
+                                                            int64_t n, double cd, float* rho) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double s = (double)(long long)W[i] * kCicUnit;
+  rho[i] = (float)((double)rho[i] + cd * s);
+}
+
 // rho[c] = f32(f64(rho[c]) + cd), counts[c] times (np.add.at semantics)
 __global__ __launch_bounds__(kBlock) void k_counts_to_rho(const uint32_t* __restrict__ cnt,
                                                            int64_t n, double cd, float* rho) {
@@ -231,6 +362,16 @@ __global__ void k_substep_finalize(bemcs_substeps_case_t one,
     ns = c > 1.0 ? (int)c : 1;
   }
   *n_sub = ns;
+}
+
+// User cross-section table (lookup_user_cs, Python/physics_engine.py:475-484):
+// 10 ** spline(clip(log10(max(E, 1e-30)), lo, hi)), deterministic log/exp.
+__device__ __forceinline__ double user_table(const double* t, const double* c, int n, double lo, double hi,
+                                             double E) {
+  E = nmax(E, 1e-30);
+  double le = det_log(E) * kInvLn10;
+  le = le < lo ? lo : (le > hi ? hi : le);
+  return det_exp(bspline3(t, c, n, le) * kLn10);
 }
 
 // ---------------------------------------------------------- fused ion step
@@ -340,7 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
       double a = vx, b = vy, c = vz;
       double v2 = a * a + b * b + c * c;
       double E_eV = (0.5 * p.m_ion * v2) / p.q;
-      double gamma = 0.05 + 1e-4 * E_eV;
+      double gamma = p.see_t ? user_table(p.see_t, p.see_c, p.see_n, p.see_lo, p.see_hi, E_eV)
+                             : 0.05 + 1e-4 * E_eV;
       gamma = gamma < 0.0 ? 0.0 : (gamma > 1.0 ? 1.0 : gamma);
       const uint32_t pid = tag & 0x7FFFFFFFu;
       u01x2 u = uniforms(p.seed, P_SEE, pid, p.step, 0u);
@@ -472,7 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
     double a = vx, b = vy, c = vz;
     double g = nmax(sqrt(a * a + b * b + c * c), 1.0);
     double sigma;
-    if (p.cx_model == 1) {
+    if (p.cx_t) {
+      const double E = (0.5 * p.m_ion * (g * g)) / p.q;
+      sigma = user_table(p.cx_t, p.cx_c, p.cx_n, p.cx_lo, p.cx_hi, E);
+    } else if (p.cx_model == 1) {
       double E = (0.5 * p.m_ion * (g * g)) / p.q;
       sigma = nmax((87.3 - 13.6 * (det_log(E) * kInvLn10)) * 1e-20, 0.0);
     } else {
@@ -904,6 +1049,29 @@ int bemcs_deposit_ngp_counts(const float* x, const float* y, int64_t n, double d
   BEMCS_REQUIRE(x && y && counts, "bemcs_deposit_ngp_counts: null pointer");
   hipLaunchKernelGGL(k_deposit, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), x, y,
                      n, (const int32_t*)nullptr, (float)dx, (float)dy, nx, ny, counts);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_deposit_cic(const float* x, const float* y, int64_t n, const int32_t* n_dev, double dx,
+                      double dy, int32_t nx, int32_t ny, uint64_t* weights, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n >= 0 && nx >= 2 && ny >= 2, "bemcs_deposit_cic: bad sizes");
+  BEMCS_REQUIRE((int64_t)nx * ny < INT_MAX - 2 * (int64_t)nx, "bemcs_deposit_cic: mesh too large");
+  if (n == 0) return BEMCS_OK;
+  BEMCS_REQUIRE(x && y && weights, "bemcs_deposit_cic: null pointer");
+  BEMCS_REQUIRE(dx > 0.0 && dy > 0.0, "bemcs_deposit_cic: spacing must be positive");
+  hipLaunchKernelGGL(k_deposit_cic, dim3(grid_for(n, kCicChunk)), dim3(kBlock), 0, as_stream(stream), x,
+                     y, n, n_dev, (float)dx, (float)dy, nx, ny, (unsigned long long*)weights);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_weights_to_rho(const uint64_t* weights, int64_t n_nodes, double charge_density, float* rho,
+                         bemcs_stream_t stream) {
+  BEMCS_REQUIRE(n_nodes >= 0 && weights && rho, "bemcs_weights_to_rho: bad arguments");
+  if (n_nodes == 0) return BEMCS_OK;
+  hipLaunchKernelGGL(k_weights_to_rho, dim3(grid_for(n_nodes, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), (const unsigned long long*)weights, n_nodes, charge_density, rho);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

@@ -13,8 +13,8 @@
 // fine operator is matrix-free (one byte per node).  Precision: the outer
 // flexible CG keeps x, r, p, Ap and all dot products in float64; the
 // preconditioner is one float32 V-cycle (bilinear prolongation, Galerkin
-// 9-point coarse operators built on the host, damped Jacobi smoothing, dense
-// inverse on the coarsest level, at most kCoarseMax nodes).
+// 9-point coarse operators built on the device, damped Jacobi smoothing,
+// dense inverse on the coarsest level, at most kCoarseMax nodes).
 #include <math.h>
 #include <stdlib.h>
 
@@ -825,6 +825,8 @@ __global__ __launch_bounds__(256) void k_efield(const double* __restrict__ V, in
 struct PicardPack {
   const uint32_t* cnt_ion;
   const uint32_t* cnt_e;
+  const unsigned long long* w_ion;  // bilinear weight sums (null: NGP counts)
+  const unsigned long long* w_e;
   double cd_ion, cd_e;
   float* rho;
   const double* V_fixed;
@@ -852,8 +854,14 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
     // rho from NGP counts as the reference's two np.add.at deposits produce
     // it: per particle rho = f32(f64(rho) + cd), ions first, then electrons
     float r = 0.0f;
-    for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
-    for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
+    if (P.w_ion) {
+      // bilinear weight sums, as bemcs_weights_to_rho on a zero rho
+      r = (float)((double)r + P.cd_ion * ((double)(long long)P.w_ion[i] * kCicUnit));
+      r = (float)((double)r + P.cd_e * ((double)(long long)P.w_e[i] * kCicUnit));
+    } else {
+      for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
+      for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
+    }
     P.rho[i] = r;
   } else if (PHASE == PIC_RELAX) {
     // Dirichlet values from V_fixed, Neumann rows mirror;
@@ -1859,6 +1867,8 @@ static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p) {
   PicardPack P;
   P.cnt_ion = p->cnt_ion;
   P.cnt_e = p->cnt_e;
+  P.w_ion = (const unsigned long long*)p->w_ion;
+  P.w_e = (const unsigned long long*)p->w_e;
   P.cd_ion = p->cd_ion;
   P.cd_e = p->cd_e;
   P.rho = p->rho_out;
@@ -1927,7 +1937,8 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
   unsigned gx = 1;
   for (int c = 0; c < n; ++c) {
     const bemcs_picard_t* p = &ps[c];
-    BEMCS_REQUIRE(p->cnt_ion && p->cnt_e && p->V_fixed && p->V && p->rho_out && p->Exy,
+    BEMCS_REQUIRE(((p->cnt_ion && p->cnt_e) || (p->w_ion && p->w_e)) && p->V_fixed && p->V &&
+                      p->rho_out && p->Exy,
                   "bemcs_poisson_picard: bad arguments");
     BEMCS_REQUIRE(p->iterations == iterations && p->rtol == rtol && p->max_iter == max_iter,
                   "bemcs_poisson_batch_picard: iterations, rtol and max_iter must agree across cases");

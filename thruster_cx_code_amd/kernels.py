@@ -86,3 +86,32 @@ def accumulate_rho_hip(x, y, rho, num_p, dx, dy, nx, ny, charge_density):
           "counts_to_rho")
     if host is not None:
         host[...] = rt.cpu().numpy().reshape(host.shape)
+
+
+def accumulate_rho_cic_hip(x, y, rho, num_p, dx, dy, nx, ny, charge_density):
+    """Bilinear (cloud-in-cell) deposit of the model manual
+    (Python/docs/PY-BEMCS_Manual.tex:215-230) with accumulate_rho_cpu's
+    signature (Python/physics_engine.py:178): rho[node] += charge_density *
+    (sum of the node's bilinear weights), the weights summed exactly in
+    2^-32 fixed point and added to rho once per node in float64 (one
+    rounding to float32)."""
+    n = int(num_p)
+    if n == 0:
+        return
+    require_gpu()
+    lib = _lib.load()
+    xt, _ = _dev_view(x, n, torch.float32)
+    yt, _ = _dev_view(y, n, torch.float32)
+    if isinstance(rho, torch.Tensor):
+        rt, host = rho, None
+    else:
+        host = rho
+        rt = torch.from_numpy(np.ascontiguousarray(rho, dtype=np.float32).reshape(-1)).cuda()
+    w = torch.zeros(int(nx) * int(ny), dtype=torch.int64, device=rt.device)
+    s = stream_ptr()
+    check(lib.bemcs_deposit_cic(ptr(xt), ptr(yt), n, None, float(dx), float(dy), int(nx), int(ny), ptr(w), s),
+          "deposit_cic")
+    check(lib.bemcs_weights_to_rho(ptr(w), int(nx) * int(ny), float(charge_density), ptr(rt), s),
+          "weights_to_rho")
+    if host is not None:
+        host[...] = rt.cpu().numpy().reshape(host.shape)

@@ -76,7 +76,8 @@ class DigitalTwinSimulator:
         self.kB = 1.380649e-23
         self.eps0 = 8.854e-12
         self.m_e = self.m_XE / 100.0
-        self.user_cs = {}
+        self.user_cs = {}   # label -> {'energy', 'cs', 'spline', 'type'} as the reference's loaders build
+        self._tables = {}
         self.MATERIAL_PRESETS = MATERIAL_PRESETS
         self.macro_weight = 3e5
         self.sb_sigma = 5.67e-8
@@ -141,6 +142,10 @@ class DigitalTwinSimulator:
         n0_n = params.get("n0_plasma", 1e17)
         bulk = params.get("entire_bulk_plasma", False)
         self.seed = int(params.get("seed", 0)) & 0xFFFFFFFF
+        shape = params.get("deposit_shape", "ngp")
+        if shape not in ("ngp", "cic"):
+            raise ValueError(f"deposit_shape must be 'ngp' or 'cic' (got {shape!r})")
+        self.deposit_shape = shape
         debye = np.sqrt(self.eps0 * Te_up / (self.q * n0_n * 0.61))
         if bulk:
             n_debye = 80 if n0_n <= 1e17 else (40 if n0_n <= 4e17 else 30)
@@ -271,6 +276,9 @@ class DigitalTwinSimulator:
             self.cnt_e = zeros(nn, torch.int32, dev)
             self.cnt_i_next = zeros(nn, torch.int32, dev)
             self.cnt_e_next = zeros(nn, torch.int32, dev)
+            # bilinear deposit: per-node weight sums, 2^-32 fixed point
+            self.w_i = zeros(nn, torch.int64, dev)
+            self.w_e = zeros(nn, torch.int64, dev)
             self.ppc_d = zeros(nn, torch.int32, dev)
             self.accum_ppc_d = zeros(nn, torch.float64, dev)
             # electrode temperature (ping-pong), impact heat (ping-pong, 2^-20 K
@@ -351,6 +359,11 @@ class DigitalTwinSimulator:
         self.injected_ions = self.transmitted_ions = self.entered_optics = 0.0
         self.ppc_steps_count = 0
         self._pending = []
+        # energy conservation tracking (reference :443-449)
+        self.energy_history_t, self.energy_history_ke_i, self.energy_history_ke_e = [], [], []
+        self.energy_history_fe, self.energy_history_tot = [], []
+        self._prev_total_energy = None
+        self.energy_warning_threshold = 0.05
 
     def _set_geometry_constants(self, params):
         grids = self._grids
@@ -372,6 +385,39 @@ class DigitalTwinSimulator:
         else:
             idx = [(self.ny // 2) * self.nx + xi]
         self._saddle_idx = torch.tensor(idx, dtype=torch.int64, device=self.device)
+
+    # ------------------------------------------------ user cross-sections
+    def lookup_user_cs(self, cs_type_prefix, energy_eV):
+        """Host evaluation of a user table (reference :475-484): the first
+        label with the prefix and a fitted spline, 10 ** spline(log10 E) with
+        log10 E clipped to the table's range; None without such a table."""
+        for label, ds in self.user_cs.items():
+            if label.startswith(cs_type_prefix) and ds.get('spline') is not None:
+                log_e = np.log10(np.maximum(energy_eV, 1e-30))
+                e_min = np.log10(max(ds['energy'][0], 1e-30))
+                e_max = np.log10(ds['energy'][-1])
+                return 10.0 ** ds['spline'](np.clip(log_e, e_min, e_max))
+        return None
+
+    def _table_args(self, prefix):
+        """Device copy of the table lookup_user_cs would use (knots,
+        coefficients, log10 energy range), or None."""
+        for label, ds in self.user_cs.items():
+            if not (label.startswith(prefix) and ds.get('spline') is not None):
+                continue
+            key = (label, id(ds['spline']))
+            cached = self._tables.get(prefix)
+            if cached is None or cached[0] != key:
+                t, c, k = ds['spline']._eval_args
+                if k != 3:
+                    raise ValueError("user cross-section splines must be cubic (k=3)")
+                e = np.asarray(ds['energy'], np.float64)
+                lo, hi = float(np.log10(max(e[0], 1e-30))), float(np.log10(e[-1]))
+                td = torch.from_numpy(np.ascontiguousarray(t, np.float64)).to(self.device)
+                cd = torch.from_numpy(np.ascontiguousarray(c, np.float64)).to(self.device)
+                self._tables[prefix] = (key, td, cd, lo, hi, int(len(t)))
+            return self._tables[prefix]
+        return None
 
     # ------------------------------------------------------------ Poisson
     def recalc_poisson(self, iterations=5, params=None, counts=None):
@@ -397,6 +443,9 @@ class DigitalTwinSimulator:
         p = _lib.Picard()
         ci, ce = counts if counts is not None else (self.cnt_i, self.cnt_e)
         p.cnt_ion, p.cnt_e = ptr(ci), ptr(ce)
+        if self.deposit_shape == "cic":
+            # the weight sums always hold the last deposit
+            p.w_ion, p.w_e = ptr(self.w_i), ptr(self.w_e)
         p.V_fixed, p.V = ptr(self.V_fixed_d), ptr(self.V_d)
         p.rho_out, p.Exy = ptr(self.rho_d), ptr(self.Exy)
         p.cd_ion, p.cd_e = cd, -cd
@@ -606,6 +655,10 @@ class DigitalTwinSimulator:
             self._rf_update(params, t_current)
             self._timed_end(self.kernel_timing, "poisson_rf", ev)
         self._inject(params, t_current)
+        if self.deposit_shape == "cic" and (it % 2 == 0 or self._deposit_every_step(params, thermal)):
+            ev = self._timed_begin(self.kernel_timing)
+            self._deposit_cic()
+            self._timed_end(self.kernel_timing, "deposit_cic", ev)
         if it % 2 == 0:
             kt = self.kernel_timing
             ev = self._timed_begin(kt)
@@ -698,6 +751,23 @@ class DigitalTwinSimulator:
         ev.record()
         self._pending.append((ev, cnts, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
+
+    def _deposit_every_step(self, params, thermal):
+        """The RF Picard iterations and a thermal-bow rebuild at the start of
+        the next step use this step's deposit."""
+        return bool(params.get('rf_enable')) or thermal
+
+    def _deposit_cic(self):
+        """Bilinear deposit of the current populations into the weight sums
+        (the model manual's charge assignment, PY-BEMCS_Manual.tex:215-230)."""
+        s = stream_ptr()
+        self.w_i.zero_()
+        self.w_e.zero_()
+        I, E = self.ions[0], self.elec[0]
+        check(self.lib.bemcs_deposit_cic(ptr(I.x), ptr(I.y), self.bound_i, ptr(I.count), self.dx, self.dy,
+                                         self.nx, self.ny, ptr(self.w_i), s), "deposit_cic(ions)")
+        check(self.lib.bemcs_deposit_cic(ptr(E.x), ptr(E.y), self.bound_e, ptr(E.count), self.dx, self.dy,
+                                         self.nx, self.ny, ptr(self.w_e), s), "deposit_cic(electrons)")
 
     def _step_end(self, slot):
         se = _lib.StepEnd()
@@ -895,6 +965,13 @@ class DigitalTwinSimulator:
         ip.cx_model = 1 if params.get('cx_model', 'reference') == 'loglinear' else 0
         ip.mcc_enabled = 1
         ip.seed = self.seed
+        for prefix, names in (('CX', ('cx_t', 'cx_c', 'cx_lo', 'cx_hi', 'cx_n')),
+                              ('SEE', ('see_t', 'see_c', 'see_lo', 'see_hi', 'see_n'))):
+            tab = self._table_args(prefix)
+            if tab is not None:
+                _, td, cd, lo, hi, n = tab
+                for name, v in zip(names, (ptr(td), ptr(cd), lo, hi, n)):
+                    setattr(ip, name, v)
         self._ion_params = ip
         ep = _lib.ElectronStep()
         ep.dt = self.dt
@@ -1097,11 +1174,66 @@ class DigitalTwinSimulator:
     def get_transparency(self):
         return self.transmitted_ions / self.entered_optics if self.entered_optics > 0 else 0.0
 
+    def get_third_grid_transparency_frame(self):
+        """Ions that left through the exit plane over ions that entered the
+        optics, this step only (:1869-1871, :1494)."""
+        entered = getattr(self, 'entered_optics_step', 0.0)
+        return getattr(self, 'transmitted_ions_step', 0.0) / entered if entered > 0.0 else 0.0
+
+    def get_groove_profile(self, grid_idx, thresh=None, accumulate_subcell=True, face='upstream'):
+        """Erosion depth (um) of one grid's face per mesh row, as (y_mm,
+        depth_um) (:1903-1932).  The depth map is the eroded depth plus, with
+        accumulate_subcell and thresh > 0, the sub-cell damage fraction
+        damage / thresh * dy; the face picks the first ('upstream') or last
+        ('downstream') electrode node of the grid in each row, or the maximum
+        over the grid's columns ('any').  Evaluated on the device in float64."""
+        self._resolve_surface()
+        if grid_idx < 0 or grid_idx >= self.n_grids or not self._built:
+            return np.array([]), np.array([])
+        ny, nx = self.ny, self.nx
+        mask = ((self.grid_bits_d.view(ny, nx) >> grid_idx) & 1).bool()
+        if not bool(mask.any()):
+            return np.array([]), np.array([])
+        depth = self.eroded_depth_d.view(ny, nx)
+        if accumulate_subcell and thresh and thresh > 0:
+            depth = depth + (self.damage_map_d.view(ny, nx) / thresh) * self.dy
+        has_any = mask.any(dim=1)
+        per_y = torch.zeros(ny, dtype=torch.float64, device=depth.device)
+        if face == 'any':
+            cols = mask.any(dim=0)
+            sub = depth[:, cols]
+            per_y = torch.where(has_any, sub.max(dim=1).values, per_y)
+        elif face in ('upstream', 'downstream'):
+            col = torch.arange(nx, device=depth.device)
+            if face == 'upstream':
+                idx = torch.where(mask, col, nx).min(dim=1).values.clamp(max=nx - 1)
+            else:
+                idx = torch.where(mask, col, -1).max(dim=1).values.clamp(min=0)
+            picked = depth.gather(1, idx.view(ny, 1)).view(ny)
+            per_y = torch.where(has_any, picked, per_y)
+        else:
+            raise ValueError(f"face must be 'upstream','downstream', or 'any' (got {face!r})")
+        y_mm = np.arange(ny) * self.dy
+        return y_mm, per_y.cpu().numpy() * 1000.0
+
     def has_active_particles(self):
         return self.num_p > 0 or self.num_e > 0
 
+    def mean_ion_energy_eV(self):
+        """Mean kinetic energy of the ions in eV (the headless runner's
+        'mean_particle_energy', Python/run_simulation_from_config.py:264-268),
+        summed on the device in float64."""
+        n = self.num_p
+        if n == 0:
+            return float('nan')
+        I = self.ions[0]
+        v2 = I.vx[:n].double() ** 2 + I.vy[:n].double() ** 2 + I.vz[:n].double() ** 2
+        return float(v2.sum()) * 0.5 * self.m_ion / self.q / n
+
     def get_total_energy(self):
-        """Kinetic + field energy (reference :1800-1866), summed on the device."""
+        """Kinetic + field energy (reference :1800-1866), summed on the device
+        in float64; appends to the energy history and warns when the total
+        changes by more than energy_warning_threshold in one call."""
         n, ne = self.num_p, self.num_e
         I, E = self.ions[0], self.elec[0]
         ke_i = float((I.vx[:n].double() ** 2 + I.vy[:n].double() ** 2 + I.vz[:n].double() ** 2).sum()) \
@@ -1109,8 +1241,22 @@ class DigitalTwinSimulator:
         ke_e = float((E.vx[:ne].double() ** 2 + E.vy[:ne].double() ** 2 + E.vz[:ne].double() ** 2).sum()) \
             * 0.5 * self.m_e * self.macro_weight if ne else 0.0
         e_field = float((self.Exy.double() ** 2).sum()) * 0.5 * self.eps0 * (self.dx * 1e-3) * (self.dy * 1e-3)
-        return dict(t_s=self.iteration * self.dt, ke_ions_J=ke_i, ke_elec_J=ke_e, field_J=e_field,
-                    total_J=ke_i + ke_e + e_field)
+        total = ke_i + ke_e + e_field
+        t_now = self.iteration * self.dt
+        prev = self._prev_total_energy
+        if prev is not None and prev > 0.0:
+            rel = abs(total - prev) / prev
+            if rel > self.energy_warning_threshold:
+                print(f"[Energy Warning] iter={self.iteration}: total energy changed by {rel * 100:.1f}% in one "
+                      f"step (prev={prev:.4e} J, now={total:.4e} J). KE_i={ke_i:.3e} J, KE_e={ke_e:.3e} J, "
+                      f"E_field={e_field:.3e} J")
+        self._prev_total_energy = total
+        self.energy_history_t.append(t_now)
+        self.energy_history_ke_i.append(ke_i)
+        self.energy_history_ke_e.append(ke_e)
+        self.energy_history_fe.append(e_field)
+        self.energy_history_tot.append(total)
+        return dict(t_s=t_now, ke_ions_J=ke_i, ke_elec_J=ke_e, field_J=e_field, total_J=total)
 
     def get_particle_kinematics(self):
         """(time, x, y, vx, vy, vz, energy_eV, type) rows (reference :1934-1963)."""
