@@ -161,7 +161,8 @@ def run(args):
     from thruster_cx_code_amd import DigitalTwinSimulator
 
     p = workload_params(args.workload)
-    sim = DigitalTwinSimulator()
+    sim = DigitalTwinSimulator(sort_interval=args.ion_sort_interval,
+                               electron_sort_interval=args.electron_sort_interval)
     sim.m_e = sim.m_ion / 200.0
     t_build = time.perf_counter()
     sim.build_domain(p)
@@ -250,6 +251,8 @@ def run(args):
                    "electrons_range": kstats["electron_step"]["particles_range"],
                    "n_sub_last": [rec["n_sub_ion"], rec["n_sub_e"]],
                    "num_p_last": rec["num_p"], "num_e_last": rec["num_e"],
+                   "sort_interval_steps": {"ions": args.ion_sort_interval,
+                                           "electrons": args.electron_sort_interval},
                    "remeshes": remeshes},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
@@ -438,6 +441,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5"])
+    ap.add_argument("--ion-sort-interval", type=int, default=5,
+                    help="steps between cell sorts of the ions (0: never after the preload)")
+    ap.add_argument("--electron-sort-interval", type=int, default=1,
+                    help="steps between cell sorts of the electrons")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()

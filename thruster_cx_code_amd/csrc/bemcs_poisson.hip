@@ -270,6 +270,21 @@ __device__ __forceinline__ bool block_done(const int32_t* done) {
   return s != 0;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup
+// id, and each XCD has its own L2.  The tile launches map ids so that XCD k
+// processes the k-th contiguous band of tiles (row-major) in every launch and
+// on every level: what a band writes in one launch is read back from the same
+// L2 in the next.  The map is a bijection of the (x, y) grid onto itself.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int& tx, int& ty) {
+  const int gx = (int)gridDim.x, n = gx * (int)gridDim.y;
+  const int id = (int)blockIdx.y * gx + (int)blockIdx.x;
+  const int q = n / kXcds, r = n % kXcds, xcd = id % kXcds, j = id / kXcds;
+  const int t = xcd < r ? xcd * (q + 1) + j : r * (q + 1) + (xcd - r) * q + j;
+  tx = t % gx;
+  ty = t / gx;
+}
+
 // ------------------------------------------------------------ down launch
 // Every global load of a tile is issued up front (clamped addresses, values
 // selected afterwards), so a launch waits for memory about once; the
@@ -296,7 +311,9 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const LevelDev& C = P.C;
   const CGArgs& g = P.g;
   const int tiles_x = (C.nx + TC - 1) / TC;
-  if ((int)blockIdx.x >= tiles_x || (int)blockIdx.y * TC >= C.ny) return;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  if (tx >= tiles_x || ty * TC >= C.ny) return;
   if (k > 0 && P.ca.check) {
     if (block_done(P.ca.done) || check_converged(P.ca, k)) return;
   } else if (*P.ca.done) {
@@ -310,7 +327,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   __shared__ float sst[IS9 ? 9 * N2 : 1];
   __shared__ uint8_t scode[IS9 ? 1 : N2];
   const float om = IS9 ? kOmegaC : kOmega0;
-  const int CX0 = blockIdx.x * TC, CY0 = blockIdx.y * TC;
+  const int CX0 = tx * TC, CY0 = ty * TC;
   const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
   const double* rp = pick2(g.R, (k + 1) & 1);
   double alpha = 0.0;
@@ -395,7 +412,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
       F.b[gi] = (float)rn;
       rr += rn * rn;
     }
-    block_partials(rr, 0.0, g.pr, blockIdx.y * tiles_x + blockIdx.x);
+    block_partials(rr, 0.0, g.pr, ty * tiles_x + tx);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < N2; t += 256) {
@@ -486,7 +503,9 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   const LevelDev& C = P.C;
   const CGArgs& g = P.g;
   const int tiles_x = (F.nx + TF - 1) / TF;
-  if ((int)blockIdx.x >= tiles_x || (int)blockIdx.y * TF >= F.ny) return;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  if (tx >= tiles_x || ty * TF >= F.ny) return;
   if (*g.done) return;
   constexpr int W0 = TF + 2 * SWEEPS, W1 = W0 - 2;  // the coefficient region is W1
   constexpr int N0 = W0 * W0, N1 = W1 * W1;
@@ -495,7 +514,7 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   __shared__ float sst[IS9 ? 9 * N1 : 1];
   __shared__ uint8_t scode[IS9 ? 1 : N1];
   const float om = IS9 ? kOmegaC : kOmega0;
-  const int ox = blockIdx.x * TF - SWEEPS, oy = blockIdx.y * TF - SWEEPS;
+  const int ox = tx * TF - SWEEPS, oy = ty * TF - SWEEPS;
   {
     int vg[IT0];
     float vx[IT0], vb[IT0], vid[IT0], vc[IT0][4], vw[IT0][4];
@@ -592,60 +611,69 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
     }
   }
-  if (DOT) block_partials(rz, roz, pick2(g.pz, k & 1), blockIdx.y * tiles_x + blockIdx.x);
+  if (DOT) block_partials(rz, roz, pick2(g.pz, k & 1), ty * tiles_x + tx);
 }
 
-// coarsest level: x = C^-1 b with the inverse stored column-major.  A block
-// owns 64 rows (one per lane); its 16 waves split the columns, and the partial
-// sums are added in LDS, so each thread runs only nc/16 independent loads.
-constexpr int kCoarseWaves = 16;
+// coarsest level: x = C^-1 b.  The inverse is stored column-major and is
+// symmetric up to rounding, so row i is read as the contiguous column i: one
+// coalesced 256-byte load per wavefront and step.  One wavefront per row and
+// four per work-group spread the matrix reads over many CUs; a row's
+// work-group has the same id, hence the same XCD, in every launch, so its
+// rows stay in that XCD's L2 between V-cycles.  The lane partial sums are
+// added by a fixed butterfly (deterministic).
+constexpr int kCoarseRowsPerBlock = 4;
 template <bool B>
-__global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_solve(CoarsePack one,
-                                                                    const CoarsePack* __restrict__ many,
-                                                                    int k) {
-  __shared__ float sb[kDenseMax];
-  __shared__ float part[kCoarseWaves][64];
+__global__ __launch_bounds__(64 * kCoarseRowsPerBlock) void k_coarse_solve(CoarsePack one,
+                                                                           const CoarsePack* __restrict__ many,
+                                                                           int k) {
   const CoarsePack P = B ? many[blockIdx.z] : one;
   const LevelDev& C = P.C;
-  const float* __restrict__ cinv = P.cinv;
-  if ((int)blockIdx.x * 64 >= C.n) return;
+  if ((int)blockIdx.x * kCoarseRowsPerBlock >= C.n) return;
   if (k > 0 && P.ca.check) {
-    if (block_done(P.ca.done) || check_converged<64 * kCoarseWaves>(P.ca, k)) return;
+    if (block_done(P.ca.done) || check_converged<64 * kCoarseRowsPerBlock>(P.ca, k)) return;
   } else if (*P.ca.done) {
     return;
   }
   const int nc = C.n;
-  for (int j = threadIdx.x; j < nc; j += blockDim.x) sb[j] = C.b[j];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int row = blockIdx.x * 64 + lane;
-  float s = 0.0f;
-  if (row < nc) {
-    const int chunk = (nc + kCoarseWaves - 1) / kCoarseWaves;
-    const int j0 = w * chunk, j1 = min(nc, j0 + chunk);
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int j = j0;
-    for (; j + 4 <= j1; j += 4) {
-      s0 += cinv[(size_t)j * nc + row] * sb[j];
-      s1 += cinv[(size_t)(j + 1) * nc + row] * sb[j + 1];
-      s2 += cinv[(size_t)(j + 2) * nc + row] * sb[j + 2];
-      s3 += cinv[(size_t)(j + 3) * nc + row] * sb[j + 3];
-    }
-    for (; j < j1; ++j) s0 += cinv[(size_t)j * nc + row] * sb[j];
-    s = (s0 + s1) + (s2 + s3);
+  const int lane = lane_id();
+  const int row = (int)blockIdx.x * kCoarseRowsPerBlock + (int)(threadIdx.x >> 6);
+  if (row >= nc) return;
+  const float* __restrict__ col = P.cinv + (size_t)row * nc;
+  const float* __restrict__ b = C.b;
+  float s0 = 0.0f, s1 = 0.0f;
+  int j = lane;
+  for (; j + 64 < nc; j += 128) {
+    s0 += col[j] * b[j];
+    s1 += col[j + 64] * b[j + 64];
   }
-  part[w][lane] = s;
-  __syncthreads();
-  if (w == 0 && row < nc) {
-    float t = 0.0f;
-    for (int q = 0; q < kCoarseWaves; ++q) t += part[q][lane];
-    C.x[row] = t;
-  }
+  if (j < nc) s0 += col[j] * b[j];
+  float s = s0 + s1;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) C.x[row] = s;
 }
 
 // ----------------------------------------------------------- CG kernels
-// grid-stride loops over g.nparts blocks, one partial pair per block
-#define GRID_LOOP(i, n) for (int i = blockIdx.x * 256 + threadIdx.x; i < (n); i += g.nparts * 256)
+// Node loops over g.nparts blocks, one partial pair per block.  Nodes are
+// split into min(8, nparts) contiguous bands (row-major, so bands of rows
+// like the tile bands of the V-cycle launches) and band k is strided over
+// the blocks whose id is k modulo the band count, which run on XCD k.
+__device__ __forceinline__ int band_count(int nparts) { return nparts < kXcds ? nparts : kXcds; }
+__device__ __forceinline__ int band_start(int n, int nparts) {
+  const int nb = band_count(nparts), band = (int)blockIdx.x % nb, j = (int)blockIdx.x / nb;
+  return (int)((long long)n * band / nb) + j * 256 + (int)threadIdx.x;
+}
+__device__ __forceinline__ int band_end(int n, int nparts) {
+  const int nb = band_count(nparts), band = (int)blockIdx.x % nb;
+  return (int)((long long)n * (band + 1) / nb);
+}
+__device__ __forceinline__ int band_step(int nparts) {
+  const int nb = band_count(nparts), band = (int)blockIdx.x % nb;
+  return ((nparts - band + nb - 1) / nb) * 256;
+}
+#define GRID_LOOP(i, n)                                                                  \
+  for (int i = band_start((n), g.nparts), i##_end = band_end((n), g.nparts),            \
+           i##_step = band_step(g.nparts);                                              \
+       i < i##_end; i += i##_step)
 #define CG_CASE()                               \
   const CGArgs g = B ? many[blockIdx.z] : one;  \
   if ((int)blockIdx.x >= g.nparts) return
@@ -893,7 +921,7 @@ __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const Pic
   if ((int)blockIdx.x >= g.nparts) return;
   const PicardPack P = B ? pmany[blockIdx.z] : pone;
   double rr = 0.0, bb = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < P.n; i += g.nparts * 256) {
+  GRID_LOOP(i, P.n) {
     const uint8_t kd = P.kind[i];
     double v = P.V[i];
     if (RELAX) {
@@ -1565,7 +1593,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
       P.C = h->lev[h->nlev - 1];
       P.cinv = h->cinv;
       P.ca = CheckArgs{h->pr, S->cg[c].nparts, h->nlev <= 2 ? 1 : 0, S->cg[c].scal, S->cg[c].done};
-      gx = std::max(gx, cdiv(P.C.n, 64));
+      gx = std::max(gx, cdiv(P.C.n, kCoarseRowsPerBlock));
       packs.push_back(P);
     }
     Op op{};
@@ -1660,8 +1688,8 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
     }
     case OP_COARSE: {
       const CoarsePack* pm = (const CoarsePack*)op.packs_dev;
-      if (B) hipLaunchKernelGGL(k_coarse_solve<true>, op.grid, dim3(64 * kCoarseWaves), 0, s, op.coarse, pm, k);
-      else hipLaunchKernelGGL(k_coarse_solve<false>, op.grid, dim3(64 * kCoarseWaves), 0, s, op.coarse, pm, k);
+      if (B) hipLaunchKernelGGL(k_coarse_solve<true>, op.grid, dim3(64 * kCoarseRowsPerBlock), 0, s, op.coarse, pm, k);
+      else hipLaunchKernelGGL(k_coarse_solve<false>, op.grid, dim3(64 * kCoarseRowsPerBlock), 0, s, op.coarse, pm, k);
       break;
     }
     case OP_UP: {
