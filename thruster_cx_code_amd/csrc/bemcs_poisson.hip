@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <utility>
 #include <vector>
 
@@ -1466,7 +1467,12 @@ struct bemcs_poisson_batch_s {
   bemcs::PicardPack* pic_host;  // pinned staging
   hipStream_t cap_stream;
   std::vector<hipGraphExec_t> iter_graph;
-  int pred_iters;
+  // PCG iterations launched with a solve's first graph: the count the same
+  // solve needed last time, keyed by the Picard call's length and the solve's
+  // index in it (within a call the counts follow a stable pattern, e.g. 5 2 2
+  // 2 2: the first solve sees the new charge, the next ones only the
+  // relaxation), so a solve usually needs no extra launch and host round trip
+  std::map<int, std::vector<int>> pred;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
@@ -1733,7 +1739,6 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   S->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
   const char* pg = getenv("BEMCS_POISSON_GRAPHS");
   S->use_graphs = !(pg && pg[0] == '0');
-  S->pred_iters = 8;
   S->tile0 = kTC;
   for (int c = 0; c < n; ++c)
     if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
@@ -1842,8 +1847,11 @@ static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
 // x.  Iterations advance for all cases together; converged cases skip.  The
 // initial V-cycle and the predicted number of iterations go out as one graph;
 // further iterations, if needed, one graph each.
-static int pcg(SolveSet* S, double rtol, int max_iter, int* iters, double* relres, hipStream_t s) {
-  int launched = std::min(std::max(1, S->pred_iters), max_iter);
+static int pcg(SolveSet* S, int call_len, int index, double rtol, int max_iter, int* iters, double* relres,
+               hipStream_t s) {
+  std::vector<int>& pred = S->pred[call_len];
+  if ((int)pred.size() <= index) pred.resize(index + 1, 8);
+  int launched = std::min(std::max(1, pred[index]), max_iter);
   int rc = enqueue_pcg_head(S, launched, rtol, s);
   if (rc) return rc;
   BEMCS_CHECK_LAUNCH();
@@ -1882,7 +1890,7 @@ static int pcg(SolveSet* S, double rtol, int max_iter, int* iters, double* relre
       bad_rel = rel;
     }
   }
-  S->pred_iters = worst_it;
+  pred[index] = worst_it;
   if (bad >= 0) {
     set_error("Poisson MG-PCG: case %d relative residual %.3e above tolerance %.3e after %d iterations",
               bad, bad_rel, rtol, launched);
@@ -1993,7 +2001,7 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
     if (k == 0) launch_picard_start<false>(S, s);
     else launch_picard_start<true>(S, s);
     BEMCS_CHECK_LAUNCH();
-    rc = pcg(S, rtol, max_iter, it.data(), rel.data(), s);
+    rc = pcg(S, iterations, k, rtol, max_iter, it.data(), rel.data(), s);
     for (int c = 0; c < n; ++c) {
       total[c] += it[c];
       worst[c] = std::max(worst[c], rel[c]);
@@ -2167,7 +2175,7 @@ int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rt
   int it = 0;
   double rel = 0;
   launch_cg_init(h->own, s);
-  int rc = pcg(h->own, rtol, max_iter, &it, &rel, s);
+  int rc = pcg(h->own, 0, 0, rtol, max_iter, &it, &rel, s);
   if (iters) *iters = it;
   if (relres) *relres = rel;
   hipLaunchKernelGGL(k_assemble, dim3(grid_for(n, 256)), dim3(256), 0, s, h->x, b, h->kind,
