@@ -132,6 +132,8 @@ constexpr int kTFL = 2 * kTCL;      // which every consumer block adds up
 constexpr int kLargeMesh = 600000;  // nodes above which a solve set uses the large tiles
 constexpr int kTCc = 8;             // the same on the coarser levels (more,
 constexpr int kTFc = 2 * kTCc;      // smaller tiles: these launches are latency bound)
+constexpr int kTCs = 4;             // small coarse-level tiles (fewer rounds per phase)
+constexpr int kTFs = 2 * kTCs;
 
 static_assert(kTC * kTC <= 256 && kTCc * kTCc <= 256, "one coarse node per thread in the restriction");
 
@@ -1440,6 +1442,7 @@ enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_COARSE
 struct Op {
   OpKind kind;
   int level;
+  int tile;   // coarse tile edge of a down / up launch
   dim3 grid;  // grid.z = number of cases in the op
   void* packs_dev;
   // the single-case path passes the (only) pack by value
@@ -1474,6 +1477,7 @@ struct bemcs_poisson_batch_s {
   // relaxation), so a solve usually needs no extra launch and host round trip
   std::map<int, std::vector<int>> pred;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
+  int tilec;  // coarse tile edge forced on the coarser levels (kTCc or kTCs), 0: per level
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
 };
@@ -1533,6 +1537,18 @@ struct PackUpload {
   size_t index;
 };
 
+// Coarse tile edge of the launches between level l and l+1 (l >= 1): 8x8
+// coarse tiles while the launch has at least 1024 of them (four per CU),
+// 4x4 below, where each launch is a chain of latency-bound phases and
+// smaller tiles take fewer rounds per phase.
+static int coarse_tile(const SolveSet* S, int l) {
+  if (S->tilec) return S->tilec;
+  long tiles = 0;
+  for (const bemcs_poisson_s* h : S->cases)
+    if (h->nlev - 1 > l) tiles += (long)cdiv(h->lev[l + 1].nx, kTCc) * cdiv(h->lev[l + 1].ny, kTCc);
+  return tiles >= 1024 ? kTCc : kTCs;
+}
+
 static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
                       std::vector<PackUpload>& uploads) {
   const int n = S->n;
@@ -1566,7 +1582,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
   for (int l = 0; l + 1 < lmax; ++l) {
     std::vector<DownPack> packs;
     unsigned gx = 1, gy = 1;
-    const int tc = l == 0 ? S->tile0 : kTCc;
+    const int tc = l == 0 ? S->tile0 : coarse_tile(S, l);
     for (int c = 0; c < n; ++c) {
       bemcs_poisson_s* h = S->cases[c];
       if (h->nlev - 1 <= l) continue;
@@ -1583,6 +1599,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     Op op{};
     op.kind = OP_DOWN;
     op.level = l;
+    op.tile = tc;
     op.grid = dim3(gx, gy, (unsigned)packs.size());
     op.down = packs[0];
     std::vector<uint8_t> bytes(sizeof(DownPack) * packs.size());
@@ -1614,7 +1631,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
   for (int l = lmax - 2; l >= 0; --l) {
     std::vector<UpPack> packs;
     unsigned gx = 1, gy = 1;
-    const int tf = l == 0 ? 2 * S->tile0 : kTFc;
+    const int tf = l == 0 ? 2 * S->tile0 : 2 * coarse_tile(S, l);
     for (int c = 0; c < n; ++c) {
       bemcs_poisson_s* h = S->cases[c];
       if (h->nlev - 1 <= l) continue;
@@ -1631,6 +1648,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     Op op{};
     op.kind = OP_UP;
     op.level = l;
+    op.tile = tf / 2;
     op.grid = dim3(gx, gy, (unsigned)packs.size());
     op.up = packs[0];
     std::vector<uint8_t> bytes(sizeof(UpPack) * packs.size());
@@ -1686,6 +1704,9 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
           if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, op.down, pm, k);
           else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, op.down, pm, k);
         }
+      } else if (op.tile == kTCs) {
+        if (B) hipLaunchKernelGGL((k_down<kTCs, true, false, true>), op.grid, blk, 0, s, op.down, pm, k);
+        else hipLaunchKernelGGL((k_down<kTCs, true, false, false>), op.grid, blk, 0, s, op.down, pm, k);
       } else {
         if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, op.down, pm, k);
         else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, op.down, pm, k);
@@ -1716,6 +1737,14 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
           if (B) hipLaunchKernelGGL((k_up<kTF, 1, false, true, true>), op.grid, blk, 0, s, op.up, pm, k);
           else hipLaunchKernelGGL((k_up<kTF, 1, false, true, false>), op.grid, blk, 0, s, op.up, pm, k);
         }
+      } else if (op.tile == kTCs) {
+        if (two) {
+          if (B) hipLaunchKernelGGL((k_up<kTFs, 2, true, false, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFs, 2, true, false, false>), op.grid, blk, 0, s, op.up, pm, k);
+        } else {
+          if (B) hipLaunchKernelGGL((k_up<kTFs, 1, true, false, true>), op.grid, blk, 0, s, op.up, pm, k);
+          else hipLaunchKernelGGL((k_up<kTFs, 1, true, false, false>), op.grid, blk, 0, s, op.up, pm, k);
+        }
       } else {
         if (two) {
           if (B) hipLaunchKernelGGL((k_up<kTFc, 2, true, false, true>), op.grid, blk, 0, s, op.up, pm, k);
@@ -1744,6 +1773,9 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
     if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
   const char* tl = getenv("BEMCS_MG_TILE0");
   if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
+  S->tilec = 0;
+  const char* tcs = getenv("BEMCS_MG_TILEC");
+  if (tcs && (atoi(tcs) == kTCc || atoi(tcs) == kTCs)) S->tilec = atoi(tcs);
   bool ok = true;
   auto A = [&](size_t bytes) -> void* {
     void* p = nullptr;
