@@ -79,7 +79,9 @@ struct bemcs_poisson_s {
 
 namespace bemcs {
 
-enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_COUNT = 8 };
+// per-case scalars; S_RZ0 + (k & 1) holds r_{k-1}.z_{k-1} of iteration k,
+// summed once by block 0 of the direction launch for the later consumers
+enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_RZ0, S_RZ1, S_COUNT = 8 };
 
 // ------------------------------------------------------------- 1D transfer
 __host__ __device__ __forceinline__ float w1d(int f, int I, int nc, bool periodic) {
@@ -231,14 +233,18 @@ struct CheckArgs {
   int32_t* done;
 };
 
-// Convergence test of iteration k (every block of a case computes it
-// identically; all NT threads of the block take part).
+// Convergence test of iteration k, by the first work-group of the case (all
+// NT threads take part): it records r.r and sets the stop flag, and the
+// launches that follow exit.  The other work-groups of this launch carry on;
+// after the last iteration their output is scratch that nothing reads, and
+// they save every work-group adding up all partial sums.
 template <int NT = 256>
 __device__ __forceinline__ bool check_converged(const CheckArgs& ca, int k) {
+  if (blockIdx.x != 0 || blockIdx.y != 0) return false;
   double rr, unused;
   block_sum2<NT>(ca.pr, ca.nparts, rr, unused);
   const bool stop = !(rr > ca.scal[S_TOL2]);  // converged, or NaN
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
     ca.scal[S_RR] = rr;
     ca.scal[S_ITERS] = (double)k;
     if (stop) *ca.done = 1;
@@ -335,8 +341,8 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const double* rp = pick2(g.R, (k + 1) & 1);
   double alpha = 0.0;
   if (STEP) {
-    double rz, pap, u;
-    block_sum2(pick2(g.pz, (k + 1) & 1), g.nparts, rz, u);
+    double pap, u;
+    const double rz = g.scal[S_RZ0 + (k & 1)];
     block_sum2(g.pa, g.nparts, pap, u);
     alpha = pap != 0.0 ? rz / pap : 0.0;
   }
@@ -719,11 +725,14 @@ __global__ __launch_bounds__(256) void k_cg_dir(CGArgs one, const CGArgs* __rest
   CG_CASE();
   if (*g.done) return;
   double beta = 0.0;
-  if (k > 1) {
-    double rz, roz, rz_old, u;
+  if (k > 1 || blockIdx.x == 0) {
+    double rz, roz;
     block_sum2(pick2(g.pz, (k + 1) & 1), g.nparts, rz, roz);
-    block_sum2(pick2(g.pz, k & 1), g.nparts, rz_old, u);
-    beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
+    if (k > 1) {
+      const double rz_old = g.scal[S_RZ0 + ((k - 1) & 1)];  // block 0 of iteration k-1
+      beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[S_RZ0 + (k & 1)] = rz;
   }
   double s = 0;
   GRID_LOOP(i, g.n) {
