@@ -279,6 +279,20 @@ __device__ __forceinline__ bool block_done(const int32_t* done) {
   return s != 0;
 }
 
+// Tolerance and initial convergence test of a solve from the partial sums
+// (r0.r0, b.b) of the r0 launch; all threads of the work-group take part.
+__device__ __forceinline__ void init_scalars(const CGArgs& g, double rtol) {
+  double rr, bb;
+  block_sum2(g.pi, g.nparts, rr, bb);
+  if (threadIdx.x != 0) return;
+  g.scal[S_RTOL] = rtol;
+  g.scal[S_RR] = rr;
+  g.scal[S_BB] = bb;
+  g.scal[S_TOL2] = rtol * rtol * bb;
+  g.scal[S_ITERS] = 0.0;
+  *g.done = (rr <= g.scal[S_TOL2]) ? 1 : 0;
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup
 // id, and each XCD has its own L2.  The tile launches map ids so that XCD k
 // processes the k-th contiguous band of tiles (row-major) in every launch and
@@ -314,7 +328,7 @@ __device__ __forceinline__ float tile_apply(const float* c, int W, const float* 
 
 template <int TC, bool IS9, bool STEP, bool B>
 __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __restrict__ many,
-                                              int k) {
+                                              int k, double rtol) {
   const DownPack P = B ? many[blockIdx.z] : one;
   const LevelDev& F = P.F;
   const LevelDev& C = P.C;
@@ -323,7 +337,12 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   int tx, ty;
   xcd_tile(tx, ty);
   if (tx >= tiles_x || ty * TC >= C.ny) return;
-  if (k > 0 && P.ca.check) {
+  if (!IS9 && !STEP) {
+    // level-0 launch of the initial V-cycle: its first work-group sets the
+    // tolerance and the initial stop flag; the others carry on (if the
+    // initial guess has converged, their output is scratch nobody reads)
+    if (blockIdx.x == 0 && blockIdx.y == 0) init_scalars(g, rtol);
+  } else if (k > 0 && P.ca.check) {
     if (block_done(P.ca.done) || check_converged(P.ca, k)) return;
   } else if (*P.ca.done) {
     return;
@@ -708,15 +727,7 @@ template <bool B>
 __global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs one, const CGArgs* __restrict__ many,
                                                         double rtol) {
   CG_CASE();
-  double rr, bb;
-  block_sum2(g.pi, g.nparts, rr, bb);
-  if (threadIdx.x != 0) return;
-  g.scal[S_RTOL] = rtol;
-  g.scal[S_RR] = rr;
-  g.scal[S_BB] = bb;
-  g.scal[S_TOL2] = rtol * rtol * bb;
-  g.scal[S_ITERS] = 0.0;
-  *g.done = (rr <= g.scal[S_TOL2]) ? 1 : 0;
+  init_scalars(g, rtol);
 }
 
 // p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap
@@ -1583,7 +1594,9 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     if (S->cases[c]->nlev == 1) plain.push_back(c);
   }
   if (!iteration) {
-    add_cg(OP_INIT_SCALAR, all, dim3(1, 1));  // the caller has launched the r0 kernel
+    // the caller has launched the r0 kernel; multi-level cases take their
+    // initial scalars in the level-0 down launch
+    add_cg(OP_INIT_SCALAR, plain, dim3(1, 1));
   } else {
     add_cg(OP_DIR, all, dim3(nparts_max, 1));
     add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
@@ -1699,26 +1712,26 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
       const DownPack* pm = (const DownPack*)op.packs_dev;
       if (op.level == 0 && S->tile0 == kTCL) {
         if (k > 0) {
-          if (B) hipLaunchKernelGGL((k_down<kTCL, false, true, true>), op.grid, blk, 0, s, op.down, pm, k);
-          else hipLaunchKernelGGL((k_down<kTCL, false, true, false>), op.grid, blk, 0, s, op.down, pm, k);
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, true, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTCL, false, true, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
         } else {
-          if (B) hipLaunchKernelGGL((k_down<kTCL, false, false, true>), op.grid, blk, 0, s, op.down, pm, k);
-          else hipLaunchKernelGGL((k_down<kTCL, false, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTCL, false, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
         }
       } else if (op.level == 0) {
         if (k > 0) {
-          if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, op.down, pm, k);
-          else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, op.down, pm, k);
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
         } else {
-          if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, op.down, pm, k);
-          else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
         }
       } else if (op.tile == kTCs) {
-        if (B) hipLaunchKernelGGL((k_down<kTCs, true, false, true>), op.grid, blk, 0, s, op.down, pm, k);
-        else hipLaunchKernelGGL((k_down<kTCs, true, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+        if (B) hipLaunchKernelGGL((k_down<kTCs, true, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+        else hipLaunchKernelGGL((k_down<kTCs, true, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
       } else {
-        if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, op.down, pm, k);
-        else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, op.down, pm, k);
+        if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+        else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
       }
       break;
     }
