@@ -426,3 +426,26 @@ def test_surface_modes_deterministic():
         assert np.array_equal(a, b)
     for a, b in zip(out[0][4], out[1][4]):
         assert np.array_equal(a, b)
+
+
+def test_headless_runner_on_reference_config(tmp_path, capsys):
+    """The headless runner on the reference's own config.json (three grids,
+    sim_mode Thermal, pitch forwarded): prints the reference's output line
+    every 10 iterations and writes the GUI's history CSV."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import csv
+    import os
+    from thruster_cx_code_amd import run_config
+    cfg = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config", "config.json")
+    out_csv = tmp_path / "history.csv"
+    print("[gpu] headless runner 20 steps", flush=True)
+    assert run_config.main([cfg, "--steps", "20", "--csv", str(out_csv)]) == 0
+    text = capsys.readouterr().out
+    lines = [l for l in text.splitlines() if l.startswith("Iteration: ")]
+    assert len(lines) == 2 and lines[1].startswith("Iteration: 20 | Grid Temps: [G1: ")
+    for key in ("Divergence: ", "Saddle Potential: ", "Mean Ion Energy: ", "Step Time: "):
+        assert key in lines[1]
+    rows = list(csv.reader(open(out_csv, encoding="utf-8")))
+    assert rows[0][:3] == ["iteration", "t_sim_s", "minpotential"] and rows[0][-1] == "grid_3_temp_K"
+    assert len(rows) == 21 and rows[20][0] == "20"
