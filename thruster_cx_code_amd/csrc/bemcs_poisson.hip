@@ -61,6 +61,7 @@ struct bemcs_poisson_s {
   int32_t* mirror;   // final mirror (unknown or Dirichlet) of Neumann nodes, -1 otherwise
   int32_t* mirror1;  // first hop of Neumann nodes, -1 otherwise
   double *x, *p, *Ap, *bred, *Vnew;
+  double *p1, *Ap1, *Az;  // second direction buffers and A z (multi-level path)
   double* R[2];       // residuals r_k, alternating
   int nparts;         // partial pairs per reduction buffer (level-0 tiles)
   double *pa, *pr, *pi;
@@ -79,9 +80,10 @@ struct bemcs_poisson_s {
 
 namespace bemcs {
 
-// per-case scalars; S_RZ0 + (k & 1) holds r_{k-1}.z_{k-1} of iteration k,
-// summed once by block 0 of the direction launch for the later consumers
-enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_RZ0, S_RZ1, S_COUNT = 8 };
+// per-case scalars; S_RZ0 + (k & 1) and S_PAP0 + (k & 1) hold r_{k-1}.z_{k-1}
+// and p_k.Ap_k of iteration k, stored by the first work-group of the launch
+// that sums them for the next iteration
+enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_RZ0, S_RZ1, S_PAP0, S_PAP1, S_COUNT = 10 };
 
 // ------------------------------------------------------------- 1D transfer
 __host__ __device__ __forceinline__ float w1d(int f, int I, int nc, bool periodic) {
@@ -167,6 +169,49 @@ __device__ __forceinline__ void block_partials(double a, double b, double* part,
   }
 }
 
+// Four partial sums per block (the level-0 up launch: r.z, r_old.z, z.Az,
+// z.Ap) and their fixed-order total.
+__device__ __forceinline__ void block_partials4(double a, double b, double c, double d, double* part,
+                                                int slot) {
+  __shared__ double s4[4][4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  d = wave_sum(d);
+  if (lane_id() == 0) {
+    const int w = threadIdx.x / 64;
+    s4[0][w] = a;
+    s4[1][w] = b;
+    s4[2][w] = c;
+    s4[3][w] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) part[4 * slot + threadIdx.x] =
+      (s4[threadIdx.x][0] + s4[threadIdx.x][1]) + (s4[threadIdx.x][2] + s4[threadIdx.x][3]);
+}
+
+__device__ __forceinline__ void block_sum4(const double* __restrict__ part, int n, double out[4]) {
+  __shared__ double sq[4][256];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n; k += 256) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] += part[4 * k + q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sq[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sq[q][threadIdx.x] += sq[q][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = sq[q][0];
+  __syncthreads();
+}
+
 template <int NT = 256>
 __device__ __forceinline__ void block_sum2(const double* __restrict__ part, int n, double& a_out,
                                            double& b_out) {
@@ -206,7 +251,10 @@ __device__ __forceinline__ void block_sum2(const double* __restrict__ part, int 
 struct CGArgs {
   double* x;
   double* R[2];
-  double *p, *Ap;
+  double *p, *Ap;  // single-level path; the multi-level path alternates P[k&1], AP[k&1]
+  double* P[2];
+  double* AP[2];
+  double* Az;      // A z of the last V-cycle (level-0 up launch)
   const double* bred;
   float* b0;       // level-0 right-hand side of the V-cycle
   const float* z;  // level-0 V-cycle output
@@ -359,16 +407,44 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
   const double* rp = pick2(g.R, (k + 1) & 1);
   double alpha = 0.0;
-  if (STEP) {
+  double beta = 0.0;
+  // Small level-0 tiles (meshes up to kLargeMesh nodes) form the CG direction
+  // in this launch; large tiles take it from the direction launch, which is
+  // cheaper there than the wider halo loads of the fused form.
+  constexpr bool FUSED = STEP && TC == kTC;
+  if (STEP && !FUSED) {
     double pap, u;
-    const double rz = g.scal[S_RZ0 + (k & 1)];
+    const double rz = g.scal[S_RZ0 + (k & 1)];  // block 0 of the direction launch
     block_sum2(g.pa, g.nparts, pap, u);
     alpha = pap != 0.0 ? rz / pap : 0.0;
   }
+  if (FUSED) {
+    // direction and step of iteration k in one launch: beta from the r.z
+    // partial sums of the last V-cycle, p.Ap from the expansion
+    // (z + beta p).(Az + beta Ap) = z.Az + 2 beta z.Ap + beta^2 p.Ap (A
+    // symmetric), so the direction needs no launch and no reduction of its own
+    double q[4];
+    block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
+    const double rz = q[0], roz = q[1], zaz = q[2], zap = q[3];
+    double pap = zaz;
+    if (k > 1) {
+      const double rz_old = g.scal[S_RZ0 + ((k - 1) & 1)];
+      beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
+      pap = zaz + 2.0 * beta * zap + beta * beta * g.scal[S_PAP0 + ((k - 1) & 1)];
+    }
+    alpha = pap != 0.0 ? rz / pap : 0.0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      g.scal[S_RZ0 + (k & 1)] = rz;
+      g.scal[S_PAP0 + (k & 1)] = pap;
+    }
+  }
+  // fused: A z and the previous Ap; otherwise the direction launch's Ap
+  const double* azp = FUSED ? g.Az : g.Ap;
+  const double* apo = pick2(g.AP, (k + 1) & 1);  // Ap of iteration k-1 (fused)
   {
     int vg[IT1];
     float vid[IT1], vb[IT1];
-    double vr[STEP ? IT1 : 1], va[STEP ? IT1 : 1];
+    double vr[STEP ? IT1 : 1], va[STEP ? IT1 : 1], vo[STEP ? IT1 : 1];
 #pragma unroll
     for (int it = 0; it < IT1; ++it) {
       const int t = min((int)threadIdx.x + it * 256, N1 - 1);
@@ -378,7 +454,8 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
       vid[it] = F.inv_d[gc];
       if (STEP) {
         vr[it] = rp[gc];
-        va[it] = g.Ap[gc];
+        va[it] = azp[gc];
+        vo[it] = FUSED && k > 1 ? apo[gc] : 0.0;
       } else {
         vb[it] = F.b[gc];
       }
@@ -406,7 +483,14 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
       const bool in = vg[it] >= 0;
       const float id = in ? vid[it] : 0.0f;
       float b = 0.0f;
-      if (in) b = STEP ? (float)(vr[it] - alpha * va[it]) : vb[it];
+      if (in) {
+        if (STEP) {
+          const double ap = FUSED && k > 1 ? va[it] + beta * vo[it] : va[it];  // Ap of iteration k
+          b = (float)(vr[it] - alpha * ap);
+        } else {
+          b = vb[it];
+        }
+      }
       sb[t] = b;
       sid[t] = id;
       s1[t] = om * id * b;
@@ -428,14 +512,28 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
     // the owned nodes take the CG step; their b is the value staged above
     double rr = 0.0;
     double* rn_out = pick2(g.R, k & 1);
+    double* pn = pick2(g.P, k & 1);
+    double* apn = pick2(g.AP, k & 1);
+    const double* po = pick2(g.P, (k + 1) & 1);
 #pragma unroll
     for (int it = 0; it < TFd * TFd / 256; ++it) {
       const int t = threadIdx.x + it * 256;
       const int gx = 2 * CX0 + t % TFd, gy = 2 * CY0 + t / TFd;
       if (gx >= F.nx || gy >= F.ny) continue;
       const int gi = gy * F.nx + gx;
-      const double rn = rp[gi] - alpha * g.Ap[gi];
-      g.x[gi] += alpha * g.p[gi];
+      double pi, api;
+      if (FUSED) {
+        const double zi = (double)g.z[gi];
+        pi = k > 1 ? zi + beta * po[gi] : zi;
+        api = k > 1 ? azp[gi] + beta * apo[gi] : azp[gi];
+        pn[gi] = pi;
+        apn[gi] = api;
+      } else {
+        pi = g.p[gi];
+        api = g.Ap[gi];
+      }
+      const double rn = rp[gi] - alpha * api;
+      g.x[gi] += alpha * pi;
       rn_out[gi] = rn;
       F.b[gi] = (float)rn;
       rr += rn * rn;
@@ -535,14 +633,20 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   xcd_tile(tx, ty);
   if (tx >= tiles_x || ty * TF >= F.ny) return;
   if (*g.done) return;
-  constexpr int W0 = TF + 2 * SWEEPS, W1 = W0 - 2;  // the coefficient region is W1
+  // the level-0 launch of the fused form (small level-0 tiles) also computes
+  // z on a ring around its tile, for A z at its own nodes
+  constexpr bool FUSED = DOT && TF == kTF;
+  constexpr int EXT = FUSED ? 1 : 0, H = SWEEPS + EXT;
+  constexpr int W0 = TF + 2 * H, W1 = W0 - 2;  // the coefficient region is W1
   constexpr int N0 = W0 * W0, N1 = W1 * W1;
   constexpr int IT0 = (N0 + 255) / 256, IT1 = (N1 + 255) / 256;
+  constexpr int TZ = TF + 2 * EXT;  // region of the final sweep
   __shared__ float sx[N0], sb[N0], sid[N0], s1[SWEEPS == 2 ? N1 : 1];
   __shared__ float sst[IS9 ? 9 * N1 : 1];
   __shared__ uint8_t scode[IS9 ? 1 : N1];
+  __shared__ float sz[FUSED ? TZ * TZ : 1];
   const float om = IS9 ? kOmegaC : kOmega0;
-  const int ox = tx * TF - SWEEPS, oy = ty * TF - SWEEPS;
+  const int ox = tx * TF - H, oy = ty * TF - H;
   {
     int vg[IT0];
     float vx[IT0], vb[IT0], vid[IT0], vc[IT0][4], vw[IT0][4];
@@ -618,12 +722,10 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     }
     __syncthreads();
   }
-  double rz = 0.0, roz = 0.0;
-  for (int t = threadIdx.x; t < TF * TF; t += 256) {
-    const int lx = t % TF + SWEEPS, ly = t / TF + SWEEPS;
+  double rz = 0.0, roz = 0.0, zaz = 0.0, zap = 0.0;
+  for (int t = threadIdx.x; t < TZ * TZ; t += 256) {
+    const int lx = t % TZ + SWEEPS, ly = t / TZ + SWEEPS;
     const int gx = ox + lx, gy = oy + ly;
-    if (gx >= F.nx || gy >= F.ny) continue;
-    const int gi = gy * F.nx + gx;
     const int li = ly * W0 + lx;
     const int t1 = (ly - 1) * W1 + (lx - 1);  // index in the coefficient region
     const float id = sid[li];
@@ -633,13 +735,47 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       const int W = SWEEPS == 2 ? W1 : W0;
       v = c[0] + om * id * (sb[li] - tile_apply<IS9>(c, W, sst, N1, t1, IS9 ? 0 : scode[t1]));
     }
+    if (FUSED) {
+      sz[t] = v;
+      continue;
+    }
+    if (gx >= F.nx || gy >= F.ny) continue;
+    const int gi = gy * F.nx + gx;
     F.t[gi] = v;
     if (DOT) {
       rz += pick2(g.R, k & 1)[gi] * (double)v;
       roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
     }
   }
-  if (DOT) block_partials(rz, roz, pick2(g.pz, k & 1), ty * tiles_x + tx);
+  if (FUSED) {
+    __syncthreads();
+    // owned nodes: z, A z with the level-0 operator (neighbours as the
+    // single-level direction kernel takes them), partial sums
+    for (int t = threadIdx.x; t < TF * TF; t += 256) {
+      const int ux = t % TF + EXT, uy = t / TF + EXT;  // position in the sz region
+      const int gx = ox + SWEEPS + ux, gy = oy + SWEEPS + uy;
+      if (gx >= F.nx || gy >= F.ny) continue;
+      const int gi = gy * F.nx + gx;
+      const float v = sz[uy * TZ + ux];
+      F.t[gi] = v;
+      double az = 0.0;
+      const int d = F.code[gi];
+      if (d) {
+        const float zs = (gy > 0 || F.periodic) ? sz[(uy - 1) * TZ + ux] : v;
+        const float zn = (gy < F.ny - 1 || F.periodic) ? sz[(uy + 1) * TZ + ux] : v;
+        const double nb =
+            (double)sz[uy * TZ + ux - 1] + (double)sz[uy * TZ + ux + 1] + (double)zs + (double)zn;
+        az = (double)d * (double)v - nb;
+      }
+      g.Az[gi] = az;
+      rz += pick2(g.R, k & 1)[gi] * (double)v;
+      roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
+      zaz += (double)v * az;
+      if (k > 0) zap += (double)v * pick2(g.AP, k & 1)[gi];
+    }
+  }
+  if (FUSED) block_partials4(rz, roz, zaz, zap, pick2(g.pz, k & 1), ty * tiles_x + tx);
+  else if (DOT) block_partials(rz, roz, pick2(g.pz, k & 1), ty * tiles_x + tx);
 }
 
 // coarsest level: x = C^-1 b.  The inverse is stored column-major and is
@@ -1526,6 +1662,11 @@ static CGArgs cg_args_of(const bemcs_poisson_s* h, double* scal, int32_t* done, 
   g.R[1] = h->R[1];
   g.p = h->p;
   g.Ap = h->Ap;
+  g.P[0] = h->p;
+  g.P[1] = h->p1;
+  g.AP[0] = h->Ap;
+  g.AP[1] = h->Ap1;
+  g.Az = h->Az;
   g.bred = h->bred;
   g.b0 = h->lev[0].b;
   g.z = h->nlev == 1 ? h->lev[0].x : h->lev[0].t;
@@ -1598,7 +1739,9 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     // initial scalars in the level-0 down launch
     add_cg(OP_INIT_SCALAR, plain, dim3(1, 1));
   } else {
-    add_cg(OP_DIR, all, dim3(nparts_max, 1));
+    // with small level-0 tiles, multi-level cases form the direction inside
+    // the level-0 down launch
+    add_cg(OP_DIR, S->tile0 == kTC ? plain : all, dim3(nparts_max, 1));
     add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
   }
   for (int l = 0; l + 1 < lmax; ++l) {
@@ -2156,14 +2299,17 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
     h->R[1] = (double*)A(sizeof(double) * n);
     h->p = (double*)A(sizeof(double) * n);
     h->Ap = (double*)A(sizeof(double) * n);
+    h->p1 = (double*)A(sizeof(double) * n);
+    h->Ap1 = (double*)A(sizeof(double) * n);
+    h->Az = (double*)A(sizeof(double) * n);
     h->bred = (double*)A(sizeof(double) * n);
     h->Vnew = (double*)A(sizeof(double) * n);
     h->nparts = ((nx + kTF - 1) / kTF) * ((ny + kTF - 1) / kTF);
     h->pa = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pr = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pi = (double*)A(sizeof(double) * 2 * h->nparts);
-    h->pz[0] = (double*)A(sizeof(double) * 2 * h->nparts);
-    h->pz[1] = (double*)A(sizeof(double) * 2 * h->nparts);
+    h->pz[0] = (double*)A(sizeof(double) * 4 * h->nparts);
+    h->pz[1] = (double*)A(sizeof(double) * 4 * h->nparts);
     // set-up workspace, kept for in-place updates
     const int n1 = nlev > 1 ? h->lev[1].n : 1;
     h->su_st[0] = (double*)A(sizeof(double) * 9 * n1);
