@@ -62,6 +62,7 @@ struct bemcs_poisson_s {
   int32_t* mirror1;  // first hop of Neumann nodes, -1 otherwise
   double *x, *p, *Ap, *bred, *Vnew;
   double *p1, *Ap1, *Az;  // second direction buffers and A z (multi-level path)
+  double* xg;             // solution of the solve before last (Picard extrapolation)
   double* R[2];       // residuals r_k, alternating
   int nparts;         // partial pairs per reduction buffer (level-0 tiles)
   double *pa, *pr, *pi;
@@ -137,6 +138,7 @@ constexpr int kLargeMesh = 600000;  // nodes above which a solve set uses the la
 constexpr int kTCc = 8;             // the same on the coarser levels (more,
 constexpr int kTFc = 2 * kTCc;      // smaller tiles: these launches are latency bound)
 constexpr int kTCs = 4;             // small coarse-level tiles (fewer rounds per phase)
+constexpr double kPicardExtrap = 0.5;  // initial-guess extrapolation of Picard solves
 constexpr int kTFs = 2 * kTCs;
 
 static_assert(kTC * kTC <= 256 && kTCc * kTCc <= 256, "one coarse node per thread in the restriction");
@@ -1023,6 +1025,9 @@ struct PicardPack {
   const int32_t* mirror;
   double* bred;
   const double* x;
+  double* xg;       // x of the solve before last (initial-guess extrapolation)
+  const double* xr; // last solution, relaxed into V at the start of a sweep
+  double* xw;       // x, writable (initial-guess extrapolation)
   int nx, ny, per, n;
   double coeff, Vp, Te, qn0, one_m_omega, omega, dx_m, dy_m;
   float* Exy;
@@ -1068,6 +1073,21 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
   }
 }
 
+// Initial guess of Picard solve j >= 2 from the last two solutions:
+// x_j^0 = x_{j-1} + theta (x_{j-1} - x_{j-2}); the solutions of successive
+// Picard sweeps change by a nearly constant factor, so the guess starts the
+// solve closer to its solution.  theta = 0 only saves x_{j-1}.
+template <bool B>
+__global__ __launch_bounds__(256) void k_picard_extrap(PicardPack one, const PicardPack* __restrict__ many,
+                                                       double theta) {
+  const PicardPack P = B ? many[blockIdx.z] : one;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  const double x1 = P.xw[i];
+  if (theta != 0.0) P.xw[i] = x1 + theta * (x1 - P.xg[i]);
+  P.xg[i] = x1;
+}
+
 // One launch for the start of Picard iteration j: the relaxation of
 // iteration j-1 (RELAX), the right-hand side of iteration j and the initial
 // PCG residual r0 = b - A x with its partials (r0.r0, b.b).  Each node needs
@@ -1085,11 +1105,11 @@ __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const Pic
     double v = P.V[i];
     if (RELAX) {
       double vn;
-      if (kd == K_UNKNOWN) vn = P.x[i];
+      if (kd == K_UNKNOWN) vn = P.xr[i];
       else if (kd == K_BOUND) vn = P.V_fixed[i];
       else {
         const int m = P.mirror[i];
-        vn = P.kind[m] == K_UNKNOWN ? P.x[m] : P.V_fixed[m];
+        vn = P.kind[m] == K_UNKNOWN ? P.xr[m] : P.V_fixed[m];
       }
       P.Vnew[i] = vn;
       v = P.one_m_omega * v + P.omega * vn;
@@ -1634,6 +1654,7 @@ struct bemcs_poisson_batch_s {
   std::map<int, std::vector<int>> pred;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
   int tilec;  // coarse tile edge forced on the coarser levels (kTCc or kTCs), 0: per level
+  double extrap;  // theta of the Picard initial-guess extrapolation (0: off)
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
 };
@@ -1938,6 +1959,8 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
     if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
   const char* tl = getenv("BEMCS_MG_TILE0");
   if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
+  const char* ex = getenv("BEMCS_PICARD_EXTRAP");
+  S->extrap = ex ? atof(ex) : kPicardExtrap;
   S->tilec = 0;
   const char* tcs = getenv("BEMCS_MG_TILEC");
   if (tcs && (atoi(tcs) == kTCc || atoi(tcs) == kTCs)) S->tilec = atoi(tcs);
@@ -2096,7 +2119,7 @@ static int pcg(SolveSet* S, int call_len, int index, double rtol, int max_iter, 
   return BEMCS_OK;
 }
 
-static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p) {
+static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p, bool extrap) {
   PicardPack P;
   P.cnt_ion = p->cnt_ion;
   P.cnt_e = p->cnt_e;
@@ -2112,6 +2135,11 @@ static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p) {
   P.mirror = h->mirror;
   P.bred = h->bred;
   P.x = h->x;
+  P.xw = h->x;
+  P.xg = h->xg;
+  // with extrapolation the solve's x holds the next initial guess when a
+  // sweep starts, and the last solution has been saved to xg
+  P.xr = extrap ? h->xg : h->x;
   P.nx = h->nx;
   P.ny = h->ny;
   P.per = h->periodic;
@@ -2160,6 +2188,15 @@ static void launch_picard(SolveSet* S, unsigned gx, hipStream_t s) {
                        (const PicardPack*)S->pic_dev);
 }
 
+static void launch_picard_extrap(SolveSet* S, unsigned gx, double theta, hipStream_t s) {
+  if (S->n > 1)
+    hipLaunchKernelGGL(k_picard_extrap<true>, dim3(gx, 1, S->n), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev, theta);
+  else
+    hipLaunchKernelGGL(k_picard_extrap<false>, dim3(gx, 1, 1), dim3(256), 0, s, S->pic_host[0],
+                       (const PicardPack*)S->pic_dev, theta);
+}
+
 // The Picard loop of every case of the set (:886-902), then E = -grad V.
 static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, double* worst_relres,
                   hipStream_t s) {
@@ -2175,7 +2212,7 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
                   "bemcs_poisson_picard: bad arguments");
     BEMCS_REQUIRE(p->iterations == iterations && p->rtol == rtol && p->max_iter == max_iter,
                   "bemcs_poisson_batch_picard: iterations, rtol and max_iter must agree across cases");
-    S->pic_host[c] = picard_pack(S->cases[c], p);
+    S->pic_host[c] = picard_pack(S->cases[c], p, S->extrap != 0.0);
     gx = std::max(gx, grid_for(S->pic_host[c].n, 256));
   }
   if (n > 1)
@@ -2194,6 +2231,7 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
   std::vector<double> rel(n, 0.0), worst(n, 0.0);
   int rc = BEMCS_OK;
   for (int k = 0; k < iterations && rc == BEMCS_OK; ++k) {
+    if (k >= 1 && S->extrap != 0.0) launch_picard_extrap(S, gx, k >= 2 ? S->extrap : 0.0, s);
     // relaxation of the previous iteration + this right-hand side + r0
     if (k == 0) launch_picard_start<false>(S, s);
     else launch_picard_start<true>(S, s);
@@ -2302,6 +2340,7 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
     h->p1 = (double*)A(sizeof(double) * n);
     h->Ap1 = (double*)A(sizeof(double) * n);
     h->Az = (double*)A(sizeof(double) * n);
+    h->xg = (double*)A(sizeof(double) * n);
     h->bred = (double*)A(sizeof(double) * n);
     h->Vnew = (double*)A(sizeof(double) * n);
     h->nparts = ((nx + kTF - 1) / kTF) * ((ny + kTF - 1) / kTF);
