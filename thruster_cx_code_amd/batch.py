@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import ptr, stream_ptr
+from .device import stream_ptr
 from .physics_engine import RING, DigitalTwinSimulator, check
 
 
@@ -248,16 +248,18 @@ class BatchedSimulator:
                 u.dx, u.dy, u.q_m, u.dt, u.dt2, u.frac = sim.dx, sim.dy, qm, sim.dt, sim.dt ** 2, 0.25
                 u.ds_max_bits, u.n_sub_out = dsb + off, nsb + off // 2
             ip = sim._ion_params
-            ip.Exy, ip.is_bound, ip.n_sub = ptr(sim.Exy), ptr(sim.is_bound_d), C.c_void_p(nsb)
-            ip.cnt_next, ip.ppc = ptr(sim.cnt_i_next), ptr(sim.ppc_d)
-            ip.crossed_angle, ip.diag = ptr(sim.crossed_angle), C.c_void_p(diag_ptr)
-            ip.e_cnt_next = ptr(sim.cnt_e_next)
+            exy, isb = sim.Exy.data_ptr(), sim.is_bound_d.data_ptr()
+            cen = sim.cnt_e_next.data_ptr()
+            ip.Exy, ip.is_bound, ip.n_sub = exy, isb, nsb
+            ip.cnt_next, ip.ppc = sim.cnt_i_next.data_ptr(), sim.ppc_d.data_ptr()
+            ip.crossed_angle, ip.diag = sim.crossed_angle.data_ptr(), diag_ptr
+            ip.e_cnt_next = cen
             ip.step = it & 0xFFFFFFFF
             setattr(ic[c], "in", I0.struct())
             ic[c].out, ic[c].e_out, ic[c].p = I1.struct(), E1.struct(), ip
             ep = sim._elec_params
-            ep.Exy, ep.is_bound, ep.n_sub = ptr(sim.Exy), ptr(sim.is_bound_d), C.c_void_p(nsb + 4)
-            ep.cnt_next, ep.overflow = ptr(sim.cnt_e_next), ptr(sim.overflow_d)
+            ep.Exy, ep.is_bound, ep.n_sub = exy, isb, nsb + 4
+            ep.cnt_next, ep.overflow = cen, sim.overflow_d.data_ptr()
             setattr(ec[c], "in", E0.struct())
             ec[c].out, ec[c].p = E1.struct(), ep
             # after the step the output buffers become the inputs

@@ -47,10 +47,15 @@ class SpeciesBuffers:
         self.vz = empty(cap, torch.float32, device)
         self.tag = empty(cap, torch.int32, device) if with_tag else None
         self.count = zeros(1, torch.int32, device)
+        self._struct = None
 
     def struct(self):
-        return _lib.Species(ptr(self.x), ptr(self.y), ptr(self.vx), ptr(self.vy), ptr(self.vz),
-                            ptr(self.tag), ptr(self.count), self.cap)
+        """The bemcs_species_t of these buffers (cached until `grow`; callers
+        copy it or pass it by const pointer)."""
+        if self._struct is None:
+            self._struct = _lib.Species(ptr(self.x), ptr(self.y), ptr(self.vx), ptr(self.vy), ptr(self.vz),
+                                        ptr(self.tag), ptr(self.count), self.cap)
+        return self._struct
 
     def arrays(self):
         out = [self.x, self.y, self.vx, self.vy, self.vz]
@@ -70,6 +75,7 @@ class SpeciesBuffers:
                 new[:n_valid].copy_(old[:n_valid])
             setattr(self, name, new)
         self.cap = new_cap
+        self._struct = None
 
 
 class CountBound:

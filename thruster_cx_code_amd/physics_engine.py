@@ -553,7 +553,7 @@ class DigitalTwinSimulator:
             n0 = params.get('n0_plasma', 1e17)
             Te_up = params.get('Te_up', 3.0)
             Ti = params.get('Ti', 0.1)
-            vb = np.sqrt(self.q_ion * Te_up / self.m_ion)
+            vb = math.sqrt(self.q_ion * Te_up / self.m_ion)
             area = self.Ly * 1e-3 * 0.005
             bf = 1.0 if params.get('entire_bulk_plasma', False) else 0.61
             I_ion = self.q_ion * bf * n0 * vb * area
@@ -581,7 +581,7 @@ class DigitalTwinSimulator:
                     p.y_lo[0], p.y_hi[0] = self.dy, self.Ly - self.dy
                 p.x0 = self.dx * 1.5
                 p.v_drift = float(vb)
-                p.v_spread = float(np.sqrt(self.q_ion * Ti / self.m_ion))
+                p.v_spread = math.sqrt(self.q_ion * Ti / self.m_ion)
                 p.pid_base = self.next_ion_pid & 0xFFFFFFFF
                 plans.append((True, p, n))
                 self.next_ion_pid += n
@@ -605,12 +605,12 @@ class DigitalTwinSimulator:
         ne = int(params.get('neut_rate', 30))
         if ne > 0:
             Te = params.get('Te', 5.0)
-            nxp = float(np.clip(params.get('neut_x', self.Lx - 0.5), self.dx, self.Lx - self.dx))
-            nrp = float(np.clip(params.get('neut_r', self.Ly), self.dy, self.Ly))
+            nxp = float(min(max(params.get('neut_x', self.Lx - 0.5), self.dx), self.Lx - self.dx))
+            nrp = float(min(max(params.get('neut_r', self.Ly), self.dy), self.Ly))
             p = self._inject_struct(1, self.cnt_e)
             p.y_lo[0], p.y_hi[0] = 0.0, nrp
             p.x0 = nxp
-            p.v_spread = float(np.sqrt(2.0 * self.q * Te / self.m_e))
+            p.v_spread = math.sqrt(2.0 * self.q * Te / self.m_e)
             plans.append((False, p, ne))
         return plans
 
