@@ -84,7 +84,7 @@ namespace bemcs {
 // per-case scalars; S_RZ0 + (k & 1) and S_PAP0 + (k & 1) hold r_{k-1}.z_{k-1}
 // and p_k.Ap_k of iteration k, stored by the first work-group of the launch
 // that sums them for the next iteration
-enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_RZ0, S_RZ1, S_PAP0, S_PAP1, S_COUNT = 10 };
+enum { S_RR = 0, S_BB, S_TOL2, S_ITERS, S_RTOL, S_RZ0, S_RZ1, S_PAP0, S_PAP1, S_Q0, S_COUNT = 14 };
 
 // ------------------------------------------------------------- 1D transfer
 __host__ __device__ __forceinline__ float w1d(int f, int I, int nc, bool periodic) {
@@ -263,6 +263,7 @@ struct CGArgs {
   const uint8_t* code;
   int nx, ny, per, n;
   int nparts;      // partial pairs per buffer (one per level-0 tile)
+  int presum;      // the level-0 up launch's partial sums are added by a launch of their own
   double* pa;      // p.Ap
   double* pr;      // r.r
   double* pz[2];   // (r_k.z_k, r_{k-1}.z_k) of V-cycle k in pz[k&1]
@@ -426,7 +427,12 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
     // (z + beta p).(Az + beta Ap) = z.Az + 2 beta z.Ap + beta^2 p.Ap (A
     // symmetric), so the direction needs no launch and no reduction of its own
     double q[4];
-    block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
+    if (g.presum) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = g.scal[S_Q0 + j];
+    } else {
+      block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
+    }
     const double rz = q[0], roz = q[1], zaz = q[2], zap = q[3];
     double pap = zaz;
     if (k > 1) {
@@ -866,6 +872,17 @@ __global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs one, const CGArgs
                                                         double rtol) {
   CG_CASE();
   init_scalars(g, rtol);
+}
+
+// A batch's sums of the level-0 up launch's partial quads (r.z, r_old.z,
+// z.Az, z.Ap) of iteration k, one work-group per case, for the step launch.
+template <bool B>
+__global__ __launch_bounds__(256) void k_sum_quads(CGArgs one, const CGArgs* __restrict__ many, int k) {
+  const CGArgs g = B ? many[blockIdx.z] : one;
+  if (*g.done) return;
+  double q[4];
+  block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
+  if (threadIdx.x < 4) g.scal[S_Q0 + threadIdx.x] = q[threadIdx.x];
 }
 
 // p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap
@@ -1613,7 +1630,7 @@ static void destroy(bemcs_poisson_s* h) {
 // Per case, the set owns the PCG scalars and the stop flag; vectors and
 // partial buffers belong to the handle (a handle must not be used by two
 // sets at the same time).
-enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_COARSE, OP_UP, OP_DOTZ };
+enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_COARSE, OP_UP, OP_DOTZ, OP_SUMQ };
 
 struct Op {
   OpKind kind;
@@ -1763,6 +1780,12 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     // with small level-0 tiles, multi-level cases form the direction inside
     // the level-0 down launch
     add_cg(OP_DIR, S->tile0 == kTC ? plain : all, dim3(nparts_max, 1));
+    if (S->tile0 == kTC && n > 1) {
+      std::vector<int> multi;
+      for (int c = 0; c < n; ++c)
+        if (S->cases[c]->nlev > 1) multi.push_back(c);
+      add_cg(OP_SUMQ, multi, dim3(1, 1));
+    }
     add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
   }
   for (int l = 0; l + 1 < lmax; ++l) {
@@ -1855,6 +1878,10 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
     case OP_INIT:
       if (B) hipLaunchKernelGGL(k_cg_init<true>, op.grid, blk, 0, s, op.cg, cgm);
       else hipLaunchKernelGGL(k_cg_init<false>, op.grid, blk, 0, s, op.cg, cgm);
+      break;
+    case OP_SUMQ:
+      if (B) hipLaunchKernelGGL(k_sum_quads<true>, op.grid, blk, 0, s, op.cg, cgm, k);
+      else hipLaunchKernelGGL(k_sum_quads<false>, op.grid, blk, 0, s, op.cg, cgm, k);
       break;
     case OP_INIT_SCALAR:
       if (B) hipLaunchKernelGGL(k_cg_init_scalar<true>, op.grid, blk, 0, s, op.cg, cgm, rtol);
@@ -1983,7 +2010,12 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   if (ok) ok = hipHostMalloc((void**)&S->pic_host, sizeof(PicardPack) * n, 0) == hipSuccess;
   if (ok) {
     for (int c = 0; c < n; ++c)
+    {
       S->cg.push_back(cg_args_of(hs[c], S->scal + S_COUNT * c, S->done + c, S->tile0));
+      // a batch adds each case's level-0 partial sums once per iteration
+      // (the same fixed-order sum as a case alone adds in every work-group)
+      S->cg.back().presum = n > 1 && S->tile0 == kTC ? 1 : 0;
+    }
     std::vector<PackUpload> uploads;
     build_ops(S, false, S->init_ops, uploads);
     build_ops(S, true, S->iter_ops, uploads);
