@@ -429,3 +429,42 @@ def test_poisson_update_rejects_invalid_map(lib):
         assert b"edge" in lib.bemcs_last_error()
     finally:
         lib.bemcs_poisson_free(h)
+
+
+@pytest.mark.parametrize("nx,ny", [(1024, 384), (2048, 768)])
+def test_two_level_down_launch_same_bits(lib, nx, ny, monkeypatch):
+    """The fused down launch of the last two coarse levels computes what two
+    single-level launches compute, in the same order: solves give identical
+    bits and iteration counts with it (default) and without it
+    (BEMCS_MG_FUSE=0)."""
+    from conftest import c1_params
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    print(f"[gpu] two-level down {nx}x{ny}", flush=True)
+    g = DigitalTwinSimulator()
+    g.build_domain(c1_params(nx=nx, ny=ny, ions=0, electrons=0))
+    ib = np.ascontiguousarray(g.isBound.astype(np.uint8))
+    handles = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("BEMCS_MG_FUSE", fuse)
+        h = C.c_void_p()
+        assert lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), nx, ny, 0, C.byref(h)) == 0
+        handles.append(h)
+    monkeypatch.delenv("BEMCS_MG_FUSE")
+    try:
+        assert lib.bemcs_poisson_levels(handles[0]) >= 5
+        r = np.random.default_rng(nx)
+        for rep in range(2):
+            bd = _dev(r.normal(0.0, 1.0, nx * ny) * np.where(ib.ravel() != 0, 100.0, 1e-3))
+            outs = []
+            for h in handles:
+                V = torch.zeros(nx * ny, dtype=torch.float64, device="cuda")
+                its, rel = C.c_int32(0), C.c_double(0)
+                rc = lib.bemcs_poisson_solve(h, _p(bd), _p(V), 1e-10, 300, C.byref(its), C.byref(rel), _stream())
+                assert rc == 0, lib.bemcs_last_error()
+                outs.append((V.cpu().numpy(), its.value))
+            print(f"  rep {rep}: iterations {outs[0][1]} / {outs[1][1]}")
+            assert outs[0][1] == outs[1][1]
+            assert np.array_equal(outs[0][0], outs[1][0])
+    finally:
+        for h in handles:
+            lib.bemcs_poisson_free(h)

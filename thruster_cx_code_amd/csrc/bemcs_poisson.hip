@@ -602,6 +602,203 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   C.b[I] = s;
 }
 
+// ----------------------------------------------- two-level down launch
+// The down launches of two coarse levels in one (F -> M -> C, 9-point
+// operators, non-periodic meshes): a work-group owns a kT2 x kT2 tile of C,
+// recomputes the right-hand side of M on the (2 kT2 + 5)^2 region its
+// M-level sweeps need (the restriction of F's residual on a region of F
+// around it), then runs the M level as the single-level launch does.  Every
+// value is computed by the same operations in the same order as two
+// single-level launches, so the V-cycle gives the same bits; one launch and
+// one global round trip fewer.
+constexpr int kT2 = 4;
+struct Down2Pack {
+  LevelDev F, M, C;
+  const int32_t* done;
+};
+
+template <bool B>
+__global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* __restrict__ many) {
+  const Down2Pack P = B ? many[blockIdx.z] : one;
+  const LevelDev& F = P.F;
+  const LevelDev& M = P.M;
+  const LevelDev& C = P.C;
+  const int tiles_x = (C.nx + kT2 - 1) / kT2;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  if (tx >= tiles_x || ty * kT2 >= C.ny) return;
+  if (*P.done) return;
+  constexpr float om = kOmegaC;
+  // M level: the single-level launch's regions for a kT2 tile of C
+  constexpr int MW1 = 2 * kT2 + 5, MW2 = MW1 - 2, MW3 = MW1 - 4;
+  constexpr int MN1 = MW1 * MW1, MN2 = MW2 * MW2, MN3 = MW3 * MW3;
+  // F level: the same for the MW1 x MW1 region of M
+  constexpr int FW1 = 2 * MW1 + 5, FW2 = FW1 - 2, FW3 = FW1 - 4;
+  constexpr int FN1 = FW1 * FW1, FN2 = FW2 * FW2, FN3 = FW3 * FW3;
+  constexpr int FIT1 = (FN1 + 255) / 256, FIT2 = (FN2 + 255) / 256;
+  __shared__ float fb[FN1], fid[FN1], f1[FN1], f2[FN2], fr[FN3];
+  __shared__ float fst[9 * FN2];
+  __shared__ float mb[MN1], mid[MN1], m1[MN1], m2[MN2], mr[MN3];
+  __shared__ float mst[9 * MN2];
+  const int CX0 = tx * kT2, CY0 = ty * kT2;
+  const int mx0 = 2 * CX0 - 3, my0 = 2 * CY0 - 3;  // M coordinates of the M region's (0, 0)
+  const int fx0 = 2 * mx0 - 3, fy0 = 2 * my0 - 3;  // F coordinates of the F region's (0, 0)
+  {
+    int vg[FIT1];
+    float vid[FIT1], vb[FIT1];
+#pragma unroll
+    for (int it = 0; it < FIT1; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, FN1 - 1);
+      const int gi = gnode(F, fx0 + t % FW1, fy0 + t / FW1);
+      const int gc = gi < 0 ? 0 : gi;
+      vg[it] = gi;
+      vid[it] = F.inv_d[gc];
+      vb[it] = F.b[gc];
+    }
+    float vst[FIT2][9];
+    int vg2[FIT2];
+#pragma unroll
+    for (int it = 0; it < FIT2; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, FN2 - 1);
+      const int gi = gnode(F, fx0 + 1 + t % FW2, fy0 + 1 + t / FW2);
+      const int gc = gi < 0 ? 0 : gi;
+      vg2[it] = gi;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) vst[it][q] = F.st[(size_t)q * F.n + gc];
+    }
+#pragma unroll
+    for (int it = 0; it < FIT1; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= FN1) break;
+      const bool in = vg[it] >= 0;
+      const float id = in ? vid[it] : 0.0f;
+      const float b = in ? vb[it] : 0.0f;
+      fb[t] = b;
+      fid[t] = id;
+      f1[t] = om * id * b;
+    }
+#pragma unroll
+    for (int it = 0; it < FIT2; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= FN2) break;
+      const bool in = vg2[it] >= 0;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) fst[q * FN2 + t] = in ? vst[it][q] : 0.0f;
+    }
+    // M level: 1/diagonal and coefficients (the right-hand side follows)
+    for (int t = threadIdx.x; t < MN1; t += 256) {
+      const int gi = gnode(M, mx0 + t % MW1, my0 + t / MW1);
+      mid[t] = gi >= 0 ? M.inv_d[gi] : 0.0f;
+    }
+    for (int t = threadIdx.x; t < MN2; t += 256) {
+      const int gi = gnode(M, mx0 + 1 + t % MW2, my0 + 1 + t / MW2);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) mst[q * MN2 + t] = gi >= 0 ? M.st[(size_t)q * M.n + gi] : 0.0f;
+    }
+  }
+  __syncthreads();
+  // F: second sweep; the F nodes under this work-group's M tile keep it
+  for (int t = threadIdx.x; t < FN2; t += 256) {
+    const int lx = t % FW2 + 1, ly = t / FW2 + 1;
+    const int li = ly * FW1 + lx;
+    const float id = fid[li];
+    float v = 0.0f;
+    if (id != 0.0f) v = f1[li] + om * id * (fb[li] - tile_apply<true>(&f1[li], FW1, fst, FN2, t, 0));
+    f2[t] = v;
+    const int gx = fx0 + lx, gy = fy0 + ly;
+    if (lx >= 9 && lx < 9 + 4 * kT2 && ly >= 9 && ly < 9 + 4 * kT2 && gx < F.nx && gy < F.ny)
+      F.x[gy * F.nx + gx] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < FN3; t += 256) {
+    const int lx = t % FW3 + 2, ly = t / FW3 + 2;
+    const int li = ly * FW1 + lx;
+    const float id = fid[li];
+    float r = 0.0f;
+    if (id != 0.0f) {
+      const int t2 = (ly - 1) * FW2 + (lx - 1);
+      r = fb[li] - tile_apply<true>(&f2[t2], FW2, fst, FN2, t2, 0);
+    }
+    fr[t] = r;
+  }
+  __syncthreads();
+  // restriction to the M region (the single-level restriction, node by node);
+  // the M nodes under this work-group's C tile store their right-hand side
+  for (int t = threadIdx.x; t < MN1; t += 256) {
+    const int cx = mx0 + t % MW1, cy = my0 + t / MW1;
+    float s = 0.0f;
+    if (cx >= 0 && cx < M.nx && cy >= 0 && cy < M.ny) {
+      const int I = cy * M.nx + cx;
+      if (M.inv_d[I] != 0.0f) {
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int fy = 2 * cy + dy;
+          if (fy < 0 || fy >= F.ny) continue;
+          const float wy = w1d(fy, cy, M.ny, false);
+          if (wy == 0.0f) continue;
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int fx = 2 * cx + dx;
+            if (fx < 0 || fx >= F.nx) continue;
+            const float wx = w1d(fx, cx, M.nx, false);
+            if (wx == 0.0f) continue;
+            s += wx * wy * fr[(fy - (fy0 + 2)) * FW3 + (fx - (fx0 + 2))];
+          }
+        }
+      }
+      const int lx = t % MW1, ly = t / MW1;
+      if (lx >= 3 && lx < 3 + 2 * kT2 && ly >= 3 && ly < 3 + 2 * kT2) M.b[I] = s;
+    }
+    mb[t] = s;
+    m1[t] = om * mid[t] * s;
+  }
+  __syncthreads();
+  // M level, as the single-level launch for the C tile
+  for (int t = threadIdx.x; t < MN2; t += 256) {
+    const int lx = t % MW2 + 1, ly = t / MW2 + 1;
+    const int li = ly * MW1 + lx;
+    const float id = mid[li];
+    float v = 0.0f;
+    if (id != 0.0f) v = m1[li] + om * id * (mb[li] - tile_apply<true>(&m1[li], MW1, mst, MN2, t, 0));
+    m2[t] = v;
+    const int gx = mx0 + lx, gy = my0 + ly;
+    if (lx >= 3 && lx < 3 + 2 * kT2 && ly >= 3 && ly < 3 + 2 * kT2 && gx < M.nx && gy < M.ny)
+      M.x[gy * M.nx + gx] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < MN3; t += 256) {
+    const int lx = t % MW3 + 2, ly = t / MW3 + 2;
+    const int li = ly * MW1 + lx;
+    const float id = mid[li];
+    float r = 0.0f;
+    if (id != 0.0f) {
+      const int t2 = (ly - 1) * MW2 + (lx - 1);
+      r = mb[li] - tile_apply<true>(&m2[t2], MW2, mst, MN2, t2, 0);
+    }
+    mr[t] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x >= kT2 * kT2) return;
+  const int cx = CX0 + (int)(threadIdx.x % kT2), cy = CY0 + (int)(threadIdx.x / kT2);
+  if (cx >= C.nx || cy >= C.ny) return;
+  const int I = cy * C.nx + cx;
+  float s = 0.0f;
+  if (C.inv_d[I] != 0.0f) {
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int fy = 2 * cy + dy;
+      if (fy < 0 || fy >= M.ny) continue;
+      const float wy = w1d(fy, cy, C.ny, false);
+      if (wy == 0.0f) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int fx = 2 * cx + dx;
+        if (fx < 0 || fx >= M.nx) continue;
+        const float wx = w1d(fx, cx, C.nx, false);
+        if (wx == 0.0f) continue;
+        s += wx * wy * mr[(fy - (2 * CY0 - 1)) * MW3 + (fx - (2 * CX0 - 1))];
+      }
+    }
+  }
+  C.b[I] = s;
+}
+
 // -------------------------------------------------------------- up launch
 // The up to four (index, weight) terms of P e at fine node i; absent terms
 // have weight 0 and index 0.
@@ -1630,7 +1827,8 @@ static void destroy(bemcs_poisson_s* h) {
 // Per case, the set owns the PCG scalars and the stop flag; vectors and
 // partial buffers belong to the handle (a handle must not be used by two
 // sets at the same time).
-enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_COARSE, OP_UP, OP_DOTZ, OP_SUMQ };
+enum OpKind { OP_INIT, OP_INIT_SCALAR, OP_DIR, OP_STEP_PLAIN, OP_DOWN, OP_DOWN2, OP_COARSE, OP_UP, OP_DOTZ,
+              OP_SUMQ };
 
 struct Op {
   OpKind kind;
@@ -1641,6 +1839,7 @@ struct Op {
   // the single-case path passes the (only) pack by value
   CGArgs cg;
   DownPack down;
+  Down2Pack down2;
   UpPack up;
   CoarsePack coarse;
 };
@@ -1671,6 +1870,7 @@ struct bemcs_poisson_batch_s {
   std::map<int, std::vector<int>> pred;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
   int tilec;  // coarse tile edge forced on the coarser levels (kTCc or kTCs), 0: per level
+  bool fuse_down2;  // the last two coarse down launches as one
   double extrap;  // theta of the Picard initial-guess extrapolation (0: off)
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
@@ -1788,7 +1988,39 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     }
     add_cg(OP_STEP_PLAIN, plain, dim3(nparts_max, 1));
   }
+  // the last two down launches as one (every case with the same depth,
+  // non-periodic, the first of them below the level that runs the
+  // convergence test)
+  bool fuse2 = S->fuse_down2 && lmax >= 5;
+  for (int c = 0; c < n; ++c)
+    fuse2 = fuse2 && S->cases[c]->nlev == lmax && !S->cases[c]->periodic;
   for (int l = 0; l + 1 < lmax; ++l) {
+    if (fuse2 && l == lmax - 3) {
+      std::vector<Down2Pack> packs;
+      unsigned gx = 1, gy = 1;
+      for (int c = 0; c < n; ++c) {
+        bemcs_poisson_s* h = S->cases[c];
+        Down2Pack P;
+        P.F = h->lev[l];
+        P.M = h->lev[l + 1];
+        P.C = h->lev[l + 2];
+        P.done = S->cg[c].done;
+        gx = std::max(gx, cdiv(P.C.nx, kT2));
+        gy = std::max(gy, cdiv(P.C.ny, kT2));
+        packs.push_back(P);
+      }
+      Op op{};
+      op.kind = OP_DOWN2;
+      op.level = l;
+      op.grid = dim3(gx, gy, (unsigned)packs.size());
+      op.down2 = packs[0];
+      std::vector<uint8_t> bytes(sizeof(Down2Pack) * packs.size());
+      memcpy(bytes.data(), packs.data(), bytes.size());
+      ops.push_back(op);
+      uploads.push_back({bytes, &ops, ops.size() - 1});
+      ++l;  // the next level's down is part of this launch
+      continue;
+    }
     std::vector<DownPack> packs;
     unsigned gx = 1, gy = 1;
     const int tc = l == 0 ? S->tile0 : coarse_tile(S, l);
@@ -1899,6 +2131,12 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
       if (B) hipLaunchKernelGGL(k_cg_dotz<true>, op.grid, blk, 0, s, op.cg, cgm, k);
       else hipLaunchKernelGGL(k_cg_dotz<false>, op.grid, blk, 0, s, op.cg, cgm, k);
       break;
+    case OP_DOWN2: {
+      const Down2Pack* pm = (const Down2Pack*)op.packs_dev;
+      if (B) hipLaunchKernelGGL(k_down2<true>, op.grid, blk, 0, s, op.down2, pm);
+      else hipLaunchKernelGGL(k_down2<false>, op.grid, blk, 0, s, op.down2, pm);
+      break;
+    }
     case OP_DOWN: {
       const DownPack* pm = (const DownPack*)op.packs_dev;
       if (op.level == 0 && S->tile0 == kTCL) {
@@ -1988,6 +2226,8 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
   const char* ex = getenv("BEMCS_PICARD_EXTRAP");
   S->extrap = ex ? atof(ex) : kPicardExtrap;
+  const char* f2 = getenv("BEMCS_MG_FUSE");
+  S->fuse_down2 = !(f2 && f2[0] == '0');
   S->tilec = 0;
   const char* tcs = getenv("BEMCS_MG_TILEC");
   if (tcs && (atoi(tcs) == kTCc || atoi(tcs) == kTCs)) S->tilec = atoi(tcs);
