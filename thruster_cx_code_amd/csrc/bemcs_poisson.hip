@@ -1071,6 +1071,18 @@ __global__ __launch_bounds__(256) void k_cg_init_scalar(CGArgs one, const CGArgs
   init_scalars(g, rtol);
 }
 
+// The PCG scalars of every case written straight into fine-grained pinned
+// host memory by the last launch of a PCG graph, which then adds 1 to a
+// pinned counter (the host clears it before launching and polls it, which
+// wakes it sooner than a blocking stream synchronisation).
+__global__ __launch_bounds__(256) void k_status(const double* __restrict__ scal, double* host, int count,
+                                                int* flag) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) host[i] = scal[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // A batch's sums of the level-0 up launch's partial quads (r.z, r_old.z,
 // z.Az, z.Ap) of iteration k, one work-group per case, for the step launch.
 template <bool B>
@@ -1857,7 +1869,10 @@ struct bemcs_poisson_batch_s {
   std::vector<void*> allocs;
   double* scal;        // [n][S_COUNT]
   int32_t* done;       // [n]
-  double* host_scal;   // pinned copy of scal
+  double* host_scal;   // fine-grained pinned copy of scal, written by k_status
+  double* host_scal_dev;  // its device address
+  int* host_flag;      // fine-grained pinned: k_status launches finished since the host cleared it
+  int* host_flag_dev;
   bemcs::PicardPack* pic_dev;
   bemcs::PicardPack* pic_host;  // pinned staging
   hipStream_t cap_stream;
@@ -1889,6 +1904,7 @@ static void destroy_set(SolveSet* S) {
   if (S->cap_stream) (void)hipStreamDestroy(S->cap_stream);
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->host_scal) (void)hipHostFree(S->host_scal);
+  if (S->host_flag) (void)hipHostFree(S->host_flag);
   if (S->pic_host) (void)hipHostFree(S->pic_host);
   delete S;
 }
@@ -2246,7 +2262,11 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   S->pic_dev = (PicardPack*)A(sizeof(PicardPack) * n);
   if (ok) ok = hipMemset(S->scal, 0, sizeof(double) * S_COUNT * n) == hipSuccess;
   if (ok) ok = hipMemset(S->done, 0, sizeof(int32_t) * n) == hipSuccess;
-  if (ok) ok = hipHostMalloc((void**)&S->host_scal, sizeof(double) * S_COUNT * n, 0) == hipSuccess;
+  if (ok)
+    ok = hipHostMalloc((void**)&S->host_scal, sizeof(double) * S_COUNT * n, hipHostMallocCoherent) == hipSuccess;
+  if (ok) ok = hipHostGetDevicePointer((void**)&S->host_scal_dev, S->host_scal, 0) == hipSuccess;
+  if (ok) ok = hipHostMalloc((void**)&S->host_flag, 64, hipHostMallocCoherent) == hipSuccess;
+  if (ok) ok = hipHostGetDevicePointer((void**)&S->host_flag_dev, S->host_flag, 0) == hipSuccess;
   if (ok) ok = hipHostMalloc((void**)&S->pic_host, sizeof(PicardPack) * n, 0) == hipSuccess;
   if (ok) {
     for (int c = 0; c < n; ++c)
@@ -2296,9 +2316,33 @@ static hipError_t capture(SolveSet* S, F enqueue, hipGraphExec_t* out) {
   return e;
 }
 
+static void launch_status(SolveSet* S, hipStream_t s) {
+  hipLaunchKernelGGL(k_status, dim3(1), dim3(256), 0, s, (const double*)S->scal, S->host_scal_dev,
+                     S_COUNT * S->n, S->host_flag_dev);
+}
+
+// Waits until `graphs` k_status launches (the ends of the graphs enqueued on
+// `s` since the flag was cleared) have written host_scal.  Polls the pinned
+// counter, and the stream every 1024 polls, so a stream that finishes (or
+// fails) without reaching the count still ends the wait.
+static int wait_status(SolveSet* S, int graphs, hipStream_t s) {
+  for (unsigned spins = 1;; ++spins) {
+    if (__atomic_load_n(S->host_flag, __ATOMIC_ACQUIRE) >= graphs) return BEMCS_OK;
+    if ((spins & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipErrorNotReady) continue;
+      BEMCS_CHECK_HIP(q);
+      if (__atomic_load_n(S->host_flag, __ATOMIC_ACQUIRE) >= graphs) return BEMCS_OK;
+      set_error("Poisson MG-PCG: stream finished without a status");
+      return BEMCS_EHIP;
+    }
+  }
+}
+
 static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
     for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+    launch_status(S, cs);
   };
   if (!S->use_graphs) {
     body(s);
@@ -2316,6 +2360,7 @@ static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
     for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
     for (int k = 1; k <= N; ++k)
       for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+    launch_status(S, cs);
   };
   if (!S->use_graphs) {
     body(s);
@@ -2344,20 +2389,26 @@ static int pcg(SolveSet* S, int call_len, int index, double rtol, int max_iter, 
   std::vector<int>& pred = S->pred[call_len];
   if ((int)pred.size() <= index) pred.resize(index + 1, 8);
   int launched = std::min(std::max(1, pred[index]), max_iter);
+  __atomic_store_n(S->host_flag, 0, __ATOMIC_RELEASE);
   int rc = enqueue_pcg_head(S, launched, rtol, s);
+  int graphs = 1;
   if (rc) return rc;
   BEMCS_CHECK_LAUNCH();
   int batch = 0;
   const int n = S->n;
   for (;;) {
     for (int j = 0; j < batch && launched < max_iter; ++j) {
+      if (j == 0) {
+        __atomic_store_n(S->host_flag, 0, __ATOMIC_RELEASE);
+        graphs = 0;
+      }
+      ++graphs;
       rc = enqueue_iter(S, ++launched, s);
       if (rc) return rc;
     }
     BEMCS_CHECK_LAUNCH();
-    BEMCS_CHECK_HIP(hipMemcpyAsync(S->host_scal, S->scal, sizeof(double) * S_COUNT * n,
-                                   hipMemcpyDeviceToHost, s));
-    BEMCS_CHECK_HIP(hipStreamSynchronize(s));
+    rc = wait_status(S, graphs, s);
+    if (rc) return rc;
     bool all = true;
     for (int c = 0; c < n && all; ++c) {
       const double* sc = S->host_scal + S_COUNT * c;
