@@ -468,3 +468,67 @@ def test_two_level_down_launch_same_bits(lib, nx, ny, monkeypatch):
     finally:
         for h in handles:
             lib.bemcs_poisson_free(h)
+
+
+def test_picard_stage_ahead_same_bits(lib, monkeypatch):
+    """The launches that follow a Picard solve go out before the host reads
+    whether the solve converged, guarded by the stop flags.  A first-graph
+    prediction of 1 iteration (every solve of the first call needs further
+    rounds, each enqueuing the guarded launches again; in the batch the cases
+    converge in different rounds) and one of 40 (launches past convergence)
+    give the same bits and iteration counts, singly and batched."""
+    from thruster_cx_code_amd import _lib as L
+    print("[gpu] picard stage ahead", flush=True)
+    specs = [(128, 48, False, 1), (257, 97, False, 2), (130, 60, True, 3)]
+    runs = {}
+    for pred in ("40", "1"):
+        monkeypatch.setenv("BEMCS_PCG_PREDICT", pred)
+        for mode in ("single", "batch"):
+            cases, hs = [], []
+            batch = C.c_void_p()
+            try:
+                for nx, ny, per, seed in specs:
+                    o, b, cd = _picard_case(lib, nx, ny, per, seed)
+                    ib = np.ascontiguousarray(o.isBound.astype(np.uint8))
+                    h = C.c_void_p()
+                    assert lib.bemcs_poisson_create(ib.ctypes.data_as(C.c_void_p), nx, ny, int(per),
+                                                    C.byref(h)) == 0
+                    hs.append(h)
+                    cases.append((o, b, cd))
+                if mode == "batch":
+                    arr = (C.c_void_p * len(specs))(*[h.value for h in hs])
+                    assert lib.bemcs_poisson_batch_create(arr, len(specs), C.byref(batch)) == 0
+                out = []
+                for call in range(2):
+                    if mode == "single":
+                        its = []
+                        for h, (o, b, cd) in zip(hs, cases):
+                            it, rel = C.c_int32(0), C.c_double(0)
+                            p = _picard_params(o, b, cd)
+                            assert lib.bemcs_poisson_picard(h, C.byref(p), C.byref(it), C.byref(rel),
+                                                            _stream()) == 0, lib.bemcs_last_error()
+                            its.append(it.value)
+                    else:
+                        ps = (L.Picard * len(specs))(*[_picard_params(o, b, cd) for o, b, cd in cases])
+                        itc = (C.c_int32 * len(specs))()
+                        rels = (C.c_double * len(specs))()
+                        assert lib.bemcs_poisson_batch_picard(batch, ps, itc, rels, _stream()) == 0, \
+                            lib.bemcs_last_error()
+                        its = list(itc)
+                    out.append((its, [(b["V"].cpu().numpy().copy(), b["E"].cpu().numpy().copy())
+                                      for _, b, _ in cases]))
+                runs[(pred, mode)] = out
+                print(f"  predict {pred} {mode}: iterations {[o[0] for o in out]}")
+            finally:
+                if batch.value:
+                    lib.bemcs_poisson_batch_free(batch)
+                for h in hs:
+                    lib.bemcs_poisson_free(h)
+    monkeypatch.delenv("BEMCS_PCG_PREDICT")
+    ref = runs[("40", "single")]
+    for key, out in runs.items():
+        for (its_r, fields_r), (its, fields) in zip(ref, out):
+            assert its == its_r, key
+            for (Vr, Er), (V, E) in zip(fields_r, fields):
+                assert np.array_equal(V.view(np.uint64), Vr.view(np.uint64)), key
+                assert np.array_equal(E.view(np.uint32), Er.view(np.uint32)), key

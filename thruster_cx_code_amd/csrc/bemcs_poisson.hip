@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <utility>
 #include <vector>
@@ -1252,6 +1253,9 @@ struct PicardPack {
   double* bred;
   const double* x;
   double* xg;       // x of the solve before last (initial-guess extrapolation)
+  // the case's PCG stop flag: the launches between two solves run only while
+  // it is 1 (the solve before them has converged and they have not run yet)
+  const int32_t* done;
   const double* xr; // last solution, relaxed into V at the start of a sweep
   double* xw;       // x, writable (initial-guess extrapolation)
   int nx, ny, per, n;
@@ -1263,11 +1267,26 @@ struct PicardPack {
 // k_picard_start)
 enum { PIC_RHO = 0, PIC_RELAX = 2, PIC_EFIELD = 3 };
 
+// The launches between two solves of a Picard call (and after the last one)
+// are enqueued before the host has read whether the solve converged within
+// the iterations it launched.  They run only for a case whose stop flag is 1;
+// a case that still needs iterations skips them, and the host enqueues them
+// again after its further iterations.  In a batch, k_mark_started then sets
+// the flag of every case that ran them to 2, so a repeat skips those cases
+// (the PCG launches skip any nonzero flag; a solve's first V-cycle resets it).
+__device__ __forceinline__ bool stage_blocked(const int32_t* done) { return *done != 1; }
+
+__global__ void k_mark_started(int32_t* done, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n && done[c] == 1) done[c] = 2;
+}
+
 template <int PHASE, bool B>
 __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack* __restrict__ many) {
   const PicardPack P = B ? many[blockIdx.z] : one;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.n) return;
+  if (PHASE != PIC_RHO && stage_blocked(P.done)) return;
   if (PHASE == PIC_RHO) {
     // rho from NGP counts as the reference's two np.add.at deposits produce
     // it: per particle rho = f32(f64(rho) + cd), ions first, then electrons
@@ -1308,7 +1327,7 @@ __global__ __launch_bounds__(256) void k_picard_extrap(PicardPack one, const Pic
                                                        double theta) {
   const PicardPack P = B ? many[blockIdx.z] : one;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P.n || stage_blocked(P.done)) return;
   const double x1 = P.xw[i];
   if (theta != 0.0) P.xw[i] = x1 + theta * (x1 - P.xg[i]);
   P.xg[i] = x1;
@@ -1324,6 +1343,7 @@ __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const Pic
                                                       CGArgs one, const CGArgs* __restrict__ many) {
   const CGArgs g = B ? many[blockIdx.z] : one;
   if ((int)blockIdx.x >= g.nparts) return;
+  if (RELAX && stage_blocked(g.done)) return;
   const PicardPack P = B ? pmany[blockIdx.z] : pone;
   double rr = 0.0, bb = 0.0;
   GRID_LOOP(i, P.n) {
@@ -1887,6 +1907,7 @@ struct bemcs_poisson_batch_s {
   int tilec;  // coarse tile edge forced on the coarser levels (kTCc or kTCs), 0: per level
   bool fuse_down2;  // the last two coarse down launches as one
   double extrap;  // theta of the Picard initial-guess extrapolation (0: off)
+  int pred0;      // iterations launched with a solve position's first graph
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
 };
@@ -2242,6 +2263,8 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
   const char* ex = getenv("BEMCS_PICARD_EXTRAP");
   S->extrap = ex ? atof(ex) : kPicardExtrap;
+  const char* pp = getenv("BEMCS_PCG_PREDICT");
+  S->pred0 = pp && atoi(pp) >= 1 ? atoi(pp) : 8;
   const char* f2 = getenv("BEMCS_MG_FUSE");
   S->fuse_down2 = !(f2 && f2[0] == '0');
   S->tilec = 0;
@@ -2383,11 +2406,13 @@ static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
 // PCG on the reduced systems already in each case's b_red with initial guess
 // x.  Iterations advance for all cases together; converged cases skip.  The
 // initial V-cycle and the predicted number of iterations go out as one graph;
-// further iterations, if needed, one graph each.
+// further iterations, if needed, one graph each.  `next`, if given, enqueues
+// the launches that follow the solve (guarded by the stop flags) after every
+// graph, before the host waits for the graph's status.
 static int pcg(SolveSet* S, int call_len, int index, double rtol, int max_iter, int* iters, double* relres,
-               hipStream_t s) {
+               hipStream_t s, const std::function<void()>& next = nullptr) {
   std::vector<int>& pred = S->pred[call_len];
-  if ((int)pred.size() <= index) pred.resize(index + 1, 8);
+  if ((int)pred.size() <= index) pred.resize(index + 1, S->pred0);
   int launched = std::min(std::max(1, pred[index]), max_iter);
   __atomic_store_n(S->host_flag, 0, __ATOMIC_RELEASE);
   int rc = enqueue_pcg_head(S, launched, rtol, s);
@@ -2406,6 +2431,7 @@ static int pcg(SolveSet* S, int call_len, int index, double rtol, int max_iter, 
       rc = enqueue_iter(S, ++launched, s);
       if (rc) return rc;
     }
+    if (next) next();
     BEMCS_CHECK_LAUNCH();
     rc = wait_status(S, graphs, s);
     if (rc) return rc;
@@ -2536,6 +2562,7 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
     BEMCS_REQUIRE(p->iterations == iterations && p->rtol == rtol && p->max_iter == max_iter,
                   "bemcs_poisson_batch_picard: iterations, rtol and max_iter must agree across cases");
     S->pic_host[c] = picard_pack(S->cases[c], p, S->extrap != 0.0);
+    S->pic_host[c].done = S->done + c;
     gx = std::max(gx, grid_for(S->pic_host[c].n, 256));
   }
   if (n > 1)
@@ -2553,25 +2580,28 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
   std::vector<int> it(n, 0), total(n, 0);
   std::vector<double> rel(n, 0.0), worst(n, 0.0);
   int rc = BEMCS_OK;
+  // the right-hand side + r0 of the first sweep
+  launch_picard_start<false>(S, s);
+  BEMCS_CHECK_LAUNCH();
   for (int k = 0; k < iterations && rc == BEMCS_OK; ++k) {
-    if (k >= 1 && S->extrap != 0.0) launch_picard_extrap(S, gx, k >= 2 ? S->extrap : 0.0, s);
-    // relaxation of the previous iteration + this right-hand side + r0
-    if (k == 0) launch_picard_start<false>(S, s);
-    else launch_picard_start<true>(S, s);
-    BEMCS_CHECK_LAUNCH();
-    rc = pcg(S, iterations, k, rtol, max_iter, it.data(), rel.data(), s);
+    // what follows solve k, enqueued before its status is read: the initial
+    // guess and start of sweep k+1 (relaxation of sweep k, right-hand side,
+    // r0), or after the last solve the relaxation and E = -grad V
+    auto next = [&]() {
+      if (k + 1 < iterations) {
+        if (S->extrap != 0.0) launch_picard_extrap(S, gx, k + 1 >= 2 ? S->extrap : 0.0, s);
+        launch_picard_start<true>(S, s);
+      } else {
+        launch_picard<PIC_RELAX>(S, gx, s);
+        launch_picard<PIC_EFIELD>(S, gx, s);
+      }
+      if (n > 1) hipLaunchKernelGGL(k_mark_started, dim3(cdiv(n, 256)), dim3(256), 0, s, S->done, n);
+    };
+    rc = pcg(S, iterations, k, rtol, max_iter, it.data(), rel.data(), s, next);
     for (int c = 0; c < n; ++c) {
       total[c] += it[c];
       worst[c] = std::max(worst[c], rel[c]);
     }
-  }
-  if (rc == BEMCS_OK) {
-    launch_picard<PIC_RELAX>(S, gx, s);
-    BEMCS_CHECK_LAUNCH();
-  }
-  if (rc == BEMCS_OK) {
-    launch_picard<PIC_EFIELD>(S, gx, s);
-    BEMCS_CHECK_LAUNCH();
   }
   for (int c = 0; c < n; ++c) {
     if (iters_total) iters_total[c] = total[c];
