@@ -1886,6 +1886,9 @@ struct bemcs_poisson_batch_s {
   std::vector<bemcs::CGArgs> cg;  // per case
   void* cg_dev;                   // the same on the device
   std::vector<bemcs::Op> init_ops, iter_ops;
+  // iter_ops[0, iter_split): the step and every case's convergence test;
+  // [iter_split, end): the rest of the V-cycle, which a converged case skips
+  size_t iter_split;
   std::vector<void*> allocs;
   double* scal;        // [n][S_COUNT]
   int32_t* done;       // [n]
@@ -2302,6 +2305,15 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
     std::vector<PackUpload> uploads;
     build_ops(S, false, S->init_ops, uploads);
     build_ops(S, true, S->iter_ops, uploads);
+    // cases with three or more levels test convergence in the level-1 down
+    // launch, cases with one or two in the coarsest solve
+    bool coarse_test = false;
+    for (int c = 0; c < n; ++c) coarse_test = coarse_test || hs[c]->nlev <= 2;
+    S->iter_split = S->iter_ops.size();
+    for (size_t j = 0; j < S->iter_ops.size(); ++j) {
+      const Op& op = S->iter_ops[j];
+      if (coarse_test ? op.kind == OP_COARSE : (op.kind == OP_DOWN && op.level == 1)) S->iter_split = j + 1;
+    }
     S->cg_dev = A(sizeof(CGArgs) * n);
     if (ok) ok = upload(S->cg_dev, S->cg.data(), sizeof(CGArgs) * n) == 0;
     for (auto& u : uploads) {
@@ -2362,9 +2374,20 @@ static int wait_status(SolveSet* S, int graphs, hipStream_t s) {
   }
 }
 
+// Launches of iteration k's op range [b, e).
+static void launch_iter_ops(SolveSet* S, int k, size_t b, size_t e, hipStream_t s) {
+  for (size_t j = b; j < e; ++j) launch_op(S, S->iter_ops[j], k, 0.0, s);
+}
+
+// A graph ends after the launch that runs the convergence test of its last
+// iteration (nothing after the test is needed if the solve has converged);
+// the next graph, if the solve needs one, first completes that V-cycle.
+// Iteration k >= 2 as one graph: the rest of iteration k-1, then iteration k
+// up to its test.
 static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
-    for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+    launch_iter_ops(S, k - 1, S->iter_split, S->iter_ops.size(), cs);
+    launch_iter_ops(S, k, 0, S->iter_split, cs);
     launch_status(S, cs);
   };
   if (!S->use_graphs) {
@@ -2377,12 +2400,12 @@ static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   return BEMCS_OK;
 }
 
-// The initial V-cycle and iterations 1..N as one graph (one host launch).
+// The initial V-cycle and iterations 1..N (the last up to its test) as one
+// graph (one host launch).
 static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
     for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
-    for (int k = 1; k <= N; ++k)
-      for (const Op& op : S->iter_ops) launch_op(S, op, k, 0.0, cs);
+    for (int k = 1; k <= N; ++k) launch_iter_ops(S, k, 0, k < N ? S->iter_ops.size() : S->iter_split, cs);
     launch_status(S, cs);
   };
   if (!S->use_graphs) {
