@@ -532,3 +532,19 @@ def test_picard_stage_ahead_same_bits(lib, monkeypatch):
             for (Vr, Er), (V, E) in zip(fields_r, fields):
                 assert np.array_equal(V.view(np.uint64), Vr.view(np.uint64)), key
                 assert np.array_equal(E.view(np.uint32), Er.view(np.uint32)), key
+
+
+def test_stream_pointer_follows_current_stream(lib):
+    """stream_ptr() returns the raw handle of torch's current stream, also
+    inside a `torch.cuda.stream` context and for an explicit stream."""
+    import torch
+    from thruster_cx_code_amd.device import stream_ptr
+
+    def raw(p):
+        return p.value or 0          # the default stream's handle is 0 (NULL)
+    assert raw(stream_ptr()) == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert raw(stream_ptr()) == s.cuda_stream != 0
+    assert raw(stream_ptr(s)) == s.cuda_stream
+    assert raw(stream_ptr()) == torch.cuda.current_stream().cuda_stream

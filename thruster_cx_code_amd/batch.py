@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import stream_ptr
+from .device import EventRing, stream_ptr
 from .physics_engine import RING, DigitalTwinSimulator, check
 
 
@@ -84,6 +84,7 @@ class BatchedSimulator:
         self.args = None
         self.poisson_iter_log = []
         self.kernel_timing = None
+        self._events = None  # count-readback events, made at the first step
         for s, p in zip(sims, params):
             self.add(s, p)
 
@@ -329,17 +330,17 @@ class BatchedSimulator:
             check(lib.bemcs_sort_by_cell_batched(sx[1], len(sort_e), post_be, self.max_nodes, s), "sort_batched")
             for c in sort_e:
                 sims[c].elec.reverse()
-        # asynchronous count readback for the host bounds (one event for all cases)
-        ev = torch.cuda.Event()
-        cnts = []
+        # the counts reach each case's pinned ring asynchronously; one event
+        # for all cases tells the host bounds when
+        if self._events is None:
+            self._events = EventRing()
         for sim in sims:
-            sl = sim.iteration % RING
-            h = torch.empty(2, dtype=torch.int32, pin_memory=True)
-            h.copy_(sim.ring_counts[2 * sl:2 * sl + 2], non_blocking=True)
-            cnts.append(h)
+            if len(sim._pending) >= len(self._events) - 2:
+                sim._refresh_bounds(block=True)
+        ev = self._events.next()
         ev.record()
-        for sim, h in zip(sims, cnts):
-            sim._pending.append((ev, h, sim._bi.snapshot(), sim._be.snapshot(), sim.iteration))
+        for sim in sims:
+            sim._pending.append((ev, sim.iteration % RING, sim._bi.snapshot(), sim._be.snapshot(), sim.iteration))
 
     # ------------------------------------------------------ diagnostics
     def records(self):

@@ -20,9 +20,36 @@ def ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+# torch's direct bindings for the current device and its current raw stream
+# (what torch's own generated code calls); torch.cuda.current_stream() costs
+# several microseconds of Python per call, and a step makes a dozen calls
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_cur_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream_ptr(stream=None):
+    """hipStream_t of `stream`, or of the current torch stream."""
+    if stream is None and _raw_stream is not None and _cur_device is not None:
+        return C.c_void_p(_raw_stream(_cur_device()))
     s = stream if stream is not None else torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
+
+
+class EventRing:
+    """Reusable events for asynchronous readbacks, handed out in turn; a
+    caller keeps fewer than `len(events) - 1` of them pending."""
+
+    def __init__(self, n=64):
+        self.events = [torch.cuda.Event() for _ in range(n)]
+        self.i = 0
+
+    def __len__(self):
+        return len(self.events)
+
+    def next(self):
+        ev = self.events[self.i]
+        self.i = (self.i + 1) % len(self.events)
+        return ev
 
 
 def empty(n, dtype, device):

@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .device import SpeciesBuffers, CountBound, ptr, stream_ptr, require_gpu, zeros
+from .device import SpeciesBuffers, CountBound, EventRing, ptr, stream_ptr, require_gpu, zeros
 
 _NP_FP = np.float32
 
@@ -316,7 +316,12 @@ class DigitalTwinSimulator:
             self.ring_theta = zeros(RING, torch.float64, dev)
             self.ring_minpot = zeros(RING, torch.float64, dev)
             self.ring_ppc = zeros(2 * RING, torch.int32, dev)
-            self.ring_counts = zeros(2 * RING, torch.int32, dev)
+            # particle counts after each step, written by k_step_end straight
+            # into pinned host memory (read once an event or a
+            # synchronisation has passed the step)
+            self.ring_counts = torch.zeros(2 * RING, dtype=torch.int32, pin_memory=True)
+            self._ring_counts_np = self.ring_counts.numpy()
+            self._events = EventRing()
             self.ring_nsub = zeros(2 * RING, torch.int32, dev)
             self.ring_nin = zeros(2 * RING, torch.int32, dev)
             self.ring_remeshed = np.zeros(RING, bool)
@@ -497,7 +502,10 @@ class DigitalTwinSimulator:
         p = _lib.Inject()
         p.is_bound = ptr(self.is_bound_d)
         p.cnt = ptr(cnt)
-        p.dx32, p.dy32 = float(np.float32(self.dx)), float(np.float32(self.dy))
+        key = (self.dx, self.dy)
+        if getattr(self, "_dxy32", (None,))[0] != key:
+            self._dxy32 = (key, float(np.float32(self.dx)), float(np.float32(self.dy)))
+        p.dx32, p.dy32 = self._dxy32[1], self._dxy32[2]
         p.nx, p.ny = self.nx, self.ny
         p.kind = kind
         p.seed = self.seed
@@ -567,11 +575,15 @@ class DigitalTwinSimulator:
                     sr = grids[0]['r']
                     if self.geometry == 'two_holes':
                         pitch = params.get('pitch_mm', 0.0)
-                        lim = np.array([[max(self.dy, 0.5 * sr), min(2.5 * sr, self.Ly - self.dy)],
-                                        [max(self.dy, 0.5 * sr + pitch), min(2.5 * sr + pitch, self.Ly - self.dy)]],
-                                       dtype=_NP_FP)
-                        p.y_lo[0], p.y_hi[0] = float(lim[0, 0]), float(lim[0, 1])
-                        p.y_lo[1], p.y_hi[1] = float(lim[1, 0]), float(lim[1, 1])
+                        key = (sr, pitch, self.dy, self.Ly)
+                        if getattr(self, "_band_lim", (None,))[0] != key:
+                            lim = np.array([[max(self.dy, 0.5 * sr), min(2.5 * sr, self.Ly - self.dy)],
+                                            [max(self.dy, 0.5 * sr + pitch),
+                                             min(2.5 * sr + pitch, self.Ly - self.dy)]], dtype=_NP_FP)
+                            self._band_lim = (key, [float(v) for v in lim.ravel()])
+                        lim = self._band_lim[1]
+                        p.y_lo[0], p.y_hi[0] = lim[0], lim[1]
+                        p.y_lo[1], p.y_hi[1] = lim[2], lim[3]
                         p.two_band, p.band_f32 = 1, 1
                     elif self.geometry == 'one_hole':
                         p.y_lo[0], p.y_hi[0] = max(self.dy, 0.5 * sr), min(2.5 * sr, self.Ly - self.dy)
@@ -627,14 +639,22 @@ class DigitalTwinSimulator:
         """Tighten the host bounds with particle counts already copied back:
         the count after step k plus every growth applied since step k bounds
         the current count (steps only remove particles otherwise)."""
+        cnts = self._ring_counts_np
         while self._pending:
-            ev, cnts, gi, ge, step = self._pending[0]
+            ev, slot, gi, ge, step = self._pending[0]
             if not block and not ev.query():
                 break
             ev.synchronize()
             self._pending.pop(0)
-            self._bi.refresh(int(cnts[0]), gi)
-            self._be.refresh(int(cnts[1]), ge)
+            self._bi.refresh(int(cnts[2 * slot]), gi)
+            self._be.refresh(int(cnts[2 * slot + 1]), ge)
+
+    def _readback_event(self):
+        """An event for this step's count readback (waits for old readbacks
+        when too many are pending to reuse one)."""
+        if len(self._pending) >= len(self._events) - 2:
+            self._refresh_bounds(block=True)
+        return self._events.next()
 
     def _one_step(self, params):
         self._resolve_surface()
@@ -744,12 +764,11 @@ class DigitalTwinSimulator:
         self._step_end(slot)
         self._sort(ions=self.sort_interval > 0 and it % self.sort_interval == 0,
                    electrons=self.electron_sort_interval > 0 and it % self.electron_sort_interval == 0)
-        # asynchronous count readback for the host bounds
-        cnts = torch.empty(2, dtype=torch.int32, pin_memory=True)
-        cnts.copy_(self.ring_counts[2 * slot:2 * slot + 2], non_blocking=True)
-        ev = torch.cuda.Event()
+        # the step's counts reach the pinned ring asynchronously; the event
+        # tells the host bounds when
+        ev = self._readback_event()
         ev.record()
-        self._pending.append((ev, cnts, self._bi.snapshot(), self._be.snapshot(), it))
+        self._pending.append((ev, slot, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
 
     def _deposit_every_step(self, params, thermal):
@@ -1032,7 +1051,7 @@ class DigitalTwinSimulator:
             rec['exit_vx_std'] = math.sqrt(max(d.sum_vx2 / nc - (d.sum_vx / nc) ** 2, 0.0))
             rec['exit_v_std'] = math.sqrt(max(d.sum_v2 / nc - (d.sum_v / nc) ** 2, 0.0))
             rec['exit_energy_mean_eV'] = d.sum_E / nc
-        counts = self.ring_counts[2 * slot:2 * slot + 2].cpu().numpy()
+        counts = self._ring_counts_np[2 * slot:2 * slot + 2]
         rec['num_p'], rec['num_e'] = int(counts[0]), int(counts[1])
         ppc = self.ring_ppc[2 * slot:2 * slot + 2].cpu().numpy()
         rec['total_active_cells'], rec['low_ppc_cells'] = int(ppc[0]), int(ppc[1])
