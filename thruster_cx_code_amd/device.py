@@ -75,6 +75,13 @@ class SpeciesBuffers:
         self.tag = empty(cap, torch.int32, device) if with_tag else None
         self.count = zeros(1, torch.int32, device)
         self._struct = None
+        self._ptrs = None
+
+    def ptrs(self):
+        """(x, y, vx, vy, vz, count) raw pointers (cached until `grow`)."""
+        if self._ptrs is None:
+            self._ptrs = (ptr(self.x), ptr(self.y), ptr(self.vx), ptr(self.vy), ptr(self.vz), ptr(self.count))
+        return self._ptrs
 
     def struct(self):
         """The bemcs_species_t of these buffers (cached until `grow`; callers
@@ -103,6 +110,7 @@ class SpeciesBuffers:
             setattr(self, name, new)
         self.cap = new_cap
         self._struct = None
+        self._ptrs = None
 
 
 class CountBound:

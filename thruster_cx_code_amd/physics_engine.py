@@ -679,16 +679,13 @@ class DigitalTwinSimulator:
             ev = self._timed_begin(self.kernel_timing)
             self._deposit_cic()
             self._timed_end(self.kernel_timing, "deposit_cic", ev)
-        if it % 2 == 0:
-            kt = self.kernel_timing
-            ev = self._timed_begin(kt)
-            self.recalc_poisson(5, params)
-            self._timed_end(kt, "poisson_picard", ev)
         I0, I1 = self.ions
         E0, E1 = self.elec
         dsb = self.ds_bits.data_ptr()
         nsb = self.n_sub_d.data_ptr()
-        # clear this step's outputs and record the pushed populations
+        # clear this step's outputs and record the pushed populations (before
+        # the Poisson call, which reads none of them, so the push follows the
+        # solve with one host step less)
         slot = it % RING
         self.ring_remeshed[slot] = False
         dsz = C.sizeof(_lib.IonDiag)
@@ -705,15 +702,15 @@ class DigitalTwinSimulator:
         sb.nin_out = self.ring_nin.data_ptr() + 8 * slot
         sb.count_out_x = self.see_stage.count.data_ptr() if erosion else None
         check(lib.bemcs_step_begin(C.byref(sb), s), "step_begin")
+        if it % 2 == 0:
+            kt = self.kernel_timing
+            ev = self._timed_begin(kt)
+            self.recalc_poisson(5, params)
+            self._timed_end(kt, "poisson_picard", ev)
         # substep counts for both species (:1177-1209)
-        check(lib.bemcs_substeps(ptr(I0.x), ptr(I0.y), ptr(I0.vx), ptr(I0.vy), ptr(I0.vz), ptr(I0.count),
-                                 self.bound_i, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
-                                 self.q_ion / self.m_ion, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb), C.c_void_p(nsb), s),
-              "substeps(ions)")
-        check(lib.bemcs_substeps(ptr(E0.x), ptr(E0.y), ptr(E0.vx), ptr(E0.vy), ptr(E0.vz), ptr(E0.count),
-                                 self.bound_e, ptr(self.Exy), self.nx, self.ny, self.dx, self.dy,
-                                 -self.q / self.m_e, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb + 8), C.c_void_p(nsb + 4), s),
-              "substeps(electrons)")
+        ai, ae = self._substep_args()
+        check(lib.bemcs_substeps(*I0.ptrs(), self.bound_i, *ai, s), "substeps(ions)")
+        check(lib.bemcs_substeps(*E0.ptrs(), self.bound_e, *ae, s), "substeps(electrons)")
         ip = self._ion_params
         ip.Exy, ip.is_bound, ip.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb)
         ip.cnt_next, ip.ppc = ptr(self.cnt_i_next), ptr(self.ppc_d)
@@ -800,7 +797,6 @@ class DigitalTwinSimulator:
     # --------------------------------------------- electrode-surface physics
     def _surface_params(self, ip, it, thermal, erosion):
         """Impact heat / damage accumulators and SEE staging of the ion step."""
-        f = lambda v: float(np.float32(v))
         ip.heat_acc = None
         ip.damage_acc = None
         ip.see_defer = 0
@@ -814,9 +810,6 @@ class DigitalTwinSimulator:
             ip.damage_acc = ptr(self.damage_acc)
             ip.see_defer = 1
             ip.see_stage = self.see_stage.struct()
-        ip.half_m32, ip.w32, ip.q32 = f(0.5 * self.m_ion), f(self.macro_weight), f(self.q)
-        ip.ccell32, ip.accel32 = f(self.C_cell), f(self.thermal_accel)
-        ip.eth32, ip.ycoef32 = f(self.sputter_E_th), f(self.sputter_Y_coeff)
 
     def _ensure_see_stage(self):
         cap = self.ions[0].cap
@@ -980,6 +973,9 @@ class DigitalTwinSimulator:
         ip.Lx32, ip.Ly32 = f(self.Lx), f(self.Ly)
         ip.dt32 = f(self.dt)
         ip.nx, ip.ny, ip.periodic_y = self.nx, self.ny, int(self.periodic_y)
+        ip.half_m32, ip.w32, ip.q32 = f(0.5 * self.m_ion), f(self.macro_weight), f(self.q)
+        ip.ccell32, ip.accel32 = f(self.C_cell), f(self.thermal_accel)
+        ip.eth32, ip.ycoef32 = f(self.sputter_E_th), f(self.sputter_Y_coeff)
         ip.neutral_model = 1 if params.get('neutral_model', 'roy') == 'uniform' else 0
         ip.cx_model = 1 if params.get('cx_model', 'reference') == 'loglinear' else 0
         ip.mcc_enabled = 1
@@ -1000,6 +996,19 @@ class DigitalTwinSimulator:
         ep.Lx32, ep.Ly32 = f(self.Lx), f(self.Ly)
         ep.nx, ep.ny, ep.periodic_y = self.nx, self.ny, int(self.periodic_y)
         self._elec_params = ep
+
+    def _substep_args(self):
+        """bemcs_substeps arguments after the particle pointers and bound, per
+        species (rebuilt when a buffer or a constant they hold changes)."""
+        dsb, nsb = self.ds_bits.data_ptr(), self.n_sub_d.data_ptr()
+        key = (self.Exy.data_ptr(), dsb, nsb, self.nx, self.ny, self.dx, self.dy, self.dt, self.m_e, self.m_ion)
+        if getattr(self, "_substep_cache", (None,))[0] != key:
+            common = (ptr(self.Exy), self.nx, self.ny, self.dx, self.dy)
+            self._substep_cache = (key, (
+                common + (self.q_ion / self.m_ion, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb), C.c_void_p(nsb)),
+                common + (-self.q / self.m_e, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb + 8),
+                          C.c_void_p(nsb + 4))))
+        return self._substep_cache[1]
 
     def advance(self, params, n_steps):
         """Run n_steps time steps without synchronising the host."""
