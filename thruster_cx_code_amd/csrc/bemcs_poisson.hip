@@ -853,6 +853,23 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   __shared__ float sz[FUSED ? TZ * TZ : 1];
   const float om = IS9 ? kOmegaC : kOmega0;
   const int ox = tx * TF - H, oy = ty * TF - H;
+  // fused: the owned nodes' r_k, r_{k-1}, Ap_{k-1} and operator code, used
+  // after the sweeps, are loaded now so their latency overlaps the sweeps
+  constexpr int NOWN = FUSED ? (TF * TF + 255) / 256 : 1;
+  double o_r[NOWN], o_ro[NOWN], o_ap[NOWN];
+  int o_code[NOWN];
+  if (FUSED) {
+#pragma unroll
+    for (int it = 0; it < NOWN; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, TF * TF - 1);
+      const int gx = ox + H + t % TF, gy = oy + H + t / TF;
+      const int gc = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
+      o_r[it] = pick2(g.R, k & 1)[gc];
+      o_ro[it] = pick2(g.R, (k + 1) & 1)[gc];
+      o_ap[it] = k > 0 ? pick2(g.AP, k & 1)[gc] : 0.0;
+      o_code[it] = F.code[gc];
+    }
+  }
   {
     int vg[IT0];
     float vx[IT0], vb[IT0], vid[IT0], vc[IT0][4], vw[IT0][4];
@@ -957,7 +974,10 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     __syncthreads();
     // owned nodes: z, A z with the level-0 operator (neighbours as the
     // single-level direction kernel takes them), partial sums
-    for (int t = threadIdx.x; t < TF * TF; t += 256) {
+#pragma unroll
+    for (int it = 0; it < NOWN; ++it) {
+      const int t = threadIdx.x + it * 256;
+      if (t >= TF * TF) break;
       const int ux = t % TF + EXT, uy = t / TF + EXT;  // position in the sz region
       const int gx = ox + SWEEPS + ux, gy = oy + SWEEPS + uy;
       if (gx >= F.nx || gy >= F.ny) continue;
@@ -965,7 +985,7 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       const float v = sz[uy * TZ + ux];
       F.t[gi] = v;
       double az = 0.0;
-      const int d = F.code[gi];
+      const int d = o_code[it];
       if (d) {
         const float zs = (gy > 0 || F.periodic) ? sz[(uy - 1) * TZ + ux] : v;
         const float zn = (gy < F.ny - 1 || F.periodic) ? sz[(uy + 1) * TZ + ux] : v;
@@ -974,10 +994,10 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
         az = (double)d * (double)v - nb;
       }
       g.Az[gi] = az;
-      rz += pick2(g.R, k & 1)[gi] * (double)v;
-      roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
+      rz += o_r[it] * (double)v;
+      roz += o_ro[it] * (double)v;
       zaz += (double)v * az;
-      if (k > 0) zap += (double)v * pick2(g.AP, k & 1)[gi];
+      if (k > 0) zap += (double)v * o_ap[it];
     }
   }
   if (FUSED) block_partials4(rz, roz, zaz, zap, pick2(g.pz, k & 1), ty * tiles_x + tx);
