@@ -1030,14 +1030,30 @@ __global__ __launch_bounds__(64 * kCoarseRowsPerBlock) void k_coarse_solve(Coars
   if (row >= nc) return;
   const float* __restrict__ col = P.cinv + (size_t)row * nc;
   const float* __restrict__ b = C.b;
-  float s0 = 0.0f, s1 = 0.0f;
-  int j = lane;
-  for (; j + 64 < nc; j += 128) {
-    s0 += col[j] * b[j];
-    s1 += col[j + 64] * b[j + 64];
+  // eight accumulators per lane over chunks of 8 x 64 columns: the 16 loads
+  // of a chunk are in flight together (the row is latency bound, not
+  // bandwidth bound: it comes from the XCD's L2)
+  constexpr int U = 8;
+  float sa[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) sa[q] = 0.0f;
+  int j0 = 0;
+  for (; j0 + U * 64 <= nc; j0 += U * 64) {
+    float cv[U], bv[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      cv[q] = col[j0 + q * 64 + lane];
+      bv[q] = b[j0 + q * 64 + lane];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) sa[q] += cv[q] * bv[q];
   }
-  if (j < nc) s0 += col[j] * b[j];
-  float s = s0 + s1;
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int j = j0 + q * 64 + lane;
+    if (j < nc) sa[q] += col[j] * b[j];
+  }
+  float s = ((sa[0] + sa[1]) + (sa[2] + sa[3])) + ((sa[4] + sa[5]) + (sa[6] + sa[7]));
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if (lane == 0) C.x[row] = s;
 }
