@@ -285,14 +285,13 @@ struct CheckArgs {
   int32_t* done;
 };
 
-// Convergence test of iteration k, by the first work-group of the case (all
-// NT threads take part): it records r.r and sets the stop flag, and the
-// launches that follow exit.  The other work-groups of this launch carry on;
-// after the last iteration their output is scratch that nothing reads, and
-// they save every work-group adding up all partial sums.
+// Convergence test of iteration k by one work-group (all NT threads take
+// part): it records r.r and sets the stop flag, and the launches that follow
+// exit.  The other work-groups of the launch carry on; after the last
+// iteration their output is scratch that nothing reads, and they save every
+// work-group adding up all partial sums.
 template <int NT = 256>
-__device__ __forceinline__ bool check_converged(const CheckArgs& ca, int k) {
-  if (blockIdx.x != 0 || blockIdx.y != 0) return false;
+__device__ __forceinline__ bool test_converged(const CheckArgs& ca, int k) {
   double rr, unused;
   block_sum2<NT>(ca.pr, ca.nparts, rr, unused);
   const bool stop = !(rr > ca.scal[S_TOL2]);  // converged, or NaN
@@ -302,6 +301,14 @@ __device__ __forceinline__ bool check_converged(const CheckArgs& ca, int k) {
     if (stop) *ca.done = 1;
   }
   return stop;
+}
+
+// The same by the first work-group of a launch (the coarsest solve of one-
+// and two-level meshes); false for the other work-groups.
+template <int NT = 256>
+__device__ __forceinline__ bool check_converged(const CheckArgs& ca, int k) {
+  if (blockIdx.x != 0 || blockIdx.y != 0) return false;
+  return test_converged<NT>(ca, k);
 }
 
 // Arguments of one case for the V-cycle launches.  A launch serves one case
@@ -386,18 +393,33 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const LevelDev& C = P.C;
   const CGArgs& g = P.g;
   const int tiles_x = (C.nx + TC - 1) / TC;
+  const int tiles_y = (C.ny + TC - 1) / TC;
   int tx, ty;
   xcd_tile(tx, ty);
-  if (tx >= tiles_x || ty * TC >= C.ny) return;
+  // The launches that compute a global scalar (the initial tolerance and stop
+  // flag in the level-0 launch of the initial V-cycle, the convergence test
+  // in the first coarse down launch of an iteration) have a grid one tile
+  // row taller: the work-group at (0, tiles_y), which has no tile, computes
+  // it, while the tile work-groups run beside it.  The other work-groups
+  // carry on (after convergence their output is scratch nobody reads).
+  const bool scalar_block = tx == 0 && ty == tiles_y;
   if (!IS9 && !STEP) {
-    // level-0 launch of the initial V-cycle: its first work-group sets the
-    // tolerance and the initial stop flag; the others carry on (if the
-    // initial guess has converged, their output is scratch nobody reads)
-    if (blockIdx.x == 0 && blockIdx.y == 0) init_scalars(g, rtol);
+    if (scalar_block) {
+      init_scalars(g, rtol);
+      return;
+    }
   } else if (k > 0 && P.ca.check) {
-    if (block_done(P.ca.done) || check_converged(P.ca, k)) return;
-  } else if (*P.ca.done) {
-    return;
+    if (scalar_block) {
+      // a solve that has converged keeps its count and residual
+      if (!block_done(P.ca.done)) test_converged(P.ca, k);
+      return;
+    }
+  }
+  if (tx >= tiles_x || ty >= tiles_y) return;
+  if ((IS9 || STEP) && !(k > 0 && P.ca.check)) {
+    if (*P.ca.done) return;
+  } else if (IS9 || STEP) {
+    if (block_done(P.ca.done)) return;
   }
   constexpr int TFd = 2 * TC;                 // owned fine nodes per edge
   constexpr int W1 = TFd + 5, W2 = W1 - 2, W3 = W1 - 4;
@@ -2113,6 +2135,9 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
       packs.push_back(P);
     }
     if (packs.empty()) continue;
+    // one more tile row for the work-group that computes the launch's global
+    // scalars (initial tolerance, or the convergence test)
+    if ((!iteration && l == 0) || (iteration && l == 1)) gy += 1;
     Op op{};
     op.kind = OP_DOWN;
     op.level = l;
