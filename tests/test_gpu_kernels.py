@@ -435,8 +435,8 @@ def test_poisson_update_rejects_invalid_map(lib):
 def test_two_level_down_launch_same_bits(lib, nx, ny, monkeypatch):
     """The fused down launch of the last two coarse levels computes what two
     single-level launches compute, in the same order: solves give identical
-    bits and iteration counts with it (default) and without it
-    (BEMCS_MG_FUSE=0)."""
+    bits and iteration counts with it (BEMCS_MG_FUSE=1) and without it (the
+    default)."""
     from conftest import c1_params
     from thruster_cx_code_amd import DigitalTwinSimulator
     print(f"[gpu] two-level down {nx}x{ny}", flush=True)

@@ -1966,7 +1966,7 @@ struct bemcs_poisson_batch_s {
   std::map<int, std::vector<int>> pred;
   int tile0;  // coarse tile edge of the level-0 launches (kTC or kTCL)
   int tilec;  // coarse tile edge forced on the coarser levels (kTCc or kTCs), 0: per level
-  bool fuse_down2;  // the last two coarse down launches as one
+  bool fuse_down2;  // the last two coarse down launches as one (opt-in: slower at 1024x384)
   double extrap;  // theta of the Picard initial-guess extrapolation (0: off)
   int pred0;      // iterations launched with a solve position's first graph
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
@@ -2330,7 +2330,7 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   const char* pp = getenv("BEMCS_PCG_PREDICT");
   S->pred0 = pp && atoi(pp) >= 1 ? atoi(pp) : 8;
   const char* f2 = getenv("BEMCS_MG_FUSE");
-  S->fuse_down2 = !(f2 && f2[0] == '0');
+  S->fuse_down2 = f2 && f2[0] == '1';
   S->tilec = 0;
   const char* tcs = getenv("BEMCS_MG_TILEC");
   if (tcs && (atoi(tcs) == kTCc || atoi(tcs) == kTCs)) S->tilec = atoi(tcs);
