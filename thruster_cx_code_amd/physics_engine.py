@@ -707,10 +707,9 @@ class DigitalTwinSimulator:
             ev = self._timed_begin(kt)
             self.recalc_poisson(5, params)
             self._timed_end(kt, "poisson_picard", ev)
-        # substep counts for both species (:1177-1209)
-        ai, ae = self._substep_args()
-        check(lib.bemcs_substeps(*I0.ptrs(), self.bound_i, *ai, s), "substeps(ions)")
-        check(lib.bemcs_substeps(*E0.ptrs(), self.bound_e, *ae, s), "substeps(electrons)")
+        # substep counts for both species (:1177-1209), as a batch of two
+        check(lib.bemcs_substeps_batched(self._substep_cases(I0, E0), 2, max(self.bound_i, self.bound_e), s),
+              "substeps")
         ip = self._ion_params
         ip.Exy, ip.is_bound, ip.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb)
         ip.cnt_next, ip.ppc = ptr(self.cnt_i_next), ptr(self.ppc_d)
@@ -997,18 +996,31 @@ class DigitalTwinSimulator:
         ep.nx, ep.ny, ep.periodic_y = self.nx, self.ny, int(self.periodic_y)
         self._elec_params = ep
 
-    def _substep_args(self):
-        """bemcs_substeps arguments after the particle pointers and bound, per
-        species (rebuilt when a buffer or a constant they hold changes)."""
+    def _substep_cases(self, I0, E0):
+        """Device address of the two bemcs_substeps_case_t (ions, electrons)
+        of the current particle buffers.  The array is uploaded once per buffer
+        configuration (the ping-pong buffers give a few) and reused."""
         dsb, nsb = self.ds_bits.data_ptr(), self.n_sub_d.data_ptr()
-        key = (self.Exy.data_ptr(), dsb, nsb, self.nx, self.ny, self.dx, self.dy, self.dt, self.m_e, self.m_ion)
-        if getattr(self, "_substep_cache", (None,))[0] != key:
-            common = (ptr(self.Exy), self.nx, self.ny, self.dx, self.dy)
-            self._substep_cache = (key, (
-                common + (self.q_ion / self.m_ion, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb), C.c_void_p(nsb)),
-                common + (-self.q / self.m_e, self.dt, self.dt ** 2, 0.25, C.c_void_p(dsb + 8),
-                          C.c_void_p(nsb + 4))))
-        return self._substep_cache[1]
+        # every pointer and constant the structs hold (an address reused by
+        # the allocator then maps to the same contents)
+        key = tuple(t.data_ptr() for sp in (I0, E0) for t in (sp.x, sp.y, sp.vx, sp.vy, sp.vz, sp.count)) + (
+            self.Exy.data_ptr(), dsb, nsb, self.nx, self.ny, self.dx, self.dy, self.dt, self.q, self.q_ion,
+            self.m_e, self.m_ion)
+        cache = self.__dict__.setdefault("_substep_dev", {})
+        d = cache.get(key)
+        if d is None:
+            arr = (_lib.SubstepsCase * 2)()
+            for u, sp, qm, off in ((arr[0], I0, self.q_ion / self.m_ion, 0), (arr[1], E0, -self.q / self.m_e, 8)):
+                u.x, u.y, u.vx, u.vy, u.vz = (sp.x.data_ptr(), sp.y.data_ptr(), sp.vx.data_ptr(),
+                                              sp.vy.data_ptr(), sp.vz.data_ptr())
+                u.n_dev, u.Exy, u.nx, u.ny = sp.count.data_ptr(), self.Exy.data_ptr(), self.nx, self.ny
+                u.dx, u.dy, u.q_m, u.dt, u.dt2, u.frac = self.dx, self.dy, qm, self.dt, self.dt ** 2, 0.25
+                u.ds_max_bits, u.n_sub_out = dsb + off, nsb + off // 2
+            if len(cache) >= 16:
+                cache.clear()
+            host = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+            d = cache[key] = host.to(self.device)
+        return C.c_void_p(d.data_ptr())
 
     def advance(self, params, n_steps):
         """Run n_steps time steps without synchronising the host."""
