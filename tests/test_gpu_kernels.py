@@ -192,6 +192,16 @@ def test_percentile_bit_exact(lib):
         ad = _dev(a)
         assert lib.bemcs_percentile(_p(ad), _p(nd), n, 95.0, 5, None, _p(out), _stream()) == 0
         assert np.float32(out.item()) == np.percentile(a, 95), n
+    # heavy ties: the ranks either side of the 95th percentile fall on the
+    # same value or on neighbouring values
+    for n, levels in [(100, 3), (1000, 4), (4097, 2), (65536, 7), (20, 20)]:
+        r = np.random.default_rng(n + levels)
+        a = r.integers(0, levels, n).astype(np.float32) * np.float32(0.37)
+        nd = torch.tensor([n], dtype=torch.int32, device="cuda")
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        ad = _dev(a)
+        assert lib.bemcs_percentile(_p(ad), _p(nd), n, 95.0, 5, None, _p(out), _stream()) == 0
+        assert np.float32(out.item()) == np.percentile(a, 95), (n, levels)
     nd = torch.tensor([5], dtype=torch.int32, device="cuda")
     out = torch.zeros(1, dtype=torch.float64, device="cuda")
     ad = _dev(a)

@@ -109,8 +109,26 @@ __global__ __launch_bounds__(1024) void k_percentile(bemcs_percentile_case_t one
   if (lo > n - 1) lo = n - 1;
   uint32_t hi = lo + 1 < n ? lo + 1 : n - 1;
   const float g = vi - fl;
-  uint32_t klo = select_kth(keys, n, lo, hist);
-  uint32_t khi = select_kth(keys, n, hi, hist);
+  const uint32_t klo = select_kth(keys, n, lo, hist);
+  // rank hi = lo + 1 (or lo at the end): klo again when more than hi keys
+  // are <= klo, else the smallest key above klo; one pass instead of a
+  // second selection
+  __shared__ uint32_t s_le, s_next;
+  if (threadIdx.x == 0) {
+    s_le = 0;
+    s_next = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  uint32_t le = 0, next = 0xFFFFFFFFu;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t v = keys[i];
+    le += v <= klo;
+    if (v > klo && v < next) next = v;
+  }
+  if (le) atomicAdd(&s_le, le);
+  if (next != 0xFFFFFFFFu) atomicMin(&s_next, next);
+  __syncthreads();
+  const uint32_t khi = hi == lo || s_le > hi ? klo : s_next;
   if (threadIdx.x == 0) {
     float va = __uint_as_float(klo), vb = __uint_as_float(khi);
     float d = vb - va;
@@ -132,11 +150,22 @@ __global__ __launch_bounds__(1024) void k_ppc_stats(bemcs_ppc_case_t one,
   double* accum = c.accum;
   uint32_t* stats = c.stats;
   uint32_t occ = 0, low = 0;
-  for (int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 1024) {
-    const uint32_t c = ppc[i];
-    if (accum) accum[i] += (double)c;
-    occ += c > 0;
-    low += c > 0 && c < 3;
+  // two nodes per round, both loaded before either is used (the counts are
+  // integers and each accumulator element is its own, so order is free)
+  const int64_t stride = (int64_t)gridDim.x * 1024;
+  for (int64_t i0 = (int64_t)blockIdx.x * 1024 + threadIdx.x; i0 < n; i0 += 2 * stride) {
+    const int64_t i1 = i0 + stride;
+    const bool has1 = i1 < n;
+    const uint32_t c0 = ppc[i0];
+    const uint32_t c1 = has1 ? ppc[i1] : 0u;
+    const double a0 = accum ? accum[i0] : 0.0;
+    const double a1 = accum && has1 ? accum[i1] : 0.0;
+    if (accum) {
+      accum[i0] = a0 + (double)c0;
+      if (has1) accum[i1] = a1 + (double)c1;
+    }
+    occ += (c0 > 0) + (c1 > 0);
+    low += (c0 > 0 && c0 < 3) + (c1 > 0 && c1 < 3);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
