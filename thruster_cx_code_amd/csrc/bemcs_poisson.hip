@@ -283,7 +283,22 @@ struct CheckArgs {
   int check;  // this launch directly follows the step: it runs the convergence test
   double* scal;
   int32_t* done;
+  // non-null in the last test launch of a graph of a one-case set: the test
+  // work-group also writes the scalars to pinned host memory and counts
+  // itself in the host's flag (no separate status launch)
+  double* host;
+  int* flag;
 };
+
+// All S_COUNT scalars of the case into fine-grained pinned host memory, then
+// the flag (the caller's thread 0 may just have written some of them).
+__device__ __forceinline__ void write_status(const CheckArgs& ca) {
+  __syncthreads();
+  if (threadIdx.x < S_COUNT) ca.host[threadIdx.x] = ca.scal[threadIdx.x];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(ca.flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Convergence test of iteration k by one work-group (all NT threads take
 // part): it records r.r and sets the stop flag, and the launches that follow
@@ -412,6 +427,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
     if (scalar_block) {
       // a solve that has converged keeps its count and residual
       if (!block_done(P.ca.done)) test_converged(P.ca, k);
+      if (P.ca.host) write_status(P.ca);
       return;
     }
   }
@@ -1969,6 +1985,9 @@ struct bemcs_poisson_batch_s {
   bool fuse_down2;  // the last two coarse down launches as one (opt-in: slower at 1024x384)
   double extrap;  // theta of the Picard initial-guess extrapolation (0: off)
   int pred0;      // iterations launched with a solve position's first graph
+  // one case with three or more levels: a graph's last launch (the level-1
+  // down launch with the convergence test) writes the status itself
+  bool status_in_test;
   std::vector<hipGraphExec_t> pcg_graph;  // init + iterations 1..N in one graph, per N
   std::vector<double> pcg_rtol;
 };
@@ -2202,7 +2221,9 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
   return true;
 }
 
-static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t s) {
+// `status`: this launch ends a graph of a set that writes its status from
+// the test work-group (set->status_in_test).
+static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t s, bool status = false) {
   const bool B = S->n > 1;
   const dim3 blk(256);
   const CGArgs* cgm = (const CGArgs*)op.packs_dev;
@@ -2240,28 +2261,33 @@ static void launch_op(SolveSet* S, const Op& op, int k, double rtol, hipStream_t
     }
     case OP_DOWN: {
       const DownPack* pm = (const DownPack*)op.packs_dev;
+      DownPack d = op.down;  // the one-case pack goes by value
+      if (status) {
+        d.ca.host = S->host_scal_dev;
+        d.ca.flag = S->host_flag_dev;
+      }
       if (op.level == 0 && S->tile0 == kTCL) {
         if (k > 0) {
-          if (B) hipLaunchKernelGGL((k_down<kTCL, false, true, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-          else hipLaunchKernelGGL((k_down<kTCL, false, true, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, true, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTCL, false, true, false>), op.grid, blk, 0, s, d, pm, k, rtol);
         } else {
-          if (B) hipLaunchKernelGGL((k_down<kTCL, false, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-          else hipLaunchKernelGGL((k_down<kTCL, false, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          if (B) hipLaunchKernelGGL((k_down<kTCL, false, false, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTCL, false, false, false>), op.grid, blk, 0, s, d, pm, k, rtol);
         }
       } else if (op.level == 0) {
         if (k > 0) {
-          if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-          else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, true, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTC, false, true, false>), op.grid, blk, 0, s, d, pm, k, rtol);
         } else {
-          if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-          else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+          if (B) hipLaunchKernelGGL((k_down<kTC, false, false, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+          else hipLaunchKernelGGL((k_down<kTC, false, false, false>), op.grid, blk, 0, s, d, pm, k, rtol);
         }
       } else if (op.tile == kTCs) {
-        if (B) hipLaunchKernelGGL((k_down<kTCs, true, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-        else hipLaunchKernelGGL((k_down<kTCs, true, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+        if (B) hipLaunchKernelGGL((k_down<kTCs, true, false, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+        else hipLaunchKernelGGL((k_down<kTCs, true, false, false>), op.grid, blk, 0, s, d, pm, k, rtol);
       } else {
-        if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, op.down, pm, k, rtol);
-        else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, op.down, pm, k, rtol);
+        if (B) hipLaunchKernelGGL((k_down<kTCc, true, false, true>), op.grid, blk, 0, s, d, pm, k, rtol);
+        else hipLaunchKernelGGL((k_down<kTCc, true, false, false>), op.grid, blk, 0, s, d, pm, k, rtol);
       }
       break;
     }
@@ -2370,6 +2396,7 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
     // launch, cases with one or two in the coarsest solve
     bool coarse_test = false;
     for (int c = 0; c < n; ++c) coarse_test = coarse_test || hs[c]->nlev <= 2;
+    S->status_in_test = n == 1 && hs[0]->nlev >= 3;
     S->iter_split = S->iter_ops.size();
     for (size_t j = 0; j < S->iter_ops.size(); ++j) {
       const Op& op = S->iter_ops[j];
@@ -2440,6 +2467,18 @@ static void launch_iter_ops(SolveSet* S, int k, size_t b, size_t e, hipStream_t 
   for (size_t j = b; j < e; ++j) launch_op(S, S->iter_ops[j], k, 0.0, s);
 }
 
+// The end of a graph: iteration k up to its convergence test, then the
+// status (from the test launch itself when the set allows it).
+static void launch_graph_tail(SolveSet* S, int k, hipStream_t s) {
+  if (S->status_in_test) {
+    launch_iter_ops(S, k, 0, S->iter_split - 1, s);
+    launch_op(S, S->iter_ops[S->iter_split - 1], k, 0.0, s, true);
+  } else {
+    launch_iter_ops(S, k, 0, S->iter_split, s);
+    launch_status(S, s);
+  }
+}
+
 // A graph ends after the launch that runs the convergence test of its last
 // iteration (nothing after the test is needed if the solve has converged);
 // the next graph, if the solve needs one, first completes that V-cycle.
@@ -2448,8 +2487,7 @@ static void launch_iter_ops(SolveSet* S, int k, size_t b, size_t e, hipStream_t 
 static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
     launch_iter_ops(S, k - 1, S->iter_split, S->iter_ops.size(), cs);
-    launch_iter_ops(S, k, 0, S->iter_split, cs);
-    launch_status(S, cs);
+    launch_graph_tail(S, k, cs);
   };
   if (!S->use_graphs) {
     body(s);
@@ -2466,8 +2504,8 @@ static int enqueue_iter(SolveSet* S, int k, hipStream_t s) {
 static int enqueue_pcg_head(SolveSet* S, int N, double rtol, hipStream_t s) {
   auto body = [&](hipStream_t cs) {
     for (const Op& op : S->init_ops) launch_op(S, op, 0, rtol, cs);
-    for (int k = 1; k <= N; ++k) launch_iter_ops(S, k, 0, k < N ? S->iter_ops.size() : S->iter_split, cs);
-    launch_status(S, cs);
+    for (int k = 1; k < N; ++k) launch_iter_ops(S, k, 0, S->iter_ops.size(), cs);
+    launch_graph_tail(S, N, cs);
   };
   if (!S->use_graphs) {
     body(s);
