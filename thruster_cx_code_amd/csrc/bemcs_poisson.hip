@@ -1337,9 +1337,25 @@ struct PicardPack {
   float* Exy;
 };
 
-// Picard phases launched on their own (the right-hand side is part of
-// k_picard_start)
-enum { PIC_RHO = 0, PIC_RELAX = 2, PIC_EFIELD = 3 };
+// Picard phases launched on their own (rho from the deposit and the
+// right-hand side are part of k_picard_start)
+enum { PIC_RELAX = 2, PIC_EFIELD = 3 };
+
+// rho at node i from NGP counts as the reference's two np.add.at deposits
+// produce it: per particle rho = f32(f64(rho) + cd), ions first, then
+// electrons; or from the bilinear weight sums, as bemcs_weights_to_rho on a
+// zero rho
+__device__ __forceinline__ float deposit_rho(const PicardPack& P, int i) {
+  float r = 0.0f;
+  if (P.w_ion) {
+    r = (float)((double)r + P.cd_ion * ((double)(long long)P.w_ion[i] * kCicUnit));
+    r = (float)((double)r + P.cd_e * ((double)(long long)P.w_e[i] * kCicUnit));
+  } else {
+    for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
+    for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
+  }
+  return r;
+}
 
 // The launches between two solves of a Picard call (and after the last one)
 // are enqueued before the host has read whether the solve converged within
@@ -1360,21 +1376,8 @@ __global__ __launch_bounds__(256) void k_picard(PicardPack one, const PicardPack
   const PicardPack P = B ? many[blockIdx.z] : one;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.n) return;
-  if (PHASE != PIC_RHO && stage_blocked(P.done)) return;
-  if (PHASE == PIC_RHO) {
-    // rho from NGP counts as the reference's two np.add.at deposits produce
-    // it: per particle rho = f32(f64(rho) + cd), ions first, then electrons
-    float r = 0.0f;
-    if (P.w_ion) {
-      // bilinear weight sums, as bemcs_weights_to_rho on a zero rho
-      r = (float)((double)r + P.cd_ion * ((double)(long long)P.w_ion[i] * kCicUnit));
-      r = (float)((double)r + P.cd_e * ((double)(long long)P.w_e[i] * kCicUnit));
-    } else {
-      for (uint32_t k = P.cnt_ion[i]; k > 0; --k) r = (float)((double)r + P.cd_ion);
-      for (uint32_t k = P.cnt_e[i]; k > 0; --k) r = (float)((double)r + P.cd_e);
-    }
-    P.rho[i] = r;
-  } else if (PHASE == PIC_RELAX) {
+  if (stage_blocked(P.done)) return;
+  if (PHASE == PIC_RELAX) {
     // Dirichlet values from V_fixed, Neumann rows mirror;
     // V = (1-omega) V + omega V_new
     const uint8_t k = P.kind[i];
@@ -1435,10 +1438,18 @@ __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const Pic
       v = P.one_m_omega * v + P.omega * vn;
       P.V[i] = v;
     }
+    // the first sweep of a call computes rho from the deposit (every node)
+    float rho_i;
+    if (RELAX) {
+      rho_i = kd == K_UNKNOWN ? P.rho[i] : 0.0f;
+    } else {
+      rho_i = deposit_rho(P, i);
+      P.rho[i] = rho_i;
+    }
     double b = 0.0;
     if (kd == K_UNKNOWN) {
       const double rho_e = P.qn0 * det_exp(((v < P.Vp ? v : P.Vp) - P.Vp) / P.Te);
-      const double rt = (double)P.rho[i] + rho_e;
+      const double rt = (double)rho_i + rho_e;
       b = -((-P.coeff) * rt);
       const int xx = i % P.nx, y = i / P.nx;
       const int nb[4] = {i - 1, i + 1, nbr_s(i, xx, y, P.nx, P.ny, P.per != 0),
@@ -2690,7 +2701,6 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
   if (n > 1)
     BEMCS_CHECK_HIP(hipMemcpyAsync(S->pic_dev, S->pic_host, sizeof(PicardPack) * n,
                                    hipMemcpyHostToDevice, s));
-  launch_picard<PIC_RHO>(S, gx, s);
   for (int c = 0; c < n; ++c) {
     bemcs_poisson_s* h = S->cases[c];
     if (h->have_guess) continue;
