@@ -1415,9 +1415,12 @@ __global__ __launch_bounds__(256) void k_picard_extrap(PicardPack one, const Pic
 // PCG residual r0 = b - A x with its partials (r0.r0, b.b).  Each node needs
 // only its own V (relaxed here) and neighbouring x and V_fixed, which no
 // thread writes, so the three steps fuse exactly.
+// save (second sweep of a call with extrapolation): relax from the solve's x
+// and save it as the solution before last, which a separate extrapolation
+// launch with theta = 0 would otherwise copy first.
 template <bool RELAX, bool B>
 __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const PicardPack* __restrict__ pmany,
-                                                      CGArgs one, const CGArgs* __restrict__ many) {
+                                                      CGArgs one, const CGArgs* __restrict__ many, int save) {
   const CGArgs g = B ? many[blockIdx.z] : one;
   if ((int)blockIdx.x >= g.nparts) return;
   if (RELAX && stage_blocked(g.done)) return;
@@ -1427,12 +1430,14 @@ __global__ __launch_bounds__(256) void k_picard_start(PicardPack pone, const Pic
     const uint8_t kd = P.kind[i];
     double v = P.V[i];
     if (RELAX) {
+      const double* xr = save ? P.x : P.xr;
+      if (save) P.xg[i] = P.x[i];
       double vn;
-      if (kd == K_UNKNOWN) vn = P.xr[i];
+      if (kd == K_UNKNOWN) vn = xr[i];
       else if (kd == K_BOUND) vn = P.V_fixed[i];
       else {
         const int m = P.mirror[i];
-        vn = P.kind[m] == K_UNKNOWN ? P.xr[m] : P.V_fixed[m];
+        vn = P.kind[m] == K_UNKNOWN ? xr[m] : P.V_fixed[m];
       }
       P.Vnew[i] = vn;
       v = P.one_m_omega * v + P.omega * vn;
@@ -2639,16 +2644,16 @@ static PicardPack picard_pack(bemcs_poisson_s* h, const bemcs_picard_t* p, bool 
 }
 
 template <bool RELAX>
-static void launch_picard_start(SolveSet* S, hipStream_t s) {
+static void launch_picard_start(SolveSet* S, hipStream_t s, int save = 0) {
   unsigned gx = 1;
   for (int c = 0; c < S->n; ++c) gx = std::max(gx, (unsigned)S->cg[c].nparts);
   const CGArgs* cgm = (const CGArgs*)S->cg_dev;
   if (S->n > 1)
     hipLaunchKernelGGL((k_picard_start<RELAX, true>), dim3(gx, 1, S->n), dim3(256), 0, s, S->pic_host[0],
-                       (const PicardPack*)S->pic_dev, S->cg[0], cgm);
+                       (const PicardPack*)S->pic_dev, S->cg[0], cgm, save);
   else
     hipLaunchKernelGGL((k_picard_start<RELAX, false>), dim3(gx, 1, 1), dim3(256), 0, s, S->pic_host[0],
-                       (const PicardPack*)S->pic_dev, S->cg[0], cgm);
+                       (const PicardPack*)S->pic_dev, S->cg[0], cgm, save);
 }
 
 // r0 = b - A x of a solve without a Picard right-hand side
@@ -2721,8 +2726,11 @@ static int picard(SolveSet* S, const bemcs_picard_t* ps, int32_t* iters_total, d
     // r0), or after the last solve the relaxation and E = -grad V
     auto next = [&]() {
       if (k + 1 < iterations) {
-        if (S->extrap != 0.0) launch_picard_extrap(S, gx, k + 1 >= 2 ? S->extrap : 0.0, s);
-        launch_picard_start<true>(S, s);
+        // the second sweep has one solution to save and none to extrapolate
+        // from: its start launch saves it
+        const bool save = S->extrap != 0.0 && k + 1 == 1;
+        if (S->extrap != 0.0 && !save) launch_picard_extrap(S, gx, S->extrap, s);
+        launch_picard_start<true>(S, s, save ? 1 : 0);
       } else {
         launch_picard<PIC_RELAX>(S, gx, s);
         launch_picard<PIC_EFIELD>(S, gx, s);
