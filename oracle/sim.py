@@ -27,9 +27,18 @@ the reference keeps them; electrons in x <= Lx); ``seed``;
 ``discharge_chamber.pitch_mm`` is honoured as the GUI does
 (Python/main.py:1689).
 
-Parity status: the composition under the reference's own MT19937 stream is
-"parity unpinned" (the reference cannot be imported here: it imports Taichi,
-which the image lacks).  Each component is pinned in its own module.
+Pins: each kernel is pinned in its own module.  The composition of the step
+(injection bands, the order SEE spawn -> electron purge -> CEX -> PPC, the
+entered / crossed / hit / out-of-bounds / keep masks, the bookkeeping of both
+species, the saddle-point index, the preload placement and the electrode
+rasterisation) is pinned by tests/test_oracle_step.py: closed-form
+single-particle scenarios in a field-free domain, a scalar per-particle
+restatement of sections D, E, H and the PPC map run against this class's
+intermediate states over many steps (tests/literal_step.py), and planted
+potentials.  Mistakes planted in this file (a mask threshold, a comparison, a
+boundary, the order of the purge) fail those tests.  What stays open: the
+reference's own MT19937 draws (replaced by Philox by design; the reference
+cannot be imported here, it needs Taichi).
 """
 import math
 import numpy as np
