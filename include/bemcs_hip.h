@@ -105,6 +105,19 @@ double bemcs_det_exp(double x);
 double bemcs_det_log(double x);
 void bemcs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Replaces thermal_conduction_taichi(T, T_new, mask, nx, ny, Fo_x, Fo_y)
+ * (Python/physics_engine.py:147-171, called 10 times with T/T_new swapped at
+ * :1586-1597): one explicit conduction sweep over device float32 [ny][nx]
+ * maps with the reference NumPy loop's semantics (:1598-1613): T_new = T on
+ * every node except interior electrode nodes (mask == 1, 1 <= ix <= nx-2,
+ * 1 <= iy <= ny-2), which take max(T + dT, 300) with non-electrode
+ * neighbours replaced by the node itself, in float32 and the loop's
+ * operation order (bit-identical).  The Taichi kernel also updates electrode
+ * nodes on the outer ring; the NumPy loop, the parity target, does not.
+ * mask is int32 (isBound.astype(np.int32), as the reference passes it). */
+int bemcs_thermal_conduction(const float* T, float* T_new, const int32_t* mask, int32_t nx,
+                             int32_t ny, float fo_x, float fo_y, bemcs_stream_t stream);
+
 /* Cell-index division.  The push divides positions by float32(dx) and
  * float32(dy) (IEEE, as NumPy).  For a given divisor the library uses a
  * reciprocal-multiply-plus-remainder form instead, after checking on the

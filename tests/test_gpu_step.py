@@ -134,19 +134,78 @@ def test_steps_parity(pair):
             assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
 
 
-def test_mcc_events_happen_and_match():
-    """Dense neutrals so that CEX events occur; masks must match exactly."""
-    p = c1_params(nx=128, ny=48, ions=20000, electrons=0, neut_rate=0)
+@pytest.mark.parametrize("neutral_model,cx_model", [("uniform", "loglinear"), ("roy", "reference")])
+def test_mcc_events_happen_and_match(neutral_model, cx_model):
+    """Dense neutrals so that CEX events occur; masks must match exactly.
+    ("roy", "reference") is the reference's own default physics: the Roy
+    plume density and sigma = (-0.8821 ln g + 15.1262)^2 1e-20
+    (Python/physics_engine.py:1690-1719)."""
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=0, neut_rate=0, neutral_model=neutral_model,
+                  cx_model=cx_model)
     p["n0"] = 3e22
     g, o = _mk(p)
     _sync(o, g)
     g.step(p)
     o.step(p)
     rec = g.record()
-    print(f"[gpu] mcc events {rec['mcc_events']}")
+    print(f"[gpu] mcc events {rec['mcc_events']} ({neutral_model}, {cx_model})", flush=True)
     assert rec["mcc_events"] > 50
     assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
     _compare_ions(g, o)
+
+
+def _steps_vs_oracle(p, g, o, n_steps, check_mcc=True):
+    """n_steps steps, the oracle fed the GPU's state before each; particles
+    bit-exact on odd steps, within 1e-6 on even ones (E after a solve)."""
+    events = 0
+    for _ in range(n_steps):
+        _sync(o, g)
+        print(f"[gpu] step {g.iteration + 1}", flush=True)
+        rg = g.step(p)
+        ro = o.step(p)
+        rec = g.record()
+        assert rec["n_sub_ion"] == o.n_sub_ion and rec["n_sub_e"] == o.n_sub_e
+        even = g.iteration % 2 == 0
+        if even:
+            err = np.max(np.abs(g.V - o.V)) / np.max(np.abs(o.V))
+            assert err < 1e-9
+        _compare_ions(g, o, exact=not even)
+        _compare_electrons(g, o, exact=not even)
+        if not even:
+            assert rec["entered"] == o.entered_optics_step
+            assert rec["crossed"] == o.exit_count_step
+            assert rec["lost_to_grid"] == o.lost_to_grid_step
+            assert rec["see_spawned"] == o.see_count
+            assert rg[1] == ro[1]
+            assert np.array_equal(g.current_ppc_map, o.current_ppc_map)
+            if check_mcc:
+                assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
+            events += rec["mcc_events"]
+    return events
+
+
+def test_reference_default_physics_steps():
+    """Four steps with the reference's default neutral and cross-section
+    models (Roy plume, built-in sigma(g)) and dense neutrals."""
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=20000, neutral_model="roy", cx_model="reference")
+    p["n0"] = 3e22
+    g, o = _mk(p)
+    events = _steps_vs_oracle(p, g, o, 4)
+    print(f"  CEX events on odd steps: {events}", flush=True)
+    assert events > 50
+
+
+def test_two_holes_steps():
+    """two_holes: two injection bands, periodic wrap over one pitch."""
+    p = c1_params(nx=128, ny=120, ions=8000, electrons=8000, geometry="two_holes")
+    p["pitch_mm"] = 3.0
+    g, o = _mk(p)
+    assert np.array_equal(g.isBound, o.isBound)
+    _compare_ions(g, o)
+    _steps_vs_oracle(p, g, o, 3)
+    ys = g.p_y[g.p_pid >= 8000]
+    assert len(ys) > 50
+    assert np.any(ys < 1.5 * 0.95 + 1.0) and np.any(ys > 1.5 * 0.95 + 2.0)
 
 
 def test_periodic_geometry_step():

@@ -449,3 +449,35 @@ def test_headless_runner_on_reference_config(tmp_path, capsys):
     rows = list(csv.reader(open(out_csv, encoding="utf-8")))
     assert rows[0][:3] == ["iteration", "t_sim_s", "minpotential"] and rows[0][-1] == "grid_3_temp_K"
     assert len(rows) == 21 and rows[20][0] == "20"
+
+
+@pytest.mark.parametrize("nx,ny,seed", [(40, 24, 11), (301, 157, 12)])
+def test_thermal_conduction_drop_in_bit_exact(hip_lib, nx, ny, seed):
+    """thermal_conduction_hip with thermal_conduction_taichi's signature,
+    called 10 times with T / T_new swapped as the reference call site does
+    (:1586-1597), equals the reference NumPy loop (:1598-1613) bit for bit:
+    the literal loop on the small map, its vectorised twin on the large one."""
+    from thruster_cx_code_amd import thermal_conduction_hip
+    print(f"[gpu] thermal_conduction_hip {nx}x{ny}", flush=True)
+    T, B, _, _ = _thermal_case(nx, ny, seed)
+    T[B] = np.maximum(T[B], 300.0)
+    T[1, 1] = np.nan if nx < 100 else T[1, 1]
+    fx, fy = 0.2, 0.137
+    want = (S.conduction_loop if nx < 100 else S.conduction)(T, B, fx, fy, 10)
+    # NumPy arrays in, as the reference passes them
+    Ta, Tb = T.copy(), np.zeros_like(T)
+    mask = B.astype(np.int32)
+    for _ in range(10):
+        thermal_conduction_hip(Ta, Tb, mask, nx, ny, np.float32(fx), np.float32(fy))
+        Ta, Tb = Tb, Ta
+    assert np.array_equal(Ta.view(np.uint32), want.view(np.uint32))
+    # device tensors in (no copies), through the C ABI
+    Td, Tn = _dev(T.ravel()), torch.zeros(nx * ny, dtype=torch.float32, device="cuda")
+    md = _dev(mask.ravel())
+    for _ in range(10):
+        assert hip_lib.bemcs_thermal_conduction(_ptr(Td), _ptr(Tn), _ptr(md), nx, ny, float(F32(fx)),
+                                                float(F32(fy)), _stream()) == 0
+        Td, Tn = Tn, Td
+    torch.cuda.synchronize()
+    assert np.array_equal(Td.cpu().numpy().view(np.uint32), want.ravel().view(np.uint32))
+    assert hip_lib.bemcs_thermal_conduction(_ptr(Td), _ptr(Td), _ptr(md), nx, ny, 0.2, 0.2, _stream()) != 0

@@ -12,14 +12,15 @@ device buffers and streams only.
 KERNEL_BACKEND = "hip"
 
 __all__ = ["KERNEL_BACKEND", "DigitalTwinSimulator", "push_particles_boris_hip", "accumulate_rho_hip",
-           "accumulate_rho_cic_hip"]
+           "accumulate_rho_cic_hip", "thermal_conduction_hip"]
 
 
 def __getattr__(name):
     if name == "DigitalTwinSimulator":
         from .physics_engine import DigitalTwinSimulator
         return DigitalTwinSimulator
-    if name in ("push_particles_boris_hip", "accumulate_rho_hip", "accumulate_rho_cic_hip"):
+    if name in ("push_particles_boris_hip", "accumulate_rho_hip", "accumulate_rho_cic_hip",
+                "thermal_conduction_hip"):
         from . import kernels
         return getattr(kernels, name)
     raise AttributeError(name)

@@ -185,6 +185,7 @@ _SIGS = {
     "bemcs_thermal_workspace_bytes": (i64, [i32, i32, i32]),
     "bemcs_thermal": (i32, [vp, vp]),
     "bemcs_rf_set": (i32, [vp, vp, i64, i32, vp, i32, f64, vp]),
+    "bemcs_thermal_conduction": (i32, [vp, vp, vp, i32, i32, C.c_float, C.c_float, vp]),
     "bemcs_electrodes": (i32, [vp, vp]),
     "bemcs_grid_means": (i32, [vp, vp, i32, i32, i32, vp, vp, vp]),
     "bemcs_ion_step_batched": (i32, [vp, i32, i64, vp, vp]),

@@ -8,6 +8,8 @@
 //                 300 K floor + per-grid partial sums, one launch
 //                 (:1359-1388, :1563-1617)
 //   k_grid_means  per-grid mean temperatures from the partial sums
+//   k_conduction_sweep  one conduction sweep with the drop-in signature of
+//                 thermal_conduction_taichi (:147-171)
 //   k_rf_set      V_fixed of the RF grid = Vdc + A sin(2 pi f t) (:984-991)
 //   k_electrodes  electrode map, Dirichlet values and grid masks of the grids,
 //                 bow included (:791-843)
@@ -273,6 +275,33 @@ __global__ __launch_bounds__(kBlock) void k_grid_means(const double* __restrict_
   }
 }
 
+// One explicit conduction sweep with thermal_conduction_taichi's signature
+// (Python/physics_engine.py:147-171) and the NumPy loop's semantics (:1598-1613):
+// T_new = T everywhere except interior electrode nodes (mask == 1, not on the
+// outer ring), which take max(T + dT, 300) with insulated (non-electrode)
+// neighbours replaced by the node itself.  float32 in the loop's operation
+// order: dT = Fo_x ((l - 2c) + r) + Fo_y ((d - 2c) + u).
+__global__ __launch_bounds__(kBlock) void k_conduction_sweep(const float* __restrict__ T,
+                                                             float* __restrict__ T_new,
+                                                             const int32_t* __restrict__ mask,
+                                                             int nx, int ny, float fx, float fy) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)nx * ny) return;
+  const int ix = (int)(i % nx), iy = (int)(i / nx);
+  const float c = T[i];
+  float o = c;
+  if (mask[i] == 1 && ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2) {
+    const float l = mask[i - 1] == 1 ? T[i - 1] : c;
+    const float r = mask[i + 1] == 1 ? T[i + 1] : c;
+    const float d = mask[i - nx] == 1 ? T[i - nx] : c;
+    const float u = mask[i + nx] == 1 ? T[i + nx] : c;
+    const float dT = fx * ((l - 2.0f * c) + r) + fy * ((d - 2.0f * c) + u);
+    const float t = c + dT;
+    o = (300.0f > t) ? 300.0f : t;  // Python max(t, 300.0): NaN stays
+  }
+  T_new[i] = o;
+}
+
 // RF grid nodes: their DC potential is that of the last grid covering them
 // (grids are rasterised in order, later ones overwrite, :826-827).
 struct GridV {
@@ -429,6 +458,17 @@ int bemcs_thermal(const bemcs_thermal_t* p, bemcs_stream_t stream) {
                        reinterpret_cast<const double*>(p->workspace), nb, p->n_grids, p->T_grids);
     BEMCS_CHECK_LAUNCH();
   }
+  return BEMCS_OK;
+}
+
+int bemcs_thermal_conduction(const float* T, float* T_new, const int32_t* mask, int32_t nx, int32_t ny,
+                             float fo_x, float fo_y, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(T && T_new && mask && nx >= 1 && ny >= 1 && T != T_new,
+                "bemcs_thermal_conduction: bad arguments (T and T_new distinct, nx, ny >= 1)");
+  const int64_t n = (int64_t)nx * ny;
+  hipLaunchKernelGGL(k_conduction_sweep, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream),
+                     T, T_new, mask, (int)nx, (int)ny, fo_x, fo_y);
+  BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }
 

@@ -1,9 +1,10 @@
 """Drop-in kernel functions with the reference's Python signatures.
 
-``push_particles_boris_hip`` and ``accumulate_rho_hip`` take exactly the
-arguments of ``push_particles_boris_cpu`` / ``accumulate_rho_cpu``
-(Python/physics_engine.py:188-190, :178) and mutate their array arguments in
-place.  Arrays may be NumPy (copied to the device and back, as the Taichi path
+``push_particles_boris_hip``, ``accumulate_rho_hip`` and
+``thermal_conduction_hip`` take exactly the arguments of
+``push_particles_boris_cpu`` / ``accumulate_rho_cpu`` /
+``thermal_conduction_taichi`` (Python/physics_engine.py:188-190, :178,
+:147-156) and mutate their array arguments in place.  Arrays may be NumPy (copied to the device and back, as the Taichi path
 does at :1215-1232) or torch tensors already on the GPU (no copies).
 """
 import ctypes as C
@@ -115,3 +116,32 @@ def accumulate_rho_cic_hip(x, y, rho, num_p, dx, dy, nx, ny, charge_density):
           "weights_to_rho")
     if host is not None:
         host[...] = rt.cpu().numpy().reshape(host.shape)
+
+
+def thermal_conduction_hip(T, T_new, mask, nx, ny, Fo_x, Fo_y):
+    """One conduction sweep with thermal_conduction_taichi's signature
+    (Python/physics_engine.py:147-156): writes T_new from T.  Semantics and
+    float32 arithmetic of the reference NumPy loop (:1598-1613), which the
+    call site at :1586-1597 uses when Taichi is off: only interior electrode
+    nodes (mask == 1) change, to max(T + dT, 300), insulated towards
+    non-electrode neighbours; every other node copies T.  Bit-identical to
+    that loop.  Arrays may be NumPy (copied to the device and back) or float32
+    / int32 CUDA tensors of shape (ny, nx)."""
+    require_gpu()
+    lib = _lib.load()
+    nx, ny = int(nx), int(ny)
+    n = nx * ny
+    tin, _ = _dev_view(T, n, torch.float32)
+    if isinstance(T_new, torch.Tensor):
+        tout, host = _dev_view(T_new, n, torch.float32)[0], None
+    else:
+        host = T_new
+        tout = torch.empty(n, dtype=torch.float32, device=tin.device)
+    if isinstance(mask, torch.Tensor):
+        mt, _ = _dev_view(mask, n, torch.int32)
+    else:
+        mt = torch.from_numpy(np.ascontiguousarray(np.asarray(mask).reshape(-1)[:n], dtype=np.int32)).cuda()
+    check(lib.bemcs_thermal_conduction(ptr(tin), ptr(tout), ptr(mt), nx, ny, float(np.float32(Fo_x)),
+                                       float(np.float32(Fo_y)), stream_ptr()), "thermal_conduction")
+    if host is not None:
+        host.reshape(-1)[:n] = tout.cpu().numpy()
