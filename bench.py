@@ -44,7 +44,8 @@ def workload_params(name, ions=None, electrons=None):
         "n0": 1e18, "neutral_model": "uniform", "cx_model": "loglinear",
         "neut_rate": 30, "sim_mode": "Trajectories", "mesh": {"nx": nx, "ny": ny},
         "preload": {"ions": npre if ions is None else ions,
-                    "electrons": npre if electrons is None else electrons},
+                    "electrons": npre if electrons is None else electrons,
+                    "hold": name == "c2"},
         "seed": 0,
     }
 
@@ -89,31 +90,108 @@ def cpu_worker(workload, ions, electrons, max_seconds, max_steps):
     print(json.dumps({"value": pushed / wall, "steps": steps, "wall_s": wall, "pushed": pushed}))
 
 
-def cpu_baseline(workload, ions, electrons, max_seconds=20.0, max_steps=200):
-    env = dict(os.environ)
-    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
-        env[k] = "1"
-    env["HIP_VISIBLE_DEVICES"] = ""
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", workload, str(ions), str(electrons),
-           str(max_seconds), str(max_steps)]
-    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=max_seconds * 20 + 600)
-    if out.returncode != 0:
-        raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
-    rec = json.loads(out.stdout.strip().splitlines()[-1])
-    cpu = "unknown"
+def cpu_worker_composed(workload, n_sub_i, n_sub_e):
+    """Per-step time of the NumPy oracle (reference dtype semantics) at the
+    GPU line's own population, composed from measured pieces on one core:
+    one full odd step with every species pushed once (injection, hold, NGP
+    deposit, substep counts, diagnostics, purge, CEX, PPC), one push substep
+    per species, and one 5-iteration SuperLU Picard call (every 2nd step):
+        t_step = t_odd + t_picard / 2 + (n_sub_i - 1) t_push_i + (n_sub_e - 1) t_push_e
+    with the mean substep counts the GPU measured over its timed steps."""
+    from oracle import kernels as K
+    from oracle.sim import OracleSimulator
+
+    class OneSubstep(OracleSimulator):
+        """Runs the push of each species once per step (timing only)."""
+        def step(self, params):
+            real = K.push_boris
+            calls = {}
+
+            def once(x, *a, **kw):
+                key = id(x)
+                if key not in calls:
+                    calls[key] = True
+                    real(x, *a, **kw)
+            import oracle.sim as osim
+            osim.push_boris = once
+            try:
+                return super().step(params)
+            finally:
+                osim.push_boris = real
+
+    p = workload_params(workload)
+    o = OneSubstep(mode="ref32", mcc_math="ref32")
+    o.m_e = o.m_ion / 200.0
+    t0 = time.perf_counter()
+    o.build_domain(p)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step(p)                       # iteration 1: odd, no Poisson call
+    t_odd = time.perf_counter() - t0
+    pushed = o.pushed
+    t0 = time.perf_counter()
+    o.recalc_poisson(5, p)
+    t_picard = time.perf_counter() - t0
+    qm_i, qm_e = o.q_ion / o.m_ion, -o.q / o.m_e
+    t0 = time.perf_counter()
+    K.push_boris(o.p_x, o.p_y, o.p_vx, o.p_vy, o.p_vz, o.Ex, o.Ey, o.Bx, o.By, o.Bz, o.num_p, o.dx, o.dy,
+                 o.nx, o.ny, o.dt / max(n_sub_i, 1), qm_i)
+    t_push_i = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    K.push_boris(o.e_x, o.e_y, o.e_vx, o.e_vy, o.e_vz, o.Ex, o.Ey, o.Bx, o.By, o.Bz, o.num_e, o.dx, o.dy,
+                 o.nx, o.ny, o.dt / max(n_sub_e, 1), qm_e)
+    t_push_e = time.perf_counter() - t0
+    t_step = t_odd + 0.5 * t_picard + (n_sub_i - 1) * t_push_i + (n_sub_e - 1) * t_push_e
+    print(json.dumps({"value": pushed / t_step, "pushed": pushed, "t_step_s": t_step, "t_odd_s": t_odd,
+                      "t_picard_s": t_picard, "t_push_i_s": t_push_i, "t_push_e_s": t_push_e,
+                      "n_sub": [n_sub_i, n_sub_e], "build_s": t_build}))
+
+
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
                 if ln.startswith("model name"):
-                    cpu = ln.split(":", 1)[1].strip()
-                    break
+                    return ln.split(":", 1)[1].strip()
     except OSError:
         pass
+    return "unknown"
+
+
+def _run_cpu(args, timeout):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    env["HIP_VISIBLE_DEVICES"] = ""
+    cmd = [sys.executable, os.path.abspath(__file__)] + [str(a) for a in args]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
+    if out.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(workload, ions, electrons, max_seconds=20.0, max_steps=200):
+    rec = _run_cpu(["--cpu-worker", workload, ions, electrons, max_seconds, max_steps], max_seconds * 20 + 600)
     return {
         "value": rec["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
         "sample": (f"NumPy oracle (reference dtype semantics, SciPy SuperLU) on the same "
                    f"{workload} mesh and physics with {ions}+{electrons} preloaded particles, "
-                   f"{rec['steps']} steps in {rec['wall_s']:.1f} s on 1 core of {cpu}"),
+                   f"{rec['steps']} steps in {rec['wall_s']:.1f} s on 1 core of {_cpu_model()}"),
+    }
+
+
+def cpu_baseline_composed(workload, n_sub_i, n_sub_e):
+    rec = _run_cpu(["--cpu-composed", workload, n_sub_i, n_sub_e], 1800)
+    return {
+        "value": rec["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        "sample": (f"NumPy oracle (reference dtype semantics, SciPy SuperLU) at the GPU line's population "
+                   f"({rec['pushed']} particles pushed per step) on 1 core of {_cpu_model()}; per-step time "
+                   f"{rec['t_step_s']:.1f} s composed from measured pieces: one odd step with one push per "
+                   f"species {rec['t_odd_s']:.2f} s + half a 5-iteration Picard call "
+                   f"({rec['t_picard_s']:.2f} s) + (n_sub - 1) further push substeps per species at "
+                   f"{rec['t_push_i_s']:.2f} s (ions) and {rec['t_push_e_s']:.2f} s (electrons), with the "
+                   f"GPU's mean n_sub {n_sub_i}/{n_sub_e} over the timed steps"),
+        "breakdown": rec,
     }
 
 
@@ -268,8 +346,12 @@ def run(args):
     }
     if not args.no_cpu_baseline:
         _progress("cpu baseline")
-        ions = electrons = 100_000
-        out["cpu_baseline"] = cpu_baseline(args.workload, ions, electrons, args.cpu_seconds)
+        if args.workload == "c1":
+            out["cpu_baseline"] = cpu_baseline(args.workload, 100_000, 100_000, args.cpu_seconds)
+        else:
+            ns_i = int(round(float(np.mean(nsub[:, 0]))))
+            ns_e = int(round(float(np.mean(nsub[:, 1]))))
+            out["cpu_baseline"] = cpu_baseline_composed(args.workload, ns_i, ns_e)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
@@ -437,6 +519,9 @@ def main():
         sp.add_argument("--sweep-steps", type=int, default=3000)
         sp.add_argument("--sweep-seq-cases", type=int, default=16)
         sweep_bench(sp.parse_args())
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-composed":
+        cpu_worker_composed(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
         w, ni, ne, secs, steps = sys.argv[2:7]

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 4
+#define BEMCS_ABI_VERSION 5
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -270,7 +270,18 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
 /* Injection of `n` particles at the end of `sp` (count is advanced), adding
  * their NGP counts to cnt.  kind: 0 = beam ions (:1020-1064), 1 = neutraliser
  * electrons (:1085-1099), 2 = RF source electrons (:1066-1083), 3 = preloaded
- * ions, 4 = preloaded electrons (uniform over non-electrode nodes). */
+ * ions, 4 = preloaded electrons (uniform over non-electrode nodes).
+ *
+ * Population hold (kinds 3 and 4 with target > 0; no reference counterpart,
+ * a synthetic workload option beside the preload): the launch tops the
+ * device count up to `target`, adding need = min(max(target - count, 0), n)
+ * particles placed and drawn as the preload does, from the streams
+ * (seed, 0x12 ions / 0x13 electrons) with ids *id_next .. *id_next + need - 1
+ * (a device counter advanced by need; ion ids are 0x40000000 | id, so they
+ * never meet the host-assigned ids below 2^30).  The host needs no count.
+ *
+ * overflow (may be null): bit 8 is set when the count had to be clamped at
+ * the capacity (the host's count bound was wrong). */
 typedef struct {
   uint8_t const* is_bound;
   uint32_t* cnt;
@@ -287,6 +298,9 @@ typedef struct {
   uint32_t seed, step;
   uint32_t pid_base;         /* ions: persistent id of the first particle */
   uint32_t pad_;
+  uint32_t* overflow;        /* may be null */
+  uint32_t* id_next;         /* population hold: device id counter */
+  int64_t target;            /* population hold: count to top up to (0: off) */
 } bemcs_inject_t;
 
 int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,

@@ -24,6 +24,8 @@ from .dmath import det_exp, box_muller
 # purpose tags (the same numbers appear in the HIP header)
 P_INIT_ION = 0x10
 P_INIT_ELEC = 0x11
+P_HOLD_ION = 0x12
+P_HOLD_ELEC = 0x13
 P_INJ_COUNT = 0x20
 P_INJ_ION = 0x21
 P_NEUT = 0x22

@@ -22,7 +22,8 @@ sum once, which differs from this only by the rounding of the per-node adds.
 Additive options (absent keys give the reference's behaviour):
 ``mesh = {"nx", "ny"}`` fixes the node count; ``preload = {"ions", "electrons"}``
 seeds particles uniformly over non-electrode nodes (ions in x <= x_plume, where
-the reference keeps them; electrons in x <= Lx); ``seed``;
+the reference keeps them; electrons in x <= Lx); ``seed``; ``preload.hold`` tops both species up to their preloaded counts
+after each step's injections (population hold);
 ``neutral_model in {"roy", "uniform"}``; ``cx_model in {"reference", "loglinear"}``;
 ``discharge_chamber.pitch_mm`` is honoured as the GUI does
 (Python/main.py:1689).
@@ -53,6 +54,7 @@ from . import surface
 from . import spline
 
 F32 = np.float32
+HOLD_ID_BIT = 0x40000000
 
 
 class OracleSimulator:
@@ -272,48 +274,77 @@ class OracleSimulator:
         self.poisson = SuperLUPoisson(self.isBound, self.periodic_y)
         self.recalc_poisson(30 if not preserve_state else 10, params)
         pre = params.get('preload')
+        if not preserve_state:
+            self.hold_ids = [0, 0]
+            self.hold_targets = None
         if pre and not preserve_state:
             self._preload(params, int(pre.get('ions', 0)), int(pre.get('electrons', 0)))
+            if pre.get('hold'):
+                self.hold_targets = (int(pre.get('ions', 0)), int(pre.get('electrons', 0)))
 
     # ------------------------------------------------------------------
-    def _preload_positions(self, purpose, count, x_max):
-        pid = np.arange(count, dtype=np.uint64)
+    def _preload_positions(self, purpose, count, x_max, first_id=0):
+        pid = first_id + np.arange(count, dtype=np.uint64)
         x = np.zeros(count, F32)
         y = np.zeros(count, F32)
-        todo = np.ones(count, bool)
+        todo = np.arange(count)
         for t in range(32):
-            u0, u1 = rng.uniforms(self.seed, purpose, pid, t, 0)
+            if len(todo) == 0:
+                break
+            u0, u1 = rng.uniforms(self.seed, purpose, pid[todo], t, 0)
             cx = (x_max * u0).astype(F32)
             cy = (self.Ly * u1).astype(F32)
             ok = ~self.isBound[nearest_index(cy, self.dy, self.ny), nearest_index(cx, self.dx, self.nx)]
-            take = todo & ok
-            x[take], y[take] = cx[take], cy[take]
-            last = todo & (t == 31)
-            x[last], y[last] = cx[last], cy[last]
-            todo &= ~ok
+            if t == 31:
+                ok[:] = True
+            x[todo[ok]], y[todo[ok]] = cx[ok], cy[ok]
+            todo = todo[~ok]
         return pid, x, y
 
-    def _preload(self, params, n_i, n_e):
+    def _preload(self, params, n_i, n_e, hold=False):
+        """Preloaded particles; with hold, the top-up of the population hold
+        (ids from the hold counters, streams P_HOLD_ION / P_HOLD_ELEC, ion
+        pids 0x40000000 | id)."""
         Te_up = params.get('Te_up', 3.0)
         Ti = params.get('Ti', 0.1)
         if n_i > 0:
             grids = params.get('grids', [])
             x_max = self.grid_x_ends[-1] + grids[-1]["gap"] if grids else self.Lx
-            pid, x, y = self._preload_positions(rng.P_INIT_ION, n_i, x_max)
+            purpose = rng.P_HOLD_ION if hold else rng.P_INIT_ION
+            first = self.hold_ids[0] if hold else 0
+            pid, x, y = self._preload_positions(purpose, n_i, x_max, first)
             vb = np.sqrt(self.q_ion * Te_up / self.m_ion)
             vs = np.sqrt(self.q_ion * Ti / self.m_ion)
-            z0, z1 = rng.normals(self.seed, rng.P_INIT_ION, pid, 0, 1)
-            z2, _ = rng.normals(self.seed, rng.P_INIT_ION, pid, 0, 2)
+            z0, z1 = rng.normals(self.seed, purpose, pid, 0, 1)
+            z2, _ = rng.normals(self.seed, purpose, pid, 0, 2)
+            if hold:
+                tag = (np.uint64(HOLD_ID_BIT) | (pid & np.uint64(HOLD_ID_BIT - 1))).astype(np.uint32)
+                self.hold_ids[0] = (self.hold_ids[0] + n_i) & 0xFFFFFFFF
+            else:
+                tag = pid.astype(np.uint32)
+                self.next_ion_pid = n_i
             self._add_ions(x, y, (vb + z0 * vs).astype(F32), (z1 * vs).astype(F32),
-                           (z2 * vs).astype(F32), np.zeros(n_i, bool), pid.astype(np.uint32))
-            self.next_ion_pid = n_i
+                           (z2 * vs).astype(F32), np.zeros(n_i, bool), tag)
         if n_e > 0:
-            pid, x, y = self._preload_positions(rng.P_INIT_ELEC, n_e, self.Lx)
+            purpose = rng.P_HOLD_ELEC if hold else rng.P_INIT_ELEC
+            first = self.hold_ids[1] if hold else 0
+            pid, x, y = self._preload_positions(purpose, n_e, self.Lx, first)
             vt = np.sqrt(2.0 * self.q * params.get('Te', 5.0) / self.m_e)
-            z0, z1 = rng.normals(self.seed, rng.P_INIT_ELEC, pid, 0, 1)
-            z2, _ = rng.normals(self.seed, rng.P_INIT_ELEC, pid, 0, 2)
+            z0, z1 = rng.normals(self.seed, purpose, pid, 0, 1)
+            z2, _ = rng.normals(self.seed, purpose, pid, 0, 2)
+            if hold:
+                self.hold_ids[1] = (self.hold_ids[1] + n_e) & 0xFFFFFFFF
             self._add_electrons(x, y, (z0 * vt).astype(F32), (z1 * vt).astype(F32),
                                 (z2 * vt).astype(F32))
+
+    def _hold(self, params):
+        """Population hold (preload {"hold": true}, a synthetic workload
+        option with no reference counterpart): after the step's injections,
+        top both species up to their preloaded counts."""
+        if not self.hold_targets:
+            return
+        ti, te = self.hold_targets
+        self._preload(params, max(ti - self.num_p, 0), max(te - self.num_e, 0), hold=True)
 
     # ------------------------------------------------------------------
     def recalc_poisson(self, iterations=5, params=None):
@@ -449,6 +480,7 @@ class OracleSimulator:
         # A. injection
         self._inject(params, t_current)
         self._neutralizer(params)
+        self._hold(params)
         # B. deposit + Poisson
         self.deposit()
         if self.iteration % 2 == 0:

@@ -42,3 +42,34 @@ def test_count_bound_tight_after_refresh():
     b.grow(5)
     b.refresh(40, snap)
     assert b.bound == 45
+
+
+def test_count_bound_with_population_hold():
+    """Histories that mix additive growth, removals and top-ups to a target
+    (count' = max(count, target)): the bound always covers the true count
+    and is at most target + the additive growth not yet observed."""
+    rng = random.Random(1)
+    for trial in range(200):
+        b = CountBound()
+        target = rng.randrange(10, 200)
+        true = 0
+        history = []
+        for step in range(60):
+            inj = rng.randrange(0, 30)
+            b.grow(inj)
+            true += inj
+            b.top_up(target)
+            true = max(true, target)
+            true -= rng.randrange(0, true + 1) if rng.random() < 0.7 else 0
+            see = rng.randrange(0, 10)
+            b.grow(see)
+            true += rng.randrange(0, see + 1)
+            history.append((true, b.snapshot()))
+            while history and rng.random() < 0.5:
+                c, snap = history.pop(0)
+                b.refresh(c, snap)
+            assert b.bound >= true
+        while history:
+            c, snap = history.pop(0)
+            b.refresh(c, snap)
+        assert true <= b.bound <= max(true, target)

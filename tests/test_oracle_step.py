@@ -517,3 +517,36 @@ def test_electrode_rasterisation_literal(geometry, cham, defl):
     lit = electrode_mask(o, p["grids"], geometry, 3.0)
     assert np.array_equal(lit, o.isBound)
     assert lit.sum() > 500
+
+
+def test_population_hold_tops_up_both_species():
+    """preload {"hold": true}: every step pushes exactly the preloaded
+    populations (after the step's injections the counts are topped up), the
+    added ions carry fresh ids with bit 30 set, and their positions follow the
+    preload's placement law from their own streams."""
+    p = c1_params(nx=128, ny=48, ions=3000, electrons=3000)
+    p["preload"]["hold"] = True
+    o = _sim(p, Capture)
+    seen = set(int(v) for v in o.p_pid)
+    for _ in range(4):
+        before_ids = list(o.hold_ids)
+        o.step(p)
+        assert o.pushed >= 3000 + 3000
+        assert len(o.cap["x"]) >= 3000
+        new = o.cap["pid"] >= 0x40000000
+        ids = (o.cap["pid"][new] & 0x3FFFFFFF).astype(np.int64)
+        assert len(ids) == o.hold_ids[0] - before_ids[0]
+        assert np.array_equal(np.sort(ids), np.arange(before_ids[0], o.hold_ids[0]))
+        assert not (set(int(v) for v in o.cap["pid"][new]) & seen)
+        seen |= set(int(v) for v in o.cap["pid"])
+    # placement of the last top-up, literal
+    x_plume = o.grid_x_ends[-1] + p["grids"][-1]["gap"]
+    k_new = np.where(new)[0][:50]
+    for k in k_new:
+        i = int(o.cap["pid"][k]) & 0x3FFFFFFF
+        for t in range(32):
+            u0, u1 = rng.uniforms(o.seed, rng.P_HOLD_ION, np.uint64(i), t, 0)
+            cx, cy = F32(x_plume * float(u0)), F32(o.Ly * float(u1))
+            if not o.isBound[nearest(cy, o.dy, o.ny), nearest(cx, o.dx, o.nx)] or t == 31:
+                break
+        assert o.cap["x_old"][k] == cx and o.cap["y_old"][k] == cy

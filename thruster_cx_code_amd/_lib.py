@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -64,7 +64,7 @@ class Inject(C.Structure):
                 ("v_drift", f64), ("v_spread", f64), ("Lx", f64), ("Ly", f64),
                 ("dx32", f32), ("dy32", f32), ("nx", i32), ("ny", i32), ("kind", i32),
                 ("two_band", i32), ("band_f32", i32), ("seed", u32), ("step", u32),
-                ("pid_base", u32), ("pad_", u32)]
+                ("pid_base", u32), ("pad_", u32), ("overflow", vp), ("id_next", vp), ("target", i64)]
 
 
 class Picard(C.Structure):

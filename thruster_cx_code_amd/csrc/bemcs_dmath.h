@@ -157,6 +157,8 @@ BEMCS_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 enum : uint32_t {
   P_INIT_ION = 0x10,
   P_INIT_ELEC = 0x11,
+  P_HOLD_ION = 0x12,
+  P_HOLD_ELEC = 0x13,
   P_INJ_COUNT = 0x20,
   P_INJ_ION = 0x21,
   P_NEUT = 0x22,

@@ -733,22 +733,34 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
 }
 
 // --------------------------------------------------------------- injection
+// Population hold: particles to add so that the count reaches target, at most
+// the launch size n.
+constexpr uint32_t kHoldIdBit = 0x40000000u;
+__device__ __forceinline__ int64_t hold_need(int64_t target, int64_t count, int64_t n) {
+  int64_t need = target - count;
+  if (need < 0) need = 0;
+  return need < n ? need : n;
+}
+
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_inject(bemcs_inject_case_t one,
                                                     const bemcs_inject_case_t* __restrict__ many) {
   const bemcs_inject_case_t a = B ? many[blockIdx.z] : one;
   const bemcs_species_t& sp = a.sp;
   const bemcs_inject_t& p = a.p;
-  const int64_t n = a.n;
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t base = *sp.count;
+  const bool hold = p.target > 0 && (p.kind == 3 || p.kind == 4);
+  const int64_t n = hold ? hold_need(p.target, base, a.n) : a.n;
+  const uint32_t id0 = hold ? *p.id_next : 0u;
   int key = -1;
   if (i < n && base + i < sp.cap) {
-    const uint32_t id = (uint32_t)i;
+    const uint32_t id = id0 + (uint32_t)i;
     float x, y;
     double vx, vy, vz;
     if (p.kind == 3 || p.kind == 4) {
-      const uint32_t purpose = p.kind == 3 ? P_INIT_ION : P_INIT_ELEC;
+      const uint32_t purpose = p.kind == 3 ? (hold ? P_HOLD_ION : P_INIT_ION)
+                                           : (hold ? P_HOLD_ELEC : P_INIT_ELEC);
       x = 0.f;
       y = 0.f;
       for (uint32_t t = 0; t < 32; ++t) {
@@ -798,7 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_inject(bemcs_inject_case_t one,
     sp.vx[s] = (float)vx;
     sp.vy[s] = (float)vy;
     sp.vz[s] = (float)vz;
-    if (sp.tag) sp.tag[s] = p.pid_base + id;
+    if (sp.tag) sp.tag[s] = hold ? (kHoldIdBit | (id & (kHoldIdBit - 1u))) : p.pid_base + id;
     if (p.cnt) {
       int dix = round_i32_np(x / p.dx32, 1, p.nx - 2);
       int diy = round_i32_np(y / p.dy32, 1, p.ny - 2);
@@ -812,14 +824,16 @@ template <bool B>
 __global__ void k_add_count(bemcs_inject_case_t one, const bemcs_inject_case_t* __restrict__ many) {
   const bemcs_inject_case_t a = B ? many[blockIdx.z] : one;
   int32_t* count = a.sp.count;
-  const int64_t n = a.n, cap = a.sp.cap;
-  uint32_t* overflow = nullptr;
+  const int64_t cap = a.sp.cap;
+  const bool hold = a.p.target > 0 && (a.p.kind == 3 || a.p.kind == 4);
+  const int64_t n = hold ? hold_need(a.p.target, *count, a.n) : a.n;
   int64_t c = (int64_t)*count + n;
   if (c > cap) {
-    if (overflow) atomicOr(overflow, 8u);
+    if (a.p.overflow) atomicOr(a.p.overflow, 8u);
     c = cap;
   }
   *count = (int32_t)c;
+  if (hold) *a.p.id_next += (uint32_t)n;
 }
 
 // --------------------------------------------------------------- cell sort
@@ -1166,7 +1180,12 @@ int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
   BEMCS_REQUIRE(p->kind >= 0 && p->kind <= 4, "bemcs_inject: bad kind");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_inject: mesh too small");
   BEMCS_REQUIRE((p->kind != 3 && p->kind != 4) || p->is_bound, "bemcs_inject: preload needs is_bound");
-  BEMCS_REQUIRE(n_bound_before + n <= sp->cap, "bemcs_inject: capacity exceeded");
+  BEMCS_REQUIRE(p->target <= 0 || ((p->kind == 3 || p->kind == 4) && p->id_next && n >= p->target),
+                "bemcs_inject: population hold needs kind 3 or 4, id_next and n >= target");
+  // a top-up ends at max(count, target); other injections add n
+  BEMCS_REQUIRE(p->target > 0 ? (n_bound_before <= sp->cap && p->target <= sp->cap)
+                              : n_bound_before + n <= sp->cap,
+                "bemcs_inject: capacity exceeded");
   if (n <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
   const bemcs_inject_case_t a{*sp, *p, n, n_bound_before};

@@ -124,16 +124,25 @@ class CountBound:
     def __init__(self):
         self.bound = 0
         self.grown = 0
+        self.floor = 0
 
     def grow(self, n):
         self.bound += int(n)
         self.grown += int(n)
 
+    def top_up(self, target):
+        """A launch that raises the count to at least `target` (population
+        hold): count' = max(count, target) <= max(bound, target).  Later
+        refreshes bound the count by max(observed, target) + growth since,
+        since every hold since the observation raised it to target at most."""
+        self.floor = max(self.floor, int(target))
+        self.bound = max(self.bound, int(target))
+
     def snapshot(self):
         return self.grown
 
     def refresh(self, count, snap):
-        self.bound = min(self.bound, int(count) + (self.grown - snap))
+        self.bound = min(self.bound, max(int(count), self.floor) + (self.grown - snap))
 
     def reset(self, exact):
         self.bound = int(exact)

@@ -361,6 +361,11 @@ class DigitalTwinSimulator:
         # host upper bounds of the device counts (no per-step synchronisation)
         self._bi = CountBound()
         self._be = CountBound()
+        # population hold (synthetic workload option): targets and the
+        # device id counters of the particles it adds
+        hold = bool(pre.get('hold', False))
+        self.hold_targets = (ni, ne) if hold and (ni > 0 or ne > 0) else None
+        self.hold_ids = torch.zeros(2, dtype=torch.int32, device=dev)
         self.injected_ions = self.transmitted_ions = self.entered_optics = 0.0
         self.ppc_steps_count = 0
         self._pending = []
@@ -511,6 +516,7 @@ class DigitalTwinSimulator:
         p.seed = self.seed
         p.step = self.iteration & 0xFFFFFFFF
         p.Lx, p.Ly = self.Lx, self.Ly
+        p.overflow = ptr(self.overflow_d)
         return p
 
     def _launch_inject(self, sp_idx_ions, p, n):
@@ -518,7 +524,38 @@ class DigitalTwinSimulator:
         bound = self.bound_i if sp_idx_ions else self.bound_e
         check(self.lib.bemcs_inject(C.byref(sp.struct()), C.byref(p), int(n), int(bound), stream_ptr()),
               "inject")
-        (self._bi if sp_idx_ions else self._be).grow(n)
+        cb = self._bi if sp_idx_ions else self._be
+        if p.target > 0:
+            cb.top_up(p.target)
+        else:
+            cb.grow(n)
+
+    def _hold_plans(self, params):
+        """Population hold (preload {"hold": true}): top both species up to
+        their preloaded counts after the step's injections, with the preload's
+        placement and velocity laws.  No reference counterpart: a synthetic
+        workload option that keeps a configured population (the reference's
+        injection at the configured density cannot: the macro-weight rule
+        caps the ion source, and electron space charge expels electrons)."""
+        plans = []
+        if not self.hold_targets:
+            return plans
+        ti, te = self.hold_targets
+        Te_up = params.get('Te_up', 3.0)
+        Ti = params.get('Ti', 0.1)
+        if ti > 0:
+            p = self._inject_struct(3, self.cnt_i)
+            p.Lx = self.x_plume
+            p.v_drift = float(np.sqrt(self.q_ion * Te_up / self.m_ion))
+            p.v_spread = float(np.sqrt(self.q_ion * Ti / self.m_ion))
+            p.id_next, p.target = self.hold_ids.data_ptr(), ti
+            plans.append((True, p, ti))
+        if te > 0:
+            p = self._inject_struct(4, self.cnt_e)
+            p.v_spread = float(np.sqrt(2.0 * self.q * params.get('Te', 5.0) / self.m_e))
+            p.id_next, p.target = self.hold_ids.data_ptr() + 4, te
+            plans.append((False, p, te))
+        return plans
 
     def _preload(self, params, n_i, n_e):
         Te_up = params.get('Te_up', 3.0)
@@ -539,8 +576,9 @@ class DigitalTwinSimulator:
             self._launch_inject(False, p, n_e)
 
     def _inject(self, params, t_current):
-        """Injection of one step (:996-1099): plan, reserve capacity, launch."""
-        plans = self._inject_plan(params, t_current)
+        """Injection of one step (:996-1099), then the population hold when
+        configured: plan, reserve capacity, launch."""
+        plans = self._inject_plan(params, t_current) + self._hold_plans(params)
         self._reserve_step(plans)
         for is_ion, p, n in plans:
             self._launch_inject(is_ion, p, n)
@@ -548,8 +586,14 @@ class DigitalTwinSimulator:
     def _reserve_step(self, plans):
         """Capacity for this step's injections and for one SEE electron per
         ion, reserved before anything of the step is launched."""
-        ni = self.bound_i + sum(n for is_ion, _, n in plans if is_ion)
-        ne = self.bound_e + sum(n for is_ion, _, n in plans if not is_ion)
+        ni = self.bound_i + sum(n for is_ion, p, n in plans if is_ion and p.target <= 0)
+        ne = self.bound_e + sum(n for is_ion, p, n in plans if not is_ion and p.target <= 0)
+        for is_ion, p, n in plans:
+            if p.target > 0:
+                if is_ion:
+                    ni = max(ni, int(p.target))
+                else:
+                    ne = max(ne, int(p.target))
         self._ensure_capacity(ni, ne + ni)
 
     def _inject_plan(self, params, t_current):
@@ -1052,6 +1096,7 @@ class DigitalTwinSimulator:
         dsz = C.sizeof(_lib.IonDiag)
         raw = self.ring_diag[slot * dsz:(slot + 1) * dsz].cpu().numpy().tobytes()
         d = _lib.IonDiag.from_buffer_copy(raw)
+        self._check_overflow_word()
         if d.overflow & 32:
             raise _lib.BemcsError(_lib.EOVERFLOW, "impact heat or sputter damage outside the fixed-point "
                                   "accumulator range (non-finite or beyond 2^53 units)")
@@ -1072,13 +1117,18 @@ class DigitalTwinSimulator:
             rec['exit_vx_std'] = math.sqrt(max(d.sum_vx2 / nc - (d.sum_vx / nc) ** 2, 0.0))
             rec['exit_v_std'] = math.sqrt(max(d.sum_v2 / nc - (d.sum_v / nc) ** 2, 0.0))
             rec['exit_energy_mean_eV'] = d.sum_E / nc
+        # mean exit speed of primary and of CEX ions (:1470-1480), NaN without any
+        ncp = d.crossed_primary
+        rec['exit_v_mean_primary'] = d.sum_v_primary / ncp if ncp > 0 else float('nan')
+        rec['exit_v_mean_cex'] = d.sum_v_cex / (nc - ncp) if nc - ncp > 0 else float('nan')
         counts = self._ring_counts_np[2 * slot:2 * slot + 2]
         rec['num_p'], rec['num_e'] = int(counts[0]), int(counts[1])
         ppc = self.ring_ppc[2 * slot:2 * slot + 2].cpu().numpy()
         rec['total_active_cells'], rec['low_ppc_cells'] = int(ppc[0]), int(ppc[1])
         nsub = self.ring_nsub[2 * slot:2 * slot + 2].cpu().numpy()
         rec['n_sub_ion'], rec['n_sub_e'] = int(nsub[0]), int(nsub[1])
-        for k in ('exit_vx_mean', 'exit_v_mean', 'exit_vx_std', 'exit_v_std', 'exit_energy_mean_eV'):
+        for k in ('exit_vx_mean', 'exit_v_mean', 'exit_vx_std', 'exit_v_std', 'exit_energy_mean_eV',
+                  'exit_v_mean_primary', 'exit_v_mean_cex'):
             setattr(self, k, rec.get(k, float('nan')))
         self.exit_count_step = nc
         self.total_active_cells = rec['total_active_cells']
@@ -1090,7 +1140,18 @@ class DigitalTwinSimulator:
 
     def particle_counts(self):
         torch.cuda.current_stream().synchronize()
+        self._check_overflow_word()
         return int(self.ions[0].count.item()), int(self.elec[0].count.item())
+
+    def _check_overflow_word(self):
+        """Flags of the electron step and of injection (bemcs_electron_step_t
+        overflow, bemcs_inject_t overflow): 4 a survivor beyond capacity, 8 an
+        injected count clamped at capacity, 16 a count above the launch
+        bound.  Each means particles were dropped; the state is not usable."""
+        f = int(self.overflow_d.item())
+        if f:
+            raise _lib.BemcsError(_lib.EOVERFLOW, f"particle buffer overflow in the electron step or "
+                                  f"injection (flags {f})")
 
     # ---------------------------------------------------- host mirrors
     def _ion_host(self, name):
