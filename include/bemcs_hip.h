@@ -247,7 +247,13 @@ int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
                    int64_t n_bound, bemcs_stream_t stream);
 
 /* Fused electron step: n_sub pushes, periodic wrap, purge (nearest-node hit
- * or outside [0,Lx]x[0,Ly] or NaN, :1534-1561), removal, NGP counts. */
+ * or outside [0,Lx]x[0,Ly] or NaN, :1534-1561), removal, NGP counts.
+ * With sub_chunks = K > 1 the substeps are split into K chunks, substeps
+ * [floor(n_sub j / K), floor(n_sub (j+1) / K)) in chunk j; the fused step then
+ * runs only the substeps from chunk sub_first on (earlier chunks ran through
+ * bemcs_electron_push_chunk, with cell sorts between them to keep the
+ * gathers of a wavefront in a few cells).  Each electron's substeps depend
+ * only on its own state, so the split changes no result. */
 typedef struct {
   const float* Exy;
   const uint8_t* is_bound;
@@ -260,12 +266,20 @@ typedef struct {
   float dx32, dy32;
   float Lx32, Ly32;
   int32_t nx, ny, periodic_y;
+  int32_t sub_first;         /* first chunk the fused step runs (0: all substeps) */
+  int32_t sub_chunks;        /* K (<= 1: no split) */
   int32_t pad_;
 } bemcs_electron_step_t;
 
 int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
                         const bemcs_electron_step_t* p, int64_t n_bound,
                         bemcs_stream_t stream);
+
+/* Chunk `chunk` of p->sub_chunks of the electron substeps (the push of
+ * :1251-1283, n_sub read on the device), in place on the first
+ * min(*sp->count, n_bound) electrons: no wrap, purge or removal. */
+int bemcs_electron_push_chunk(const bemcs_species_t* sp, const bemcs_electron_step_t* p, int32_t chunk,
+                              int64_t n_bound, bemcs_stream_t stream);
 
 /* Injection of `n` particles at the end of `sp` (count is advanced), adding
  * their NGP counts to cnt.  kind: 0 = beam ions (:1020-1064), 1 = neutraliser

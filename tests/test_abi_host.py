@@ -48,7 +48,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(_lib.IonDiag) == 10 * 4 + 7 * 8
     assert C.sizeof(_lib.IonStep) == (8 * 8 + 11 * 8 + 8 * 4 + 8 * 4 + 2 * 8 + C.sizeof(_lib.Species) + 8 * 4
                                       + 4 * 8 + 4 * 8 + 2 * 4)
-    assert C.sizeof(_lib.ElectronStep) == 5 * 8 + 3 * 8 + 4 * 4 + 4 * 4
+    assert C.sizeof(_lib.ElectronStep) == 5 * 8 + 3 * 8 + 4 * 4 + 6 * 4
     assert C.sizeof(_lib.Inject) == 2 * 8 + 4 * 8 + 5 * 8 + 2 * 4 + 9 * 4 + 4 + 3 * 8
     assert C.sizeof(_lib.Picard) == 6 * 8 + 11 * 8 + 2 * 4 + 2 * 8
 

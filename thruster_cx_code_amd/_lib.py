@@ -56,7 +56,8 @@ class ElectronStep(C.Structure):
     _fields_ = [("Exy", vp), ("is_bound", vp), ("n_sub", vp), ("cnt_next", vp), ("overflow", vp),
                 ("dt", f64), ("q_m", f64), ("Ly", f64),
                 ("dx32", f32), ("dy32", f32), ("Lx32", f32), ("Ly32", f32),
-                ("nx", i32), ("ny", i32), ("periodic_y", i32), ("pad_", i32)]
+                ("nx", i32), ("ny", i32), ("periodic_y", i32), ("sub_first", i32), ("sub_chunks", i32),
+                ("pad_", i32)]
 
 
 class Inject(C.Structure):
@@ -161,6 +162,7 @@ _SIGS = {
     "bemcs_fastdiv_enabled": (i32, [C.c_float, vp]),
     "bemcs_ion_step": (i32, [vp, vp, vp, vp, i64, vp]),
     "bemcs_electron_step": (i32, [vp, vp, vp, i64, vp]),
+    "bemcs_electron_push_chunk": (i32, [vp, vp, i32, i64, vp]),
     "bemcs_inject": (i32, [vp, vp, i64, i64, vp]),
     "bemcs_sort_by_cell": (i32, [vp, vp, i64, i32, i32, f64, f64, vp, vp]),
     "bemcs_poisson_create": (i32, [vp, i32, i32, i32, vp]),

@@ -670,6 +670,42 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
 }
 
 // ------------------------------------------------------ fused electron step
+// First substep of chunk j of k over n substeps: floor(n j / k).
+__device__ __forceinline__ int chunk_begin(int n, int j, int k) {
+  if (k <= 1 || j <= 0) return 0;
+  if (j >= k) return n;
+  return (int)(((int64_t)n * j) / k);
+}
+
+// Substeps of one chunk, in place (no wrap, purge or removal).
+template <bool FAST>
+__global__ __launch_bounds__(kBlock) void k_electron_push(bemcs_species_t sp, bemcs_electron_step_t p,
+                                                          int chunk, int64_t n_bound) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t n = *sp.count;
+  if (n > sp.cap) n = sp.cap;
+  if (n > n_bound) n = n_bound;
+  if (i >= n) return;
+  const int nsub = *p.n_sub;
+  const int s0 = chunk_begin(nsub, chunk, p.sub_chunks);
+  const int s1 = chunk_begin(nsub, chunk + 1, p.sub_chunks);
+  if (s0 >= s1) return;
+  const Divisor ddx{p.dx32, 1.0f / p.dx32}, ddy{p.dy32, 1.0f / p.dy32};
+  const double dts = p.dt / (double)nsub;
+  const double h = 0.5 * p.q_m * dts;
+  const float dts32 = (float)dts;
+  EField<true> E{p.Exy, p.Exy + 1};
+  float x = sp.x[i], y = sp.y[i], vx = sp.vx[i], vy = sp.vy[i], vz = sp.vz[i];
+  for (int s = s0; s < s1; ++s)
+    boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, p.nx, p.ny, ddx,
+                                     ddy, h, dts32);
+  sp.x[i] = x;
+  sp.y[i] = y;
+  sp.vx[i] = vx;
+  sp.vy[i] = vy;
+  sp.vz[i] = vz;
+}
+
 template <bool FAST, bool B>
 __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t one,
                                                            const bemcs_electron_case_t* __restrict__ many) {
@@ -699,8 +735,9 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
     const double h = 0.5 * p.q_m * dts;
     const float dts32 = (float)dts;
     EField<true> E{p.Exy, p.Exy + 1};
+    const int s0 = chunk_begin(nsub, p.sub_first, p.sub_chunks);
     if (active)
-      for (int s = 0; s < nsub; ++s)
+      for (int s = s0; s < nsub; ++s)
         boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
                                          ddx, ddy, h, dts32);
   }
@@ -954,8 +991,24 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(bemcs_sort_case_t one,
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *in.count;
   if (n > a.n_bound) n = a.n_bound;
+  // runs of equal keys in consecutive lanes (input already close to cell
+  // order) reserve their slots with one atomic
+  const int key = i < n ? (int)sv.keys[i] : -1;
+  const int lane = lane_id();
+  const int prev = __shfl_up(key, 1);
+  const bool head = (lane == 0) || (key != prev);
+  const uint64_t heads = __ballot(head);
+  uint32_t base = 0;
+  if (head && key >= 0) {
+    const uint64_t after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+    const int run = after ? __ffsll((unsigned long long)after) : (64 - lane);
+    base = atomicAdd(&sv.hist[key], (uint32_t)run);
+  }
+  const uint64_t upto = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
+  const int leader = 63 - __clzll((long long)upto);
+  base = __shfl(base, leader);
   if (i >= n) return;
-  uint32_t d = atomicAdd(&sv.hist[sv.keys[i]], 1u);
+  const uint32_t d = base + (uint32_t)(lane - leader);
   out.x[d] = in.x[i];
   out.y[d] = in.y[i];
   out.vx[d] = in.vx[i];
@@ -1170,6 +1223,21 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   auto kern = fast ? k_electron_step<true, false> : k_electron_step<false, false>;
   hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
                      (const bemcs_electron_case_t*)nullptr);
+  BEMCS_CHECK_LAUNCH();
+  return BEMCS_OK;
+}
+
+int bemcs_electron_push_chunk(const bemcs_species_t* sp, const bemcs_electron_step_t* p, int32_t chunk,
+                              int64_t n_bound, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(sp && p && sp->count && p->Exy && p->n_sub, "bemcs_electron_push_chunk: null argument");
+  BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_electron_push_chunk: mesh too small");
+  BEMCS_REQUIRE(p->sub_chunks >= 1 && chunk >= 0 && chunk < p->sub_chunks,
+                "bemcs_electron_push_chunk: 0 <= chunk < sub_chunks");
+  if (n_bound <= 0) return BEMCS_OK;
+  hipStream_t s = as_stream(stream);
+  const bool fast = fastdiv_ok(p->dx32, s) && fastdiv_ok(p->dy32, s);
+  auto kern = fast ? k_electron_push<true> : k_electron_push<false>;
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, *sp, *p, (int)chunk, n_bound);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
 }

@@ -20,6 +20,7 @@ in DESIGN.md-independent terms in the module docstrings of the kernels.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import torch
@@ -62,6 +63,10 @@ class DigitalTwinSimulator:
         # substeps), so their order decays within 2-3 steps; ions drift slowly.
         self.sort_interval = int(sort_interval)
         self.electron_sort_interval = int(electron_sort_interval)
+        # electron substeps per chunk between in-step cell sorts (0: no
+        # chunking), and the population below which no chunking is done
+        self.electron_chunk = int(os.environ.get("BEMCS_ECHUNK", "256"))
+        self.electron_chunk_min = int(os.environ.get("BEMCS_ECHUNK_MIN", "262144"))
         self.poisson_rtol = float(poisson_rtol)
         self.poisson_max_iter = int(poisson_max_iter)
         # physical constants and defaults as the reference (:310-401)
@@ -322,7 +327,11 @@ class DigitalTwinSimulator:
             self.ring_counts = torch.zeros(2 * RING, dtype=torch.int32, pin_memory=True)
             self._ring_counts_np = self.ring_counts.numpy()
             self._events = EventRing()
-            self.ring_nsub = zeros(2 * RING, torch.int32, dev)
+            # substep counts of each step, written by k_step_end into pinned
+            # host memory (the electron chunking reads the last known one)
+            self.ring_nsub = torch.zeros(2 * RING, dtype=torch.int32, pin_memory=True)
+            self._ring_nsub_np = self.ring_nsub.numpy()
+            self._nsub_e_hint = 0
             self.ring_nin = zeros(2 * RING, torch.int32, dev)
             self.ring_remeshed = np.zeros(RING, bool)
             self._surface_pending = None
@@ -692,6 +701,7 @@ class DigitalTwinSimulator:
             self._pending.pop(0)
             self._bi.refresh(int(cnts[2 * slot]), gi)
             self._be.refresh(int(cnts[2 * slot + 1]), ge)
+            self._nsub_e_hint = int(self._ring_nsub_np[2 * slot + 1])
 
     def _readback_event(self):
         """An event for this step's count readback (waits for old readbacks
@@ -772,6 +782,7 @@ class DigitalTwinSimulator:
         ep = self._elec_params
         ep.Exy, ep.is_bound, ep.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb + 4)
         ep.cnt_next, ep.overflow = ptr(self.cnt_e_next), ptr(self.overflow_d)
+        E0 = self._electron_chunks(ep, kt, s)
         ev = self._timed_begin(kt)
         check(lib.bemcs_electron_step(C.byref(E0.struct()), C.byref(E1.struct()), C.byref(ep),
                                       self.bound_e, s), "electron_step")
@@ -810,6 +821,44 @@ class DigitalTwinSimulator:
         ev.record()
         self._pending.append((ev, slot, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
+
+    def _electron_chunks(self, ep, kt, s):
+        """Electrons take hundreds of substeps per step and cross many cells,
+        so a wavefront's gathers spread over the mesh.  With a large
+        population the substeps are split into K chunks of about
+        electron_chunk substeps (K from the last step whose substep count the
+        host has seen), with a cell sort after each chunk but the last; the
+        fused step then runs the last chunk.  Returns the buffer holding the
+        electrons for the fused step."""
+        E0 = self.elec[0]
+        ep.sub_first, ep.sub_chunks = 0, 1
+        c = self.electron_chunk
+        if c <= 0 or self.bound_e < self.electron_chunk_min or self._nsub_e_hint < 2 * c:
+            return E0
+        k = min(max(int(round(self._nsub_e_hint / c)), 2), 64)
+        ep.sub_chunks = k
+        tmp = self._elec_spare()
+        scratch = ptr(self.sort_scratch_for(self.bound_e))
+        for j in range(k - 1):
+            ev = self._timed_begin(kt)
+            check(self.lib.bemcs_electron_push_chunk(C.byref(E0.struct()), C.byref(ep), j, self.bound_e, s),
+                  "electron_push_chunk")
+            self._timed_end(kt, "electron_push", ev)
+            ev = self._timed_begin(kt)
+            check(self.lib.bemcs_sort_by_cell(C.byref(E0.struct()), C.byref(tmp.struct()), self.bound_e,
+                                              self.nx, self.ny, self.dx, self.dy, scratch, s), "sort_by_cell")
+            self._timed_end(kt, "electron_sort", ev)
+            E0, tmp = tmp, E0
+        self.elec[0], self._e_spare = E0, tmp
+        ep.sub_first = k - 1
+        return E0
+
+    def _elec_spare(self):
+        cap = self.elec[0].cap
+        sp = getattr(self, '_e_spare', None)
+        if sp is None or sp.cap < cap:
+            sp = self._e_spare = SpeciesBuffers(cap, self.device, False)
+        return sp
 
     def _deposit_every_step(self, params, thermal):
         """The RF Picard iterations and a thermal-bow rebuild at the start of
