@@ -327,6 +327,15 @@ int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, in
                        int32_t nx, int32_t ny, double dx, double dy, uint32_t* scratch,
                        bemcs_stream_t stream);
 
+/* The same counting sort by gather tile: tiles of 2^shift_x x 2^shift_y
+ * cells (key (iy0 >> shift_y) * tiles_x + (ix0 >> shift_x)), which keeps a
+ * wavefront's gathers in a few cache lines with fewer, longer runs to
+ * scatter than a per-cell sort.  Order within a tile is unspecified.
+ * 0 <= shift <= 8; scratch as bemcs_sort_by_cell. */
+int bemcs_sort_by_tile(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
+                       int32_t nx, int32_t ny, double dx, double dy, int32_t shift_x, int32_t shift_y,
+                       uint32_t* scratch, bemcs_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Poisson: matrix-free MG-PCG on the reference operator.
  * ---------------------------------------------------------------------- */
@@ -629,6 +638,7 @@ typedef struct {
   int32_t nx, ny;
   double dx, dy;
   uint32_t* scratch;       /* as bemcs_sort_by_cell */
+  int32_t shift_x, shift_y; /* tile of 2^shift_x x 2^shift_y cells (0, 0: one cell) */
 } bemcs_sort_case_t;
 
 typedef struct {

@@ -282,21 +282,25 @@ def test_electron_chunks_match_oracle():
     assert len(g.kernel_timing.get("electron_sort", [])) > 0
 
 
-def test_electron_chunks_identical_to_one_pass():
-    """Chunked and unchunked runs of the same case agree bit for bit over
-    several steps (potential, ions by id, electrons as a multiset)."""
+def test_electron_step_variants_identical():
+    """The electron step's execution variants agree bit for bit over several
+    steps (potential, ions by id, electrons as a multiset): substeps in one
+    pass or in chunks with sorts between them, with or without a presort."""
     p = c1_params(nx=256, ny=96, ions=100000, electrons=100000)
     from thruster_cx_code_amd import DigitalTwinSimulator
     out = []
-    for chunk in (0, 16):
+    variants = [(0, False), (16, True), (0, True), (16, False)]
+    for chunk, presort in variants:
+        print(f"[gpu] electron variant chunk={chunk} presort={presort}", flush=True)
         g = DigitalTwinSimulator()
         g.m_e = g.m_ion / 200
-        g.electron_chunk, g.electron_chunk_min = chunk, 0
+        g.electron_chunk, g.electron_chunk_min, g.electron_presort = chunk, 0, presort
         g.build_domain(p)
         g.advance(p, 6)
         out.append((g.V.copy(), _ions_by_pid(g.p_x, g.p_y, g.p_vx, g.p_vy, g.p_vz, g.p_pid, g.p_isCEX),
                     _elec_sorted(g.e_x, g.e_y, g.e_vx, g.e_vy, g.e_vz)))
-    assert np.array_equal(out[0][0], out[1][0])
-    for a, b in zip(out[0][1], out[1][1]):
-        assert np.array_equal(a, b)
-    assert np.array_equal(out[0][2], out[1][2])
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        for a, b in zip(out[0][1], o[1]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(out[0][2], o[2])

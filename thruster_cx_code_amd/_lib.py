@@ -107,7 +107,7 @@ class InjectCase(C.Structure):
 
 class SortCase(C.Structure):
     _fields_ = [("in", Species), ("out", Species), ("n_bound", i64), ("nx", i32), ("ny", i32),
-                ("dx", f64), ("dy", f64), ("scratch", vp)]
+                ("dx", f64), ("dy", f64), ("scratch", vp), ("shift_x", i32), ("shift_y", i32)]
 
 
 class PpcCase(C.Structure):
@@ -165,6 +165,7 @@ _SIGS = {
     "bemcs_electron_push_chunk": (i32, [vp, vp, i32, i64, vp]),
     "bemcs_inject": (i32, [vp, vp, i64, i64, vp]),
     "bemcs_sort_by_cell": (i32, [vp, vp, i64, i32, i32, f64, f64, vp, vp]),
+    "bemcs_sort_by_tile": (i32, [vp, vp, i64, i32, i32, f64, f64, i32, i32, vp, vp]),
     "bemcs_poisson_create": (i32, [vp, i32, i32, i32, vp]),
     "bemcs_poisson_create_opt": (i32, [vp, i32, i32, i32, i32, vp]),
     "bemcs_poisson_free": (i32, [vp]),

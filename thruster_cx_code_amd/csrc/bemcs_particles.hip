@@ -25,16 +25,66 @@ namespace bemcs {
 
 constexpr int kBlock = 256;
 
+// ---------------------------------------------------- LDS-privatised counts
+// Per-tile privatisation of the NGP and PPC count histograms: a work-group
+// counts its particles' nodes in an LDS window of kWinX x kWinY nodes anchored
+// at its smallest (ix, iy) (cell-sorted particles of one work-group lie in a
+// few cells), with runs of equal nodes in a wavefront merged before the LDS
+// atomics, then adds each touched window node to the global histogram with one
+// atomic.  Nodes outside the window go straight to global atomics.  Integer
+// counts, so the totals do not depend on the order of the adds.
+constexpr int kWinX = 64, kWinY = 32;
+struct LdsHist {
+  uint32_t bins[kWinX * kWinY];
+  int mins[2][kBlock / 64];
+};
+
+// Every thread of the work-group must call it; ix < 0 adds nothing.
+__device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int nx, int ix, int iy) {
+  const bool valid = ix >= 0;
+  int mx = valid ? ix : INT_MAX, my = valid ? iy : INT_MAX;
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = min(mx, __shfl_xor(mx, o));
+    my = min(my, __shfl_xor(my, o));
+  }
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    h.mins[0][wave] = mx;
+    h.mins[1][wave] = my;
+  }
+  for (int k = threadIdx.x; k < kWinX * kWinY; k += kBlock) h.bins[k] = 0u;
+  __syncthreads();
+  int x0 = h.mins[0][0], y0 = h.mins[1][0];
+  for (int w = 1; w < kBlock / 64; ++w) {
+    x0 = min(x0, h.mins[0][w]);
+    y0 = min(y0, h.mins[1][w]);
+  }
+  const int lx = ix - x0, ly = iy - y0;
+  const bool in_win = valid && lx < kWinX && ly < kWinY;
+  wave_hist_add(h.bins, in_win ? ly * kWinX + lx : -1);
+  wave_hist_add(hist, (valid && !in_win) ? iy * nx + ix : -1);
+  __syncthreads();
+  if (x0 != INT_MAX) {
+    for (int k = threadIdx.x; k < kWinX * kWinY; k += kBlock) {
+      const uint32_t c = h.bins[k];
+      if (c) atomicAdd(&hist[(y0 + k / kWinX) * nx + (x0 + k % kWinX)], c);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ push
-template <bool INTERLEAVED>
+// The field at the gather corners: float32, either two arrays or
+// interleaved (Ex, Ey) pairs.
+template <bool INTERLEAVED, typename T = float>
 struct EField {
-  const float* ex;
-  const float* ey;
-  __device__ __forceinline__ void corner(int idx, float& a, float& b) const {
+  using value_type = T;
+  const T* ex;
+  const T* ey;
+  __device__ __forceinline__ void corner(int idx, T& a, T& b) const {
     if (INTERLEAVED) {
-      float2 v = reinterpret_cast<const float2*>(ex)[idx];
-      a = v.x;
-      b = v.y;
+      const T* q = ex + 2 * (int64_t)idx;
+      a = q[0];
+      b = q[1];
     } else {
       a = ex[idx];
       b = ey[idx];
@@ -44,11 +94,12 @@ struct EField {
 
 // One gather + Boris + drift substep.  h = 0.5*q_m*dt (f64), dt32 = f32(dt);
 // dx, dy are the float32 cell sizes.
-template <bool INTERLEAVED, bool HAS_B, bool FAST>
+template <class FieldT, bool HAS_B, bool FAST>
 __device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, float& vy, float& vz,
-                                              const EField<INTERLEAVED>& E, const float* Bx,
+                                              const FieldT& E, const float* Bx,
                                               const float* By, const float* Bz, int nx, int ny,
                                               Divisor dx, Divisor dy, double h, float dt32) {
+  using FT = typename FieldT::value_type;
   float gx = div_rn<FAST>(x, dx);
   float gy = div_rn<FAST>(y, dy);
   int i0 = floor_i32_np(gx, nx - 2);
@@ -63,7 +114,7 @@ __device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, flo
   double a10 = fx * (1.0 - fy);
   double a01 = (1.0 - fx) * fy;
   double a11 = fx * fy;
-  float e00x, e00y, e10x, e10y, e01x, e01y, e11x, e11y;
+  FT e00x, e00y, e10x, e10y, e01x, e01y, e11x, e11y;
   E.corner(c00, e00x, e00y);
   E.corner(c10, e10x, e10y);
   E.corner(c01, e01x, e01y);
@@ -118,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_push_boris(float* __restrict__ x, fl
   float px = x[i], py = y[i], pvx = vx[i], pvy = vy[i], pvz = vz[i];
   EField<false> E{Ex, Ey};
   for (int r = 0; r < n_rep; ++r)
-    boris_substep<false, HAS_B, FAST>(px, py, pvx, pvy, pvz, E, Bx, By, Bz, nx, ny, dx, dy, h, dt32);
+    boris_substep<EField<false>, HAS_B, FAST>(px, py, pvx, pvy, pvz, E, Bx, By, Bz, nx, ny, dx, dy, h, dt32);
   x[i] = px;
   y[i] = py;
   vx[i] = pvx;
@@ -289,9 +340,17 @@ __global__ __launch_bounds__(kBlock) void k_counts_to_rho(const uint32_t* __rest
 // ds = |v| dt + 0.5 |qm E| dt^2 with the reference's mixed float32/float64
 // evaluation (:940-962); max over particles folded into *ds_bits (ds >= 0,
 // so the float64 bit pattern orders like the value).
+// work-groups of the substep-count kernel: enough to fill the chip, few
+// enough that their maxima meet in a few thousand atomics
+inline unsigned substep_grid(int64_t n) {
+  const unsigned g = grid_for(n, kBlock);
+  return g < 2048u ? g : 2048u;
+}
+
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_substep_ds(bemcs_substeps_case_t one,
-                                                        const bemcs_substeps_case_t* __restrict__ many) {
+                                                        const bemcs_substeps_case_t* __restrict__ many,
+                                                        int64_t max_bound) {
   __shared__ unsigned long long smax[kBlock / 64];
   const bemcs_substeps_case_t a = B ? many[blockIdx.z] : one;
   const float* __restrict__ x = a.x;
@@ -305,12 +364,14 @@ __global__ __launch_bounds__(kBlock) void k_substep_ds(bemcs_substeps_case_t one
   const float dx32 = (float)a.dx, dy32 = (float)a.dy, dt32 = (float)a.dt;
   const double qm = a.q_m, dt2 = a.dt2;
   unsigned long long* ds_bits = (unsigned long long*)a.ds_max_bits;
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int64_t n = *n_dev;
-  const int64_t lim = (int64_t)gridDim.x * kBlock;
-  if (n > lim) n = lim;
+  if (n > max_bound) n = max_bound;
+  // a capped grid strides over the particles, so the block maxima meet in a
+  // few thousand atomics on the one result word instead of one per 256
+  // particles
   double ds = 0.0;
-  if (i < n) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     float px = x[i], py = y[i];
     float gx = px / dx32, gy = py / dy32;
     int i0 = floor_i64_np(gx, nx - 2);
@@ -329,8 +390,8 @@ __global__ __launch_bounds__(kBlock) void k_substep_ds(bemcs_substeps_case_t one
     double a_mag = sqrt(ax * ax + ay * ay);
     float a = vx[i], b = vy[i], c = vz[i];
     float v_mag = sqrtf(a * a + b * b + c * c);
-    ds = (double)(v_mag * dt32) + 0.5 * a_mag * dt2;
-    if (!(ds >= 0.0)) ds = 0.0;  // NaN particles do not set the substep count
+    double d = (double)(v_mag * dt32) + 0.5 * a_mag * dt2;
+    if (d >= 0.0 && d > ds) ds = d;  // NaN particles do not set the substep count
   }
   unsigned long long bits = (unsigned long long)__double_as_longlong(ds);
   for (int o = 32; o > 0; o >>= 1) {
@@ -413,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
     EField<true> E{p.Exy, p.Exy + 1};
     if (active)
       for (int s = 0; s < nsub; ++s)
-        boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
+        boris_substep<EField<true>, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
                                          ddx, ddy, h, dts32);
   }
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
@@ -646,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
 
   // ---- write survivors, NGP counts for the next deposit, PPC counts
   int64_t slot = wave_append(out.count, keep);
-  int dkey = -1, pkey = -1;
+  int dix = -1, diy = -1, pix = -1, piy = -1;
   if (keep) {
     if (slot < out.cap) {
       out.x[slot] = x;
@@ -655,18 +716,18 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
       out.vy[slot] = vy;
       out.vz[slot] = vz;
       out.tag[slot] = tag;
-      int dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
-      int diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
-      dkey = diy * nx + dix;
-      int pix = trunc_i64_np(div_rn<FAST>(x, ddx), nx - 1);
-      int piy = trunc_i64_np(div_rn<FAST>(y, ddy), ny - 1);
-      pkey = piy * nx + pix;
+      const float gx = div_rn<FAST>(x, ddx), gy = div_rn<FAST>(y, ddy);
+      dix = round_i32_np(gx, 1, nx - 2);
+      diy = round_i32_np(gy, 1, ny - 2);
+      pix = trunc_i64_np(gx, nx - 1);
+      piy = trunc_i64_np(gy, ny - 1);
     } else {
       atomicOr(&p.diag->overflow, 2u);
     }
   }
-  wave_hist_add(p.cnt_next, dkey);
-  wave_hist_add(p.ppc, pkey);
+  __shared__ LdsHist hcnt, hppc;
+  block_hist_add(hcnt, p.cnt_next, nx, dix, diy);
+  block_hist_add(hppc, p.ppc, nx, pix, piy);
 }
 
 // ------------------------------------------------------ fused electron step
@@ -678,6 +739,16 @@ __device__ __forceinline__ int chunk_begin(int n, int j, int k) {
 }
 
 // Substeps of one chunk, in place (no wrap, purge or removal).
+// Substeps [s0, s1) of one particle in registers.
+template <class FieldT, bool FAST>
+__device__ __forceinline__ void push_range(float& x, float& y, float& vx, float& vy, float& vz,
+                                           const FieldT& E, int nx, int ny, Divisor ddx, Divisor ddy,
+                                           double h, float dts32, int s0, int s1) {
+  for (int s = s0; s < s1; ++s)
+    boris_substep<FieldT, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, ddx, ddy, h,
+                                       dts32);
+}
+
 template <bool FAST>
 __global__ __launch_bounds__(kBlock) void k_electron_push(bemcs_species_t sp, bemcs_electron_step_t p,
                                                           int chunk, int64_t n_bound) {
@@ -694,11 +765,9 @@ __global__ __launch_bounds__(kBlock) void k_electron_push(bemcs_species_t sp, be
   const double dts = p.dt / (double)nsub;
   const double h = 0.5 * p.q_m * dts;
   const float dts32 = (float)dts;
-  EField<true> E{p.Exy, p.Exy + 1};
   float x = sp.x[i], y = sp.y[i], vx = sp.vx[i], vy = sp.vy[i], vz = sp.vz[i];
-  for (int s = s0; s < s1; ++s)
-    boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, p.nx, p.ny, ddx,
-                                     ddy, h, dts32);
+  const EField<true> E{p.Exy, p.Exy + 1};
+  push_range<EField<true>, FAST>(x, y, vx, vy, vz, E, p.nx, p.ny, ddx, ddy, h, dts32, s0, s1);
   sp.x[i] = x;
   sp.y[i] = y;
   sp.vx[i] = vx;
@@ -734,12 +803,9 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
     const double dts = p.dt / (double)nsub;
     const double h = 0.5 * p.q_m * dts;
     const float dts32 = (float)dts;
-    EField<true> E{p.Exy, p.Exy + 1};
     const int s0 = chunk_begin(nsub, p.sub_first, p.sub_chunks);
-    if (active)
-      for (int s = s0; s < nsub; ++s)
-        boris_substep<true, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
-                                         ddx, ddy, h, dts32);
+    const EField<true> E{p.Exy, p.Exy + 1};
+    if (active) push_range<EField<true>, FAST>(x, y, vx, vy, vz, E, nx, ny, ddx, ddy, h, dts32, s0, nsub);
   }
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
   bool keep = false;
@@ -751,7 +817,7 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
     keep = !(hit || outb);
   }
   int64_t slot = wave_append(out.count, keep);
-  int dkey = -1;
+  int dix = -1, diy = -1;
   if (keep) {
     if (slot < out.cap) {
       out.x[slot] = x;
@@ -759,14 +825,14 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
       out.vx[slot] = vx;
       out.vy[slot] = vy;
       out.vz[slot] = vz;
-      int dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
-      int diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
-      dkey = diy * nx + dix;
+      dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
+      diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
     } else {
       atomicOr(p.overflow, 4u);
     }
   }
-  wave_hist_add(p.cnt_next, dkey);
+  __shared__ LdsHist hcnt;
+  block_hist_add(hcnt, p.cnt_next, nx, dix, diy);
 }
 
 // --------------------------------------------------------------- injection
@@ -878,11 +944,19 @@ __global__ void k_add_count(bemcs_inject_case_t one, const bemcs_inject_case_t* 
 // block sums [1024 + nx*ny/1024], keys [n_bound].
 struct SortView {
   uint32_t *hist, *sums, *keys;
-  int64_t ncell;
+  int64_t ncell;  // bins: tiles of 2^shift_x x 2^shift_y gather cells
+  int tiles_x;
 };
+__host__ __device__ __forceinline__ int sort_tiles_x(const bemcs_sort_case_t& a) {
+  return ((a.nx - 2) >> a.shift_x) + 1;
+}
+__host__ __device__ __forceinline__ int64_t sort_bins(const bemcs_sort_case_t& a) {
+  return (int64_t)sort_tiles_x(a) * (((a.ny - 2) >> a.shift_y) + 1);
+}
 __device__ __forceinline__ SortView sort_view(const bemcs_sort_case_t& a) {
   SortView v;
-  v.ncell = (int64_t)a.nx * a.ny;
+  v.ncell = sort_bins(a);
+  v.tiles_x = sort_tiles_x(a);
   v.hist = a.scratch;
   v.sums = a.scratch + v.ncell;
   v.keys = v.sums + 1024 + ((v.ncell + 1023) / 1024);
@@ -914,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(bemcs_sort_case_t one,
   if (i < n) {
     int i0 = floor_i32_np(a.in.x[i] / (float)a.dx, a.nx - 2);
     int j0 = floor_i32_np(a.in.y[i] / (float)a.dy, a.ny - 2);
-    key = j0 * a.nx + i0;
+    key = (j0 >> a.shift_y) * v.tiles_x + (i0 >> a.shift_x);
     v.keys[i] = (uint32_t)key;
   }
   wave_hist_add(v.hist, key);
@@ -1164,8 +1238,8 @@ int bemcs_substeps(const float* x, const float* y, const float* vx, const float*
   const bemcs_substeps_case_t a{x, y, vx, vy, vz, n_dev, Exy, nx, ny, dx, dy, q_m, dt, dt2, frac,
                                 ds_max_bits, n_sub_out};
   if (n_bound > 0)
-    hipLaunchKernelGGL(k_substep_ds<false>, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
-                       (const bemcs_substeps_case_t*)nullptr);
+    hipLaunchKernelGGL(k_substep_ds<false>, dim3(substep_grid(n_bound)), dim3(kBlock), 0, s, a,
+                       (const bemcs_substeps_case_t*)nullptr, n_bound);
   hipLaunchKernelGGL(k_substep_finalize<false>, dim3(1), dim3(1), 0, s, a,
                      (const bemcs_substeps_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
@@ -1268,12 +1342,20 @@ int bemcs_inject(const bemcs_species_t* sp, const bemcs_inject_t* p, int64_t n,
 int bemcs_sort_by_cell(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
                        int32_t nx, int32_t ny, double dx, double dy, uint32_t* scratch,
                        bemcs_stream_t stream) {
+  return bemcs_sort_by_tile(in, out, n_bound, nx, ny, dx, dy, 0, 0, scratch, stream);
+}
+
+int bemcs_sort_by_tile(const bemcs_species_t* in, const bemcs_species_t* out, int64_t n_bound,
+                       int32_t nx, int32_t ny, double dx, double dy, int32_t shift_x, int32_t shift_y,
+                       uint32_t* scratch, bemcs_stream_t stream) {
   BEMCS_REQUIRE(in && out && scratch && in->count && out->count, "bemcs_sort_by_cell: null argument");
   BEMCS_REQUIRE(out->cap >= n_bound && in->cap >= n_bound, "bemcs_sort_by_cell: capacity");
+  BEMCS_REQUIRE(nx >= 3 && ny >= 3 && shift_x >= 0 && shift_x <= 8 && shift_y >= 0 && shift_y <= 8,
+                "bemcs_sort_by_tile: mesh too small or tile shift outside 0..8");
   hipStream_t s = as_stream(stream);
-  const bemcs_sort_case_t a{*in, *out, n_bound, nx, ny, dx, dy, scratch};
+  const bemcs_sort_case_t a{*in, *out, n_bound, nx, ny, dx, dy, scratch, shift_x, shift_y};
   const bemcs_sort_case_t* none = nullptr;
-  const int64_t ncell = (int64_t)nx * ny;
+  const int64_t ncell = sort_bins(a);
   hipLaunchKernelGGL(k_sort_prepare<false>, dim3(grid_for(ncell, kBlock)), dim3(kBlock), 0, s, a, none);
   if (n_bound > 0) {
     const unsigned nb = grid_for(ncell, 1024);
@@ -1325,8 +1407,8 @@ int bemcs_substeps_batched(const bemcs_substeps_case_t* cases, int32_t n, int64_
   BEMCS_REQUIRE(cases && n >= 1, "bemcs_substeps_batched: bad arguments");
   hipStream_t s = as_stream(stream);
   if (max_bound > 0)
-    hipLaunchKernelGGL(k_substep_ds<true>, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0,
-                       s, bemcs_substeps_case_t{}, cases);
+    hipLaunchKernelGGL(k_substep_ds<true>, dim3(substep_grid(max_bound), 1, n), dim3(kBlock), 0,
+                       s, bemcs_substeps_case_t{}, cases, max_bound);
   hipLaunchKernelGGL(k_substep_finalize<true>, dim3(1, 1, n), dim3(1), 0, s, bemcs_substeps_case_t{},
                      cases);
   BEMCS_CHECK_LAUNCH();

@@ -65,8 +65,13 @@ class DigitalTwinSimulator:
         self.electron_sort_interval = int(electron_sort_interval)
         # electron substeps per chunk between in-step cell sorts (0: no
         # chunking), and the population below which no chunking is done
-        self.electron_chunk = int(os.environ.get("BEMCS_ECHUNK", "256"))
+        self.electron_chunk = int(os.environ.get("BEMCS_ECHUNK", "2048"))
         self.electron_chunk_min = int(os.environ.get("BEMCS_ECHUNK_MIN", "262144"))
+        # tile of the sorts between chunks: 2^sx x 2^sy cells
+        self.electron_tile = tuple(int(v) for v in os.environ.get("BEMCS_ETILE", "0,0").split(","))
+        # sort before the first chunk too (the step starts from the last
+        # step's order plus appended refills and SEE electrons)
+        self.electron_presort = os.environ.get("BEMCS_EPRESORT", "1") == "1"
         self.poisson_rtol = float(poisson_rtol)
         self.poisson_max_iter = int(poisson_max_iter)
         # physical constants and defaults as the reference (:310-401)
@@ -825,30 +830,42 @@ class DigitalTwinSimulator:
     def _electron_chunks(self, ep, kt, s):
         """Electrons take hundreds of substeps per step and cross many cells,
         so a wavefront's gathers spread over the mesh.  With a large
-        population the substeps are split into K chunks of about
-        electron_chunk substeps (K from the last step whose substep count the
-        host has seen), with a cell sort after each chunk but the last; the
-        fused step then runs the last chunk.  Returns the buffer holding the
-        electrons for the fused step."""
+        population (electron_chunk_min) the electrons are sorted by cell
+        first (the step starts from the last step's order with refills and
+        SEE electrons appended), and the substeps are split into K chunks of
+        about electron_chunk substeps (K from the last step whose substep
+        count the host has seen) with a sort after each chunk but the last;
+        the fused step then runs the last chunk.  Returns the buffer holding
+        the electrons for the fused step."""
         E0 = self.elec[0]
         ep.sub_first, ep.sub_chunks = 0, 1
         c = self.electron_chunk
-        if c <= 0 or self.bound_e < self.electron_chunk_min or self._nsub_e_hint < 2 * c:
+        if self.bound_e < self.electron_chunk_min or (c <= 0 and not self.electron_presort):
             return E0
-        k = min(max(int(round(self._nsub_e_hint / c)), 2), 64)
-        ep.sub_chunks = k
+        k = 1
+        if c > 0 and self._nsub_e_hint >= 2 * c:
+            k = min(max(int(round(self._nsub_e_hint / c)), 2), 64)
         tmp = self._elec_spare()
         scratch = ptr(self.sort_scratch_for(self.bound_e))
+        sx, sy = self.electron_tile
+
+        def sort(E0, tmp):
+            ev = self._timed_begin(kt)
+            check(self.lib.bemcs_sort_by_tile(C.byref(E0.struct()), C.byref(tmp.struct()), self.bound_e,
+                                              self.nx, self.ny, self.dx, self.dy, sx, sy, scratch, s),
+                  "sort_by_tile")
+            self._timed_end(kt, "electron_sort", ev)
+            return tmp, E0
+
+        if self.electron_presort:
+            E0, tmp = sort(E0, tmp)
+        ep.sub_chunks = k
         for j in range(k - 1):
             ev = self._timed_begin(kt)
             check(self.lib.bemcs_electron_push_chunk(C.byref(E0.struct()), C.byref(ep), j, self.bound_e, s),
                   "electron_push_chunk")
             self._timed_end(kt, "electron_push", ev)
-            ev = self._timed_begin(kt)
-            check(self.lib.bemcs_sort_by_cell(C.byref(E0.struct()), C.byref(tmp.struct()), self.bound_e,
-                                              self.nx, self.ny, self.dx, self.dy, scratch, s), "sort_by_cell")
-            self._timed_end(kt, "electron_sort", ev)
-            E0, tmp = tmp, E0
+            E0, tmp = sort(E0, tmp)
         self.elec[0], self._e_spare = E0, tmp
         ep.sub_first = k - 1
         return E0
