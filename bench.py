@@ -3,19 +3,20 @@ one MI355X.
 
 Workload (default "c2"): two-grid Xe+ extraction (screen +1000 V, 0.38 mm,
 r 0.95 mm; accel -200 V, 0.5 mm, r 0.55 mm; 0.6 mm gap), 1024 x 384 nodes,
-8M macro-ions + 8M macro-electrons preloaded (synthetic, seed 0), Ti 0.2 eV,
-electrons Maxwellian 5 eV with m_e = M_Xe/200, uniform neutral Xe 1e18 m^-3
-at 300 K, log-linear CEX cross-section, dt 5e-10 s, particles sorted by cell
-every 20 steps (and right after the preload),
-MG-PCG Poisson to 1e-9 relative residual, 5 Picard iterations every 2nd step.
+8M macro-ions + 8M macro-electrons (synthetic, seed 0), Ti 0.2 eV, electrons
+Maxwellian 5 eV with m_e = M_Xe/200, uniform neutral Xe 1e18 m^-3 at 300 K,
+log-linear CEX cross-section, dt 5e-10 s, MG-PCG Poisson to 1e-9 relative
+residual, 5 Picard iterations every 2nd step.  The population is held at
+8M + 8M: after each step's beam injection and neutraliser the device tops
+both species up with preload-distributed particles (the reference's own
+injection at this mesh holds about 1M of each: the macro-weight rule caps the
+ion source and electron space charge expels electrons).  Ions are sorted by
+cell every 20 steps, electrons at the start of every step.
 
-Prints one JSON line.  The timed region is `--steps` steps (default 5000)
+Prints one JSON line.  The timed region is `--steps` steps (default 2000)
 after `--warmup` steps (default 50), bracketed by device synchronisation;
 `value` counts the particles pushed in each timed step (after injection)
-divided by the wall time.  The preloaded populations decay (ions leave
-through the grids, electrons are purged at electrodes and walls) towards the
-injection-fed steady state, so the particle count falls over the run; the
-ranges are reported in `config`.
+divided by the wall time; the pushed populations are reported in `config`.
 """
 import argparse
 import json
@@ -214,19 +215,32 @@ def algorithmic_bytes(name, n_ions, n_elec, n_nodes):
     return 0
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on the
-    default bench command; None when no summary is committed."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
+# MI355X vector issue rates per CU per clock (published vector peaks: FP64
+# 78.6 TFLOP/s, FP32 157.3 TFLOP/s at 2.4 GHz over 256 CUs)
+CU_CLOCKS_PER_S = 256 * 2.4e9
+VALU_F64_LANES_PER_CLK = 64
+VALU_B32_LANES_PER_CLK = 128
+
+
+def read_profile_value(name, key):
+    """A figure from a committed rocprofv3 summary under profiles/ (a line
+    `key: value`); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             for ln in fh:
-                if ln.startswith("traffic_bytes_per_launch:"):
-                    return float(ln.split(":", 1)[1])
+                if ln.startswith(key + ":"):
+                    return float(ln.split(":", 1)[1].split()[0])
     except OSError:
         pass
     return None
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on the
+    driver's bench command; None when no summary is committed."""
+    return read_profile_value(f"traffic_{kernel}_{workload}.txt", "traffic_bytes_per_launch")
 
 
 def _progress(msg):
@@ -301,14 +315,31 @@ def run(args):
     per_launch = kstats[dom]["algorithmic_bytes_per_launch"]
     achieved = per_launch / (kstats[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = read_traffic(dom, args.workload)
+    # second bound of the dominant kernel: its substep loop issues a fixed
+    # number of VALU instructions per particle-substep (SQ_INSTS_VALU per
+    # substep, from the committed counter summary of the same command)
+    vfile = f"valu_{dom}_{args.workload}.txt"
+    valu_per_sub = read_profile_value(vfile, "valu_instructions_per_substep")
+    v64 = read_profile_value(vfile, "valu_f64_per_substep")
+    v32 = read_profile_value(vfile, "valu_b32_per_substep")
+    sub_rate = kstats[dom].get("particle_substeps_per_s")
+    substep_roofline = None
+    if v64 and v32 and sub_rate:
+        peak_sub = CU_CLOCKS_PER_S / (v64 / VALU_F64_LANES_PER_CLK + v32 / VALU_B32_LANES_PER_CLK)
+        substep_roofline = {"bound": "valu", "kernel": dom, "achieved": sub_rate, "peak": peak_sub,
+                            "unit": "particle-substeps/s", "frac": sub_rate / peak_sub,
+                            "valu_instructions_per_substep": valu_per_sub,
+                            "valu_f64_per_substep": v64, "valu_b32_per_substep": v32,
+                            "peak_basis": "256 CUs x 2.4 GHz / (f64 ops / 64 + 32-bit ops / 128 lanes per clock)"}
     step_bytes = int(sum(BYTES_PER_ION * r[0] + BYTES_PER_ELECTRON * r[1] + BYTES_PER_NODE_STEP * n_nodes
                          for r in nin))
     step_achieved = step_bytes / (wall_ms * 1e-3) / 1e9
     rec = sim.record()
     labels = {
         "c2": ("particle-steps/s incl. Poisson at 1024x384",
-               "c2: two-grid Xe+ extraction, 1024x384 nodes, 8M ions + 8M electrons "
-               f"preloaded, steps {first}-{first + args.steps - 1}, dt 5e-10 s, "
+               "c2: two-grid Xe+ extraction, 1024x384 nodes, 8M ions + 8M electrons held (preload "
+               "topped up to 8M + 8M each step after injection, synthetic), "
+               f"steps {first}-{first + args.steps - 1}, dt 5e-10 s, "
                f"MG-PCG to {sim.poisson_rtol:g}, 5 Picard iterations every 2nd step"),
         "c1": ("particle-steps/s incl. Poisson at 256x96", "c1: 256x96 nodes, 1e5 + 1e5 preloaded"),
         "c5": ("particle-steps/s incl. Poisson, thermal, erosion and RF at 2048x768",
@@ -330,6 +361,7 @@ def run(args):
         "dtype": "f32 state, f64 gather/kick, f64 CG + f32 multigrid",
         "data": "synthetic (seeded preload, Philox streams)",
         "config": {"workload": labels[args.workload][1],
+                   "electron_sort": {"presort": sim.electron_presort, "chunk_substeps": sim.electron_chunk},
                    "nodes": n_nodes, "particles_pushed": pushed, "build_s": t_build,
                    "ions_range": kstats["ion_step"]["particles_range"],
                    "electrons_range": kstats["electron_step"]["particles_range"],
@@ -340,6 +372,7 @@ def run(args):
                    "remeshes": remeshes},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "substep_roofline": substep_roofline,
         "step_roofline": {"bytes_per_step": step_bytes / args.steps, "achieved": step_achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_achieved / HBM_PEAK_GBS},
         "kernels": kstats,
@@ -529,7 +562,7 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5"])
     ap.add_argument("--ion-sort-interval", type=int, default=None,
