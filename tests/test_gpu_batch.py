@@ -113,3 +113,68 @@ def test_cases_join_and_leave_between_steps():
             alone[c].advance(params[c], 1)
             _same(alone[c], together[c], (step, c))
     assert together[0].iteration == 7 and together[1].iteration == 6
+
+
+class _OracleCheckedBatch:
+    """Mixin for BatchedSimulator: before every batch step each case's oracle
+    is fed the case's state; after it the oracle steps too and the two are
+    compared (particles bit-exact on steps without a Poisson solve, within
+    1e-6 otherwise; potential within 1e-9 of SuperLU on solve steps)."""
+
+    def _check_init(self):
+        self.oracles = {}
+        self.checked = 0
+
+    def advance(self, n_steps):
+        from oracle.sim import OracleSimulator
+        from test_gpu_step import _compare_electrons, _compare_ions, _sync
+        for _ in range(int(n_steps)):
+            pairs = []
+            for sim, p in zip(self.sims, self.params):
+                o = self.oracles.get(id(sim))
+                if o is None:
+                    o = OracleSimulator()
+                    o.m_e = sim.m_e
+                    o.build_domain(p)
+                    self.oracles[id(sim)] = o
+                _sync(o, sim)
+                pairs.append((sim, o, p))
+            super().advance(1)
+            for sim, o, p in pairs:
+                o.step(p)
+                rec = sim.record()
+                even = sim.iteration % 2 == 0
+                assert rec["n_sub_ion"] == o.n_sub_ion and rec["n_sub_e"] == o.n_sub_e
+                if even:
+                    err = np.max(np.abs(sim.V - o.V)) / np.max(np.abs(o.V))
+                    assert err < 1e-9
+                _compare_ions(sim, o, exact=not even)
+                _compare_electrons(sim, o, exact=not even)
+                if not even:
+                    assert rec["entered"] == o.entered_optics_step
+                    assert rec["crossed"] == o.exit_count_step
+                    assert rec["lost_to_grid"] == o.lost_to_grid_step
+                    assert rec["mcc_events"] == int(np.count_nonzero(o.mcc_mask))
+                self.checked += 1
+
+
+def _checked_batch_class():
+    from thruster_cx_code_amd.batch import BatchedSimulator
+
+    class Checked(_OracleCheckedBatch, BatchedSimulator):
+        def __init__(self, *a, **kw):
+            BatchedSimulator.__init__(self, *a, **kw)
+            self._check_init()
+    return Checked
+
+
+def test_batched_steps_match_oracle():
+    """Three uneven cases (one periodic) advanced as one batch: every case's
+    step equals the oracle's step on the same state, odd and even steps."""
+    params = _cases()
+    sims = _build(params)
+    batch = _checked_batch_class()(sims, params)
+    for step in range(1, 5):
+        print(f"[gpu] batch vs oracle step {step}", flush=True)
+        batch.advance(1)
+    assert batch.checked == 12

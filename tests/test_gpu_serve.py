@@ -58,3 +58,20 @@ def test_admission_respects_memory_budget():
     print("[gpu] serve budget", flush=True)
     res, srv = _serve(reqs, hbm_budget=budget, max_active=8)
     assert sorted(res) == [0, 1, 2]
+
+
+def test_served_steps_match_oracle():
+    """Continuous batching: the second request arrives (in real time) while
+    the first runs and joins the batch; every served step of both is checked
+    against the oracle on the same state."""
+    from test_gpu_batch import _checked_batch_class
+    from thruster_cx_code_amd.serve import SweepServer, beam_request
+    reqs = [beam_request(0, (256, 96), -200.0, 0.6, seed=0), beam_request(1, (256, 96), -300.0, 0.7, seed=1)]
+    reqs[1].arrival = 0.05
+    srv = SweepServer(max_steps=6, min_steps=6, check_every=3, chunk=1, window=20, tol_deg=0.0)
+    srv.batch = _checked_batch_class()()
+    print("[gpu] served vs oracle", flush=True)
+    res = srv.run(reqs, realtime=True)
+    assert sorted(r.id for r in res) == [0, 1] and not any(r.error for r in res)
+    assert max(n for _, n, _ in srv.log) == 2  # both on the device together
+    assert srv.batch.checked >= 12  # steps past a stop step until the next completion check count too
