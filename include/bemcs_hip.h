@@ -547,6 +547,18 @@ typedef struct {
 int64_t bemcs_thermal_workspace_bytes(int32_t nx, int32_t ny, int32_t n_grids);
 int bemcs_thermal(const bemcs_thermal_t* p, bemcs_stream_t stream);
 
+/* bemcs_thermal without a workspace and without any allocation: p->workspace
+ * is ignored and the per-tile partial sums go to a 1 MiB scratch array the
+ * library holds in device memory (one per GPU, from the first call on that
+ * device until the process ends).  No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  Meshes with bemcs_thermal_workspace_bytes(nx, ny, n_grids) above
+ * 1048576 bytes are rejected with BEMCS_EINVAL (at 8 grids, up to 8192
+ * tiles of 64 x 32 nodes, e.g. 4096 x 4096).  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (different streams, no
+ * ordering between them) race and must not be issued. */
+int bemcs_thermal_noalloc(const bemcs_thermal_t* p, bemcs_stream_t stream);
+
 /* RF modulation of a grid potential (:984-991): on every node of grid rf_grid
  * (bit rf_grid of grid_bits), V_fixed = Vdc + v_rf, Vdc being the potential
  * of the last grid covering the node (grid_V: host array of n_grids values). */

@@ -187,6 +187,7 @@ _SIGS = {
                                C.c_float, i64, vp]),
     "bemcs_thermal_workspace_bytes": (i64, [i32, i32, i32]),
     "bemcs_thermal": (i32, [vp, vp]),
+    "bemcs_thermal_noalloc": (i32, [vp, vp]),
     "bemcs_rf_set": (i32, [vp, vp, i64, i32, vp, i32, f64, vp]),
     "bemcs_thermal_conduction": (i32, [vp, vp, vp, i32, i32, C.c_float, C.c_float, vp]),
     "bemcs_electrodes": (i32, [vp, vp]),

@@ -14,7 +14,18 @@
 //   k_electrodes  electrode map, Dirichlet values and grid masks of the grids,
 //                 bow included (:791-843)
 //   k_grid_sums   per-grid temperature sums (grid means at a domain build)
+#include <mutex>
+
 #include "bemcs_internal.h"
+
+// Scratch of bemcs_thermal_noalloc: the per-tile partial sums of the grid
+// means, in a file-scope device array instead of a caller workspace, for
+// callers that cannot allocate.  Each GPU holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in, not the device
+// array: host code uses only the address hipGetSymbolAddress returns for it on
+// each device (thermal_scratch below).
+constexpr int64_t kThermalScratchBytes = 1 << 20;
+static __device__ __align__(256) unsigned char scratch[kThermalScratchBytes];
 
 namespace bemcs {
 namespace {
@@ -493,6 +504,42 @@ int bemcs_electrodes(const bemcs_electrodes_t* p, bemcs_stream_t stream) {
   hipLaunchKernelGGL(k_electrodes, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *p);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static int thermal_scratch(void** out) {
+  int dev = 0;
+  BEMCS_CHECK_HIP(hipGetDevice(&dev));
+  BEMCS_REQUIRE(dev >= 0 && dev < kMaxDevices, "bemcs_thermal_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    BEMCS_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return BEMCS_OK;
+}
+
+int bemcs_thermal_noalloc(const bemcs_thermal_t* p, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(p, "bemcs_thermal_noalloc: null argument");
+  const int64_t need = bemcs_thermal_workspace_bytes(p->nx, p->ny, p->n_grids);
+  BEMCS_REQUIRE(need <= kThermalScratchBytes,
+                "bemcs_thermal_noalloc: mesh too large for the 1 MiB scratch "
+                "(bemcs_thermal_workspace_bytes(nx, ny, n_grids) must be <= 1048576)");
+  void* ws = nullptr;
+  const int rc = thermal_scratch(&ws);
+  if (rc != BEMCS_OK) return rc;
+  bemcs_thermal_t q = *p;
+  q.workspace = ws;
+  return bemcs_thermal(&q, stream);
 }
 
 int bemcs_grid_means(const float* T, const uint8_t* grid_bits, int32_t nx, int32_t ny, int32_t n_grids,
