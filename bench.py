@@ -30,6 +30,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WINDOW = 4000  # timed steps per synchronised window (below the per-step record ring)
 
 
 def workload_params(name, ions=None, electrons=None):
@@ -271,26 +272,41 @@ def run(args):
     torch.cuda.synchronize()
     _progress(f"warmup {args.warmup} steps done")
     first = sim.iteration + 1
-    sim.kernel_timing = {}
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    sim.advance(p, args.steps)
-    end.record()
-    torch.cuda.synchronize()
-    wall_ms = start.elapsed_time(end)
-    _progress(f"timed {args.steps} steps: {wall_ms:.1f} ms")
     from thruster_cx_code_amd.physics_engine import RING
-    slots = [(first + k) % RING for k in range(args.steps)]
-    nin = sim.ring_nin.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
-    nsub = sim.ring_nsub.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
+    # the timed steps run in windows of at most WINDOW steps (the per-step
+    # records live in a ring of RING slots); each window is bracketed by
+    # synchronisation and timed with events, and its records and kernel
+    # timings are read back between windows, outside the timed time
+    wall_ms = 0.0
+    nin_l, nsub_l, remesh_l, durs = [], [], [], {}
+    done = 0
+    while done < args.steps:
+        k = min(WINDOW, args.steps - done)
+        sim.kernel_timing = {}
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = sim.iteration + 1
+        start.record()
+        sim.advance(p, k)
+        end.record()
+        torch.cuda.synchronize()
+        wall_ms += start.elapsed_time(end)
+        slots = [(w0 + j) % RING for j in range(k)]
+        nin_l.append(sim.ring_nin.view(RING, 2).cpu().numpy()[slots].astype(np.int64))
+        nsub_l.append(sim.ring_nsub.view(RING, 2).cpu().numpy()[slots].astype(np.int64))
+        remesh_l.append(sim.ring_remeshed[slots].copy())
+        for name, evs in sim.kernel_timing.items():
+            durs.setdefault(name, []).extend(a.elapsed_time(b) for a, b in evs)
+        done += k
+        _progress(f"timed {done} of {args.steps} steps: {wall_ms:.1f} ms")
+    nin = np.concatenate(nin_l)
+    nsub = np.concatenate(nsub_l)
     pushed = int(nin.sum())
     value = pushed / (wall_ms * 1e-3)
     n_nodes = sim.nx * sim.ny
     kstats = {}
-    for name, evs in sim.kernel_timing.items():
-        durs = [a.elapsed_time(b) for a, b in evs]
-        kstats[name] = {"launches": len(durs), "total_ms": float(sum(durs)), "avg_ms": float(np.mean(durs))}
+    for name, d in durs.items():
+        kstats[name] = {"launches": len(d), "total_ms": float(sum(d)), "avg_ms": float(np.mean(d))}
     for name, col in (("ion_step", 0), ("electron_step", 1)):
         if name in kstats:
             subs = int((nin[:, col] * nsub[:, col]).sum())
@@ -308,7 +324,7 @@ def run(args):
         kstats["pcg_iterations_per_picard_call"] = {
             "calls": n_calls, "min": int(its.min()), "median": float(np.median(its)),
             "max": int(its.max()), "total": int(its.sum())}
-    remeshes = int(sim.ring_remeshed[slots].sum())
+    remeshes = int(np.concatenate(remesh_l).sum())
     # dominant single kernel: the fused particle steps (one launch each); the
     # Poisson call is a sequence of ~30 launches per PCG iteration
     dom = max(("ion_step", "electron_step"), key=lambda k: kstats[k]["total_ms"])
@@ -495,10 +511,19 @@ def serve_bench(args):
     import torch
     from thruster_cx_code_amd.serve import SweepServer, latency_summary, poisson_stream
     reqs = poisson_stream(args.serve_requests, args.serve_rate, seed=0)
-    srv = SweepServer(max_steps=args.serve_max_steps, min_steps=int(0.75 * args.serve_max_steps),
-                      max_active=args.serve_slots)
+    # the reference sweep's rule: at least 1500 steps, then stop when the last
+    # 50 theta95 values have a standard deviation below 0.1 degree
+    # (Python/benchmarks/benchmark_perveance.py:56-60), capped at max_steps
+    min_steps = min(1500, args.serve_max_steps)
+    srv = SweepServer(max_steps=args.serve_max_steps, min_steps=min_steps, max_active=args.serve_slots)
     t0 = time.perf_counter()
-    res = srv.run(reqs, realtime=True)
+    last = [0.0]
+
+    def progress(now, active, admitted, done):
+        if now - last[0] >= 20.0:
+            last[0] = now
+            _progress(f"serve: {now:.0f} s, {done} done, {active} active, {admitted} admitted")
+    res = srv.run(reqs, realtime=True, progress=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     summ = latency_summary(res, wall)
@@ -509,11 +534,23 @@ def serve_bench(args):
         "vs_baseline": None, "data": "synthetic (Poisson arrivals, seeded cases)",
         "config": {"workload": f"{args.serve_requests} requests at {args.serve_rate}/s, meshes 256x96/512x192/"
                                f"1024x384, Va ~ U[-400,-100] V, I ~ U[0.5,3] mA, steady-state rule after "
-                               f"{int(0.75 * args.serve_max_steps)} steps, cap {args.serve_max_steps} steps",
+                               f"{min_steps} steps, cap {args.serve_max_steps} steps",
                    "slots": args.serve_slots},
         "latency": summ, "wall_s": wall,
         "max_active": max((a for _, a, _ in srv.log), default=0),
+        "capacity": capacity(res, wall, args.serve_rate),
     }))
+
+
+def capacity(results, wall, rate):
+    """Device throughput in case-steps/s over the run and the request rate it
+    sustains at the run's mean steps per request, against the arrival rate."""
+    ok = [r for r in results if not r.error]
+    steps = sum(int(r.steps) for r in ok)
+    mean_steps = steps / len(ok) if ok else float("nan")
+    cs = steps / wall
+    return {"case_steps": steps, "case_steps_per_s": cs, "mean_steps_per_request": mean_steps,
+            "sustainable_req_per_s": cs / mean_steps if ok else float("nan"), "arrival_req_per_s": rate}
 
 
 def copy_calibration(mbytes=512, reps=20):
@@ -539,9 +576,9 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--serve":
         sp = argparse.ArgumentParser()
         sp.add_argument("--serve", action="store_true")
-        sp.add_argument("--serve-requests", type=int, default=64)
+        sp.add_argument("--serve-requests", type=int, default=512)
         sp.add_argument("--serve-rate", type=float, default=4.0)
-        sp.add_argument("--serve-max-steps", type=int, default=1000)
+        sp.add_argument("--serve-max-steps", type=int, default=4000)
         sp.add_argument("--serve-slots", type=int, default=64)
         serve_bench(sp.parse_args())
         return
@@ -577,8 +614,6 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time step shards "
               f"across GPUs by case or by particles, which this build does not run.", file=sys.stderr)
         sys.exit(2)
-    if args.steps > 8000:
-        sys.exit("--steps must be <= 8000 (per-step records ring)")
     run(args)
 
 

@@ -818,8 +818,11 @@ class DigitalTwinSimulator:
         check(lib.bemcs_percentile(ptr(self.crossed_angle), C.c_void_p(diag_ptr + 4), self.bound_i, 95.0, 5,
                                    None, C.c_void_p(self.ring_theta.data_ptr() + 8 * slot), s), "percentile")
         self._step_end(slot)
+        # electrons of a population the next step presorts need no sort here
+        presorted = self.electron_presort and self.bound_e >= self.electron_chunk_min
         self._sort(ions=self.sort_interval > 0 and it % self.sort_interval == 0,
-                   electrons=self.electron_sort_interval > 0 and it % self.electron_sort_interval == 0)
+                   electrons=(not presorted and self.electron_sort_interval > 0
+                              and it % self.electron_sort_interval == 0))
         # the step's counts reach the pinned ring asynchronously; the event
         # tells the host bounds when
         ev = self._readback_event()

@@ -157,11 +157,13 @@ class SweepServer:
         case.checked_it = last
         return case.stop_step is not None
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, progress=None):
         """Serve `requests` (sorted by arrival).  realtime=True admits a
         request once its arrival time has passed on the wall clock; False
-        admits every request immediately (for equality tests).  Returns the
-        results in completion order."""
+        admits every request immediately (for equality tests).  progress, if
+        given, is called after every completion check with (seconds since
+        start, active cases, requests admitted or rejected, results).
+        Returns the results in completion order."""
         queue = sorted(requests, key=lambda r: r.arrival)
         active, results = [], []
         t0 = time.perf_counter()
@@ -204,6 +206,8 @@ class SweepServer:
                                           c.req.arrival, c.admitted, now))
                     del c.sim
                 self.log.append((now, len(active), qi))
+                if progress is not None:
+                    progress(now, len(active), qi, len(results))
         return results
 
 
