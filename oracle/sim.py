@@ -641,9 +641,13 @@ class OracleSimulator:
             self.exit_vx_std = float(np.std(vxe))
             self.exit_v_std = float(np.std(ve))
             self.exit_energy_mean_eV = float(np.mean(0.5 * self.m_ion * ve ** 2 / self.q))
+            cex = self.p_isCEX[crossed]
+            self.exit_v_mean_primary = float(np.mean(ve[~cex])) if np.any(~cex) else np.nan
+            self.exit_v_mean_cex = float(np.mean(ve[cex])) if np.any(cex) else np.nan
         else:
             self.exit_vx_mean = self.exit_v_mean = self.exit_vx_std = np.nan
             self.exit_v_std = self.exit_energy_mean_eV = np.nan
+            self.exit_v_mean_primary = self.exit_v_mean_cex = np.nan
         self.crossed_angles = (np.abs(np.arctan2(p_vy[crossed], p_vx[crossed])) * 180.0 / np.pi)
         self.current_div = (float(np.percentile(self.crossed_angles, 95)) if nc > 5 else np.nan)
         if np.any(exited):

@@ -304,3 +304,45 @@ def test_electron_step_variants_identical():
         for a, b in zip(out[0][1], o[1]):
             assert np.array_equal(a, b)
         assert np.array_equal(out[0][2], o[2])
+
+
+def test_exit_statistics_and_energy_vs_reference_formulas():
+    """Exit statistics (:1451-1480) and the energy budget (:1800-1866).  The
+    device sums the exit moments in float64 with atomics (the last bits depend
+    on the order of the adds) and forms the standard deviations from the first
+    two moments; the reference takes np.mean / np.std of float32 arrays, which
+    NumPy evaluates in float32: they agree to 2e-6 relative (float32
+    rounding of the reference's results).  get_total_energy sums in
+    float64 on the device; the reference squares and sums float32 arrays
+    (pairwise float32 summation): they agree to 1e-5 relative, and the device
+    sum matches an exact float64 sum of the same values to 1e-12."""
+    import math
+    from ensemble import case_params
+    p = case_params(3)
+    p["n0"] = 3e21
+    g, o = _mk(p)
+    seen = 0
+    for _ in range(5):
+        _sync(o, g)
+        g.step(p)
+        o.step(p)
+        rec = g.record()
+        if g.iteration % 2 == 1 and rec["crossed"] > 0:
+            seen += 1
+            for k in ("exit_vx_mean", "exit_v_mean", "exit_vx_std", "exit_v_std", "exit_energy_mean_eV",
+                      "exit_v_mean_primary", "exit_v_mean_cex"):
+                a, b = getattr(g, k), getattr(o, k)
+                assert (math.isnan(a) and math.isnan(b)) or abs(a - b) <= 2e-6 * abs(b), (k, a, b)
+    assert seen >= 2
+    e = g.get_total_energy()
+    v2 = g.p_vx ** 2 + g.p_vy ** 2 + g.p_vz ** 2
+    ke_i32 = float(np.sum(v2)) * 0.5 * g.m_ion * g.macro_weight
+    v2_64 = sum(a.astype(np.float64) ** 2 for a in (g.p_vx, g.p_vy, g.p_vz))
+    ke_i64 = math.fsum(v2_64) * 0.5 * g.m_ion * g.macro_weight
+    ve2 = g.e_vx ** 2 + g.e_vy ** 2 + g.e_vz ** 2
+    ke_e32 = float(np.sum(ve2)) * 0.5 * g.m_e * g.macro_weight
+    f32 = float(np.sum(g.Ex ** 2 + g.Ey ** 2)) * 0.5 * g.eps0 * (g.dx * 1e-3) * (g.dy * 1e-3)
+    assert abs(e["ke_ions_J"] - ke_i32) <= 1e-5 * ke_i32
+    assert abs(e["ke_ions_J"] - ke_i64) <= 1e-12 * ke_i64
+    assert abs(e["ke_elec_J"] - ke_e32) <= 1e-5 * ke_e32
+    assert abs(e["field_J"] - f32) <= 1e-5 * f32
