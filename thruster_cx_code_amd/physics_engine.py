@@ -9,14 +9,21 @@ mask_grids`` (host copies made on access).  ``advance(params, n)`` runs n steps
 without any host synchronisation for throughput use.
 
 Per step on the device (one HIP stream):
-  inject (counts drawn on the host from the keyed Philox stream)
+  inject (counts drawn on the host from the keyed Philox stream), then the
+  population top-up when preload {"hold": true}
   -> Poisson every 2nd step (MG-PCG Picard loop, E field)
-  -> substep counts -> fused ion kernel and fused electron kernel
+  -> substep counts -> fused ion kernel
+  -> electron cell sort (large populations), optional substep chunks with
+     sorts between them, fused electron kernel
   -> PPC statistics, theta95 percentile, saddle potential into a ring of
-     per-step records -> periodic cell sort.
-The random streams, the CEX probability in float64 and the additive
-``params`` keys (mesh, preload, seed, neutral_model, cx_model) are described
-in DESIGN.md-independent terms in the module docstrings of the kernels.
+     per-step records -> periodic cell sort of the ions.
+Additive ``params`` keys beside the reference's: mesh {"nx", "ny"},
+preload {"ions", "electrons", "hold"}, seed, neutral_model in {"roy",
+"uniform"}, cx_model in {"reference", "loglinear"}, deposit_shape in
+{"ngp", "cic"}.  Random draws come from Philox streams keyed by (seed,
+purpose) with counters (particle id, step, block), and the CEX probability and
+SEE yield are evaluated in float64 with deterministic exp/log, so event masks
+do not depend on thread order.
 """
 import ctypes as C
 import math
