@@ -149,6 +149,123 @@ def cpu_worker_composed(workload, n_sub_i, n_sub_e):
                       "n_sub": [n_sub_i, n_sub_e], "build_s": t_build}))
 
 
+def cpu_worker_sweep(path):
+    """CPU time of a sweep's cases on the NumPy oracle (reference dtype
+    semantics, SciPy SuperLU), one core, composed per case from measured
+    pieces and the populations the GPU run recorded:
+        t_case = t_build(m) + c_pre N_pre + steps t0(m) + a S + b P + (steps / 2) t_picard(m)
+    where, for the case's mesh m, t_build(m) and t0(m) (one odd step with
+    only the step's own injections) and t_picard(m) (a 5-iteration Picard
+    call) are measured on that mesh; b (one push substep per particle) and a
+    (the rest of a step per particle: deposit, masks, purge, CEX, PPC) are
+    measured once with 2e5 + 2e5 particles, c_pre from the same build; S is
+    the sum over steps of the pushed particles and P of the pushed particles
+    times their substep counts, both from the GPU's per-step records."""
+    from oracle import kernels as K
+    from oracle.sim import OracleSimulator
+    spec = json.load(open(path))
+    cases, stats = spec["cases"], spec["stats"]
+
+    class OneSubstep(OracleSimulator):
+        """Runs the push of each species once per step (timing only)."""
+        def step(self, params):
+            real = K.push_boris
+            calls = {}
+
+            def once(x, *a, **kw):
+                if id(x) not in calls:
+                    calls[id(x)] = True
+                    real(x, *a, **kw)
+            import oracle.sim as osim
+            osim.push_boris = once
+            try:
+                return super().step(params)
+            finally:
+                osim.push_boris = real
+
+    def build(params):
+        o = OneSubstep(mode="ref32", mcc_math="ref32")
+        o.m_e = o.m_ion / 200.0
+        t0 = time.perf_counter()
+        o.build_domain(params)
+        return o, time.perf_counter() - t0
+
+    def empty(params):
+        q = json.loads(json.dumps(params))
+        q["preload"] = {"ions": 0, "electrons": 0}
+        return q
+
+    per_mesh = {}
+    for p in cases:
+        key = (p["mesh"]["nx"], p["mesh"]["ny"])
+        if key in per_mesh:
+            continue
+        q = empty(p)
+        o, t_build = build(q)
+        t0 = time.perf_counter()
+        o.step(q)
+        t_odd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.recalc_poisson(5, q)
+        per_mesh[key] = {"t_build": t_build, "t0": t_odd, "t_picard": time.perf_counter() - t0}
+    ref = sorted(per_mesh)[len(per_mesh) // 2]
+    q = empty(next(p for p in cases if (p["mesh"]["nx"], p["mesh"]["ny"]) == ref))
+    nc = 200_000
+    q["preload"] = {"ions": nc, "electrons": nc}
+    o, t_build = build(q)
+    t0 = time.perf_counter()
+    o.step(q)
+    t_odd = time.perf_counter() - t0
+    pushed = o.pushed
+    t0 = time.perf_counter()
+    K.push_boris(o.e_x, o.e_y, o.e_vx, o.e_vy, o.e_vz, o.Ex, o.Ey, o.Bx, o.By, o.Bz, o.num_e, o.dx, o.dy,
+                 o.nx, o.ny, o.dt, -o.q / o.m_e)
+    K.push_boris(o.p_x, o.p_y, o.p_vx, o.p_vy, o.p_vz, o.Ex, o.Ey, o.Bx, o.By, o.Bz, o.num_p, o.dx, o.dy,
+                 o.nx, o.ny, o.dt, o.q_ion / o.m_ion)
+    b = (time.perf_counter() - t0) / max(o.num_e + o.num_p, 1)
+    a = max((t_odd - per_mesh[ref]["t0"] - b * pushed) / max(pushed, 1), 0.0)
+    c_pre = max((t_build - per_mesh[ref]["t_build"]) / (2 * nc), 0.0)
+    total = 0.0
+    per_case = []
+    for p, st in zip(cases, stats):
+        m = per_mesh[(p["mesh"]["nx"], p["mesh"]["ny"])]
+        n_pre = p["preload"]["ions"] + p["preload"]["electrons"]
+        t = (m["t_build"] + c_pre * n_pre + st["steps"] * m["t0"] + a * st["sum_n"] + b * st["sum_n_sub"]
+             + (st["steps"] // 2) * m["t_picard"])
+        per_case.append(t)
+        total += t
+    print(json.dumps({"total_s": total, "per_case_s": per_case, "a_s": a, "b_s": b, "c_pre_s": c_pre,
+                      "per_mesh": {f"{k[0]}x{k[1]}": v for k, v in per_mesh.items()}}))
+
+
+def cpu_sweep_baseline(cases, stats):
+    """Cases/hr of the sweep on the host's cores: the one-core composed CPU
+    time of every case (cpu_worker_sweep), spread over P = the cores this
+    process may use (at most 16, the host share of one GPU) running one case
+    each, as the reference's sweep runs one process per case.  P processes
+    at the one-core rate is an upper bound for the CPU."""
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+        json.dump({"cases": cases, "stats": stats}, f)
+        path = f.name
+    try:
+        rec = _run_cpu(["--cpu-sweep", path], 3600)
+    finally:
+        os.unlink(path)
+    P = min(len(os.sched_getaffinity(0)), 16)
+    value = len(cases) * 3600.0 / (rec["total_s"] / P)
+    return {
+        "value": value, "unit": "cases/hr", "cores": P, "kind": "port",
+        "sample": (f"NumPy oracle (reference dtype semantics, SciPy SuperLU) on {_cpu_model()}: per-case "
+                   f"one-core time composed from measured pieces (per mesh: build, an odd step with only "
+                   f"its injections, a 5-iteration Picard call; per particle: {rec['b_s'] * 1e9:.0f} ns "
+                   f"per push substep and {rec['a_s'] * 1e9:.0f} ns per step) and the populations and "
+                   f"substep counts the GPU recorded at every step; {rec['total_s'] / 3600:.1f} core-hours "
+                   f"for all cases, spread over {P} processes"),
+        "breakdown": rec,
+    }
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -434,13 +551,16 @@ def poisson_bench(calls=10):
 def sweep_bench(args):
     """Sweep throughput (cases/hr): `--sweep-cases` uneven two-grid cases
     (thruster_cx_code_amd.sweep.uneven_sweep), `--sweep-steps` steps each,
-    advanced as one batch with the sort-interval autotuner, against the same
+    advanced as one batch with the autotuner (sort intervals and deposit tile), against the same
     GPU running the cases one at a time (timed on every k-th case and scaled to
-    all cases).  Build (geometry, preload, initial Poisson) is included."""
+    all cases).  Build (geometry, preload, initial Poisson) is included.  The
+    CPU baseline (cpu_sweep_baseline) composes each case's one-core NumPy
+    time from the populations this run recorded."""
     import gc
+    import numpy as np
     import torch
     from thruster_cx_code_amd.batch import BatchedSimulator
-    from thruster_cx_code_amd.sweep import SortTuner, build_case, uneven_sweep
+    from thruster_cx_code_amd.sweep import Autotuner, build_case, uneven_sweep
     cases = uneven_sweep(args.sweep_cases, seed=0)
     n = len(cases)
     torch.cuda.synchronize()
@@ -450,7 +570,7 @@ def sweep_bench(args):
     t_build = time.perf_counter() - t0
     _progress(f"built {n} cases in {t_build:.1f} s")
     batch = BatchedSimulator(sims, cases)
-    tuner = SortTuner(batch)
+    tuner = Autotuner(batch)
     t0 = time.perf_counter()
     done = 0
     while done < args.sweep_steps:
@@ -461,6 +581,14 @@ def sweep_bench(args):
         _progress(f"batch: {done} steps, {time.perf_counter() - t0:.1f} s")
     t_batch = time.perf_counter() - t0
     final = [(s.record()["num_p"], s.record()["num_e"]) for s in sims]
+    # per-case populations and substep counts of every step, for the CPU model
+    from thruster_cx_code_amd.physics_engine import RING
+    stats = []
+    for s in sims:
+        slots = [k % RING for k in range(1, s.iteration + 1)]
+        nin = s.ring_nin.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
+        nsub = s.ring_nsub.view(RING, 2).cpu().numpy()[slots].astype(np.int64)
+        stats.append({"steps": int(s.iteration), "sum_n": int(nin.sum()), "sum_n_sub": int((nin * nsub).sum())})
     tuning = tuner.history
     del tuner, batch, sims
     gc.collect()
@@ -492,12 +620,15 @@ def sweep_bench(args):
                                "Vs ~ U[800,1600] V, particles per species log-uniform [2e5, 4e6]",
                    "cases": n, "steps": args.sweep_steps},
         "batched": {"cases_per_hr": batched_hr, "build_s": t_build, "steps_s": t_batch,
-                    "sort_tuning": tuning},
+                    "autotuning": tuning},
         "one_at_a_time": {"cases_per_hr": seq_hr, "timed_cases": sample, "timed_s": t_seq_sample,
                           "scaled_s": t_seq},
         "speedup": batched_hr / seq_hr,
         "final_counts_first_cases": final[:4],
     }
+    if not args.no_cpu_baseline:
+        _progress("cpu baseline (composed per case)")
+        out["cpu_baseline"] = cpu_sweep_baseline(cases, stats)
     print(json.dumps(out))
 
 
@@ -588,7 +719,11 @@ def main():
         sp.add_argument("--sweep-cases", type=int, default=64)
         sp.add_argument("--sweep-steps", type=int, default=3000)
         sp.add_argument("--sweep-seq-cases", type=int, default=16)
+        sp.add_argument("--no-cpu-baseline", action="store_true")
         sweep_bench(sp.parse_args())
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-sweep":
+        cpu_worker_sweep(sys.argv[2])
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-composed":
         cpu_worker_composed(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 5
+#define BEMCS_ABI_VERSION 6
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -240,6 +240,15 @@ typedef struct {
   const double* see_c;
   double cx_lo, cx_hi, see_lo, see_hi;
   int32_t cx_n, see_n;
+  /* LDS window of the NGP and PPC count histograms (a work-group counts its
+   * particles' nodes in a window of W x H nodes anchored at their smallest
+   * (ix, iy) and adds each touched node to the global counts once; nodes
+   * outside the window go straight to global atomics): 0 = 64 x 32, else
+   * log2(W) | log2(H) << 8 | 1 << 16 with W x H <= 2048.  Counts are
+   * integers, so the window changes no result, only the number of global
+   * atomics. */
+  int32_t count_window;
+  int32_t pad_;
 } bemcs_ion_step_t;
 
 int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
@@ -268,7 +277,7 @@ typedef struct {
   int32_t nx, ny, periodic_y;
   int32_t sub_first;         /* first chunk the fused step runs (0: all substeps) */
   int32_t sub_chunks;        /* K (<= 1: no split) */
-  int32_t pad_;
+  int32_t count_window;      /* LDS window of the NGP counts, as in bemcs_ion_step_t */
 } bemcs_electron_step_t;
 
 int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,

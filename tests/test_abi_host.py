@@ -47,7 +47,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(_lib.Species) == 7 * 8 + 8
     assert C.sizeof(_lib.IonDiag) == 10 * 4 + 7 * 8
     assert C.sizeof(_lib.IonStep) == (8 * 8 + 11 * 8 + 8 * 4 + 8 * 4 + 2 * 8 + C.sizeof(_lib.Species) + 8 * 4
-                                      + 4 * 8 + 4 * 8 + 2 * 4)
+                                      + 4 * 8 + 4 * 8 + 2 * 4 + 2 * 4)
     assert C.sizeof(_lib.ElectronStep) == 5 * 8 + 3 * 8 + 4 * 4 + 6 * 4
     assert C.sizeof(_lib.Inject) == 2 * 8 + 4 * 8 + 5 * 8 + 2 * 4 + 9 * 4 + 4 + 3 * 8
     assert C.sizeof(_lib.Picard) == 6 * 8 + 11 * 8 + 2 * 4 + 2 * 8
@@ -127,3 +127,35 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         _lib.load()
+
+
+def test_count_window_codes_and_rejection(lib):
+    """The count-window code packs log2 width | log2 height << 8 | 1 << 16; windows
+    that are not powers of two or exceed 2048 nodes are refused by the host
+    code and by the entry points (before anything reaches the device)."""
+    from thruster_cx_code_amd.physics_engine import count_window_code
+    assert count_window_code(64, 32) == 6 | 5 << 8 | 1 << 16
+    assert count_window_code(512, 4) == 9 | 2 << 8 | 1 << 16
+    assert count_window_code(1, 1) == 1 << 16
+    for bad in ((48, 32), (64, 64), (4096, 1), (0, 8)):
+        with pytest.raises(ValueError):
+            count_window_code(*bad)
+    fake = C.c_void_p(256)  # never dereferenced: validation fails first
+    sp = _lib.Species()
+    for name in ("x", "y", "vx", "vy", "vz", "tag", "count"):
+        setattr(sp, name, fake)
+    sp.cap = 1000
+    ip = _lib.IonStep()
+    for name in ("Exy", "is_bound", "n_sub", "cnt_next", "ppc", "crossed_angle", "diag", "e_cnt_next"):
+        setattr(ip, name, fake)
+    ip.nx, ip.ny = 16, 16
+    ip.count_window = 6 | 6 << 8 | 1 << 16  # 64 x 64 = 4096 nodes
+    rc = lib.bemcs_ion_step(C.byref(sp), C.byref(sp), C.byref(sp), C.byref(ip), 10, None)
+    assert rc == _lib.EINVAL and b"count_window" in lib.bemcs_last_error()
+    ep = _lib.ElectronStep()
+    for name in ("Exy", "is_bound", "n_sub", "cnt_next", "overflow"):
+        setattr(ep, name, fake)
+    ep.nx, ep.ny = 16, 16
+    ep.count_window = 6 | 5 << 8  # no explicit flag
+    rc = lib.bemcs_electron_step(C.byref(sp), C.byref(sp), C.byref(ep), 10, None)
+    assert rc == _lib.EINVAL and b"count_window" in lib.bemcs_last_error()

@@ -91,6 +91,29 @@ def _same(a, b, where):
     assert np.array_equal(a.V.view(np.uint64), b.V.view(np.uint64)), where
 
 
+def test_autotuned_batch_matches_untuned_batch():
+    """The autotuner changes sort intervals and the count window of every
+    case between steps; the results equal those of the same batch run with
+    the defaults (sorting and the window change no result)."""
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    from thruster_cx_code_amd.sweep import Autotuner
+    params = _cases()
+    plain = _build(params)
+    tuned = _build(params)
+    b_plain = BatchedSimulator(plain, params)
+    b_tuned = BatchedSimulator(tuned, params)
+    tuner = Autotuner(b_tuned, ion_candidates=(1, 3), electron_candidates=(1, 2),
+                      window_candidates=((64, 32), (512, 4), (1, 1), (32, 16)), trial=2, every=10)
+    print("[gpu] autotuned batch", flush=True)
+    tuner.advance(60)  # tunes of 16 steps with 10 steps between them: two tunes
+    b_plain.advance(60)
+    assert len(tuner.history) == 2
+    assert {tuple(h["count_window"]) for h in tuner.history} <= {(64, 32), (512, 4), (1, 1), (32, 16)}
+    for c, (a, b) in enumerate(zip(plain, tuned)):
+        _same(a, b, c)
+        assert torch.equal(a.cnt_i, b.cnt_i) and torch.equal(a.cnt_e, b.cnt_e), c
+
+
 def test_cases_join_and_leave_between_steps():
     """A case admitted later (its own iteration counter, opposite Poisson
     parity) and a case retired midway match their alone twins."""

@@ -282,6 +282,22 @@ def test_electron_chunks_match_oracle():
     assert len(g.kernel_timing.get("electron_sort", [])) > 0
 
 
+@pytest.mark.parametrize("window", [(1, 1), (512, 4), (32, 16), (2048, 1)])
+def test_count_windows_match_oracle(window):
+    """Any LDS count window (one node, wide and flat, small, one row) gives
+    the oracle's NGP and PPC counts and particles, step for step."""
+    p = c1_params(nx=128, ny=48, ions=20000, electrons=20000)
+    g, o = _mk(p)
+    g.count_window = window
+    assert g.count_window == window
+    print(f"[gpu] count window {window}", flush=True)
+    _steps_vs_oracle(p, g, o, 3)
+    ci = K.deposit_counts(o.p_x, o.p_y, o.dx, o.dy, o.nx, o.ny)
+    ce = K.deposit_counts(o.e_x, o.e_y, o.dx, o.dy, o.nx, o.ny)
+    assert np.array_equal(g.cnt_i.cpu().numpy().astype(np.uint32), ci)
+    assert np.array_equal(g.cnt_e.cpu().numpy().astype(np.uint32), ce)
+
+
 def test_electron_step_variants_identical():
     """The electron step's execution variants agree bit for bit over several
     steps (potential, ions by id, electrons as a multiset): substeps in one

@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -49,7 +49,7 @@ class IonStep(C.Structure):
                 ("eth32", f32), ("ycoef32", f32), ("see_defer", i32),
                 ("cx_t", vp), ("cx_c", vp), ("see_t", vp), ("see_c", vp),
                 ("cx_lo", f64), ("cx_hi", f64), ("see_lo", f64), ("see_hi", f64),
-                ("cx_n", i32), ("see_n", i32)]
+                ("cx_n", i32), ("see_n", i32), ("count_window", i32), ("pad_", i32)]
 
 
 class ElectronStep(C.Structure):
@@ -57,7 +57,7 @@ class ElectronStep(C.Structure):
                 ("dt", f64), ("q_m", f64), ("Ly", f64),
                 ("dx32", f32), ("dy32", f32), ("Lx32", f32), ("Ly32", f32),
                 ("nx", i32), ("ny", i32), ("periodic_y", i32), ("sub_first", i32), ("sub_chunks", i32),
-                ("pad_", i32)]
+                ("count_window", i32)]
 
 
 class Inject(C.Structure):

@@ -27,20 +27,36 @@ constexpr int kBlock = 256;
 
 // ---------------------------------------------------- LDS-privatised counts
 // Per-tile privatisation of the NGP and PPC count histograms: a work-group
-// counts its particles' nodes in an LDS window of kWinX x kWinY nodes anchored
-// at its smallest (ix, iy) (cell-sorted particles of one work-group lie in a
-// few cells), with runs of equal nodes in a wavefront merged before the LDS
+// counts its particles' nodes in an LDS window of W x H nodes anchored at its
+// smallest (ix, iy) (cell-sorted particles of one work-group lie in a few
+// cells), with runs of equal nodes in a wavefront merged before the LDS
 // atomics, then adds each touched window node to the global histogram with one
 // atomic.  Nodes outside the window go straight to global atomics.  Integer
-// counts, so the totals do not depend on the order of the adds.
-constexpr int kWinX = 64, kWinY = 32;
+// counts, so the totals do not depend on the order of the adds or on the
+// window shape, which the caller picks (count_window: 0 = 64 x 32, else
+// log2 W | log2 H << 8 | 1 << 16, W x H <= kWinBins; anything else falls back
+// to 64 x 32).
+constexpr int kWinLog2 = 11, kWinBins = 1 << kWinLog2;
 struct LdsHist {
-  uint32_t bins[kWinX * kWinY];
+  uint32_t bins[kWinBins];
   int mins[2][kBlock / 64];
 };
+struct CountWindow {
+  int xs, ys;  // log2 of the window's width and height
+};
+__host__ __device__ __forceinline__ bool count_window_valid(int32_t w) {
+  const int xs = w & 0xff, ys = (w >> 8) & 0xff;
+  return w == 0 || ((w >> 16) == 1 && xs + ys <= kWinLog2);
+}
+__device__ __forceinline__ CountWindow count_window(int32_t w) {
+  if (w == 0 || !count_window_valid(w)) return CountWindow{6, 5};
+  return CountWindow{w & 0xff, (w >> 8) & 0xff};
+}
 
 // Every thread of the work-group must call it; ix < 0 adds nothing.
-__device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int nx, int ix, int iy) {
+__device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int nx, int ix, int iy,
+                                               CountWindow win) {
+  const int W = 1 << win.xs, H = 1 << win.ys, NB = W * H;
   const bool valid = ix >= 0;
   int mx = valid ? ix : INT_MAX, my = valid ? iy : INT_MAX;
   for (int o = 32; o > 0; o >>= 1) {
@@ -52,7 +68,7 @@ __device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int n
     h.mins[0][wave] = mx;
     h.mins[1][wave] = my;
   }
-  for (int k = threadIdx.x; k < kWinX * kWinY; k += kBlock) h.bins[k] = 0u;
+  for (int k = threadIdx.x; k < NB; k += kBlock) h.bins[k] = 0u;
   __syncthreads();
   int x0 = h.mins[0][0], y0 = h.mins[1][0];
   for (int w = 1; w < kBlock / 64; ++w) {
@@ -60,14 +76,14 @@ __device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int n
     y0 = min(y0, h.mins[1][w]);
   }
   const int lx = ix - x0, ly = iy - y0;
-  const bool in_win = valid && lx < kWinX && ly < kWinY;
-  wave_hist_add(h.bins, in_win ? ly * kWinX + lx : -1);
+  const bool in_win = valid && lx < W && ly < H;
+  wave_hist_add(h.bins, in_win ? (ly << win.xs) + lx : -1);
   wave_hist_add(hist, (valid && !in_win) ? iy * nx + ix : -1);
   __syncthreads();
   if (x0 != INT_MAX) {
-    for (int k = threadIdx.x; k < kWinX * kWinY; k += kBlock) {
+    for (int k = threadIdx.x; k < NB; k += kBlock) {
       const uint32_t c = h.bins[k];
-      if (c) atomicAdd(&hist[(y0 + k / kWinX) * nx + (x0 + k % kWinX)], c);
+      if (c) atomicAdd(&hist[(y0 + (k >> win.xs)) * nx + (x0 + (k & (W - 1)))], c);
     }
   }
 }
@@ -726,8 +742,9 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
     }
   }
   __shared__ LdsHist hcnt, hppc;
-  block_hist_add(hcnt, p.cnt_next, nx, dix, diy);
-  block_hist_add(hppc, p.ppc, nx, pix, piy);
+  const CountWindow win = count_window(p.count_window);
+  block_hist_add(hcnt, p.cnt_next, nx, dix, diy, win);
+  block_hist_add(hppc, p.ppc, nx, pix, piy, win);
 }
 
 // ------------------------------------------------------ fused electron step
@@ -832,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
     }
   }
   __shared__ LdsHist hcnt;
-  block_hist_add(hcnt, p.cnt_next, nx, dix, diy);
+  block_hist_add(hcnt, p.cnt_next, nx, dix, diy, count_window(p.count_window));
 }
 
 // --------------------------------------------------------------- injection
@@ -1271,6 +1288,7 @@ int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
                     p->diag && p->e_cnt_next,
                 "bemcs_ion_step: null buffer in parameters");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_ion_step: mesh too small");
+  BEMCS_REQUIRE(count_window_valid(p->count_window), "bemcs_ion_step: count_window beyond 2048 nodes");
   BEMCS_REQUIRE(n_bound <= in->cap && out->cap >= n_bound, "bemcs_ion_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
@@ -1289,6 +1307,7 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(p->Exy && p->is_bound && p->n_sub && p->cnt_next && p->overflow,
                 "bemcs_electron_step: null buffer in parameters");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_electron_step: mesh too small");
+  BEMCS_REQUIRE(count_window_valid(p->count_window), "bemcs_electron_step: count_window beyond 2048 nodes");
   BEMCS_REQUIRE(n_bound <= in->cap, "bemcs_electron_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);

@@ -43,6 +43,16 @@ P_RF_COUNT = 0x23
 
 RING = 8192  # per-step diagnostic records kept on the device
 
+
+def count_window_code(width, height):
+    """The count_window code of the step structs for a width x height LDS
+    window (powers of two with width x height <= 2048): log2 width |
+    log2 height << 8 | 1 << 16 (0 selects the default 64 x 32)."""
+    w, h = int(width), int(height)
+    if w < 1 or h < 1 or w & (w - 1) or h & (h - 1) or w * h > 2048:
+        raise ValueError(f"count window {width} x {height}: powers of two, at most 2048 nodes")
+    return (w.bit_length() - 1) | ((h.bit_length() - 1) << 8) | (1 << 16)
+
 MATERIAL_PRESETS = {
     'Molybdenum': {'k': 138.0, 'rho': 10280.0, 'cp': 250.0, 'emissivity': 0.80,
                    'alpha': 4.8e-6, 'E_mod': 329e9, 'Y_coeff': 1.05e-4, 'E_th': 30.0},
@@ -79,6 +89,8 @@ class DigitalTwinSimulator:
         # sort before the first chunk too (the step starts from the last
         # step's order plus appended refills and SEE electrons)
         self.electron_presort = os.environ.get("BEMCS_EPRESORT", "1") == "1"
+        # LDS window of the NGP / PPC count histograms (0: 64 x 32 nodes)
+        self._count_window = 0
         self.poisson_rtol = float(poisson_rtol)
         self.poisson_max_iter = int(poisson_max_iter)
         # physical constants and defaults as the reference (:310-401)
@@ -1099,6 +1111,7 @@ class DigitalTwinSimulator:
         ip.cx_model = 1 if params.get('cx_model', 'reference') == 'loglinear' else 0
         ip.mcc_enabled = 1
         ip.seed = self.seed
+        ip.count_window = self._count_window
         for prefix, names in (('CX', ('cx_t', 'cx_c', 'cx_lo', 'cx_hi', 'cx_n')),
                               ('SEE', ('see_t', 'see_c', 'see_lo', 'see_hi', 'see_n'))):
             tab = self._table_args(prefix)
@@ -1114,7 +1127,25 @@ class DigitalTwinSimulator:
         ep.dx32, ep.dy32 = f(self.dx), f(self.dy)
         ep.Lx32, ep.Ly32 = f(self.Lx), f(self.Ly)
         ep.nx, ep.ny, ep.periodic_y = self.nx, self.ny, int(self.periodic_y)
+        ep.count_window = self._count_window
         self._elec_params = ep
+
+    @property
+    def count_window(self):
+        """(width, height) in nodes of the LDS window in which a work-group
+        of the fused ion and electron steps privatises its NGP and PPC counts
+        before the global atomics (powers of two, width x height <= 2048;
+        default 64 x 32).  Counts are integers, so the window changes no
+        result, only the number of global atomics."""
+        c = self._count_window
+        return (64, 32) if c == 0 else (1 << (c & 0xFF), 1 << ((c >> 8) & 0xFF))
+
+    @count_window.setter
+    def count_window(self, wh):
+        self._count_window = count_window_code(*wh)
+        for sp in (self.__dict__.get('_ion_params'), self.__dict__.get('_elec_params')):
+            if sp is not None:
+                sp.count_window = self._count_window
 
     def _substep_cases(self, I0, E0):
         """Device address of the two bemcs_substeps_case_t (ions, electrons)

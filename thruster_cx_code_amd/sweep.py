@@ -1,5 +1,5 @@
-"""Sweep workloads on the batched engine: case generation, a sort-interval
-autotuner, and batched vs one-at-a-time timing.
+"""Sweep workloads on the batched engine: case generation, the autotuner (sort
+intervals and deposit tile), and batched vs one-at-a-time timing.
 
 The reference sweeps (Python/benchmarks/benchmark_perveance.py:11-75,
 benchmark_ebs.py, benchmark_impingement.py) build one DigitalTwinSimulator
@@ -54,30 +54,37 @@ def build_case(params, m_e_ratio=200.0):
     return s
 
 
-class SortTuner:
-    """Project autotuner for the cell-sort intervals of a batch.  Sorting
-    changes no result (every result is independent of particle order), only
-    gather locality and its own cost, so candidates are tried on the run's own
-    steps: each candidate pair runs `trial` timed steps and the fastest is kept
-    until the next re-tune (`every` steps), since populations change as the
-    preload decays."""
+class Autotuner:
+    """Project autotuner of a batch: the ion and electron cell-sort intervals
+    and the deposit tile, i.e. the LDS window (width x height nodes) in which
+    each work-group privatises its NGP and PPC counts.  None of them changes a
+    result (every result is independent of particle order, and counts are
+    integers), only gather locality, sorting cost and the number of global
+    atomics, so candidates are tried on the run's own steps: each candidate
+    runs `trial` timed steps, one parameter at a time (ion sort interval,
+    electron sort interval, then window), and the fastest is kept until the
+    next re-tune (`every` steps), since populations change as the preload
+    decays."""
 
     def __init__(self, batch, ion_candidates=(1, 2, 5, 10, 20), electron_candidates=(1, 2, 4),
+                 window_candidates=((64, 32), (128, 16), (256, 8), (512, 4), (32, 16)),
                  trial=6, every=500):
         self.batch = batch
         self.ion_candidates = tuple(ion_candidates)
         self.electron_candidates = tuple(electron_candidates)
+        self.window_candidates = tuple(tuple(w) for w in window_candidates)
         self.trial = int(trial)
         self.every = int(every)
         self.history = []
         self._since = None
 
-    def _set(self, si, se):
+    def _set(self, si, se, win):
         for s in self.batch.sims:
             s.sort_interval, s.electron_sort_interval = int(si), int(se)
+            s.count_window = win
 
-    def _time(self, si, se):
-        self._set(si, se)
+    def _time(self, si, se, win):
+        self._set(si, se, win)
         # a whole number of Poisson periods, so candidates see the same work
         steps = self.trial + (self.trial % 2)
         torch.cuda.synchronize()
@@ -88,27 +95,26 @@ class SortTuner:
 
     def tune(self):
         done = 0
-        best_i, best_t = None, math.inf
-        se0 = self.batch.sims[0].electron_sort_interval
-        for si in self.ion_candidates:
-            t, k = self._time(si, se0)
-            done += k
-            if t < best_t:
-                best_i, best_t = si, t
-        best_e = se0
-        for se in self.electron_candidates:
-            t, k = self._time(best_i, se)
-            done += k
-            if t < best_t:
-                best_e, best_t = se, t
-        self._set(best_i, best_e)
-        self.history.append({"ion_sort": best_i, "electron_sort": best_e, "ms_per_step": best_t * 1e3})
+        s0 = self.batch.sims[0]
+        best = [s0.sort_interval, s0.electron_sort_interval, tuple(s0.count_window)]
+        best_t = math.inf
+        for k, cands in enumerate((self.ion_candidates, self.electron_candidates, self.window_candidates)):
+            for c in cands:
+                trial = list(best)
+                trial[k] = c
+                t, n = self._time(*trial)
+                done += n
+                if t < best_t:
+                    best, best_t = trial, t
+        self._set(*best)
+        self.history.append({"ion_sort": best[0], "electron_sort": best[1], "count_window": list(best[2]),
+                             "ms_per_step": best_t * 1e3})
         self._since = 0
         return done
 
     def tune_cost(self):
         per = self.trial + (self.trial % 2)
-        return per * (len(self.ion_candidates) + len(self.electron_candidates))
+        return per * (len(self.ion_candidates) + len(self.electron_candidates) + len(self.window_candidates))
 
     def advance(self, n_steps):
         left = int(n_steps)
@@ -121,3 +127,4 @@ class SortTuner:
             self.batch.advance(k)
             self._since = (self._since or 0) + k
             left -= k
+
