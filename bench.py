@@ -333,11 +333,12 @@ def algorithmic_bytes(name, n_ions, n_elec, n_nodes):
     return 0
 
 
-# MI355X vector issue rates per CU per clock (published vector peaks: FP64
-# 78.6 TFLOP/s, FP32 157.3 TFLOP/s at 2.4 GHz over 256 CUs)
-CU_CLOCKS_PER_S = 256 * 2.4e9
-VALU_F64_LANES_PER_CLK = 64
-VALU_B32_LANES_PER_CLK = 128
+# MI355X vector issue: 256 CUs x 4 SIMDs at 2.4 GHz; a SIMD has 16 lanes, so
+# one wave64 VALU instruction (float64 or 32-bit, unpacked) occupies it for 4
+# clocks (the published FP64 vector peak, 78.6 TFLOP/s, is that rate with
+# FMAs; the FP32 peak of twice that needs packed instructions)
+SIMD_CLOCKS_PER_S = 1024 * 2.4e9
+CLOCKS_PER_WAVE_VALU = 4
 
 
 def read_profile_value(name, key):
@@ -453,17 +454,17 @@ def run(args):
     # substep, from the committed counter summary of the same command)
     vfile = f"valu_{dom}_{args.workload}.txt"
     valu_per_sub = read_profile_value(vfile, "valu_instructions_per_substep")
-    v64 = read_profile_value(vfile, "valu_f64_per_substep")
-    v32 = read_profile_value(vfile, "valu_b32_per_substep")
+    valu_busy = read_profile_value(vfile, "valu_busy")
     sub_rate = kstats[dom].get("particle_substeps_per_s")
     substep_roofline = None
-    if v64 and v32 and sub_rate:
-        peak_sub = CU_CLOCKS_PER_S / (v64 / VALU_F64_LANES_PER_CLK + v32 / VALU_B32_LANES_PER_CLK)
+    if valu_per_sub and sub_rate:
+        # wave64 VALU instructions per 64 particle-substeps, each 4 SIMD clocks
+        peak_sub = SIMD_CLOCKS_PER_S / CLOCKS_PER_WAVE_VALU / valu_per_sub * 64
         substep_roofline = {"bound": "valu", "kernel": dom, "achieved": sub_rate, "peak": peak_sub,
                             "unit": "particle-substeps/s", "frac": sub_rate / peak_sub,
-                            "valu_instructions_per_substep": valu_per_sub,
-                            "valu_f64_per_substep": v64, "valu_b32_per_substep": v32,
-                            "peak_basis": "256 CUs x 2.4 GHz / (f64 ops / 64 + 32-bit ops / 128 lanes per clock)"}
+                            "valu_instructions_per_substep": valu_per_sub, "valu_busy_counters": valu_busy,
+                            "peak_basis": "1024 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU instruction / "
+                                          "VALU instructions per wave-substep x 64 particles"}
     step_bytes = int(sum(BYTES_PER_ION * r[0] + BYTES_PER_ELECTRON * r[1] + BYTES_PER_NODE_STEP * n_nodes
                          for r in nin))
     step_achieved = step_bytes / (wall_ms * 1e-3) / 1e9

@@ -92,6 +92,17 @@ __device__ __forceinline__ int floor_i32_np(float g, int hi) {
   return i < 0 ? 0 : (i > hi ? hi : i);
 }
 
+// floor_i32_np(g, hi) in five instructions, for the push's inner loop: the
+// cast maps values it cannot hold (NaN, f >= 2^31, f < -2^31) to INT32_MIN,
+// which the clip takes to 0; here NaN and f >= 2^31 become -1 and every
+// negative value clips to 0, so every g gives the same index.  hi_f =
+// (float)hi, exact for hi < 2^24.
+__device__ __forceinline__ int floor_i32_np_med3(float g, float hi_f) {
+  const float f = floorf(g);
+  const float t = f < 2147483648.0f ? f : -1.0f;
+  return (int)__builtin_amdgcn_fmed3f(t, 0.0f, hi_f);
+}
+
 // np.clip(np.floor(g).astype(int), 0, hi)   (int64)
 __device__ __forceinline__ int floor_i64_np(float g, int hi) {
   float f = floorf(g);

@@ -53,12 +53,20 @@ __device__ __forceinline__ CountWindow count_window(int32_t w) {
   return CountWindow{w & 0xff, (w >> 8) & 0xff};
 }
 
-// Every thread of the work-group must call it; ix < 0 adds nothing.
-__device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int nx, int ix, int iy,
-                                               CountWindow win) {
+// Every thread of the work-group must call it with K nodes (ix[k] < 0 adds
+// nothing).
+template <int K>
+__device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int nx, const int (&ix)[K],
+                                               const int (&iy)[K], CountWindow win) {
   const int W = 1 << win.xs, H = 1 << win.ys, NB = W * H;
-  const bool valid = ix >= 0;
-  int mx = valid ? ix : INT_MAX, my = valid ? iy : INT_MAX;
+  int mx = INT_MAX, my = INT_MAX;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (ix[k] >= 0) {
+      mx = min(mx, ix[k]);
+      my = min(my, iy[k]);
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) {
     mx = min(mx, __shfl_xor(mx, o));
     my = min(my, __shfl_xor(my, o));
@@ -75,10 +83,14 @@ __device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int n
     x0 = min(x0, h.mins[0][w]);
     y0 = min(y0, h.mins[1][w]);
   }
-  const int lx = ix - x0, ly = iy - y0;
-  const bool in_win = valid && lx < W && ly < H;
-  wave_hist_add(h.bins, in_win ? (ly << win.xs) + lx : -1);
-  wave_hist_add(hist, (valid && !in_win) ? iy * nx + ix : -1);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool valid = ix[k] >= 0;
+    const int lx = ix[k] - x0, ly = iy[k] - y0;
+    const bool in_win = valid && lx < W && ly < H;
+    wave_hist_add(h.bins, in_win ? (ly << win.xs) + lx : -1);
+    wave_hist_add(hist, (valid && !in_win) ? iy[k] * nx + ix[k] : -1);
+  }
   __syncthreads();
   if (x0 != INT_MAX) {
     for (int k = threadIdx.x; k < NB; k += kBlock) {
@@ -91,6 +103,7 @@ __device__ __forceinline__ void block_hist_add(LdsHist& h, uint32_t* hist, int n
 // ------------------------------------------------------------------ push
 // The field at the gather corners: float32, either two arrays or
 // interleaved (Ex, Ey) pairs.
+typedef float f32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
 template <bool INTERLEAVED, typename T = float>
 struct EField {
   using value_type = T;
@@ -106,7 +119,52 @@ struct EField {
       b = ey[idx];
     }
   }
+  // (Ex, Ey) at nodes (i0, j0), (i0 + 1, j0), (i0, j0 + 1), (i0 + 1, j0 + 1)
+  // as e[0..7] = e00x, e00y, e10x, e10y, e01x, e01y, e11x, e11y.  Interleaved
+  // float32 fields load each row pair with one 16-byte load at a 32-bit byte
+  // offset from the base (24-bit row product: 0 <= i0, j0 and nx below 2^24,
+  // meshes below 2^28 nodes, which the entry points check).
+  __device__ __forceinline__ void corners(int i0, int j0, int nx, T (&e)[8]) const {
+    if constexpr (INTERLEAVED && sizeof(T) == 4) {
+      const uint32_t off = (__umul24((uint32_t)j0, (uint32_t)nx) + (uint32_t)i0) << 3;
+      const char* b = reinterpret_cast<const char*>(ex);
+      const f32x4_a8 r0 = *reinterpret_cast<const f32x4_a8*>(b + off);
+      const f32x4_a8 r1 = *reinterpret_cast<const f32x4_a8*>(b + (off + ((uint32_t)nx << 3)));
+      e[0] = r0.x; e[1] = r0.y; e[2] = r0.z; e[3] = r0.w;
+      e[4] = r1.x; e[5] = r1.y; e[6] = r1.z; e[7] = r1.w;
+    } else {
+      const int c00 = j0 * nx + i0, c01 = c00 + nx;
+      corner(c00, e[0], e[1]);
+      corner(c00 + 1, e[2], e[3]);
+      corner(c01, e[4], e[5]);
+      corner(c01 + 1, e[6], e[7]);
+    }
+  }
 };
+
+// The float64 part of a substep without magnetic field: bilinear weights of
+// the fractions (fx, fy) = (gx - i0, gy - j0) in the cell (i0, j0), the field
+// e[8] at its corners (as
+// EField::corners returns them) gathered, the half kicks (with B = 0 the
+// rotation adds exact zeros: v+ = v-) and the float32 drift.
+template <typename FT>
+__device__ __forceinline__ void kick_drift(double fx, double fy, const FT (&e)[8], float& x, float& y,
+                                           float& vx, float& vy, double h, float dt32) {
+  double a00 = (1.0 - fx) * (1.0 - fy);
+  double a10 = fx * (1.0 - fy);
+  double a01 = (1.0 - fx) * fy;
+  double a11 = fx * fy;
+  double ex = (double)e[0] * a00 + (double)e[2] * a10 + (double)e[4] * a01 + (double)e[6] * a11;
+  double ey = (double)e[1] * a00 + (double)e[3] * a10 + (double)e[5] * a01 + (double)e[7] * a11;
+  double kx = h * ex;
+  double ky = h * ey;
+  double ux = (double)vx + kx;
+  double uy = (double)vy + ky;
+  vx = (float)(ux + kx);
+  vy = (float)(uy + ky);
+  x = x + (vx * dt32) * 1000.0f;
+  y = y + (vy * dt32) * 1000.0f;
+}
 
 // One gather + Boris + drift substep.  h = 0.5*q_m*dt (f64), dt32 = f32(dt);
 // dx, dy are the float32 cell sizes.
@@ -118,34 +176,29 @@ __device__ __forceinline__ void boris_substep(float& x, float& y, float& vx, flo
   using FT = typename FieldT::value_type;
   float gx = div_rn<FAST>(x, dx);
   float gy = div_rn<FAST>(y, dy);
-  int i0 = floor_i32_np(gx, nx - 2);
-  int j0 = floor_i32_np(gy, ny - 2);
-  double fx = (double)gx - (double)i0;
-  double fy = (double)gy - (double)j0;
-  int c00 = j0 * nx + i0;
-  int c10 = c00 + 1;
-  int c01 = c00 + nx;
-  int c11 = c01 + 1;
-  double a00 = (1.0 - fx) * (1.0 - fy);
-  double a10 = fx * (1.0 - fy);
-  double a01 = (1.0 - fx) * fy;
-  double a11 = fx * fy;
-  FT e00x, e00y, e10x, e10y, e01x, e01y, e11x, e11y;
-  E.corner(c00, e00x, e00y);
-  E.corner(c10, e10x, e10y);
-  E.corner(c01, e01x, e01y);
-  E.corner(c11, e11x, e11y);
-  double ex = (double)e00x * a00 + (double)e10x * a10 + (double)e01x * a01 + (double)e11x * a11;
-  double ey = (double)e00y * a00 + (double)e10y * a10 + (double)e01y * a01 + (double)e11y * a11;
-  double kx = h * ex;
-  double ky = h * ey;
-  double ux = (double)vx + kx;
-  double uy = (double)vy + ky;
+  int i0 = floor_i32_np_med3(gx, (float)(nx - 2));
+  int j0 = floor_i32_np_med3(gy, (float)(ny - 2));
+  FT e[8];
+  E.corners(i0, j0, nx, e);
   if (!HAS_B) {
-    // with B = 0 the rotation adds exact zeros: v+ = v-
-    vx = (float)(ux + kx);
-    vy = (float)(uy + ky);
+    kick_drift((double)gx - (double)i0, (double)gy - (double)j0, e, x, y, vx, vy, h, dt32);
+    return;
   } else {
+    double fx = (double)gx - (double)i0;
+    double fy = (double)gy - (double)j0;
+    double a00 = (1.0 - fx) * (1.0 - fy);
+    double a10 = fx * (1.0 - fy);
+    double a01 = (1.0 - fx) * fy;
+    double a11 = fx * fy;
+    const FT e00x = e[0], e00y = e[1], e10x = e[2], e10y = e[3], e01x = e[4], e01y = e[5], e11x = e[6],
+             e11y = e[7];
+    double ex = (double)e00x * a00 + (double)e10x * a10 + (double)e01x * a01 + (double)e11x * a11;
+    double ey = (double)e00y * a00 + (double)e10y * a10 + (double)e01y * a01 + (double)e11y * a11;
+    double kx = h * ex;
+    double ky = h * ey;
+    double ux = (double)vx + kx;
+    double uy = (double)vy + ky;
+    const int c00 = j0 * nx + i0, c10 = c00 + 1, c01 = c00 + nx, c11 = c01 + 1;
     double bx = (double)Bx[c00] * a00 + (double)Bx[c10] * a10 + (double)Bx[c01] * a01 +
                 (double)Bx[c11] * a11;
     double by = (double)By[c00] * a00 + (double)By[c10] * a10 + (double)By[c01] * a01 +
@@ -451,6 +504,70 @@ __device__ __forceinline__ double user_table(const double* t, const double* c, i
   return det_exp(bspline3(t, c, n, le) * kLn10);
 }
 
+// Substeps [s0, s1) of K particles of one thread (no magnetic field, the
+// interleaved float32 field).  A substep takes a short path when, for every
+// particle of the wavefront, both dividends lie in [2^-100, 2^100] (the
+// certified division is exact there) and the cell coordinates g = x / dx are
+// at most twice the largest cell index h (nx - 2 or ny - 2).  Then the floor
+// index is the clip of floor(g) to [0, h] (no int32 overflow or NaN to map),
+// and g - i is exact in float32: the fractional part when i = floor(g), g
+// itself when i = 0, and by Sterbenz's lemma when i = h <= g <= 2 h, so the
+// fraction needs one conversion to float64 instead of two and a float64
+// subtraction.  Otherwise the wavefront takes boris_substep.  Each substep
+// computes every particle's cell and issues its field loads before the
+// float64 arithmetic.  The results equal boris_substep's bit for bit.
+constexpr int kElecPerThread = 1;
+template <bool FAST, int K>
+__device__ __forceinline__ void push_range_k(float (&x)[K], float (&y)[K], float (&vx)[K], float (&vy)[K],
+                                             const EField<true>& E, int nx, int ny, Divisor dx, Divisor dy,
+                                             double h, float dt32, int s0, int s1) {
+  const float hx = (float)(nx - 2), hy = (float)(ny - 2);
+  const float gx_max = 2.0f * hx, gy_max = 2.0f * hy;
+  for (int s = s0; s < s1; ++s) {
+    if (FAST) {
+      float gx[K], gy[K];
+      uint64_t ok = __builtin_amdgcn_read_exec();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        float q = x[k] * dx.r;
+        gx[k] = fmaf(fmaf(-q, dx.b, x[k]), dx.r, q);
+        q = y[k] * dy.r;
+        gy[k] = fmaf(fmaf(-q, dy.b, y[k]), dy.r, q);
+        // lane masks of the conditions, combined in scalar registers
+        ok &= __builtin_amdgcn_ballot_w64(fabsf(x[k]) >= 0x1p-100f) &
+              __builtin_amdgcn_ballot_w64(fabsf(x[k]) <= 0x1p100f) &
+              __builtin_amdgcn_ballot_w64(fabsf(y[k]) >= 0x1p-100f) &
+              __builtin_amdgcn_ballot_w64(fabsf(y[k]) <= 0x1p100f) &
+              __builtin_amdgcn_ballot_w64(gx[k] <= gx_max) & __builtin_amdgcn_ballot_w64(gy[k] <= gy_max);
+      }
+      if (ok == __builtin_amdgcn_read_exec()) {
+        int i0[K], j0[K];
+        double fx[K], fy[K];
+        float e[K][8];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float cx = __builtin_amdgcn_fmed3f(floorf(gx[k]), 0.0f, hx);
+          const float cy = __builtin_amdgcn_fmed3f(floorf(gy[k]), 0.0f, hy);
+          i0[k] = (int)cx;
+          j0[k] = (int)cy;
+          fx[k] = (double)(gx[k] - cx);
+          fy[k] = (double)(gy[k] - cy);
+          E.corners(i0[k], j0[k], nx, e[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) kick_drift(fx[k], fy[k], e[k], x[k], y[k], vx[k], vy[k], h, dt32);
+        continue;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      float vz_unused = 0.0f;
+      boris_substep<EField<true>, false, FAST>(x[k], y[k], vx[k], vy[k], vz_unused, E, nullptr, nullptr,
+                                               nullptr, nx, ny, dx, dy, h, dt32);
+    }
+  }
+}
+
 // ---------------------------------------------------------- fused ion step
 template <bool FAST, bool B>
 __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
@@ -488,10 +605,14 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
     const double h = 0.5 * p.q_m * dts;
     const float dts32 = (float)dts;
     EField<true> E{p.Exy, p.Exy + 1};
-    if (active)
-      for (int s = 0; s < nsub; ++s)
-        boris_substep<EField<true>, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny,
-                                         ddx, ddy, h, dts32);
+    if (active) {
+      float xs[1] = {x}, ys[1] = {y}, vxs[1] = {vx}, vys[1] = {vy};
+      push_range_k<FAST, 1>(xs, ys, vxs, vys, E, nx, ny, ddx, ddy, h, dts32, 0, nsub);
+      x = xs[0];
+      y = ys[0];
+      vx = vxs[0];
+      vy = vys[0];
+    }
   }
   if (PERIODIC && active) y = np_modf(y, p.Ly32);
 
@@ -743,8 +864,9 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
   }
   __shared__ LdsHist hcnt, hppc;
   const CountWindow win = count_window(p.count_window);
-  block_hist_add(hcnt, p.cnt_next, nx, dix, diy, win);
-  block_hist_add(hppc, p.ppc, nx, pix, piy, win);
+  const int dixk[1] = {dix}, diyk[1] = {diy}, pixk[1] = {pix}, piyk[1] = {piy};
+  block_hist_add<1>(hcnt, p.cnt_next, nx, dixk, diyk, win);
+  block_hist_add<1>(hppc, p.ppc, nx, pixk, piyk, win);
 }
 
 // ------------------------------------------------------ fused electron step
@@ -757,15 +879,6 @@ __device__ __forceinline__ int chunk_begin(int n, int j, int k) {
 
 // Substeps of one chunk, in place (no wrap, purge or removal).
 // Substeps [s0, s1) of one particle in registers.
-template <class FieldT, bool FAST>
-__device__ __forceinline__ void push_range(float& x, float& y, float& vx, float& vy, float& vz,
-                                           const FieldT& E, int nx, int ny, Divisor ddx, Divisor ddy,
-                                           double h, float dts32, int s0, int s1) {
-  for (int s = s0; s < s1; ++s)
-    boris_substep<FieldT, false, FAST>(x, y, vx, vy, vz, E, nullptr, nullptr, nullptr, nx, ny, ddx, ddy, h,
-                                       dts32);
-}
-
 template <bool FAST>
 __global__ __launch_bounds__(kBlock) void k_electron_push(bemcs_species_t sp, bemcs_electron_step_t p,
                                                           int chunk, int64_t n_bound) {
@@ -782,37 +895,47 @@ __global__ __launch_bounds__(kBlock) void k_electron_push(bemcs_species_t sp, be
   const double dts = p.dt / (double)nsub;
   const double h = 0.5 * p.q_m * dts;
   const float dts32 = (float)dts;
-  float x = sp.x[i], y = sp.y[i], vx = sp.vx[i], vy = sp.vy[i], vz = sp.vz[i];
+  // no magnetic field: vz is unchanged
+  float x[1] = {sp.x[i]}, y[1] = {sp.y[i]}, vx[1] = {sp.vx[i]}, vy[1] = {sp.vy[i]};
   const EField<true> E{p.Exy, p.Exy + 1};
-  push_range<EField<true>, FAST>(x, y, vx, vy, vz, E, p.nx, p.ny, ddx, ddy, h, dts32, s0, s1);
-  sp.x[i] = x;
-  sp.y[i] = y;
-  sp.vx[i] = vx;
-  sp.vy[i] = vy;
-  sp.vz[i] = vz;
+  push_range_k<FAST, 1>(x, y, vx, vy, E, p.nx, p.ny, ddx, ddy, h, dts32, s0, s1);
+  sp.x[i] = x[0];
+  sp.y[i] = y[0];
+  sp.vx[i] = vx[0];
+  sp.vy[i] = vy[0];
 }
 
 template <bool FAST, bool B>
 __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t one,
                                                            const bemcs_electron_case_t* __restrict__ many) {
+  // kElecPerThread electrons per thread: electron k of the thread is
+  // blockIdx.x * kElecPerThread * kBlock + k * kBlock + threadIdx.x
+  constexpr int K = kElecPerThread;
   const bemcs_electron_case_t a = B ? many[blockIdx.z] : one;
   const bemcs_species_t& in = a.in;
   const bemcs_species_t& out = a.out;
   const bemcs_electron_step_t& p = a.p;
   const bool PERIODIC = p.periodic_y != 0;
   const Divisor ddx{p.dx32, 1.0f / p.dx32}, ddy{p.dy32, 1.0f / p.dy32};
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i_first = (int64_t)blockIdx.x * (K * kBlock) + threadIdx.x;
   const int64_t cnt = *in.count;
   const int64_t n = cnt < in.cap ? cnt : in.cap;
-  if (i == 0 && cnt > (int64_t)gridDim.x * kBlock) atomicOr(p.overflow, 16u);
-  const bool active = i < n;
-  float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
-  if (active) {
-    x = in.x[i];
-    y = in.y[i];
-    vx = in.vx[i];
-    vy = in.vy[i];
-    vz = in.vz[i];
+  if (i_first == 0 && cnt > (int64_t)gridDim.x * (K * kBlock)) atomicOr(p.overflow, 16u);
+  float x[K], y[K], vx[K], vy[K], vz[K];
+  bool active[K];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t i = i_first + (int64_t)k * kBlock;
+    active[k] = i < n;
+    any = any || active[k];
+    // an inactive slot pushes zeros (every field index stays clamped to the
+    // mesh) and is dropped afterwards
+    x[k] = active[k] ? in.x[i] : 0.0f;
+    y[k] = active[k] ? in.y[i] : 0.0f;
+    vx[k] = active[k] ? in.vx[i] : 0.0f;
+    vy[k] = active[k] ? in.vy[i] : 0.0f;
+    vz[k] = active[k] ? in.vz[i] : 0.0f;
   }
   const int nx = p.nx, ny = p.ny;
   {
@@ -822,34 +945,39 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
     const float dts32 = (float)dts;
     const int s0 = chunk_begin(nsub, p.sub_first, p.sub_chunks);
     const EField<true> E{p.Exy, p.Exy + 1};
-    if (active) push_range<EField<true>, FAST>(x, y, vx, vy, vz, E, nx, ny, ddx, ddy, h, dts32, s0, nsub);
+    if (any) push_range_k<FAST, K>(x, y, vx, vy, E, nx, ny, ddx, ddy, h, dts32, s0, nsub);
   }
-  if (PERIODIC && active) y = np_modf(y, p.Ly32);
-  bool keep = false;
-  if (active) {
-    int ix = round_i64_np(div_rn<FAST>(x, ddx), 0, nx - 1);
-    int iy = round_i64_np(div_rn<FAST>(y, ddy), 0, ny - 1);
-    bool hit = p.is_bound[iy * nx + ix] != 0;
-    bool outb = (x < 0.0f) || (x > p.Lx32) || (y < 0.0f) || (y > p.Ly32) || isnan(x);
-    keep = !(hit || outb);
-  }
-  int64_t slot = wave_append(out.count, keep);
-  int dix = -1, diy = -1;
-  if (keep) {
-    if (slot < out.cap) {
-      out.x[slot] = x;
-      out.y[slot] = y;
-      out.vx[slot] = vx;
-      out.vy[slot] = vy;
-      out.vz[slot] = vz;
-      dix = round_i32_np(div_rn<FAST>(x, ddx), 1, nx - 2);
-      diy = round_i32_np(div_rn<FAST>(y, ddy), 1, ny - 2);
-    } else {
-      atomicOr(p.overflow, 4u);
+  int dix[K], diy[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (PERIODIC && active[k]) y[k] = np_modf(y[k], p.Ly32);
+    bool keep = false;
+    if (active[k]) {
+      int ix = round_i64_np(div_rn<FAST>(x[k], ddx), 0, nx - 1);
+      int iy = round_i64_np(div_rn<FAST>(y[k], ddy), 0, ny - 1);
+      bool hit = p.is_bound[iy * nx + ix] != 0;
+      bool outb = (x[k] < 0.0f) || (x[k] > p.Lx32) || (y[k] < 0.0f) || (y[k] > p.Ly32) || isnan(x[k]);
+      keep = !(hit || outb);
+    }
+    int64_t slot = wave_append(out.count, keep);
+    dix[k] = -1;
+    diy[k] = -1;
+    if (keep) {
+      if (slot < out.cap) {
+        out.x[slot] = x[k];
+        out.y[slot] = y[k];
+        out.vx[slot] = vx[k];
+        out.vy[slot] = vy[k];
+        out.vz[slot] = vz[k];
+        dix[k] = round_i32_np(div_rn<FAST>(x[k], ddx), 1, nx - 2);
+        diy[k] = round_i32_np(div_rn<FAST>(y[k], ddy), 1, ny - 2);
+      } else {
+        atomicOr(p.overflow, 4u);
+      }
     }
   }
   __shared__ LdsHist hcnt;
-  block_hist_add(hcnt, p.cnt_next, nx, dix, diy, count_window(p.count_window));
+  block_hist_add<K>(hcnt, p.cnt_next, nx, dix, diy, count_window(p.count_window));
 }
 
 // --------------------------------------------------------------- injection
@@ -1288,6 +1416,7 @@ int bemcs_ion_step(const bemcs_species_t* in, const bemcs_species_t* out,
                     p->diag && p->e_cnt_next,
                 "bemcs_ion_step: null buffer in parameters");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_ion_step: mesh too small");
+  BEMCS_REQUIRE((int64_t)p->nx * p->ny < (int64_t{1} << 28), "bemcs_ion_step: mesh of 2^28 nodes or more");
   BEMCS_REQUIRE(count_window_valid(p->count_window), "bemcs_ion_step: count_window beyond 2048 nodes");
   BEMCS_REQUIRE(n_bound <= in->cap && out->cap >= n_bound, "bemcs_ion_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
@@ -1307,6 +1436,7 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   BEMCS_REQUIRE(p->Exy && p->is_bound && p->n_sub && p->cnt_next && p->overflow,
                 "bemcs_electron_step: null buffer in parameters");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_electron_step: mesh too small");
+  BEMCS_REQUIRE((int64_t)p->nx * p->ny < (int64_t{1} << 28), "bemcs_electron_step: mesh of 2^28 nodes or more");
   BEMCS_REQUIRE(count_window_valid(p->count_window), "bemcs_electron_step: count_window beyond 2048 nodes");
   BEMCS_REQUIRE(n_bound <= in->cap, "bemcs_electron_step: capacity below bound");
   if (n_bound <= 0) return BEMCS_OK;
@@ -1314,7 +1444,7 @@ int bemcs_electron_step(const bemcs_species_t* in, const bemcs_species_t* out,
   const bool fast = fastdiv_ok(p->dx32, s) && fastdiv_ok(p->dy32, s);
   const bemcs_electron_case_t a{*in, *out, *p};
   auto kern = fast ? k_electron_step<true, false> : k_electron_step<false, false>;
-  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kBlock)), dim3(kBlock), 0, s, a,
+  hipLaunchKernelGGL(kern, dim3(grid_for(n_bound, kElecPerThread * kBlock)), dim3(kBlock), 0, s, a,
                      (const bemcs_electron_case_t*)nullptr);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;
@@ -1324,6 +1454,7 @@ int bemcs_electron_push_chunk(const bemcs_species_t* sp, const bemcs_electron_st
                               int64_t n_bound, bemcs_stream_t stream) {
   BEMCS_REQUIRE(sp && p && sp->count && p->Exy && p->n_sub, "bemcs_electron_push_chunk: null argument");
   BEMCS_REQUIRE(p->nx >= 3 && p->ny >= 3, "bemcs_electron_push_chunk: mesh too small");
+  BEMCS_REQUIRE((int64_t)p->nx * p->ny < (int64_t{1} << 28), "bemcs_electron_push_chunk: mesh of 2^28 nodes or more");
   BEMCS_REQUIRE(p->sub_chunks >= 1 && chunk >= 0 && chunk < p->sub_chunks,
                 "bemcs_electron_push_chunk: 0 <= chunk < sub_chunks");
   if (n_bound <= 0) return BEMCS_OK;
@@ -1415,7 +1546,7 @@ int bemcs_electron_step_batched(const bemcs_electron_case_t* cases, int32_t n, i
   if (max_bound <= 0) return BEMCS_OK;
   hipStream_t s = as_stream(stream);
   auto kern = all_fast(divisors_host, n, s) ? k_electron_step<true, true> : k_electron_step<false, true>;
-  hipLaunchKernelGGL(kern, dim3(grid_for(max_bound, kBlock), 1, n), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(kern, dim3(grid_for(max_bound, kElecPerThread * kBlock), 1, n), dim3(kBlock), 0, s,
                      bemcs_electron_case_t{}, cases);
   BEMCS_CHECK_LAUNCH();
   return BEMCS_OK;

@@ -202,6 +202,8 @@ class DigitalTwinSimulator:
             self.ny = int(self.Ly / self.dy) + 1
         if self.nx < 4 or self.ny < 4:
             raise ValueError(f"mesh too small: {self.nx} x {self.ny}")
+        if self.nx * self.ny >= 1 << 28:
+            raise ValueError(f"mesh too large: {self.nx} x {self.ny} (at most 2^28 - 1 nodes)")
         self.dx = self.Lx / (self.nx - 1)
         self.dy = self.Ly / (self.ny - 1)
         if self.dx * 1e-3 > debye or self.dy * 1e-3 > debye:
