@@ -164,6 +164,35 @@ __device__ __forceinline__ int64_t wave_append(T* counter, bool pred) {
   return pred ? (int64_t)base + off : -1;
 }
 
+// Append slot of this lane's element in a compaction with one atomic per
+// work-group instead of one per wavefront (every wavefront of a dense
+// compaction appends, and the adds to one counter serialise at its L2
+// channel).  Every thread of the work-group must call it with the same
+// counter; at most 16 wavefronts per work-group.  Returns -1 where pred is
+// false.
+template <typename T>
+__device__ __forceinline__ int64_t block_append(T* counter, bool pred) {
+  __shared__ int s_cnt[16];
+  __shared__ T s_base;
+  const uint64_t mask = __ballot(pred);
+  const int wave = (int)(threadIdx.x >> 6), lane = lane_id();
+  const int nwaves = (int)((blockDim.x + 63) >> 6);
+  if (lane == 0) s_cnt[wave] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int w = 0; w < nwaves; ++w) total += s_cnt[w];
+    s_base = total ? atomicAdd(counter, (T)total) : (T)0;
+  }
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += s_cnt[w];
+  const int64_t base = (int64_t)s_base + before;
+  __syncthreads();  // s_cnt and s_base are reused by the next call
+  const uint64_t below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+  return pred ? base + __popcll(below) : -1;
+}
+
 // Count of lanes with pred set, added once per wave.
 __device__ __forceinline__ void wave_count(uint32_t* counter, bool pred) {
   uint64_t mask = __ballot(pred);

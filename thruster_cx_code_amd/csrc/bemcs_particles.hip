@@ -843,7 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
   wave_count(&p.diag->mcc_events, coll);
 
   // ---- write survivors, NGP counts for the next deposit, PPC counts
-  int64_t slot = wave_append(out.count, keep);
+  int64_t slot = block_append(out.count, keep);
   int dix = -1, diy = -1, pix = -1, piy = -1;
   if (keep) {
     if (slot < out.cap) {
@@ -959,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_electron_step(bemcs_electron_case_t 
       bool outb = (x[k] < 0.0f) || (x[k] > p.Lx32) || (y[k] < 0.0f) || (y[k] > p.Ly32) || isnan(x[k]);
       keep = !(hit || outb);
     }
-    int64_t slot = wave_append(out.count, keep);
+    int64_t slot = block_append(out.count, keep);
     dix[k] = -1;
     diy[k] = -1;
     if (keep) {

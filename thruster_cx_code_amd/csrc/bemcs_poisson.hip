@@ -382,6 +382,24 @@ __device__ __forceinline__ void xcd_tile(int& tx, int& ty) {
   ty = t / gx;
 }
 
+// Batched launches cover the largest case's tile grid (columns rounded up to
+// a multiple of 8, so a work-group's XCD is its linear id within its case's
+// z-slice modulo 8).  Each case maps its own tiles_x x tiles_y tiles as
+// xcd_tile maps a single grid: XCD k takes the k-th contiguous band of the
+// case's tiles.  (Bands of the largest grid would leave the upper bands of
+// small cases empty and load the first XCDs with the first rows of every
+// case.)  Returns false for the work-groups beyond the case's tiles.
+__device__ __forceinline__ bool xcd_case_tile(int tiles_x, int tiles_y, int& tx, int& ty) {
+  const int id = (int)blockIdx.y * (int)gridDim.x + (int)blockIdx.x;
+  const int n = tiles_x * tiles_y;
+  const int q = n / kXcds, r = n % kXcds, xcd = id % kXcds, j = id / kXcds;
+  if (j >= (xcd < r ? q + 1 : q)) return false;
+  const int t = xcd < r ? xcd * (q + 1) + j : r * (q + 1) + (xcd - r) * q + j;
+  tx = t % tiles_x;
+  ty = t / tiles_x;
+  return true;
+}
+
 // ------------------------------------------------------------ down launch
 // Every global load of a tile is issued up front (clamped addresses, values
 // selected afterwards), so a launch waits for memory about once; the
@@ -410,7 +428,13 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const int tiles_x = (C.nx + TC - 1) / TC;
   const int tiles_y = (C.ny + TC - 1) / TC;
   int tx, ty;
-  xcd_tile(tx, ty);
+  if (B) {
+    // the launches with a scalar work-group have one more tile row (below)
+    const bool scalar_row = (!IS9 && !STEP) || (k > 0 && P.ca.check);
+    if (!xcd_case_tile(tiles_x, tiles_y + (scalar_row ? 1 : 0), tx, ty)) return;
+  } else {
+    xcd_tile(tx, ty);
+  }
   // The launches that compute a global scalar (the initial tolerance and stop
   // flag in the level-0 launch of the initial V-cycle, the convergence test
   // in the first coarse down launch of an iteration) have a grid one tile
@@ -664,7 +688,11 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
   const LevelDev& C = P.C;
   const int tiles_x = (C.nx + kT2 - 1) / kT2;
   int tx, ty;
-  xcd_tile(tx, ty);
+  if (B) {
+    if (!xcd_case_tile(tiles_x, (C.ny + kT2 - 1) / kT2, tx, ty)) return;
+  } else {
+    xcd_tile(tx, ty);
+  }
   if (tx >= tiles_x || ty * kT2 >= C.ny) return;
   if (*P.done) return;
   constexpr float om = kOmegaC;
@@ -874,7 +902,11 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   const CGArgs& g = P.g;
   const int tiles_x = (F.nx + TF - 1) / TF;
   int tx, ty;
-  xcd_tile(tx, ty);
+  if (B) {
+    if (!xcd_case_tile(tiles_x, (F.ny + TF - 1) / TF, tx, ty)) return;
+  } else {
+    xcd_tile(tx, ty);
+  }
   if (tx >= tiles_x || ty * TF >= F.ny) return;
   if (*g.done) return;
   // the level-0 launch of the fused form (small level-0 tiles) also computes
@@ -2081,6 +2113,10 @@ static int coarse_tile(const SolveSet* S, int l) {
   return tiles >= 1024 ? kTCc : kTCs;
 }
 
+// Batched tile launches: grid columns rounded up to a multiple of the XCD
+// count, so every case's z-slice starts on XCD 0 (xcd_case_tile).
+static unsigned round_up8(unsigned v) { return (v + kXcds - 1) / kXcds * kXcds; }
+
 static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
                       std::vector<PackUpload>& uploads) {
   const int n = S->n;
@@ -2145,7 +2181,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
       Op op{};
       op.kind = OP_DOWN2;
       op.level = l;
-      op.grid = dim3(gx, gy, (unsigned)packs.size());
+      op.grid = dim3(n > 1 ? round_up8(gx) : gx, gy, (unsigned)packs.size());
       op.down2 = packs[0];
       std::vector<uint8_t> bytes(sizeof(Down2Pack) * packs.size());
       memcpy(bytes.data(), packs.data(), bytes.size());
@@ -2177,7 +2213,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     op.kind = OP_DOWN;
     op.level = l;
     op.tile = tc;
-    op.grid = dim3(gx, gy, (unsigned)packs.size());
+    op.grid = dim3(n > 1 ? round_up8(gx) : gx, gy, (unsigned)packs.size());
     op.down = packs[0];
     std::vector<uint8_t> bytes(sizeof(DownPack) * packs.size());
     memcpy(bytes.data(), packs.data(), bytes.size());
@@ -2226,7 +2262,7 @@ static bool build_ops(SolveSet* S, bool iteration, std::vector<Op>& ops,
     op.kind = OP_UP;
     op.level = l;
     op.tile = tf / 2;
-    op.grid = dim3(gx, gy, (unsigned)packs.size());
+    op.grid = dim3(n > 1 ? round_up8(gx) : gx, gy, (unsigned)packs.size());
     op.up = packs[0];
     std::vector<uint8_t> bytes(sizeof(UpPack) * packs.size());
     memcpy(bytes.data(), packs.data(), bytes.size());

@@ -93,6 +93,9 @@ class DigitalTwinSimulator:
         self._count_window = 0
         self.poisson_rtol = float(poisson_rtol)
         self.poisson_max_iter = int(poisson_max_iter)
+        # largest coarsest multigrid level (nodes, 16..4096; None: 2048, or 512
+        # when the surface physics rebuilds the operator often)
+        self.poisson_coarse_max = None
         # physical constants and defaults as the reference (:310-401)
         self.dt = 5e-10
         self.injection_stop_time = None
@@ -374,7 +377,7 @@ class DigitalTwinSimulator:
         ib = np.ascontiguousarray(self.is_bound_d.cpu().numpy())
         # surface modes rebuild the operator every few steps: a smaller
         # coarsest level makes each rebuild cheaper
-        coarse = 512 if self._rebuilds_expected else 2048
+        coarse = self.poisson_coarse_max or (512 if self._rebuilds_expected else 2048)
         check(self.lib.bemcs_poisson_create_opt(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
                                                 int(self.periodic_y), coarse, C.byref(h)), "poisson_create")
         self._poisson = h
