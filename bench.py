@@ -84,10 +84,14 @@ def cpu_worker(workload, ions, electrons, max_seconds, max_steps):
     pushed = 0
     steps = 0
     t0 = time.perf_counter()
+    last = t0
     while steps < max_steps and time.perf_counter() - t0 < max_seconds:
         o.step(p)
         pushed += o.pushed
         steps += 1
+        if time.perf_counter() - last >= 30.0:
+            last = time.perf_counter()
+            print(f"[cpu] {steps} steps, {last - t0:.0f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     print(json.dumps({"value": pushed / wall, "steps": steps, "wall_s": wall, "pushed": pushed}))
 
@@ -283,14 +287,16 @@ def _run_cpu(args, timeout):
         env[k] = "1"
     env["HIP_VISIBLE_DEVICES"] = ""
     cmd = [sys.executable, os.path.abspath(__file__)] + [str(a) for a in args]
-    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
+    # progress lines of long runs pass through on stderr
+    out = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=timeout)
     if out.returncode != 0:
-        raise RuntimeError("cpu baseline failed: " + out.stderr[-2000:])
+        raise RuntimeError(f"cpu baseline failed (exit {out.returncode})")
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline(workload, ions, electrons, max_seconds=20.0, max_steps=200):
-    rec = _run_cpu(["--cpu-worker", workload, ions, electrons, max_seconds, max_steps], max_seconds * 20 + 600)
+    rec = _run_cpu(["--cpu-worker", workload, ions, electrons, max_seconds, max_steps],
+                   min(max_seconds * 20 + 600, 7200))
     return {
         "value": rec["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
         "sample": (f"NumPy oracle (reference dtype semantics, SciPy SuperLU) on the same "
@@ -514,7 +520,11 @@ def run(args):
     if not args.no_cpu_baseline:
         _progress("cpu baseline")
         if args.workload == "c1":
-            out["cpu_baseline"] = cpu_baseline(args.workload, 100_000, 100_000, args.cpu_seconds)
+            if args.cpu_steps:
+                # the whole run: every step of the CPU restatement, no time bound
+                out["cpu_baseline"] = cpu_baseline(args.workload, 100_000, 100_000, 1e9, args.cpu_steps)
+            else:
+                out["cpu_baseline"] = cpu_baseline(args.workload, 100_000, 100_000, args.cpu_seconds)
         else:
             ns_i = int(round(float(np.mean(nsub[:, 0]))))
             ns_e = int(round(float(np.mean(nsub[:, 1]))))
@@ -745,6 +755,8 @@ def main():
                     help="steps between cell sorts of the electrons; default 20 (c2), 1 for c5")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="c1: time this many whole CPU steps instead of a --cpu-seconds sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time step shards "
