@@ -519,26 +519,19 @@ class DigitalTwinSimulator:
         return self._be.bound
 
     def _ensure_capacity(self, need_i, need_e):
-        """Grow particle buffers when the host bound could exceed capacity."""
-        if need_i <= self.ions[0].cap and need_e <= self.elec[0].cap:
-            return
-        torch.cuda.synchronize(self.device)
-        ni = int(self.ions[0].count.item())
-        ne = int(self.elec[0].count.item())
-        extra_i, extra_e = need_i - self.bound_i, need_e - self.bound_e
-        self._bi.reset(ni)
-        self._be.reset(ne)
-        self._pending.clear()
-        need_i, need_e = ni + extra_i, ne + extra_e
+        """Grow particle buffers when the host bound could exceed capacity.
+        The host bounds are upper bounds of the device counts, so copying the
+        first min(capacity, bound) entries keeps every live particle; the
+        copies are queued on the stream after the launches already queued,
+        so growing needs no synchronisation."""
         if need_i > self.ions[0].cap:
             cap = max(int(self.ions[0].cap * 1.5), need_i + 65536)
-            self.ions[0].grow(cap, ni)
+            self.ions[0].grow(cap, min(self.ions[0].cap, self.bound_i))
             self.ions[1].grow(cap, 0)
             self.crossed_angle = torch.empty(cap, dtype=torch.float32, device=self.device)
-        cap_e_need = max(need_e, ne) + self.ions[0].cap
-        if cap_e_need > self.elec[0].cap:
-            cap = max(int(self.elec[0].cap * 1.5), cap_e_need + 65536)
-            self.elec[0].grow(cap, ne)
+        if need_e > self.elec[0].cap:
+            cap = max(int(self.elec[0].cap * 1.5), need_e + 65536)
+            self.elec[0].grow(cap, min(self.elec[0].cap, self.bound_e))
             self.elec[1].grow(cap, 0)
 
     def _inject_struct(self, kind, cnt):
