@@ -114,6 +114,30 @@ def test_autotuned_batch_matches_untuned_batch():
         assert torch.equal(a.cnt_i, b.cnt_i) and torch.equal(a.cnt_e, b.cnt_e), c
 
 
+def test_held_population_in_a_batch_matches_alone():
+    """A case with a population hold (top-up to its preload each step) steps
+    in a batch exactly as alone, and stays held."""
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    params = _cases()[:2]
+    params[0]["preload"]["hold"] = True
+    alone = _build(params)
+    together = _build(params)
+    batch = BatchedSimulator(together, params)
+    print("[gpu] batch with a held case", flush=True)
+    for step in range(6):
+        for s, p in zip(alone, params):
+            s.advance(p, 1)
+        batch.advance(1)
+    for c, (a, b) in enumerate(zip(alone, together)):
+        _same(a, b, c)
+    # every step pushed at least the held populations
+    from thruster_cx_code_amd.physics_engine import RING
+    nin = together[0].ring_nin.view(-1, 2).cpu().numpy()
+    held = (params[0]["preload"]["ions"], params[0]["preload"]["electrons"])
+    for it in range(1, 7):
+        assert tuple(nin[it % RING]) >= held
+
+
 def test_cases_join_and_leave_between_steps():
     """A case admitted later (its own iteration counter, opposite Poisson
     parity) and a case retired midway match their alone twins."""

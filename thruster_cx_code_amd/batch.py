@@ -11,6 +11,8 @@ per-case argument structs are uploaded once per step from pinned memory.
 Every case's results are bit-identical to stepping it alone.
 """
 import ctypes as C
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -60,6 +62,16 @@ class _ArgRing:
         ev = torch.cuda.Event()
         ev.record()
         self.events[self.slot] = ev
+
+
+_COUNT_POOL = None
+
+
+def _count_pool():
+    global _COUNT_POOL
+    if _COUNT_POOL is None:
+        _COUNT_POOL = ThreadPoolExecutor(max_workers=max(1, min(8, (os.cpu_count() or 1) // 2)))
+    return _COUNT_POOL
 
 
 class BatchedSimulator:
@@ -149,6 +161,34 @@ class BatchedSimulator:
         for _ in range(int(n_steps)):
             self._step()
 
+    def _ion_counts(self, lams):
+        """Every case's beam-ion count of the step.  The sampler costs O(lambda)
+        (one uniform per unit of the mean), so batches whose cases inject
+        thousands of ions per step run it on host threads (the C sampler
+        releases the GIL); the counts are the same either way."""
+        todo = [c for c, lam in enumerate(lams) if lam is not None]
+        out = [None] * len(lams)
+        work = sum(lams[c] for c in todo)
+        if work < 20000 or len(todo) < 2:
+            for c in todo:
+                out[c] = self.sims[c]._ion_injection_count(lams[c])
+            return out
+        pool = _count_pool()
+        nchunk = min(len(todo), pool._max_workers)
+        chunks = [[] for _ in range(nchunk)]
+        load = [0.0] * nchunk
+        for c in sorted(todo, key=lambda c: -lams[c]):  # largest first, to the lightest chunk
+            j = load.index(min(load))
+            chunks[j].append(c)
+            load[j] += lams[c] + 1.0
+
+        def run(chunk):
+            return [(c, self.sims[c]._ion_injection_count(lams[c])) for c in chunk]
+        for res in pool.map(run, chunks):
+            for c, n in res:
+                out[c] = n
+        return out
+
     def _picard(self, idx):
         """Batched Picard update of the cases idx (their own even steps)."""
         key = tuple(self.sims[c]._poisson.value for c in idx)
@@ -177,11 +217,15 @@ class BatchedSimulator:
         A = self.args
         A.begin()
         # ---- host planning: bounds, injection plans, capacity (before any launch)
-        plans = []
+        lams = []
         for sim, p in zip(sims, self.params):
             sim.iteration += 1
+            lams.append(sim._ion_injection_lambda(p, sim.iteration * sim.dt))
+        counts = self._ion_counts(lams)
+        plans = []
+        for sim, p, n_ion in zip(sims, self.params, counts):
             sim._refresh_bounds()
-            pl = sim._inject_plan(p, sim.iteration * sim.dt)
+            pl = sim._inject_plan(p, sim.iteration * sim.dt, n_ion=n_ion) + sim._hold_plans(p)
             sim._reserve_step(pl)
             plans.append(pl)
         # injection groups: the k-th injection into one species of every case
@@ -203,7 +247,11 @@ class BatchedSimulator:
                 arr[j].p = ip
                 arr[j].n = int(cnt)
                 arr[j].n_bound_before = int(sim.bound_i if key[0] else sim.bound_e)
-                (sim._bi if key[0] else sim._be).grow(cnt)
+                cb = sim._bi if key[0] else sim._be
+                if ip.target > 0:
+                    cb.top_up(ip.target)  # population hold: count' = max(count, target)
+                else:
+                    cb.grow(cnt)
                 max_n = max(max_n, int(cnt))
             inj_launch.append((dptr, len(members), max_n))
         # ---- per-step argument structs of every case

@@ -627,22 +627,37 @@ class DigitalTwinSimulator:
                     ne = max(ne, int(p.target))
         self._ensure_capacity(ni, ne + ni)
 
-    def _inject_plan(self, params, t_current):
+    def _ion_injection_lambda(self, params, t_current):
+        """Mean beam-ion count of the step, I dt / (q w) with
+        I = q 0.61 n0 v_B A (:1005-1018); None when injection is off."""
+        if not (self.injection_enabled and
+                (self.injection_stop_time is None or t_current <= self.injection_stop_time)):
+            return None
+        vb = math.sqrt(self.q_ion * params.get('Te_up', 3.0) / self.m_ion)
+        area = self.Ly * 1e-3 * 0.005
+        bf = 1.0 if params.get('entire_bulk_plasma', False) else 0.61
+        I_ion = self.q_ion * bf * params.get('n0_plasma', 1e17) * vb * area
+        return (I_ion * self.dt) / (self.q_ion * self.macro_weight)
+
+    def _ion_injection_count(self, lam):
+        """Poisson sample of the step's beam-ion count (stream P_INJ_COUNT)."""
+        return int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, self.iteration & 0xFFFFFFFF, lam))
+
+    def _inject_plan(self, params, t_current, n_ion=None):
         """The step's injections as (is_ion, bemcs_inject_t, n), in launch
-        order; count samples and particle ids are drawn here."""
+        order; count samples and particle ids are drawn here (n_ion: the
+        beam-ion count when the caller has already sampled it)."""
         plans = []
         grids = params.get('grids', [])
-        if self.injection_enabled and (self.injection_stop_time is None or t_current <= self.injection_stop_time):
+        lam = self._ion_injection_lambda(params, t_current)
+        if lam is not None:
             n0 = params.get('n0_plasma', 1e17)
             Te_up = params.get('Te_up', 3.0)
             Ti = params.get('Ti', 0.1)
             vb = math.sqrt(self.q_ion * Te_up / self.m_ion)
             area = self.Ly * 1e-3 * 0.005
-            bf = 1.0 if params.get('entire_bulk_plasma', False) else 0.61
-            I_ion = self.q_ion * bf * n0 * vb * area
             cpm = self.q_ion * self.macro_weight
-            lam = (I_ion * self.dt) / cpm
-            n = int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, self.iteration & 0xFFFFFFFF, lam))
+            n = self._ion_injection_count(lam) if n_ion is None else int(n_ion)
             self.injected_ions_step = float(n)
             if n > 0:
                 p = self._inject_struct(0, self.cnt_i)
