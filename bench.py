@@ -591,6 +591,7 @@ def sweep_bench(args):
         torch.cuda.synchronize()
         _progress(f"batch: {done} steps, {time.perf_counter() - t0:.1f} s")
     t_batch = time.perf_counter() - t0
+    captures = batch.graph_captures
     final = [(s.record()["num_p"], s.record()["num_e"]) for s in sims]
     # per-case populations and substep counts of every step, for the CPU model
     from thruster_cx_code_amd.physics_engine import RING
@@ -631,7 +632,7 @@ def sweep_bench(args):
                                "Vs ~ U[800,1600] V, particles per species log-uniform [2e5, 4e6]",
                    "cases": n, "steps": args.sweep_steps},
         "batched": {"cases_per_hr": batched_hr, "build_s": t_build, "steps_s": t_batch,
-                    "autotuning": tuning},
+                    "autotuning": tuning, "graph_captures": captures},
         "one_at_a_time": {"cases_per_hr": seq_hr, "timed_cases": sample, "timed_s": t_seq_sample,
                           "scaled_s": t_seq},
         "speedup": batched_hr / seq_hr,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 6
+#define BEMCS_ABI_VERSION 7
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -699,6 +699,20 @@ int bemcs_step_end_batched(const bemcs_step_end_t* cases, int32_t n, bemcs_strea
 int bemcs_ppc_stats_batched(const bemcs_ppc_case_t* cases, int32_t n, bemcs_stream_t stream);
 int bemcs_percentile_batched(const bemcs_percentile_case_t* cases, int32_t n,
                              bemcs_stream_t stream);
+
+/* HIP graphs of a launch sequence, for the batched sweep's per-step kernels.
+ * No counterpart in the reference, which runs sweep cases as one process each
+ * (Python/benchmarks/benchmark_perveance.py:102-113).  After
+ * bemcs_graph_capture_begin, launches on `stream` from the calling thread are
+ * recorded, not run, until bemcs_graph_capture_end instantiates them into
+ * *exec_out (abort != 0 discards them, after a launch that failed).  Kernel
+ * arguments are fixed at capture: a caller that replays a graph with
+ * bemcs_graph_launch updates the argument structs in device memory that the
+ * captured kernels read.  The graphs hold no memset nodes. */
+int bemcs_graph_capture_begin(bemcs_stream_t stream);
+int bemcs_graph_capture_end(bemcs_stream_t stream, int32_t abort, void** exec_out);
+int bemcs_graph_launch(void* exec, bemcs_stream_t stream);
+int bemcs_graph_destroy(void* exec);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,32 @@ def test_autotuned_batch_matches_untuned_batch():
         assert torch.equal(a.cnt_i, b.cnt_i) and torch.equal(a.cnt_e, b.cnt_e), c
 
 
+def test_graph_replayed_batch_matches_direct_launches():
+    """The per-step kernels replayed from captured HIP graphs (launch sizes
+    rounded up to powers of two) give the bits of direct launches, step after
+    step, across a membership change that forces a new capture; a graph is
+    captured once and replayed on the following steps."""
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    params = _cases()
+    direct = _build(params)
+    graphed = _build(params)
+    b_direct = BatchedSimulator(direct[:2], params[:2], graphs=False)
+    b_graph = BatchedSimulator(graphed[:2], params[:2], graphs=True)
+    for step in range(1, 11):
+        print(f"[gpu] graph batch step {step}", flush=True)
+        if step == 6:  # the third case joins: new argument layout, new graph
+            b_direct.add(direct[2], params[2])
+            b_graph.add(graphed[2], params[2])
+        b_direct.advance(1)
+        b_graph.advance(1)
+        for c, (a, b) in enumerate(zip(b_direct.sims, b_graph.sims)):
+            _same(a, b, (step, c))
+            assert torch.equal(a.cnt_i, b.cnt_i) and torch.equal(a.cnt_e, b.cnt_e), (step, c)
+            assert np.array_equal(a.Ex.view(np.uint32), b.Ex.view(np.uint32)), (step, c)
+    assert b_direct.graph_captures == 0
+    assert 2 <= b_graph.graph_captures < 10
+
+
 def test_held_population_in_a_batch_matches_alone():
     """A case with a population hold (top-up to its preload each step) steps
     in a batch exactly as alone, and stays held."""

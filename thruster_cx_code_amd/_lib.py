@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -201,6 +201,10 @@ _SIGS = {
     "bemcs_step_end_batched": (i32, [vp, i32, vp]),
     "bemcs_ppc_stats_batched": (i32, [vp, i32, vp]),
     "bemcs_percentile_batched": (i32, [vp, i32, vp]),
+    "bemcs_graph_capture_begin": (i32, [vp]),
+    "bemcs_graph_capture_end": (i32, [vp, i32, vp]),
+    "bemcs_graph_launch": (i32, [vp, vp]),
+    "bemcs_graph_destroy": (i32, [vp]),
 }
 
 EXPORTED = tuple(_SIGS)

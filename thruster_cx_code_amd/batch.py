@@ -8,7 +8,14 @@ each case keeps its own device state (built by its own build_domain), and a
 step of the batch issues the same sequence of launches as one case's step,
 each over all cases: the case index is the launch's z dimension and the
 per-case argument structs are uploaded once per step from pinned memory.
-Every case's results are bit-identical to stepping it alone.
+The per-step kernels (step begin, substep counts, ion and electron steps,
+PPC statistics, percentile, step end) run as one captured HIP graph, replayed
+every step: they read their argument structs from fixed places in device
+memory, and their launch sizes come from the population bounds rounded up to
+a power of two, so a graph is captured again only when the membership or a
+bound's power of two changes.  The Poisson update of the cases due runs as
+the solver's own captured graphs.  Every case's results are bit-identical to
+stepping it alone.
 """
 import ctypes as C
 import os
@@ -86,13 +93,19 @@ class BatchedSimulator:
     Poisson solves of the cases due in a step form one batched solve.
     """
 
-    def __init__(self, sims=(), params=()):
+    def __init__(self, sims=(), params=(), graphs=None):
         if len(params) != len(sims):
             raise ValueError("one params dict per case")
         self.sims, self.params = [], []
         self.device = None
         self.lib = _lib.load()
         self._sets = {}
+        # per-step kernels as captured HIP graphs (BEMCS_BATCH_GRAPHS=0: direct launches)
+        self.graphs = os.environ.get("BEMCS_BATCH_GRAPHS", "1") != "0" if graphs is None else bool(graphs)
+        self._graph_cache = {}
+        self._cap_stream = None
+        self._divisors_checked = False
+        self.graph_captures = 0
         self.args = None
         self.poisson_iter_log = []
         self.kernel_timing = None
@@ -131,6 +144,8 @@ class BatchedSimulator:
 
     def _membership_changed(self):
         self._free_sets()
+        self._free_graphs()
+        self._divisors_checked = False
         if self.device is None:
             return
         n = max(1, self.n)
@@ -150,9 +165,20 @@ class BatchedSimulator:
                 pass
         self._sets = {}
 
+    def _free_graphs(self):
+        if self._graph_cache and self.device is not None:
+            torch.cuda.synchronize(self.device)  # no replay of them is still queued
+        for g in self._graph_cache.values():
+            try:
+                self.lib.bemcs_graph_destroy(g)
+            except Exception:
+                pass
+        self._graph_cache = {}
+
     def __del__(self):
         try:
             self._free_sets()
+            self._free_graphs()
         except Exception:
             pass
 
@@ -216,6 +242,17 @@ class BatchedSimulator:
         s = stream_ptr()
         A = self.args
         A.begin()
+        # the argument structs of the per-step kernels come first in the
+        # buffer, so their device addresses (the captured graphs' arguments)
+        # depend only on the number of cases
+        dsz = C.sizeof(_lib.IonDiag)
+        sb, sb_d = A.alloc(_lib.StepBegin, n)
+        sub, sub_d = A.alloc(_lib.SubstepsCase, 2 * n)
+        ic, ic_d = A.alloc(_lib.IonCase, n)
+        ec, ec_d = A.alloc(_lib.ElectronCase, n)
+        pc, pc_d = A.alloc(_lib.PpcCase, n)
+        qc, qc_d = A.alloc(_lib.PercentileCase, n)
+        se, se_d = A.alloc(_lib.StepEnd, n)
         # ---- host planning: bounds, injection plans, capacity (before any launch)
         lams = []
         for sim, p in zip(sims, self.params):
@@ -255,14 +292,6 @@ class BatchedSimulator:
                 max_n = max(max_n, int(cnt))
             inj_launch.append((dptr, len(members), max_n))
         # ---- per-step argument structs of every case
-        dsz = C.sizeof(_lib.IonDiag)
-        sb, sb_d = A.alloc(_lib.StepBegin, n)
-        sub, sub_d = A.alloc(_lib.SubstepsCase, 2 * n)
-        ic, ic_d = A.alloc(_lib.IonCase, n)
-        ec, ec_d = A.alloc(_lib.ElectronCase, n)
-        pc, pc_d = A.alloc(_lib.PpcCase, n)
-        qc, qc_d = A.alloc(_lib.PercentileCase, n)
-        se, se_d = A.alloc(_lib.StepEnd, n)
         sort_i = [c for c, m in enumerate(sims) if m.sort_interval > 0 and m.iteration % m.sort_interval == 0]
         sort_e = [c for c, m in enumerate(sims)
                   if m.electron_sort_interval > 0 and m.iteration % m.electron_sort_interval == 0]
@@ -352,18 +381,11 @@ class BatchedSimulator:
             ev = DigitalTwinSimulator._timed_begin(kt)
             self._picard(poisson)
             DigitalTwinSimulator._timed_end(kt, "poisson_picard", ev)
-        check(lib.bemcs_step_begin_batched(sb_d, n, self.max_nodes, s), "step_begin_batched")
-        check(lib.bemcs_substeps_batched(sub_d, 2 * n, max(max_bi, max_be), s), "substeps_batched")
-        kt = self.kernel_timing
-        ev = DigitalTwinSimulator._timed_begin(kt)
-        check(lib.bemcs_ion_step_batched(ic_d, n, max_bi, self._div, s), "ion_step_batched")
-        DigitalTwinSimulator._timed_end(kt, "ion_step", ev)
-        ev = DigitalTwinSimulator._timed_begin(kt)
-        check(lib.bemcs_electron_step_batched(ec_d, n, max_be, self._div, s), "electron_step_batched")
-        DigitalTwinSimulator._timed_end(kt, "electron_step", ev)
-        check(lib.bemcs_ppc_stats_batched(pc_d, n, s), "ppc_stats_batched")
-        check(lib.bemcs_percentile_batched(qc_d, n, s), "percentile_batched")
-        check(lib.bemcs_step_end_batched(se_d, n, s), "step_end_batched")
+        core = (sb_d, sub_d, ic_d, ec_d, pc_d, qc_d, se_d)
+        if self.graphs and self.kernel_timing is None:
+            self._core_graph(core, n, max_bi, max_be, s)
+        else:
+            self._core_launches(core, n, max_bi, max_be, s, self.kernel_timing)
         for sim in sims:
             sim.ions.reverse()
             sim.elec.reverse()
@@ -389,6 +411,63 @@ class BatchedSimulator:
         ev.record()
         for sim in sims:
             sim._pending.append((ev, sim.iteration % RING, sim._bi.snapshot(), sim._be.snapshot(), sim.iteration))
+
+    def _core_launches(self, core, n, bi, be, s, kt=None):
+        """The per-step kernels of every case: clears and records, substep
+        counts, the fused ion and electron steps, PPC statistics, the
+        divergence percentile and the step-end records.  bi and be bound the
+        ion and electron counts (they size the launches; the kernels read the
+        counts on the device)."""
+        lib = self.lib
+        sb_d, sub_d, ic_d, ec_d, pc_d, qc_d, se_d = core
+        check(lib.bemcs_step_begin_batched(sb_d, n, self.max_nodes, s), "step_begin_batched")
+        check(lib.bemcs_substeps_batched(sub_d, 2 * n, max(bi, be), s), "substeps_batched")
+        ev = DigitalTwinSimulator._timed_begin(kt)
+        check(lib.bemcs_ion_step_batched(ic_d, n, bi, self._div, s), "ion_step_batched")
+        DigitalTwinSimulator._timed_end(kt, "ion_step", ev)
+        ev = DigitalTwinSimulator._timed_begin(kt)
+        check(lib.bemcs_electron_step_batched(ec_d, n, be, self._div, s), "electron_step_batched")
+        DigitalTwinSimulator._timed_end(kt, "electron_step", ev)
+        check(lib.bemcs_ppc_stats_batched(pc_d, n, s), "ppc_stats_batched")
+        check(lib.bemcs_percentile_batched(qc_d, n, s), "percentile_batched")
+        check(lib.bemcs_step_end_batched(se_d, n, s), "step_end_batched")
+
+    @staticmethod
+    def _pow2(b):
+        b = int(b)
+        return 0 if b <= 0 else 1 << (b - 1).bit_length()
+
+    def _core_graph(self, core, n, bi, be, s):
+        """Replay (capturing first if needed) the graph of _core_launches for
+        the bounds rounded up to powers of two.  A larger launch only adds
+        work-groups that find no particle; the results do not change."""
+        key = (self._pow2(bi), self._pow2(be))
+        g = self._graph_cache.get(key)
+        if g is None:
+            lib = self.lib
+            if not self._divisors_checked:
+                # the division check of a new divisor synchronises: never inside a capture
+                for d in self._div:
+                    lib.bemcs_fastdiv_enabled(d, s)
+                self._divisors_checked = True
+            if self._cap_stream is None:
+                # capture cannot run on the legacy default stream; the graph is
+                # launched on the step's stream
+                self._cap_stream = torch.cuda.Stream(device=self.device)
+            cs = C.c_void_p(self._cap_stream.cuda_stream)
+            if len(self._graph_cache) >= 8:
+                self._free_graphs()
+            check(lib.bemcs_graph_capture_begin(cs), "graph_capture_begin")
+            try:
+                self._core_launches(core, n, key[0], key[1], cs)
+            except BaseException:
+                lib.bemcs_graph_capture_end(cs, 1, C.byref(C.c_void_p()))
+                raise
+            g = C.c_void_p()
+            check(lib.bemcs_graph_capture_end(cs, 0, C.byref(g)), "graph_capture_end")
+            self._graph_cache[key] = g
+            self.graph_captures += 1
+        check(self.lib.bemcs_graph_launch(g, s), "graph_launch")
 
     # ------------------------------------------------------ diagnostics
     def records(self):

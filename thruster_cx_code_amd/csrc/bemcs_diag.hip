@@ -240,6 +240,45 @@ int bemcs_abi_version(void) { return BEMCS_ABI_VERSION; }
 
 const char* bemcs_last_error(void) { return g_err; }
 
+// ------------------------------------------------------------ HIP graphs
+// Thread-local capture, so launches of other host threads (and their
+// allocations) are unaffected; the stream may have work pending, which the
+// graph does not depend on and which its launches follow in stream order.
+int bemcs_graph_capture_begin(bemcs_stream_t stream) {
+  BEMCS_CHECK_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+  return BEMCS_OK;
+}
+
+int bemcs_graph_capture_end(bemcs_stream_t stream, int32_t abort, void** exec_out) {
+  BEMCS_REQUIRE(exec_out, "bemcs_graph_capture_end: null argument");
+  *exec_out = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(as_stream(stream), &g);
+  if (abort || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (abort) return BEMCS_OK;
+    BEMCS_CHECK_HIP(e);
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  BEMCS_CHECK_HIP(ie);
+  *exec_out = x;
+  return BEMCS_OK;
+}
+
+int bemcs_graph_launch(void* exec, bemcs_stream_t stream) {
+  BEMCS_REQUIRE(exec, "bemcs_graph_launch: null graph");
+  BEMCS_CHECK_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(exec), as_stream(stream)));
+  return BEMCS_OK;
+}
+
+int bemcs_graph_destroy(void* exec) {
+  if (exec) BEMCS_CHECK_HIP(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(exec)));
+  return BEMCS_OK;
+}
+
 int bemcs_percentile(const float* angles, const uint32_t* n_dev, int64_t n_bound, double q_percent,
                      uint32_t min_count, uint32_t* scratch, double* out, bemcs_stream_t stream) {
   BEMCS_REQUIRE(angles && n_dev && out, "bemcs_percentile: null argument");

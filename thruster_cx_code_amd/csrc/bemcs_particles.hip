@@ -140,6 +140,19 @@ struct EField {
       corner(c01 + 1, e[6], e[7]);
     }
   }
+  // The same corners from the clipped cell coordinates as floats: the node
+  // index j0 * nx + i0 is fma(cy, nx, cx), exact on meshes of at most 2^24
+  // nodes, and both rows load at one 32-bit byte offset, the second from the
+  // uniform base row1 = field + 2 nx floats.
+  __device__ __forceinline__ void corners_f(float cx, float cy, float nxf, const char* row1,
+                                            T (&e)[8]) const {
+    static_assert(INTERLEAVED && sizeof(T) == 4, "float32 interleaved field only");
+    const uint32_t off = (uint32_t)fmaf(cy, nxf, cx) << 3;
+    const f32x4_a8 r0 = *reinterpret_cast<const f32x4_a8*>(reinterpret_cast<const char*>(ex) + off);
+    const f32x4_a8 r1 = *reinterpret_cast<const f32x4_a8*>(row1 + off);
+    e[0] = r0.x; e[1] = r0.y; e[2] = r0.z; e[3] = r0.w;
+    e[4] = r1.x; e[5] = r1.y; e[6] = r1.z; e[7] = r1.w;
+  }
 };
 
 // The float64 part of a substep without magnetic field: bilinear weights of
@@ -506,23 +519,30 @@ __device__ __forceinline__ double user_table(const double* t, const double* c, i
 
 // Substeps [s0, s1) of K particles of one thread (no magnetic field, the
 // interleaved float32 field).  A substep takes a short path when, for every
-// particle of the wavefront, both dividends lie in [2^-100, 2^100] (the
-// certified division is exact there) and the cell coordinates g = x / dx are
-// at most twice the largest cell index h (nx - 2 or ny - 2).  Then the floor
-// index is the clip of floor(g) to [0, h] (no int32 overflow or NaN to map),
-// and g - i is exact in float32: the fractional part when i = floor(g), g
-// itself when i = 0, and by Sterbenz's lemma when i = h <= g <= 2 h, so the
+// particle of the wavefront, both dividends are at least 2^-100 in magnitude
+// and the cell coordinates g = x / dx lie in [-2 h, 2 h], h the largest cell
+// index (nx - 2 or ny - 2).  The dividends then lie in [2^-100, 2^100], where
+// the certified division is exact: fastdiv_ok accepts divisors of at most
+// 1e18 < 2^60, so a dividend above 2^100 would give |g| > 2^39 or NaN from the
+// division formula, far above 2 h.  The floor index is the clip of floor(g)
+// to [0, h] (no int32 overflow or NaN to map), and g - i is exact in float32:
+// the fractional part when i = floor(g), g itself when i = 0 (g below 1,
+// negative g included), and by Sterbenz's lemma when i = h <= g <= 2 h, so the
 // fraction needs one conversion to float64 instead of two and a float64
 // subtraction.  Otherwise the wavefront takes boris_substep.  Each substep
 // computes every particle's cell and issues its field loads before the
-// float64 arithmetic.  The results equal boris_substep's bit for bit.
+// float64 arithmetic.  The results equal boris_substep's bit for bit.  FIDX
+// forms the node offset in float32 (meshes of at most 2^24 nodes).
 constexpr int kElecPerThread = 1;
-template <bool FAST, int K>
-__device__ __forceinline__ void push_range_k(float (&x)[K], float (&y)[K], float (&vx)[K], float (&vy)[K],
-                                             const EField<true>& E, int nx, int ny, Divisor dx, Divisor dy,
-                                             double h, float dt32, int s0, int s1) {
+template <bool FAST, int K, bool FIDX>
+__device__ __forceinline__ void push_range_impl(float (&x)[K], float (&y)[K], float (&vx)[K],
+                                                float (&vy)[K], const EField<true>& E, int nx, int ny,
+                                                Divisor dx, Divisor dy, double h, float dt32, int s0,
+                                                int s1) {
   const float hx = (float)(nx - 2), hy = (float)(ny - 2);
   const float gx_max = 2.0f * hx, gy_max = 2.0f * hy;
+  const float nxf = (float)nx;
+  const char* row1 = reinterpret_cast<const char*>(E.ex) + (size_t)nx * 8;
   for (int s = s0; s < s1; ++s) {
     if (FAST) {
       float gx[K], gy[K];
@@ -535,24 +555,24 @@ __device__ __forceinline__ void push_range_k(float (&x)[K], float (&y)[K], float
         gy[k] = fmaf(fmaf(-q, dy.b, y[k]), dy.r, q);
         // lane masks of the conditions, combined in scalar registers
         ok &= __builtin_amdgcn_ballot_w64(fabsf(x[k]) >= 0x1p-100f) &
-              __builtin_amdgcn_ballot_w64(fabsf(x[k]) <= 0x1p100f) &
+              __builtin_amdgcn_ballot_w64(fabsf(gx[k]) <= gx_max) &
               __builtin_amdgcn_ballot_w64(fabsf(y[k]) >= 0x1p-100f) &
-              __builtin_amdgcn_ballot_w64(fabsf(y[k]) <= 0x1p100f) &
-              __builtin_amdgcn_ballot_w64(gx[k] <= gx_max) & __builtin_amdgcn_ballot_w64(gy[k] <= gy_max);
+              __builtin_amdgcn_ballot_w64(fabsf(gy[k]) <= gy_max);
       }
       if (ok == __builtin_amdgcn_read_exec()) {
-        int i0[K], j0[K];
         double fx[K], fy[K];
         float e[K][8];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const float cx = __builtin_amdgcn_fmed3f(floorf(gx[k]), 0.0f, hx);
           const float cy = __builtin_amdgcn_fmed3f(floorf(gy[k]), 0.0f, hy);
-          i0[k] = (int)cx;
-          j0[k] = (int)cy;
           fx[k] = (double)(gx[k] - cx);
           fy[k] = (double)(gy[k] - cy);
-          E.corners(i0[k], j0[k], nx, e[k]);
+          if (FIDX) {
+            E.corners_f(cx, cy, nxf, row1, e[k]);
+          } else {
+            E.corners((int)cx, (int)cy, nx, e[k]);
+          }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) kick_drift(fx[k], fy[k], e[k], x[k], y[k], vx[k], vy[k], h, dt32);
@@ -566,6 +586,17 @@ __device__ __forceinline__ void push_range_k(float (&x)[K], float (&y)[K], float
                                                nullptr, nx, ny, dx, dy, h, dt32);
     }
   }
+}
+
+// The float32 node offset where the mesh allows it (a uniform branch).
+template <bool FAST, int K>
+__device__ __forceinline__ void push_range_k(float (&x)[K], float (&y)[K], float (&vx)[K], float (&vy)[K],
+                                             const EField<true>& E, int nx, int ny, Divisor dx, Divisor dy,
+                                             double h, float dt32, int s0, int s1) {
+  if (FAST && (int64_t)nx * ny <= (int64_t{1} << 24))
+    push_range_impl<FAST, K, true>(x, y, vx, vy, E, nx, ny, dx, dy, h, dt32, s0, s1);
+  else
+    push_range_impl<FAST, K, false>(x, y, vx, vy, E, nx, ny, dx, dy, h, dt32, s0, s1);
 }
 
 // ---------------------------------------------------------- fused ion step
@@ -1275,7 +1306,8 @@ static int check_divisor(float b, float r, hipStream_t s, unsigned long long* ou
 bool fastdiv_ok(float b, hipStream_t s) {
   const char* env = getenv("BEMCS_FASTDIV");
   if (env && env[0] == '0') return false;
-  if (!(fabsf(b) >= 1e-30f && fabsf(b) <= 1e30f)) return false;
+  // divisors of at most 1e18 < 2^60: the short path of the push relies on it
+  if (!(fabsf(b) >= 1e-30f && fabsf(b) <= 1e18f)) return false;
   uint32_t key;
   memcpy(&key, &b, sizeof(key));
   std::lock_guard<std::mutex> lock(g_div_mu);
