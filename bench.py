@@ -576,7 +576,7 @@ def sweep_bench(args):
     n = len(cases)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sims = [build_case(p) for p in cases]
+    sims = [build_case(p, coarse_max=args.sweep_coarse) for p in cases]
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     _progress(f"built {n} cases in {t_build:.1f} s")
@@ -671,6 +671,11 @@ def serve_bench(args):
     wall = time.perf_counter() - t0
     summ = latency_summary(res, wall)
     _progress(f"served {len(res)} requests in {wall:.1f} s")
+    cap = capacity(res, wall, args.serve_rate)
+    alone = None
+    if args.serve_seq > 0:
+        alone = serve_alone(reqs, res, args.serve_seq)
+        alone["served_over_alone"] = cap["case_steps_per_s"] / alone["case_steps_per_s"]
     print(json.dumps({
         "metric": "served sweep p50/p99 latency (continuous batching)",
         "value": summ["p50_s"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
@@ -681,8 +686,38 @@ def serve_bench(args):
                    "slots": args.serve_slots},
         "latency": summ, "wall_s": wall,
         "max_active": max((a for _, a, _ in srv.log), default=0),
-        "capacity": capacity(res, wall, args.serve_rate),
+        "capacity": cap,
+        "one_at_a_time": alone,
     }))
+
+
+def serve_alone(reqs, results, k):
+    """The same GPU serving requests one at a time: k served requests (every
+    j-th by id) each built and stepped alone for the steps it took when
+    served; case-steps/s and requests/hr over them (build included)."""
+    import gc
+    import torch
+    from thruster_cx_code_amd.sweep import build_case
+    steps = {r.id: int(r.steps) for r in results if not r.error}
+    ok = [q for q in reqs if q.id in steps]
+    sample = ok[::max(1, len(ok) // k)][:k]
+    total_s, total_steps = 0.0, 0
+    for q in sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim = build_case(q.params)
+        sim.advance(q.params, steps[q.id])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        total_s += dt
+        total_steps += steps[q.id]
+        _progress(f"alone: request {q.id} ({q.params['mesh']['nx']}x{q.params['mesh']['ny']}, "
+                  f"{steps[q.id]} steps) {dt:.1f} s")
+        del sim
+        gc.collect()
+        torch.cuda.empty_cache()
+    return {"requests": [q.id for q in sample], "seconds": total_s, "case_steps": total_steps,
+            "case_steps_per_s": total_steps / total_s, "req_per_hr": len(sample) / total_s * 3600.0}
 
 
 def capacity(results, wall, rate):
@@ -723,6 +758,8 @@ def main():
         sp.add_argument("--serve-rate", type=float, default=4.0)
         sp.add_argument("--serve-max-steps", type=int, default=4000)
         sp.add_argument("--serve-slots", type=int, default=64)
+        sp.add_argument("--serve-seq", type=int, default=0,
+                        help="also time this many served requests one at a time")
         serve_bench(sp.parse_args())
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--sweep":
@@ -731,6 +768,8 @@ def main():
         sp.add_argument("--sweep-cases", type=int, default=64)
         sp.add_argument("--sweep-steps", type=int, default=3000)
         sp.add_argument("--sweep-seq-cases", type=int, default=16)
+        sp.add_argument("--sweep-coarse", type=int, default=None,
+                        help="coarsest multigrid level of the batched cases (nodes; default the solver's)")
         sp.add_argument("--no-cpu-baseline", action="store_true")
         sweep_bench(sp.parse_args())
         return

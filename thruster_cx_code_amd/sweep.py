@@ -46,10 +46,14 @@ def uneven_sweep(n_cases=64, seed=0, n_min=200_000, n_max=4_000_000):
     return out
 
 
-def build_case(params, m_e_ratio=200.0):
+def build_case(params, m_e_ratio=200.0, coarse_max=None):
+    """A built case; coarse_max bounds the coarsest multigrid level (nodes,
+    None: the solver's default)."""
     from .physics_engine import DigitalTwinSimulator
     s = DigitalTwinSimulator()
     s.m_e = s.m_ion / m_e_ratio
+    if coarse_max is not None:
+        s.poisson_coarse_max = int(coarse_max)
     s.build_domain(params)
     return s
 
