@@ -461,6 +461,7 @@ def run(args):
     vfile = f"valu_{dom}_{args.workload}.txt"
     valu_per_sub = read_profile_value(vfile, "valu_instructions_per_substep")
     valu_busy = read_profile_value(vfile, "valu_busy")
+    ta_busy = read_profile_value(vfile, "ta_busy")
     sub_rate = kstats[dom].get("particle_substeps_per_s")
     substep_roofline = None
     if valu_per_sub and sub_rate:
@@ -469,6 +470,7 @@ def run(args):
         substep_roofline = {"bound": "valu", "kernel": dom, "achieved": sub_rate, "peak": peak_sub,
                             "unit": "particle-substeps/s", "frac": sub_rate / peak_sub,
                             "valu_instructions_per_substep": valu_per_sub, "valu_busy_counters": valu_busy,
+                            "texture_addresser_busy_counters": ta_busy,
                             "peak_basis": "1024 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU instruction / "
                                           "VALU instructions per wave-substep x 64 particles"}
     step_bytes = int(sum(BYTES_PER_ION * r[0] + BYTES_PER_ELECTRON * r[1] + BYTES_PER_NODE_STEP * n_nodes
