@@ -578,7 +578,7 @@ def sweep_bench(args):
     n = len(cases)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sims = [build_case(p, coarse_max=args.sweep_coarse) for p in cases]
+    sims = [build_case(p, coarse_max=args.sweep_coarse, level0_tile=args.sweep_tile) for p in cases]
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     _progress(f"built {n} cases in {t_build:.1f} s")
@@ -772,6 +772,8 @@ def main():
         sp.add_argument("--sweep-seq-cases", type=int, default=16)
         sp.add_argument("--sweep-coarse", type=int, default=None,
                         help="coarsest multigrid level of the batched cases (nodes; default the solver's)")
+        sp.add_argument("--sweep-tile", type=int, default=16,
+                        help="level-0 tile edge of the batched cases' Poisson solves (0: by mesh size)")
         sp.add_argument("--no-cpu-baseline", action="store_true")
         sweep_bench(sp.parse_args())
         return

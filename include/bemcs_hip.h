@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define BEMCS_ABI_VERSION 7
+#define BEMCS_ABI_VERSION 8
 
 #define BEMCS_OK 0
 #define BEMCS_EINVAL -1     /* invalid argument (sizes, null pointers) */
@@ -365,6 +365,17 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
                              int32_t coarse_max, bemcs_poisson_t* out);
 int bemcs_poisson_free(bemcs_poisson_t h);
 int bemcs_poisson_levels(bemcs_poisson_t h);
+
+/* Coarse tile edge of the level-0 V-cycle launches for the handle's own
+ * solves and for the solve sets it joins afterwards: 8 (a work-group owns
+ * 16 x 16 fine nodes), 16 (32 x 32: a quarter of the partial sums and less
+ * halo traffic, the better choice in large batches) or 0 (by mesh size: 16
+ * above 600K nodes).  A set takes 16 if any of its handles does, so a case
+ * solved alone and in a batch of handles with the same setting gives the
+ * same bits.  Solver tuning without a reference counterpart (the reference
+ * factorises with SuperLU, Python/physics_engine.py:608); call it between
+ * solves (it rebuilds the handle's own solve set). */
+int bemcs_poisson_set_level0_tile(bemcs_poisson_t h, int32_t tile);
 
 /* Rebuilds the operator of handle h for a new electrode map of the same mesh
  * (device uint8 [ny][nx]) in place, on the device: the reference re-runs

@@ -140,6 +140,32 @@ def test_graph_replayed_batch_matches_direct_launches():
     assert 2 <= b_graph.graph_captures < 10
 
 
+def test_large_level0_tiles_alone_and_batched():
+    """Cases whose Poisson handles ask for the large level-0 tiles (32 x 32
+    fine nodes per work-group, as sweeps and the server build them) give the
+    same bits alone and batched; their first potentials agree with the
+    default tiles' to the solver tolerance."""
+    from thruster_cx_code_amd import DigitalTwinSimulator
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    from thruster_cx_code_amd.sweep import build_case
+    params = _cases()
+    alone = [build_case(p, level0_tile=16) for p in params]
+    together = [build_case(p, level0_tile=16) for p in params]
+    default = _build(params)
+    for a, d in zip(alone, default):
+        # the initial Picard solves of build_domain, large tiles against default ones
+        assert np.allclose(a.V, d.V, rtol=0.0, atol=1e-8 * np.abs(d.V).max())
+    batch = BatchedSimulator(together, params)
+    for step in range(1, 7):
+        print(f"[gpu] large-tile batch step {step}", flush=True)
+        for s, p in zip(alone, params):
+            s.advance(p, 1)
+        batch.advance(1)
+        for c, (a, b) in enumerate(zip(alone, together)):
+            _same(a, b, (step, c))
+    assert DigitalTwinSimulator().poisson_level0_tile is None
+
+
 def test_held_population_in_a_batch_matches_alone():
     """A case with a population hold (top-up to its preload each step) steps
     in a batch exactly as alone, and stays held."""

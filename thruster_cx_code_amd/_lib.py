@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbemcs_hip.so")
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 OK, EINVAL, EHIP, ENOCONV, EOVERFLOW = 0, -1, -2, -3, -4
 
 vp = C.c_void_p
@@ -170,6 +170,7 @@ _SIGS = {
     "bemcs_poisson_create_opt": (i32, [vp, i32, i32, i32, i32, vp]),
     "bemcs_poisson_free": (i32, [vp]),
     "bemcs_poisson_levels": (i32, [vp]),
+    "bemcs_poisson_set_level0_tile": (i32, [vp, i32]),
     "bemcs_poisson_update": (i32, [vp, vp, vp]),
     "bemcs_poisson_solve": (i32, [vp, vp, vp, f64, i32, vp, vp, vp]),
     "bemcs_poisson_picard": (i32, [vp, vp, vp, vp, vp]),

@@ -69,6 +69,7 @@ struct bemcs_poisson_s {
   double *pa, *pr, *pi;
   double* pz[2];
   bool have_guess;
+  int tile0_pref;     // level-0 coarse tile edge asked for (0: by mesh size, kTC or kTCL)
   bemcs_poisson_batch_s* own;  // solve set of this handle alone
   // operator set-up workspace (in-place updates)
   double* su_st[2];   // float64 9-point stencils of two consecutive coarse levels
@@ -2398,9 +2399,13 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   S->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
   const char* pg = getenv("BEMCS_POISSON_GRAPHS");
   S->use_graphs = !(pg && pg[0] == '0');
+  // a handle asks for the large level-0 tiles explicitly or by its mesh size;
+  // the set takes them if any of its handles does
   S->tile0 = kTC;
-  for (int c = 0; c < n; ++c)
-    if (hs[c]->nx * hs[c]->ny > kLargeMesh) S->tile0 = kTCL;
+  for (int c = 0; c < n; ++c) {
+    const int pref = hs[c]->tile0_pref;
+    if (pref == kTCL || (pref == 0 && hs[c]->nx * hs[c]->ny > kLargeMesh)) S->tile0 = kTCL;
+  }
   const char* tl = getenv("BEMCS_MG_TILE0");
   if (tl && (atoi(tl) == kTC || atoi(tl) == kTCL)) S->tile0 = atoi(tl);
   const char* ex = getenv("BEMCS_PICARD_EXTRAP");
@@ -2827,6 +2832,7 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
   h->periodic = per ? 1 : 0;
   h->nlev = nlev;
   h->have_guess = false;
+  h->tile0_pref = 0;
   h->own = nullptr;
   h->nc = nc;
   bool ok = true;
@@ -2931,6 +2937,23 @@ int bemcs_poisson_free(bemcs_poisson_t h) {
 }
 
 int bemcs_poisson_levels(bemcs_poisson_t h) { return h ? h->nlev : 0; }
+
+int bemcs_poisson_set_level0_tile(bemcs_poisson_t h, int32_t tile) {
+  BEMCS_REQUIRE(h && (tile == 0 || tile == kTC || tile == kTCL),
+                "bemcs_poisson_set_level0_tile: tile must be 0, 8 or 16");
+  if (h->tile0_pref == tile) return BEMCS_OK;
+  const int old = h->tile0_pref;
+  h->tile0_pref = tile;
+  SolveSet* S = nullptr;
+  const int rc = create_set(&h, 1, &S);
+  if (rc != BEMCS_OK) {
+    h->tile0_pref = old;
+    return rc;
+  }
+  destroy_set(h->own);
+  h->own = S;
+  return BEMCS_OK;
+}
 
 int bemcs_poisson_solve(bemcs_poisson_t h, const double* b, double* V, double rtol,
                         int32_t max_iter, int32_t* iters, double* relres, bemcs_stream_t stream) {

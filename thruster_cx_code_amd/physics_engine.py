@@ -96,6 +96,9 @@ class DigitalTwinSimulator:
         # largest coarsest multigrid level (nodes, 16..4096; None: 2048, or 512
         # when the surface physics rebuilds the operator often)
         self.poisson_coarse_max = None
+        # level-0 tile edge of the V-cycle launches (None: by mesh size; 16 for
+        # cases stepped in large batches, alone or batched alike)
+        self.poisson_level0_tile = None
         # physical constants and defaults as the reference (:310-401)
         self.dt = 5e-10
         self.injection_stop_time = None
@@ -381,6 +384,8 @@ class DigitalTwinSimulator:
         check(self.lib.bemcs_poisson_create_opt(ib.ctypes.data_as(C.c_void_p), self.nx, self.ny,
                                                 int(self.periodic_y), coarse, C.byref(h)), "poisson_create")
         self._poisson = h
+        if self.poisson_level0_tile:
+            check(self.lib.bemcs_poisson_set_level0_tile(h, int(self.poisson_level0_tile)), "poisson_tile")
 
     def _alloc_particles(self, params):
         pre = params.get('preload') or {}

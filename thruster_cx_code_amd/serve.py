@@ -109,11 +109,13 @@ class SweepServer:
     footprint_bytes against memory already allocated); max_active: slot
     limit; max_steps: step cap per request; min_steps: earliest steady-state
     stop (the reference uses 1500); check_every: steps between completion
-    checks (the stop step itself is found exactly from the per-step records).
+    checks (the stop step itself is found exactly from the per-step records);
+    level0_tile: level-0 tile edge of the cases' Poisson solves (16: the
+    large tiles, faster in large batches).
     """
 
     def __init__(self, hbm_budget=None, max_active=64, max_steps=4000, min_steps=1500,
-                 check_every=25, tol_deg=0.1, window=50, chunk=5):
+                 check_every=25, tol_deg=0.1, window=50, chunk=5, level0_tile=16):
         if hbm_budget is None:
             free, total = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * total)
@@ -125,6 +127,7 @@ class SweepServer:
         self.tol_deg = float(tol_deg)
         self.window = int(window)
         self.chunk = int(chunk)
+        self.level0_tile = level0_tile
         self.batch = BatchedSimulator()
         self.log = []
 
@@ -177,7 +180,7 @@ class SweepServer:
                 req = queue[qi]
                 qi += 1
                 try:
-                    sim = build_case(req.params)
+                    sim = build_case(req.params, level0_tile=self.level0_tile)
                 except ValueError as exc:  # the reference's build checks reject it
                     t = time.perf_counter() - t0
                     results.append(Result(req.id, 0, False, float("nan"), [], req.arrival, t, t, str(exc)))

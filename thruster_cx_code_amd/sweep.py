@@ -46,14 +46,18 @@ def uneven_sweep(n_cases=64, seed=0, n_min=200_000, n_max=4_000_000):
     return out
 
 
-def build_case(params, m_e_ratio=200.0, coarse_max=None):
+def build_case(params, m_e_ratio=200.0, coarse_max=None, level0_tile=None):
     """A built case; coarse_max bounds the coarsest multigrid level (nodes,
-    None: the solver's default)."""
+    None: the solver's default); level0_tile sets the level-0 tile edge of its
+    Poisson solves (None: by mesh size; 16 suits cases stepped in large
+    batches, and gives the same bits alone and batched)."""
     from .physics_engine import DigitalTwinSimulator
     s = DigitalTwinSimulator()
     s.m_e = s.m_ion / m_e_ratio
     if coarse_max is not None:
         s.poisson_coarse_max = int(coarse_max)
+    if level0_tile is not None:
+        s.poisson_level0_tile = int(level0_tile)
     s.build_domain(params)
     return s
 
