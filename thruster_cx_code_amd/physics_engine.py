@@ -350,7 +350,10 @@ class DigitalTwinSimulator:
             self.overflow_d = zeros(1, torch.int32, dev)
             sz = C.sizeof(_lib.IonDiag)
             self.ring_diag = zeros(RING * sz, torch.uint8, dev)
-            self.ring_theta = zeros(RING, torch.float64, dev)
+            # theta95 of each step, written by the percentile kernel straight
+            # into pinned host memory (read after a synchronisation)
+            self.ring_theta = torch.zeros(RING, dtype=torch.float64, pin_memory=True)
+            self._ring_theta_np = self.ring_theta.numpy()
             self.ring_minpot = zeros(RING, torch.float64, dev)
             self.ring_ppc = zeros(2 * RING, torch.int32, dev)
             # particle counts after each step, written by k_step_end straight
@@ -1231,7 +1234,7 @@ class DigitalTwinSimulator:
         rec = dict(entered=float(d.entered), crossed=nc, lost_to_grid=float(d.lost_to_grid),
                    see_spawned=d.see_spawned, see_kept=d.see_kept, mcc_events=d.mcc_events,
                    transmitted=float(nc) if d.exited else 0.0)
-        rec['current_div'] = float(self.ring_theta[slot].item()) if nc > 5 else float('nan')
+        rec['current_div'] = float(self._ring_theta_np[slot]) if nc > 5 else float('nan')
         rec['min_pot'] = float(self.ring_minpot[slot].item())
         rec['trans_last_frame'] = (rec['transmitted'] / rec['entered']) if rec['entered'] > 0 else 0.0
         rec['remeshed'] = bool(self.ring_remeshed[slot])

@@ -137,13 +137,15 @@ class SweepServer:
 
     def _scan(self, case):
         """Feed the theta95 records since the last check into the case's
-        window; return True when the case is done."""
+        window; return True when the case is done.  The records sit in pinned
+        host memory: the caller synchronises the stream once before scanning
+        the cases."""
         s = case.sim
         first, last = case.checked_it + 1, s.iteration
         if last < first:
             return False
         slots = [(k % RING) for k in range(first, last + 1)]
-        vals = s.ring_theta[slots].cpu().numpy()
+        vals = s._ring_theta_np[slots]
         for k, v in zip(range(first, last + 1), vals):
             steps = k - case.start_it
             if not math.isnan(v):
@@ -198,6 +200,7 @@ class SweepServer:
             steps_since_check += self.chunk
             if steps_since_check >= self.check_every:
                 steps_since_check = 0
+                torch.cuda.current_stream().synchronize()
                 done = [c for c in active if self._scan(c)]
                 now = time.perf_counter() - t0
                 for c in done:
