@@ -73,3 +73,17 @@ def test_count_bound_with_population_hold():
             c, snap = history.pop(0)
             b.refresh(c, snap)
         assert true <= b.bound <= max(true, target)
+
+
+def test_graph_launch_bounds_cover_the_count_bounds():
+    """The batched step's captured graphs are keyed by the count bounds
+    rounded up to powers of two: every bound maps to a launch size at least
+    as large and less than twice as large (zero stays zero: no launch)."""
+    from thruster_cx_code_amd.batch import BatchedSimulator
+    p2 = BatchedSimulator._pow2
+    assert p2(0) == 0 and p2(-3) == 0 and p2(1) == 1 and p2(2) == 2 and p2(3) == 4
+    rng = random.Random(1)
+    for _ in range(10000):
+        b = rng.randrange(1, 1 << 34)
+        q = p2(b)
+        assert q >= b and q < 2 * b and q & (q - 1) == 0
