@@ -353,7 +353,12 @@ typedef struct bemcs_poisson_s* bemcs_poisson_t;
 
 /* Builds the reduced SPD system of build_sparse_matrix (:525-608) from the
  * host isBound map (uint8 [ny][nx], 1 = electrode/Dirichlet) and the geometry
- * mode, plus its multigrid hierarchy; allocates device memory. */
+ * mode, plus its multigrid hierarchy; allocates device memory.  The operator
+ * arrays (node kinds, coarse stencils, inverse diagonals, the dense coarsest
+ * inverse) depend only on the mesh and the map: a handle created for a map
+ * and mesh that a live handle was built from shares that handle's arrays
+ * (bit-identical to building them again) and allocates only its own vectors,
+ * so sweep points of one geometry read one operator. */
 int bemcs_poisson_create(const uint8_t* is_bound_host, int32_t nx, int32_t ny,
                          int32_t periodic_y, bemcs_poisson_t* out);
 /* The same with the size of the coarsest multigrid level chosen: levels halve
@@ -384,7 +389,10 @@ int bemcs_poisson_set_level0_tile(bemcs_poisson_t h, int32_t tile);
  * inverse are recomputed into the handle's existing arrays (bemcs_poisson_create
  * runs the same set-up), so results equal those of a handle created from the
  * new map; the next Picard call starts its solve from the current V.  Waits
- * for the set-up on `stream` to report an invalid map. */
+ * for the set-up on `stream` to report an invalid map.  A handle that shares
+ * its operator arrays with other handles first moves to arrays of its own;
+ * solve sets made earlier that hold it then refuse to run (BEMCS_EINVAL)
+ * until they are made again. */
 int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_stream_t stream);
 
 /* Solves the full reference system A V = b (laplacian_lu(b), :900) for

@@ -427,6 +427,65 @@ def test_poisson_update_equals_create(lib):
         lib.bemcs_poisson_free(hb)
 
 
+def test_poisson_shared_operator_and_update(lib):
+    """Handles created from one electrode map share their operator arrays;
+    updating one of them to a new map gives it arrays of its own: it then
+    solves like a handle created from the new map, the other keeps solving
+    the old map's system bit for bit, and a solve set made before the update
+    refuses to run."""
+    from oracle.poisson import SuperLUPoisson
+    from thruster_cx_code_amd import _lib as L
+    nx, ny = 257, 97
+    o = _geometry(nx, ny, False)
+    A = o.isBound.copy()
+    B = A.copy()
+    r = np.random.default_rng(7)
+    cand = np.argwhere(A[1:-1, 1:-1]) + 1
+    for iy, ix in cand[r.choice(len(cand), 30, replace=False)]:
+        B[iy, ix] = False
+    ia = np.ascontiguousarray(A.astype(np.uint8))
+    ibm = np.ascontiguousarray(B.astype(np.uint8))
+    h1, h2, h3 = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for h, m in ((h1, ia), (h2, ia), (h3, ibm)):
+        assert lib.bemcs_poisson_create(m.ctypes.data_as(C.c_void_p), nx, ny, 0, C.byref(h)) == 0
+
+    def solve(h, b):
+        V = torch.zeros(nx * ny, dtype=torch.float64, device="cuda")
+        bd = _dev(b)
+        its, rel = C.c_int32(0), C.c_double(0)
+        assert lib.bemcs_poisson_solve(h, _p(bd), _p(V), 1e-12, 300, C.byref(its), C.byref(rel),
+                                       _stream()) == 0, lib.bemcs_last_error()
+        return V.cpu().numpy()
+
+    batch = C.c_void_p()
+    try:
+        spA, spB = SuperLUPoisson(A, False), SuperLUPoisson(B, False)
+        src = r.normal(0, 1.0, nx * ny) * 1e-3
+        bA = spA.rhs_vector(src, o.V_fixed.ravel(), 1.0)
+        bB = spB.rhs_vector(src, o.V_fixed.ravel(), 1.0)
+        v1, v2 = solve(h1, bA), solve(h2, bA)
+        assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
+        assert np.max(np.abs(v1 - spA.solve(bA))) / np.max(np.abs(v1)) < 1e-10
+        arr = (C.c_void_p * 2)(h1.value, h2.value)
+        assert lib.bemcs_poisson_batch_create(arr, 2, C.byref(batch)) == 0
+        print("[gpu] shared operator: update one handle", flush=True)
+        bd_map = _dev(ibm.ravel())
+        assert lib.bemcs_poisson_update(h1, _p(bd_map), _stream()) == 0, lib.bemcs_last_error()
+        w1, w3 = solve(h1, bB), solve(h3, bB)
+        assert np.array_equal(w1.view(np.uint64), w3.view(np.uint64))
+        assert np.max(np.abs(w1 - spB.solve(bB))) / np.max(np.abs(w1)) < 1e-10
+        assert np.array_equal(solve(h2, bA).view(np.uint64), v2.view(np.uint64))
+        ps = (L.Picard * 2)()
+        its, rels = (C.c_int32 * 2)(), (C.c_double * 2)()
+        assert lib.bemcs_poisson_batch_picard(batch, ps, its, rels, _stream()) == L.EINVAL
+        assert b"create the set again" in lib.bemcs_last_error()
+    finally:
+        if batch:
+            lib.bemcs_poisson_batch_free(batch)
+        for h in (h1, h2, h3):
+            lib.bemcs_poisson_free(h)
+
+
 def test_poisson_update_rejects_invalid_map(lib):
     nx, ny = 128, 48
     o = _geometry(nx, ny, False)

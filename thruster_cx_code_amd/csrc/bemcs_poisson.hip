@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -38,6 +39,8 @@ constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels
 constexpr int kNu = 2;           // pre- and post-smoothing sweeps
 
 enum : uint8_t { K_BOUND = 0, K_UNKNOWN = 1, K_RIGHT = 2, K_TOP = 3, K_BOTTOM = 4 };
+
+struct OpData;  // operator arrays, shared by handles built from the same electrode map
 
 struct LevelDev {
   int nx, ny, n, periodic;
@@ -70,6 +73,8 @@ struct bemcs_poisson_s {
   double* pz[2];
   bool have_guess;
   int tile0_pref;     // level-0 coarse tile edge asked for (0: by mesh size, kTC or kTCL)
+  bemcs::OpData* op;  // the operator arrays this handle's pointers above lead into
+  int op_gen;         // incremented when the handle moves to operator arrays of its own
   bemcs_poisson_batch_s* own;  // solve set of this handle alone
   // operator set-up workspace (in-place updates)
   double* su_st[2];   // float64 9-point stencils of two consecutive coarse levels
@@ -1878,6 +1883,167 @@ static void* dalloc(bemcs_poisson_s* h, size_t bytes) {
   return p;
 }
 
+// ------------------------------------------------------- shared operators
+// The operator of a mesh -- node kinds and Neumann mirrors, level-0 codes,
+// the coarse stencils, every level's inverse diagonal, the dense inverse of
+// the coarsest level -- depends only on the mesh shape and the electrode map,
+// not on voltages or charge.  Sweep points of one geometry (the reference
+// sweeps beam current or voltages at fixed grids,
+// Python/benchmarks/benchmark_perveance.py:11-75) therefore share one set of
+// operator arrays: a handle created for an electrode map that an existing
+// handle was built from points at that handle's arrays instead of building
+// its own (the set-up is deterministic, so the arrays would be bit-identical).
+// Batched launches then read one operator per geometry, from the caches.  A
+// handle whose electrode map changes (bemcs_poisson_update) first moves to
+// arrays of its own when it shares them.
+struct OpData {
+  int nx, ny, per, nlev, nc;
+  std::vector<uint8_t> ib;  // electrode map the arrays were built from (keyed ops only)
+  bool keyed;               // in the registry: later handles may share it
+  int refs;
+  std::vector<void*> allocs;
+  float* inv_d[kMaxLevels];
+  uint8_t* code0;
+  float* st[kMaxLevels];
+  float* cinv;
+  uint8_t* kind;
+  int32_t *mirror, *mirror1;
+  double* su_st[2];
+  uint8_t* su_act[2];
+  double *su_A, *su_C, *su_R, *su_T, *su_d0;
+  uint8_t* su_swept;
+  int* su_err;
+};
+
+static std::mutex g_op_mu;
+static std::vector<OpData*> g_ops;
+
+static void free_op(OpData* op) {
+  for (void* p : op->allocs) (void)hipFree(p);
+  delete op;
+}
+
+// Operator arrays (and set-up workspace) for the level shapes of h.
+static OpData* new_op(const bemcs_poisson_s* h) {
+  OpData* op = new OpData();
+  op->nx = h->nx;
+  op->ny = h->ny;
+  op->per = h->periodic;
+  op->nlev = h->nlev;
+  op->nc = h->nc;
+  op->keyed = false;
+  op->refs = 1;
+  bool ok = true;
+  auto A = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (ok && hipMalloc(&p, bytes) == hipSuccess) {
+      op->allocs.push_back(p);
+      return p;
+    }
+    ok = false;
+    return nullptr;
+  };
+  const int n = h->nx * h->ny, nc = h->nc;
+  for (int l = 0; l < h->nlev; ++l) {
+    const int nl = h->lev[l].n;
+    op->inv_d[l] = (float*)A(sizeof(float) * nl);
+    op->st[l] = l == 0 ? nullptr : (float*)A(sizeof(float) * 9 * nl);
+  }
+  op->code0 = (uint8_t*)A(n);
+  op->cinv = (float*)A(sizeof(float) * (size_t)nc * nc);
+  op->kind = (uint8_t*)A(n);
+  op->mirror = (int32_t*)A(sizeof(int32_t) * n);
+  op->mirror1 = (int32_t*)A(sizeof(int32_t) * n);
+  const int n1 = h->nlev > 1 ? h->lev[1].n : 1;
+  op->su_st[0] = (double*)A(sizeof(double) * 9 * n1);
+  op->su_st[1] = (double*)A(sizeof(double) * 9 * n1);
+  op->su_act[0] = (uint8_t*)A(n1);
+  op->su_act[1] = (uint8_t*)A(n1);
+  op->su_A = (double*)A(sizeof(double) * (size_t)nc * nc);
+  op->su_C = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
+  op->su_R = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
+  op->su_T = (double*)A(sizeof(double) * kGJPanel * kGJPanel);
+  op->su_d0 = (double*)A(sizeof(double) * nc);
+  op->su_swept = (uint8_t*)A(nc);
+  op->su_err = (int*)A(sizeof(int));
+  if (!ok) {
+    free_op(op);
+    return nullptr;
+  }
+  return op;
+}
+
+// Points the handle's operator pointers at op's arrays.
+static void attach(bemcs_poisson_s* h, OpData* op) {
+  h->op = op;
+  for (int l = 0; l < h->nlev; ++l) {
+    h->lev[l].inv_d = op->inv_d[l];
+    h->lev[l].st = op->st[l];
+    h->lev[l].code = l == 0 ? op->code0 : nullptr;
+  }
+  h->cinv = op->cinv;
+  h->kind = op->kind;
+  h->mirror = op->mirror;
+  h->mirror1 = op->mirror1;
+  for (int k = 0; k < 2; ++k) {
+    h->su_st[k] = op->su_st[k];
+    h->su_act[k] = op->su_act[k];
+  }
+  h->su_A = op->su_A;
+  h->su_C = op->su_C;
+  h->su_R = op->su_R;
+  h->su_T = op->su_T;
+  h->su_d0 = op->su_d0;
+  h->su_swept = op->su_swept;
+  h->su_err = op->su_err;
+}
+
+// A keyed operator built from electrode map ib for h's shapes (one more
+// reference), or null.
+static OpData* find_op(const bemcs_poisson_s* h, const uint8_t* ib) {
+  const size_t n = (size_t)h->nx * h->ny;
+  std::lock_guard<std::mutex> lock(g_op_mu);
+  for (OpData* op : g_ops) {
+    if (op->keyed && op->nx == h->nx && op->ny == h->ny && op->per == h->periodic && op->nlev == h->nlev &&
+        op->nc == h->nc && memcmp(op->ib.data(), ib, n) == 0) {
+      ++op->refs;
+      return op;
+    }
+  }
+  return nullptr;
+}
+
+static void register_op(OpData* op, const uint8_t* ib) {
+  op->ib.assign(ib, ib + (size_t)op->nx * op->ny);
+  std::lock_guard<std::mutex> lock(g_op_mu);
+  op->keyed = true;
+  g_ops.push_back(op);
+}
+
+// Takes op out of the registry (its arrays are about to change).
+static void unregister_op(OpData* op) {
+  std::lock_guard<std::mutex> lock(g_op_mu);
+  if (!op->keyed) return;
+  op->keyed = false;
+  op->ib.clear();
+  g_ops.erase(std::remove(g_ops.begin(), g_ops.end(), op), g_ops.end());
+}
+
+static void release_op(OpData* op) {
+  if (!op) return;
+  {
+    std::lock_guard<std::mutex> lock(g_op_mu);
+    if (--op->refs > 0) return;
+    if (op->keyed) g_ops.erase(std::remove(g_ops.begin(), g_ops.end(), op), g_ops.end());
+  }
+  free_op(op);
+}
+
+static int shared_refs(const OpData* op) {
+  std::lock_guard<std::mutex> lock(g_op_mu);
+  return op->refs;
+}
+
 static FineOp fine_op(const bemcs_poisson_s* h, int l) {
   FineOp F{};
   F.nx = h->lev[l].nx;
@@ -1970,6 +2136,7 @@ static void destroy(bemcs_poisson_s* h) {
   if (!h) return;
   destroy_set(h->own);
   for (void* p : h->allocs) (void)hipFree(p);
+  release_op(h->op);
   delete h;
 }
 
@@ -2003,6 +2170,7 @@ struct Op {
 
 struct bemcs_poisson_batch_s {
   std::vector<bemcs_poisson_s*> cases;
+  std::vector<int> gens;  // each handle's op_gen when the set was made
   int n;
   int post_sweeps;
   bool use_graphs;
@@ -2395,6 +2563,7 @@ static int create_set(bemcs_poisson_s* const* hs, int n, SolveSet** out) {
   SolveSet* S = new SolveSet();
   S->n = n;
   S->cases.assign(hs, hs + n);
+  for (int c = 0; c < n; ++c) S->gens.push_back(hs[c]->op_gen);
   const char* ps = getenv("BEMCS_MG_POST_SWEEPS");
   S->post_sweeps = (ps && ps[0] == '1') ? 1 : 2;
   const char* pg = getenv("BEMCS_POISSON_GRAPHS");
@@ -2833,6 +3002,8 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
   h->nlev = nlev;
   h->have_guess = false;
   h->tile0_pref = 0;
+  h->op = nullptr;
+  h->op_gen = 0;
   h->own = nullptr;
   h->nc = nc;
   bool ok = true;
@@ -2841,20 +3012,14 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
     if (!p) ok = false;
     return p;
   };
+  // the handle's own state: multigrid vectors of every level, CG vectors,
+  // partial sums (the operator arrays come from an OpData)
   for (int l = 0; l < nlev && ok; ++l) {
     LevelDev& D = h->lev[l];
     D.nx = lnx[l];
     D.ny = lny[l];
     D.n = D.nx * D.ny;
     D.periodic = per ? 1 : 0;
-    D.inv_d = (float*)A(sizeof(float) * D.n);
-    if (l == 0) {
-      D.code = (uint8_t*)A(D.n);
-      D.st = nullptr;
-    } else {
-      D.st = (float*)A(sizeof(float) * 9 * D.n);
-      D.code = nullptr;
-    }
     D.x = (float*)A(sizeof(float) * D.n);
     D.t = (float*)A(sizeof(float) * D.n);
     D.b = (float*)A(sizeof(float) * D.n);
@@ -2863,10 +3028,6 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
     if (ok) ok = hipMemset(D.b, 0, sizeof(float) * D.n) == hipSuccess;
   }
   if (ok) {
-    h->cinv = (float*)A(sizeof(float) * (size_t)nc * nc);
-    h->kind = (uint8_t*)A(n);
-    h->mirror = (int32_t*)A(sizeof(int32_t) * n);
-    h->mirror1 = (int32_t*)A(sizeof(int32_t) * n);
     h->x = (double*)A(sizeof(double) * n);
     h->R[0] = (double*)A(sizeof(double) * n);
     h->R[1] = (double*)A(sizeof(double) * n);
@@ -2884,32 +3045,34 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
     h->pi = (double*)A(sizeof(double) * 2 * h->nparts);
     h->pz[0] = (double*)A(sizeof(double) * 4 * h->nparts);
     h->pz[1] = (double*)A(sizeof(double) * 4 * h->nparts);
-    // set-up workspace, kept for in-place updates
-    const int n1 = nlev > 1 ? h->lev[1].n : 1;
-    h->su_st[0] = (double*)A(sizeof(double) * 9 * n1);
-    h->su_st[1] = (double*)A(sizeof(double) * 9 * n1);
-    h->su_act[0] = (uint8_t*)A(n1);
-    h->su_act[1] = (uint8_t*)A(n1);
-    h->su_A = (double*)A(sizeof(double) * (size_t)nc * nc);
-    h->su_C = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
-    h->su_R = (double*)A(sizeof(double) * (size_t)nc * kGJPanel);
-    h->su_T = (double*)A(sizeof(double) * kGJPanel * kGJPanel);
-    h->su_d0 = (double*)A(sizeof(double) * nc);
-    h->su_swept = (uint8_t*)A(nc);
-    h->su_err = (int*)A(sizeof(int));
     if (ok) ok = hipMemset(h->p, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->Ap, 0, sizeof(double) * n) == hipSuccess;
     if (ok) ok = hipMemset(h->x, 0, sizeof(double) * n) == hipSuccess;
   }
-  uint8_t* ib = nullptr;
-  if (ok) ok = hipMalloc(&ib, n) == hipSuccess;
-  if (ok) ok = upload(ib, is_bound_host, n) == 0;
   int rc = BEMCS_OK;
   if (ok) {
-    rc = setup_operator(h, ib, nullptr);
-    if (rc == BEMCS_OK) rc = setup_status(h, nullptr);
+    // the operator: shared with a handle built from the same electrode map,
+    // else built here (and offered to later handles)
+    OpData* op = find_op(h, is_bound_host);
+    if (op) {
+      attach(h, op);
+    } else {
+      op = new_op(h);
+      ok = op != nullptr;
+      if (ok) {
+        attach(h, op);
+        uint8_t* ib = nullptr;
+        ok = hipMalloc(&ib, n) == hipSuccess;
+        if (ok) ok = upload(ib, is_bound_host, n) == 0;
+        if (ok) {
+          rc = setup_operator(h, ib, nullptr);
+          if (rc == BEMCS_OK) rc = setup_status(h, nullptr);
+        }
+        if (ib) (void)hipFree(ib);
+        if (ok && rc == BEMCS_OK) register_op(op, is_bound_host);
+      }
+    }
   }
-  if (ib) (void)hipFree(ib);
   if (ok && rc == BEMCS_OK) ok = create_set(&h, 1, &h->own) == BEMCS_OK;
   if (!ok || rc != BEMCS_OK) {
     destroy(h);
@@ -2926,6 +3089,32 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
 int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_stream_t stream) {
   BEMCS_REQUIRE(h && is_bound, "bemcs_poisson_update: bad arguments");
   hipStream_t s = as_stream(stream);
+  if (shared_refs(h->op) > 1) {
+    // other handles read these arrays: build into arrays of its own; the
+    // handle's own solve set is made again for the new pointers (other sets
+    // that hold the handle refuse to run until they are made again)
+    OpData* mine = new_op(h);
+    if (!mine) {
+      set_error("bemcs_poisson_update: device allocation failed");
+      return BEMCS_EHIP;
+    }
+    OpData* old = h->op;
+    attach(h, mine);
+    SolveSet* S = nullptr;
+    const int rc = create_set(&h, 1, &S);
+    if (rc != BEMCS_OK) {
+      attach(h, old);
+      free_op(mine);
+      return rc;
+    }
+    release_op(old);
+    ++h->op_gen;
+    destroy_set(h->own);
+    h->own = S;
+    S->gens[0] = h->op_gen;
+  } else {
+    unregister_op(h->op);  // its arrays no longer follow the map it was keyed by
+  }
   int rc = setup_operator(h, is_bound, s);
   if (rc != BEMCS_OK) return rc;
   return setup_status(h, s);
@@ -3002,6 +3191,10 @@ int bemcs_poisson_batch_size(bemcs_poisson_batch_t b) { return b ? b->n : 0; }
 int bemcs_poisson_batch_picard(bemcs_poisson_batch_t b, const bemcs_picard_t* params,
                                int32_t* iters_total, double* worst_relres, bemcs_stream_t stream) {
   BEMCS_REQUIRE(b && params, "bemcs_poisson_batch_picard: bad arguments");
+  for (int c = 0; c < b->n; ++c)
+    BEMCS_REQUIRE(b->cases[c]->op_gen == b->gens[c],
+                  "bemcs_poisson_batch_picard: a handle of the set moved to operator arrays of its own "
+                  "(bemcs_poisson_update of a shared operator); create the set again");
   return picard(b, params, iters_total, worst_relres, as_stream(stream));
 }
 
