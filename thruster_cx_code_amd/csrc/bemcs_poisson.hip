@@ -2020,13 +2020,18 @@ static void register_op(OpData* op, const uint8_t* ib) {
   g_ops.push_back(op);
 }
 
-// Takes op out of the registry (its arrays are about to change).
-static void unregister_op(OpData* op) {
+// True when op has no other handle: it then leaves the registry (its arrays
+// are about to change) in the same critical section, so no handle created
+// meanwhile can start sharing it.
+static bool claim_private(OpData* op) {
   std::lock_guard<std::mutex> lock(g_op_mu);
-  if (!op->keyed) return;
-  op->keyed = false;
-  op->ib.clear();
-  g_ops.erase(std::remove(g_ops.begin(), g_ops.end(), op), g_ops.end());
+  if (op->refs > 1) return false;
+  if (op->keyed) {
+    op->keyed = false;
+    op->ib.clear();
+    g_ops.erase(std::remove(g_ops.begin(), g_ops.end(), op), g_ops.end());
+  }
+  return true;
 }
 
 static void release_op(OpData* op) {
@@ -2039,10 +2044,7 @@ static void release_op(OpData* op) {
   free_op(op);
 }
 
-static int shared_refs(const OpData* op) {
-  std::lock_guard<std::mutex> lock(g_op_mu);
-  return op->refs;
-}
+
 
 static FineOp fine_op(const bemcs_poisson_s* h, int l) {
   FineOp F{};
@@ -3089,7 +3091,7 @@ int bemcs_poisson_create_opt(const uint8_t* is_bound_host, int32_t nx, int32_t n
 int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_stream_t stream) {
   BEMCS_REQUIRE(h && is_bound, "bemcs_poisson_update: bad arguments");
   hipStream_t s = as_stream(stream);
-  if (shared_refs(h->op) > 1) {
+  if (!claim_private(h->op)) {
     // other handles read these arrays: build into arrays of its own; the
     // handle's own solve set is made again for the new pointers (other sets
     // that hold the handle refuse to run until they are made again)
@@ -3112,8 +3114,6 @@ int bemcs_poisson_update(bemcs_poisson_t h, const uint8_t* is_bound, bemcs_strea
     destroy_set(h->own);
     h->own = S;
     S->gens[0] = h->op_gen;
-  } else {
-    unregister_op(h->op);  // its arrays no longer follow the map it was keyed by
   }
   int rc = setup_operator(h, is_bound, s);
   if (rc != BEMCS_OK) return rc;
