@@ -660,16 +660,27 @@ __global__ __launch_bounds__(kBlock) void k_ion_step(bemcs_ion_case_t one,
     const bool hit_final = bound[iy * nx + ix] != 0;
     impact = iy * nx + ix;
     if (!hit_final || SURF) {
+      // the eight sample nodes' electrode flags are loaded together (one
+      // memory round trip instead of eight dependent ones), then the first
+      // electrode sample is taken, as the reference's walk does
+      int cell[8];
+      uint8_t flag[8];
+#pragma unroll
       for (int k = 1; k <= 8; ++k) {
         float f = (float)k / 8.0f;
         float xi = x0 + (x - x0) * f;
         float yi = y0 + (y - y0) * f;
         int cx = round_i64_np(div_rn<FAST>(xi, ddx), 0, nx - 1);
         int cy = round_i64_np(div_rn<FAST>(yi, ddy), 0, ny - 1);
-        if (bound[cy * nx + cx]) {
+        cell[k - 1] = cy * nx + cx;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) flag[k] = bound[cell[k]];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (!hit && flag[k]) {
           hit = true;
-          impact = cy * nx + cx;
-          break;
+          impact = cell[k];
         }
       }
     }
