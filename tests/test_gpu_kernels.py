@@ -277,7 +277,7 @@ def test_efield_bit_exact(lib):
 def test_poisson_error_vs_tolerance(lib):
     """Error against SuperLU as a function of the PCG tolerance on the
     1024 x 384 two-grid mesh (documents the margin of the simulator default
-    poisson_rtol = 1e-9)."""
+    poisson_rtol = 5e-10)."""
     nx, ny = 1024, 384
     o = _geometry(nx, ny, False)
     r = np.random.default_rng(7)
