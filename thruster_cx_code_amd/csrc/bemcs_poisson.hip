@@ -34,8 +34,16 @@ namespace bemcs {
 constexpr int kMaxLevels = 20;
 constexpr int kCoarseMax = 2048;  // default: coarsening stops at or below this many nodes
 constexpr int kDenseMax = 4096;  // largest coarsest level the dense solve accepts
-constexpr float kOmega0 = 0.8f;  // Jacobi damping, 5-point fine level
-constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels
+constexpr float kOmega0 = 0.8f;  // Jacobi damping, 5-point fine level (one-sweep smoothing)
+constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels (one-sweep smoothing)
+// Two-sweep smoothing: the degree-2 Chebyshev polynomial of D^-1 A for the
+// eigenvalue interval [0.5, 2] (Gershgorin bounds D^-1 A by 2 for these
+// diagonally dominant operators, fine and Galerkin alike): sweep weights
+// 1 / (1.25 +- 0.75 cos(pi / 4)).  Its largest amplification on [0.5, 2] is
+// 1 / T2(5/3) = 0.22, against 0.36 for two Jacobi sweeps damped by 0.8; the
+// two factors commute, so the V-cycle stays symmetric whatever their order.
+constexpr float kChebA = 0.56169f;
+constexpr float kChebB = 1.38953f;
 constexpr int kNu = 2;           // pre- and post-smoothing sweeps
 
 enum : uint8_t { K_BOUND = 0, K_UNKNOWN = 1, K_RIGHT = 2, K_TOP = 3, K_BOTTOM = 4 };
@@ -474,7 +482,6 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   __shared__ float sb[N1], sid[N1], s1[N1], s2[N2], sr[N3];
   __shared__ float sst[IS9 ? 9 * N2 : 1];
   __shared__ uint8_t scode[IS9 ? 1 : N2];
-  const float om = IS9 ? kOmegaC : kOmega0;
   const int CX0 = tx * TC, CY0 = ty * TC;
   const int ox = 2 * CX0 - 3, oy = 2 * CY0 - 3;  // fine coordinates of local (0, 0)
   const double* rp = pick2(g.R, (k + 1) & 1);
@@ -570,7 +577,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
       }
       sb[t] = b;
       sid[t] = id;
-      s1[t] = om * id * b;
+      s1[t] = kChebA * id * b;
     }
 #pragma unroll
     for (int it = 0; it < IT2; ++it) {
@@ -624,7 +631,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
     const float id = sid[li];
     float v = 0.0f;
     if (id != 0.0f) {
-      v = s1[li] + om * id * (sb[li] - tile_apply<IS9>(&s1[li], W1, sst, N2, t, IS9 ? 0 : scode[t]));
+      v = s1[li] + kChebB * id * (sb[li] - tile_apply<IS9>(&s1[li], W1, sst, N2, t, IS9 ? 0 : scode[t]));
     }
     s2[t] = v;
     const int gx = ox + lx, gy = oy + ly;
@@ -701,7 +708,6 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
   }
   if (tx >= tiles_x || ty * kT2 >= C.ny) return;
   if (*P.done) return;
-  constexpr float om = kOmegaC;
   // M level: the single-level launch's regions for a kT2 tile of C
   constexpr int MW1 = 2 * kT2 + 5, MW2 = MW1 - 2, MW3 = MW1 - 4;
   constexpr int MN1 = MW1 * MW1, MN2 = MW2 * MW2, MN3 = MW3 * MW3;
@@ -748,7 +754,7 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
       const float b = in ? vb[it] : 0.0f;
       fb[t] = b;
       fid[t] = id;
-      f1[t] = om * id * b;
+      f1[t] = kChebA * id * b;
     }
 #pragma unroll
     for (int it = 0; it < FIT2; ++it) {
@@ -776,7 +782,7 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
     const int li = ly * FW1 + lx;
     const float id = fid[li];
     float v = 0.0f;
-    if (id != 0.0f) v = f1[li] + om * id * (fb[li] - tile_apply<true>(&f1[li], FW1, fst, FN2, t, 0));
+    if (id != 0.0f) v = f1[li] + kChebB * id * (fb[li] - tile_apply<true>(&f1[li], FW1, fst, FN2, t, 0));
     f2[t] = v;
     const int gx = fx0 + lx, gy = fy0 + ly;
     if (lx >= 9 && lx < 9 + 4 * kT2 && ly >= 9 && ly < 9 + 4 * kT2 && gx < F.nx && gy < F.ny)
@@ -821,7 +827,7 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
       if (lx >= 3 && lx < 3 + 2 * kT2 && ly >= 3 && ly < 3 + 2 * kT2) M.b[I] = s;
     }
     mb[t] = s;
-    m1[t] = om * mid[t] * s;
+    m1[t] = kChebA * mid[t] * s;
   }
   __syncthreads();
   // M level, as the single-level launch for the C tile
@@ -830,7 +836,7 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
     const int li = ly * MW1 + lx;
     const float id = mid[li];
     float v = 0.0f;
-    if (id != 0.0f) v = m1[li] + om * id * (mb[li] - tile_apply<true>(&m1[li], MW1, mst, MN2, t, 0));
+    if (id != 0.0f) v = m1[li] + kChebB * id * (mb[li] - tile_apply<true>(&m1[li], MW1, mst, MN2, t, 0));
     m2[t] = v;
     const int gx = mx0 + lx, gy = my0 + ly;
     if (lx >= 3 && lx < 3 + 2 * kT2 && ly >= 3 && ly < 3 + 2 * kT2 && gx < M.nx && gy < M.ny)
@@ -1016,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       const float id = sid[li];
       float v = 0.0f;
       if (id != 0.0f)
-        v = sx[li] + om * id * (sb[li] - tile_apply<IS9>(&sx[li], W0, sst, N1, t, IS9 ? 0 : scode[t]));
+        v = sx[li] + kChebB * id * (sb[li] - tile_apply<IS9>(&sx[li], W0, sst, N1, t, IS9 ? 0 : scode[t]));
       s1[t] = v;
     }
     __syncthreads();
@@ -1032,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     if (id != 0.0f) {
       const float* c = SWEEPS == 2 ? &s1[t1] : &sx[li];
       const int W = SWEEPS == 2 ? W1 : W0;
-      v = c[0] + om * id * (sb[li] - tile_apply<IS9>(c, W, sst, N1, t1, IS9 ? 0 : scode[t1]));
+      v = c[0] + (SWEEPS == 2 ? kChebA : om) * id * (sb[li] - tile_apply<IS9>(c, W, sst, N1, t1, IS9 ? 0 : scode[t1]));
     }
     if (FUSED) {
       sz[t] = v;
