@@ -328,19 +328,16 @@ def test_count_windows_match_oracle(window):
 def test_electron_step_variants_identical():
     """The electron step's execution variants agree bit for bit over several
     steps (potential, ions by id, electrons as a multiset): substeps in one
-    pass or in chunks with sorts between them, with or without a presort,
-    the presort on the side stream or in line on the main stream."""
+    pass or in chunks with sorts between them, with or without a presort."""
     p = c1_params(nx=256, ny=96, ions=100000, electrons=100000)
     from thruster_cx_code_amd import DigitalTwinSimulator
     out = []
-    variants = [(0, False, True), (16, True, True), (0, True, True), (16, False, True),
-                (0, True, False), (16, True, False)]
-    for chunk, presort, side in variants:
-        print(f"[gpu] electron variant chunk={chunk} presort={presort} side={side}", flush=True)
+    variants = [(0, False), (16, True), (0, True), (16, False)]
+    for chunk, presort in variants:
+        print(f"[gpu] electron variant chunk={chunk} presort={presort}", flush=True)
         g = DigitalTwinSimulator()
         g.m_e = g.m_ion / 200
         g.electron_chunk, g.electron_chunk_min, g.electron_presort = chunk, 0, presort
-        g.electron_presort_side = side
         g.build_domain(p)
         g.advance(p, 6)
         out.append((g.V.copy(), _ions_by_pid(g.p_x, g.p_y, g.p_vx, g.p_vy, g.p_vz, g.p_pid, g.p_isCEX),

@@ -95,12 +95,6 @@ class DigitalTwinSimulator:
         # sort before the first chunk too (the step starts from the last
         # step's order plus appended refills and SEE electrons)
         self.electron_presort = os.environ.get("BEMCS_EPRESORT", "1") == "1"
-        # run that presort on a second stream (default priority) while the
-        # main stream runs the Poisson call, the substep counts and the ion
-        # step: it reads only the step's input electrons, which none of
-        # those write
-        self.electron_presort_side = os.environ.get("BEMCS_EPRESORT_SIDE", "1") == "1"
-        self._side = None
         # LDS window of the NGP / PPC count histograms (0: 64 x 32 nodes)
         self._count_window = 0
         self.poisson_rtol = float(poisson_rtol)
@@ -792,10 +786,6 @@ class DigitalTwinSimulator:
             self._timed_end(self.kernel_timing, "deposit_cic", ev)
         I0, I1 = self.ions
         E0, E1 = self.elec
-        # E0 stays the unsorted input for the step's records and substep
-        # counts (a count and a maximum, both order-free); self.elec[0]
-        # becomes the sorted copy the electron push reads
-        presorted = self._presort_electrons(self.kernel_timing) if self.electron_presort_side else None
         dsb = self.ds_bits.data_ptr()
         nsb = self.n_sub_d.data_ptr()
         # clear this step's outputs and record the pushed populations (before
@@ -843,7 +833,7 @@ class DigitalTwinSimulator:
         ep = self._elec_params
         ep.Exy, ep.is_bound, ep.n_sub = ptr(self.Exy), ptr(self.is_bound_d), C.c_void_p(nsb + 4)
         ep.cnt_next, ep.overflow = ptr(self.cnt_e_next), ptr(self.overflow_d)
-        E0 = self._electron_chunks(ep, kt, s, presorted)
+        E0 = self._electron_chunks(ep, kt, s)
         ev = self._timed_begin(kt)
         check(lib.bemcs_electron_step(C.byref(E0.struct()), C.byref(E1.struct()), C.byref(ep),
                                       self.bound_e, s), "electron_step")
@@ -886,34 +876,7 @@ class DigitalTwinSimulator:
         self._pending.append((ev, slot, self._bi.snapshot(), self._be.snapshot(), it))
         return slot
 
-    def _presort_electrons(self, kt):
-        """Launch the cell presort of the step's input electrons on the side
-        stream, ordered after everything already on the main stream (the
-        step's injections included), and make its output self.elec[0].
-        Returns the event the main stream waits on before the electron push,
-        or None when this step presorts nothing."""
-        if not (self.electron_presort and self.bound_e >= self.electron_chunk_min):
-            return None
-        E0 = self.elec[0]
-        tmp = self._elec_spare()
-        scratch = ptr(self.sort_scratch_for(self.bound_e))
-        if self._side is None:
-            self._side = (torch.cuda.Stream(device=self.device), torch.cuda.Event(), torch.cuda.Event())
-        side, ready, done = self._side
-        # recorded after the buffers above were (re)allocated on the main stream
-        ready.record()
-        side.wait_event(ready)
-        sx, sy = self.electron_tile
-        ev = self._timed_begin(kt, side)
-        check(self.lib.bemcs_sort_by_tile(C.byref(E0.struct()), C.byref(tmp.struct()), self.bound_e,
-                                          self.nx, self.ny, self.dx, self.dy, sx, sy, scratch, stream_ptr(side)),
-              "sort_by_tile")
-        self._timed_end(kt, "electron_sort", ev, side)
-        done.record(side)
-        self.elec[0], self._e_spare = tmp, E0
-        return done
-
-    def _electron_chunks(self, ep, kt, s, presorted=None):
+    def _electron_chunks(self, ep, kt, s):
         """Electrons take hundreds of substeps per step and cross many cells,
         so a wavefront's gathers spread over the mesh.  With a large
         population (electron_chunk_min) the electrons are sorted by cell
@@ -921,22 +884,16 @@ class DigitalTwinSimulator:
         SEE electrons appended), and the substeps are split into K chunks of
         about electron_chunk substeps (K from the last step whose substep
         count the host has seen) with a sort after each chunk but the last;
-        the fused step then runs the last chunk.  `presorted` is the event
-        of a presort already launched on the side stream (self.elec[0] is
-        its output).  Returns the buffer holding the electrons for the fused
-        step."""
+        the fused step then runs the last chunk.  Returns the buffer holding
+        the electrons for the fused step."""
         E0 = self.elec[0]
         ep.sub_first, ep.sub_chunks = 0, 1
         c = self.electron_chunk
-        if presorted is not None:
-            torch.cuda.current_stream().wait_event(presorted)
-        elif self.bound_e < self.electron_chunk_min or (c <= 0 and not self.electron_presort):
+        if self.bound_e < self.electron_chunk_min or (c <= 0 and not self.electron_presort):
             return E0
         k = 1
         if c > 0 and self._nsub_e_hint >= 2 * c:
             k = min(max(int(round(self._nsub_e_hint / c)), 2), 64)
-        if k == 1 and presorted is not None:
-            return E0
         tmp = self._elec_spare()
         scratch = ptr(self.sort_scratch_for(self.bound_e))
         sx, sy = self.electron_tile
@@ -949,7 +906,7 @@ class DigitalTwinSimulator:
             self._timed_end(kt, "electron_sort", ev)
             return tmp, E0
 
-        if self.electron_presort and presorted is None:
+        if self.electron_presort:
             E0, tmp = sort(E0, tmp)
         ep.sub_chunks = k
         for j in range(k - 1):
@@ -1124,19 +1081,19 @@ class DigitalTwinSimulator:
         return 0.0
 
     @staticmethod
-    def _timed_begin(kt, stream=None):
+    def _timed_begin(kt):
         if kt is None:
             return None
         e = torch.cuda.Event(enable_timing=True)
-        e.record(stream)
+        e.record()
         return e
 
     @staticmethod
-    def _timed_end(kt, name, start, stream=None):
+    def _timed_end(kt, name, start):
         if kt is None:
             return
         e = torch.cuda.Event(enable_timing=True)
-        e.record(stream)
+        e.record()
         kt.setdefault(name, []).append((start, e))
 
     def _sort(self, ions, electrons):
