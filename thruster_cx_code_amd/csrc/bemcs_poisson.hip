@@ -491,40 +491,35 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   // in this launch; large tiles take it from the direction launch, which is
   // cheaper there than the wider halo loads of the fused form.
   constexpr bool FUSED = STEP && TC == kTC;
-  if (STEP && !FUSED) {
-    double pap, u;
-    const double rz = g.scal[S_RZ0 + (k & 1)];  // block 0 of the direction launch
-    block_sum2(g.pa, g.nparts, pap, u);
-    alpha = pap != 0.0 ? rz / pap : 0.0;
-  }
-  if (FUSED) {
-    // direction and step of iteration k in one launch: beta from the r.z
-    // partial sums of the last V-cycle, p.Ap from the expansion
-    // (z + beta p).(Az + beta Ap) = z.Az + 2 beta z.Ap + beta^2 p.Ap (A
-    // symmetric), so the direction needs no launch and no reduction of its own
-    double q[4];
-    if (g.presum) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] = g.scal[S_Q0 + j];
-    } else {
-      block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
-    }
-    const double rz = q[0], roz = q[1], zaz = q[2], zap = q[3];
-    double pap = zaz;
-    if (k > 1) {
-      const double rz_old = g.scal[S_RZ0 + ((k - 1) & 1)];
-      beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
-      pap = zaz + 2.0 * beta * zap + beta * beta * g.scal[S_PAP0 + ((k - 1) & 1)];
-    }
-    alpha = pap != 0.0 ? rz / pap : 0.0;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-      g.scal[S_RZ0 + (k & 1)] = rz;
-      g.scal[S_PAP0 + (k & 1)] = pap;
-    }
-  }
   // fused: A z and the previous Ap; otherwise the direction launch's Ap
   const double* azp = FUSED ? g.Az : g.Ap;
   const double* apo = pick2(g.AP, (k + 1) & 1);  // Ap of iteration k-1 (fused)
+  const double* po = pick2(g.P, (k + 1) & 1);
+  // The owned nodes' CG values are loaded with the staged region, and the
+  // scalar reductions run after both are issued, so the launch waits for
+  // memory about once.  This launch writes none of the values it loads.
+  constexpr int NS = STEP ? TFd * TFd / 256 : 1;
+  double o_r[NS], o_x[NS], o_p[NS], o_ap[NS], o_apo[NS];
+  float o_z[NS];
+  if (STEP) {
+#pragma unroll
+    for (int it = 0; it < NS; ++it) {
+      const int t = threadIdx.x + it * 256;
+      const int gx = 2 * CX0 + t % TFd, gy = 2 * CY0 + t / TFd;
+      const int gi = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
+      o_r[it] = rp[gi];
+      o_x[it] = g.x[gi];
+      if (FUSED) {
+        o_z[it] = g.z[gi];
+        o_p[it] = k > 1 ? po[gi] : 0.0;
+        o_ap[it] = azp[gi];
+        o_apo[it] = k > 1 ? apo[gi] : 0.0;
+      } else {
+        o_p[it] = g.p[gi];
+        o_ap[it] = g.Ap[gi];
+      }
+    }
+  }
   {
     int vg[IT1];
     float vid[IT1], vb[IT1];
@@ -558,6 +553,37 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
         for (int q = 0; q < 9; ++q) vst[it][q] = F.st[(size_t)q * F.n + gc];
       } else {
         vcode[it] = F.code[gc];
+      }
+    }
+    if (STEP && !FUSED) {
+      double pap, u;
+      const double rz = g.scal[S_RZ0 + (k & 1)];  // block 0 of the direction launch
+      block_sum2(g.pa, g.nparts, pap, u);
+      alpha = pap != 0.0 ? rz / pap : 0.0;
+    }
+    if (FUSED) {
+      // direction and step of iteration k in one launch: beta from the r.z
+      // partial sums of the last V-cycle, p.Ap from the expansion
+      // (z + beta p).(Az + beta Ap) = z.Az + 2 beta z.Ap + beta^2 p.Ap (A
+      // symmetric), so the direction needs no launch and no reduction of its own
+      double q[4];
+      if (g.presum) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = g.scal[S_Q0 + j];
+      } else {
+        block_sum4(pick2(g.pz, (k + 1) & 1), g.nparts, q);
+      }
+      const double rz = q[0], roz = q[1], zaz = q[2], zap = q[3];
+      double pap = zaz;
+      if (k > 1) {
+        const double rz_old = g.scal[S_RZ0 + ((k - 1) & 1)];
+        beta = rz_old != 0.0 ? (rz - roz) / rz_old : 0.0;
+        pap = zaz + 2.0 * beta * zap + beta * beta * g.scal[S_PAP0 + ((k - 1) & 1)];
+      }
+      alpha = pap != 0.0 ? rz / pap : 0.0;
+      if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        g.scal[S_RZ0 + (k & 1)] = rz;
+        g.scal[S_PAP0 + (k & 1)] = pap;
       }
     }
 #pragma unroll
@@ -598,26 +624,25 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
     double* rn_out = pick2(g.R, k & 1);
     double* pn = pick2(g.P, k & 1);
     double* apn = pick2(g.AP, k & 1);
-    const double* po = pick2(g.P, (k + 1) & 1);
 #pragma unroll
-    for (int it = 0; it < TFd * TFd / 256; ++it) {
+    for (int it = 0; it < NS; ++it) {
       const int t = threadIdx.x + it * 256;
       const int gx = 2 * CX0 + t % TFd, gy = 2 * CY0 + t / TFd;
       if (gx >= F.nx || gy >= F.ny) continue;
       const int gi = gy * F.nx + gx;
       double pi, api;
       if (FUSED) {
-        const double zi = (double)g.z[gi];
-        pi = k > 1 ? zi + beta * po[gi] : zi;
-        api = k > 1 ? azp[gi] + beta * apo[gi] : azp[gi];
+        const double zi = (double)o_z[it];
+        pi = k > 1 ? zi + beta * o_p[it] : zi;
+        api = k > 1 ? o_ap[it] + beta * o_apo[it] : o_ap[it];
         pn[gi] = pi;
         apn[gi] = api;
       } else {
-        pi = g.p[gi];
-        api = g.Ap[gi];
+        pi = o_p[it];
+        api = o_ap[it];
       }
-      const double rn = rp[gi] - alpha * api;
-      g.x[gi] += alpha * pi;
+      const double rn = o_r[it] - alpha * api;
+      g.x[gi] = o_x[it] + alpha * pi;
       rn_out[gi] = rn;
       F.b[gi] = (float)rn;
       rr += rn * rn;
@@ -935,12 +960,13 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   __shared__ float sz[FUSED ? TZ * TZ : 1];
   const float om = IS9 ? kOmegaC : kOmega0;
   const int ox = tx * TF - H, oy = ty * TF - H;
-  // fused: the owned nodes' r_k, r_{k-1}, Ap_{k-1} and operator code, used
-  // after the sweeps, are loaded now so their latency overlaps the sweeps
-  constexpr int NOWN = FUSED ? (TF * TF + 255) / 256 : 1;
+  // the owned nodes' r_k and r_{k-1} (and, fused, Ap_{k-1} and the operator
+  // code), used after the sweeps, are loaded now so their latency overlaps
+  // the sweeps
+  constexpr int NOWN = DOT ? (TF * TF + 255) / 256 : 1;
   double o_r[NOWN], o_ro[NOWN], o_ap[NOWN];
   int o_code[NOWN];
-  if (FUSED) {
+  if (DOT) {
 #pragma unroll
     for (int it = 0; it < NOWN; ++it) {
       const int t = min((int)threadIdx.x + it * 256, TF * TF - 1);
@@ -948,8 +974,10 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       const int gc = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
       o_r[it] = pick2(g.R, k & 1)[gc];
       o_ro[it] = pick2(g.R, (k + 1) & 1)[gc];
-      o_ap[it] = k > 0 ? pick2(g.AP, k & 1)[gc] : 0.0;
-      o_code[it] = F.code[gc];
+      if (FUSED) {
+        o_ap[it] = k > 0 ? pick2(g.AP, k & 1)[gc] : 0.0;
+        o_code[it] = F.code[gc];
+      }
     }
   }
   {
@@ -1028,7 +1056,11 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     __syncthreads();
   }
   double rz = 0.0, roz = 0.0, zaz = 0.0, zap = 0.0;
-  for (int t = threadIdx.x; t < TZ * TZ; t += 256) {
+  constexpr int NZ = (TZ * TZ + 255) / 256;
+#pragma unroll
+  for (int it = 0; it < NZ; ++it) {
+    const int t = threadIdx.x + it * 256;
+    if (t >= TZ * TZ) break;
     const int lx = t % TZ + SWEEPS, ly = t / TZ + SWEEPS;
     const int gx = ox + lx, gy = oy + ly;
     const int li = ly * W0 + lx;
@@ -1047,9 +1079,9 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
     if (gx >= F.nx || gy >= F.ny) continue;
     const int gi = gy * F.nx + gx;
     F.t[gi] = v;
-    if (DOT) {
-      rz += pick2(g.R, k & 1)[gi] * (double)v;
-      roz += pick2(g.R, (k + 1) & 1)[gi] * (double)v;
+    if (DOT && it < NOWN) {  // unfused: TZ = TF, so t indexes the owned nodes as above
+      rz += o_r[it] * (double)v;
+      roz += o_ro[it] * (double)v;
     }
   }
   if (FUSED) {
@@ -1213,11 +1245,65 @@ __global__ __launch_bounds__(256) void k_sum_quads(CGArgs one, const CGArgs* __r
   if (threadIdx.x < 4) g.scal[S_Q0 + threadIdx.x] = q[threadIdx.x];
 }
 
-// p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap
+// p = z + beta p ; Ap = A z + beta Ap ; partials p.Ap.  A thread's nodes
+// are taken kDirBatch at a time: every load of a batch is issued before any
+// of its stores (stores to p and Ap would otherwise keep the next node's
+// loads waiting), and the first batch before the reduction that gives beta.
+// Each thread adds its nodes in increasing order, as a one-node loop would.
+constexpr int kDirBatch = 4;
+struct DirNodes {
+  int d[kDirBatch];
+  float zc[kDirBatch], zw[kDirBatch], ze[kDirBatch], zs[kDirBatch], zn[kDirBatch];
+  double p[kDirBatch], ap[kDirBatch];
+};
+
+__device__ __forceinline__ void dir_load(const CGArgs& g, int k, int base, int end, int step, DirNodes& v) {
+#pragma unroll
+  for (int u = 0; u < kDirBatch; ++u) {
+    const int i = base + u * step;
+    const int c = i < end ? i : base;  // base < end: a node of this thread
+    const int y = c / g.nx;
+    const int sn = y > 0 ? c - g.nx : (g.per ? c + (g.ny - 1) * g.nx : c);
+    const int nn = y < g.ny - 1 ? c + g.nx : (g.per ? c - (g.ny - 1) * g.nx : c);
+    v.d[u] = g.code[c];
+    v.zc[u] = g.z[c];
+    // a node with a nonzero code has both x neighbours (the clamps only keep
+    // the loads of the other nodes in bounds)
+    v.zw[u] = g.z[c > 0 ? c - 1 : c];
+    v.ze[u] = g.z[c < g.n - 1 ? c + 1 : c];
+    v.zs[u] = g.z[sn];
+    v.zn[u] = g.z[nn];
+    v.p[u] = k > 1 ? g.p[c] : 0.0;
+    v.ap[u] = k > 1 ? g.Ap[c] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void dir_store(const CGArgs& g, int k, double beta, int base, int end, int step,
+                                          const DirNodes& v, double& s) {
+#pragma unroll
+  for (int u = 0; u < kDirBatch; ++u) {
+    const int i = base + u * step;
+    if (i >= end) break;
+    double az = 0.0;
+    if (v.d[u]) {
+      const double nb = (double)v.zw[u] + (double)v.ze[u] + (double)v.zs[u] + (double)v.zn[u];
+      az = (double)v.d[u] * (double)v.zc[u] - nb;
+    }
+    const double pi = k > 1 ? (double)v.zc[u] + beta * v.p[u] : (double)v.zc[u];
+    const double api = k > 1 ? az + beta * v.ap[u] : az;
+    g.p[i] = pi;
+    g.Ap[i] = api;
+    s += pi * api;
+  }
+}
+
 template <bool B>
 __global__ __launch_bounds__(256) void k_cg_dir(CGArgs one, const CGArgs* __restrict__ many, int k) {
   CG_CASE();
   if (*g.done) return;
+  const int i0 = band_start(g.n, g.nparts), end = band_end(g.n, g.nparts), step = band_step(g.nparts);
+  DirNodes v;
+  if (i0 < end) dir_load(g, k, i0, end, step, v);
   double beta = 0.0;
   if (k > 1 || blockIdx.x == 0) {
     double rz, roz;
@@ -1229,22 +1315,9 @@ __global__ __launch_bounds__(256) void k_cg_dir(CGArgs one, const CGArgs* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[S_RZ0 + (k & 1)] = rz;
   }
   double s = 0;
-  GRID_LOOP(i, g.n) {
-    double az = 0.0;
-    const int d = g.code[i];
-    if (d) {
-      const int x = i % g.nx, y = i / g.nx;
-      const int sn = y > 0 ? i - g.nx : (g.per ? i + (g.ny - 1) * g.nx : i);
-      const int nn = y < g.ny - 1 ? i + g.nx : (g.per ? i - (g.ny - 1) * g.nx : i);
-      const double nb = (double)g.z[i - 1] + (double)g.z[i + 1] + (double)g.z[sn] + (double)g.z[nn];
-      az = (double)d * (double)g.z[i] - nb;
-      (void)x;
-    }
-    const double pi = k > 1 ? (double)g.z[i] + beta * g.p[i] : (double)g.z[i];
-    const double api = k > 1 ? az + beta * g.Ap[i] : az;
-    g.p[i] = pi;
-    g.Ap[i] = api;
-    s += pi * api;
+  for (int base = i0; base < end; base += kDirBatch * step) {
+    if (base != i0) dir_load(g, k, base, end, step, v);
+    dir_store(g, k, beta, base, end, step, v, s);
   }
   block_partials(s, 0.0, g.pa, blockIdx.x);
 }
