@@ -495,8 +495,10 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
   const double* azp = FUSED ? g.Az : g.Ap;
   const double* apo = pick2(g.AP, (k + 1) & 1);  // Ap of iteration k-1 (fused)
   const double* po = pick2(g.P, (k + 1) & 1);
-  // The owned nodes' CG values are loaded with the staged region, and the
-  // scalar reductions run after both are issued, so the launch waits for
+  // The owned nodes' CG values are loaded with the staged region (unfused:
+  // p and x; r and Ap, just staged for the halo region, are read again from
+  // the caches, which keeps the register count at five waves per SIMD), and
+  // the scalar reductions run after both are issued, so the launch waits for
   // memory about once.  This launch writes none of the values it loads.
   constexpr int NS = STEP ? TFd * TFd / 256 : 1;
   double o_r[NS], o_x[NS], o_p[NS], o_ap[NS], o_apo[NS];
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
       const int t = threadIdx.x + it * 256;
       const int gx = 2 * CX0 + t % TFd, gy = 2 * CY0 + t / TFd;
       const int gi = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
-      o_r[it] = rp[gi];
+      if (FUSED) o_r[it] = rp[gi];
       o_x[it] = g.x[gi];
       if (FUSED) {
         o_z[it] = g.z[gi];
@@ -516,7 +518,6 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
         o_apo[it] = k > 1 ? apo[gi] : 0.0;
       } else {
         o_p[it] = g.p[gi];
-        o_ap[it] = g.Ap[gi];
       }
     }
   }
@@ -639,9 +640,9 @@ __global__ __launch_bounds__(256) void k_down(DownPack one, const DownPack* __re
         apn[gi] = api;
       } else {
         pi = o_p[it];
-        api = o_ap[it];
+        api = g.Ap[gi];
       }
-      const double rn = o_r[it] - alpha * api;
+      const double rn = (FUSED ? o_r[it] : rp[gi]) - alpha * api;
       g.x[gi] = o_x[it] + alpha * pi;
       rn_out[gi] = rn;
       F.b[gi] = (float)rn;
@@ -960,13 +961,13 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
   __shared__ float sz[FUSED ? TZ * TZ : 1];
   const float om = IS9 ? kOmegaC : kOmega0;
   const int ox = tx * TF - H, oy = ty * TF - H;
-  // the owned nodes' r_k and r_{k-1} (and, fused, Ap_{k-1} and the operator
-  // code), used after the sweeps, are loaded now so their latency overlaps
-  // the sweeps
+  // fused: the owned nodes' r_k, r_{k-1}, Ap_{k-1} and operator code, used
+  // after the sweeps, are loaded now so their latency overlaps the sweeps
+  // (unfused: r_k and r_{k-1} only, loaded after the staging below)
   constexpr int NOWN = DOT ? (TF * TF + 255) / 256 : 1;
   double o_r[NOWN], o_ro[NOWN], o_ap[NOWN];
   int o_code[NOWN];
-  if (DOT) {
+  if (FUSED) {
 #pragma unroll
     for (int it = 0; it < NOWN; ++it) {
       const int t = min((int)threadIdx.x + it * 256, TF * TF - 1);
@@ -974,10 +975,8 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       const int gc = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
       o_r[it] = pick2(g.R, k & 1)[gc];
       o_ro[it] = pick2(g.R, (k + 1) & 1)[gc];
-      if (FUSED) {
-        o_ap[it] = k > 0 ? pick2(g.AP, k & 1)[gc] : 0.0;
-        o_code[it] = F.code[gc];
-      }
+      o_ap[it] = k > 0 ? pick2(g.AP, k & 1)[gc] : 0.0;
+      o_code[it] = F.code[gc];
     }
   }
   {
@@ -1040,6 +1039,18 @@ __global__ __launch_bounds__(256) void k_up(UpPack one, const UpPack* __restrict
       } else {
         scode[t] = in ? vcode[it] : (uint8_t)0;
       }
+    }
+  }
+  if (DOT && !FUSED) {
+    // unfused: the owned nodes' r_k and r_{k-1}, loaded once the staged
+    // values are in LDS (fewer live registers) so the sweeps hide their latency
+#pragma unroll
+    for (int it = 0; it < NOWN; ++it) {
+      const int t = min((int)threadIdx.x + it * 256, TF * TF - 1);
+      const int gx = ox + H + t % TF, gy = oy + H + t / TF;
+      const int gc = (gx < F.nx && gy < F.ny) ? gy * F.nx + gx : 0;
+      o_r[it] = pick2(g.R, k & 1)[gc];
+      o_ro[it] = pick2(g.R, (k + 1) & 1)[gc];
     }
   }
   __syncthreads();
