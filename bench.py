@@ -578,7 +578,7 @@ def sweep_bench(args):
     n = len(cases)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sims = [build_case(p, coarse_max=args.sweep_coarse, level0_tile=args.sweep_tile) for p in cases]
+    sims = [build_case(p, coarse_max=args.sweep_coarse or None, level0_tile=args.sweep_tile) for p in cases]
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     _progress(f"built {n} cases in {t_build:.1f} s")
@@ -660,7 +660,8 @@ def serve_bench(args):
     # 50 theta95 values have a standard deviation below 0.1 degree
     # (Python/benchmarks/benchmark_perveance.py:56-60), capped at max_steps
     min_steps = min(1500, args.serve_max_steps)
-    srv = SweepServer(max_steps=args.serve_max_steps, min_steps=min_steps, max_active=args.serve_slots)
+    srv = SweepServer(max_steps=args.serve_max_steps, min_steps=min_steps, max_active=args.serve_slots,
+                      coarse_max=args.serve_coarse)
     t0 = time.perf_counter()
     last = [0.0]
 
@@ -760,6 +761,8 @@ def main():
         sp.add_argument("--serve-rate", type=float, default=4.0)
         sp.add_argument("--serve-max-steps", type=int, default=4000)
         sp.add_argument("--serve-slots", type=int, default=64)
+        sp.add_argument("--serve-coarse", type=int, default=None,
+                        help="coarsest multigrid level of the served cases (nodes; default the solver's)")
         sp.add_argument("--serve-seq", type=int, default=0,
                         help="also time this many served requests one at a time")
         serve_bench(sp.parse_args())
@@ -770,8 +773,8 @@ def main():
         sp.add_argument("--sweep-cases", type=int, default=64)
         sp.add_argument("--sweep-steps", type=int, default=3000)
         sp.add_argument("--sweep-seq-cases", type=int, default=16)
-        sp.add_argument("--sweep-coarse", type=int, default=None,
-                        help="coarsest multigrid level of the batched cases (nodes; default the solver's)")
+        sp.add_argument("--sweep-coarse", type=int, default=1024,
+                        help="coarsest multigrid level of the batched cases (nodes; 0: the solver's default)")
         sp.add_argument("--sweep-tile", type=int, default=16,
                         help="level-0 tile edge of the batched cases' Poisson solves (0: by mesh size)")
         sp.add_argument("--no-cpu-baseline", action="store_true")

@@ -111,11 +111,12 @@ class SweepServer:
     stop (the reference uses 1500); check_every: steps between completion
     checks (the stop step itself is found exactly from the per-step records);
     level0_tile: level-0 tile edge of the cases' Poisson solves (16: the
-    large tiles, faster in large batches).
+    large tiles, faster in large batches); coarse_max: bound on the nodes of
+    the coarsest multigrid level of the cases' solves (None: the solver's).
     """
 
     def __init__(self, hbm_budget=None, max_active=64, max_steps=4000, min_steps=1500,
-                 check_every=25, tol_deg=0.1, window=50, chunk=5, level0_tile=16):
+                 check_every=25, tol_deg=0.1, window=50, chunk=5, level0_tile=16, coarse_max=None):
         if hbm_budget is None:
             free, total = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * total)
@@ -128,6 +129,7 @@ class SweepServer:
         self.window = int(window)
         self.chunk = int(chunk)
         self.level0_tile = level0_tile
+        self.coarse_max = coarse_max
         self.batch = BatchedSimulator()
         self.log = []
 
@@ -182,7 +184,7 @@ class SweepServer:
                 req = queue[qi]
                 qi += 1
                 try:
-                    sim = build_case(req.params, level0_tile=self.level0_tile)
+                    sim = build_case(req.params, coarse_max=self.coarse_max, level0_tile=self.level0_tile)
                 except ValueError as exc:  # the reference's build checks reject it
                     t = time.perf_counter() - t0
                     results.append(Result(req.id, 0, False, float("nan"), [], req.arrival, t, t, str(exc)))
