@@ -44,7 +44,6 @@ constexpr float kOmegaC = 0.7f;  // Jacobi damping, Galerkin 9-point levels (one
 // two factors commute, so the V-cycle stays symmetric whatever their order.
 constexpr float kChebA = 0.56169f;
 constexpr float kChebB = 1.38953f;
-constexpr int kNu = 2;           // pre- and post-smoothing sweeps
 
 enum : uint8_t { K_BOUND = 0, K_UNKNOWN = 1, K_RIGHT = 2, K_TOP = 3, K_BOTTOM = 4 };
 
