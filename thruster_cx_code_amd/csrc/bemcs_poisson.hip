@@ -1929,29 +1929,47 @@ __global__ __launch_bounds__(256) void k_gj_panel(const double* __restrict__ A, 
 }
 
 // Part 2: the panel update of every entry from C, R and T only (the panel's
-// vanished pivots count as rows and columns outside it).
+// vanished pivots count as rows and columns outside it).  A thread owns one
+// column j of kGJRows consecutive rows and keeps that column's p entries of R
+// in registers; each entry is formed by the same operations in the same order
+// as a one-entry-per-thread update.
+constexpr int kGJRows = 4;
 __global__ __launch_bounds__(256) void k_gj_update(double* A, int n, int K0, int p,
                                                    const double* __restrict__ C,
                                                    const double* __restrict__ R,
                                                    const double* __restrict__ Tg,
                                                    const uint8_t* __restrict__ swept) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
-  const bool ki = i >= K0 && i < K0 + p && swept[i], kj = j >= K0 && j < K0 + p && swept[j];
-  double v;
-  if (ki && kj) {
-    v = Tg[(i - K0) * kGJPanel + (j - K0)];
-  } else if (ki) {
-    v = 0.0;
-    for (int a = 0; a < p; ++a) v += Tg[(i - K0) * kGJPanel + a] * R[(size_t)a * n + j];
-  } else if (kj) {
-    v = -C[(size_t)i * kGJPanel + (j - K0)];
-  } else {
-    v = A[e];
-    for (int a = 0; a < p; ++a) v -= C[(size_t)i * kGJPanel + a] * R[(size_t)a * n + j];
+  const int j = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (j >= n) return;
+  const bool kj = j >= K0 && j < K0 + p && swept[j];
+  double r[kGJPanel];
+#pragma unroll
+  for (int a = 0; a < kGJPanel; ++a) r[a] = a < p ? R[(size_t)a * n + j] : 0.0;
+  const int i0 = (int)blockIdx.y * kGJRows;
+#pragma unroll
+  for (int ii = 0; ii < kGJRows; ++ii) {
+    const int i = i0 + ii;
+    if (i >= n) break;
+    const bool ki = i >= K0 && i < K0 + p && swept[i];
+    const size_t e = (size_t)i * n + j;
+    double v;
+    if (ki && kj) {
+      v = Tg[(i - K0) * kGJPanel + (j - K0)];
+    } else if (ki) {
+      v = 0.0;
+#pragma unroll
+      for (int a = 0; a < kGJPanel; ++a)
+        if (a < p) v += Tg[(i - K0) * kGJPanel + a] * r[a];
+    } else if (kj) {
+      v = -C[(size_t)i * kGJPanel + (j - K0)];
+    } else {
+      v = A[e];
+#pragma unroll
+      for (int a = 0; a < kGJPanel; ++a)
+        if (a < p) v -= C[(size_t)i * kGJPanel + a] * r[a];
+    }
+    A[e] = v;
   }
-  A[e] = v;
 }
 
 template <bool L0>
@@ -2186,8 +2204,8 @@ static int setup_operator(bemcs_poisson_s* h, const uint8_t* is_bound, hipStream
     const unsigned gp = std::min<unsigned>(grid_for((int64_t)nc * p, 256), 1024u);
     hipLaunchKernelGGL(k_gj_panel, dim3(gp), dim3(256), 0, s, h->su_A, h->su_d0, nc, K0, p, h->su_C, h->su_R,
                        h->su_T, h->su_swept);
-    hipLaunchKernelGGL(k_gj_update, dim3(gd), dim3(256), 0, s, h->su_A, nc, K0, p, h->su_C, h->su_R,
-                       h->su_T, h->su_swept);
+    hipLaunchKernelGGL(k_gj_update, dim3((nc + 255) / 256, (nc + kGJRows - 1) / kGJRows), dim3(256), 0, s,
+                       h->su_A, nc, K0, p, h->su_C, h->su_R, h->su_T, h->su_swept);
   }
   if (lc == 0)
     hipLaunchKernelGGL(k_dense_store<true>, dim3(gd), dim3(256), 0, s, Lc, h->su_A, h->su_swept, h->cinv);
