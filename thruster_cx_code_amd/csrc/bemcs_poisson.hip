@@ -908,26 +908,30 @@ __global__ __launch_bounds__(256) void k_down2(Down2Pack one, const Down2Pack* _
 // have weight 0 and index 0.
 __device__ __forceinline__ void pe_terms(const LevelDev& F, const LevelDev& C, int i, int* idx,
                                          float* w) {
+  // bilinear prolongation, the weights w1d gives: an even fine index takes its
+  // coarse node with weight 1, an odd one the two coarse nodes beside it with
+  // 1/2 each, or the lower one alone with weight 1 where the upper one is past
+  // a non-periodic edge (periodic in y: the upper row wraps)
   const int fx = i % F.nx, fy = i / F.nx;
-  int xs[2] = {fx >> 1, -1}, ys[2] = {fy >> 1, -1};
-  if (fx & 1) {
-    xs[0] = (fx - 1) >> 1;
-    xs[1] = (fx + 1) >> 1;
-    if (xs[1] >= C.nx) xs[1] = -1;
+  const int x0 = fx >> 1, y0 = fy >> 1;
+  const bool x1ok = (fx & 1) && x0 + 1 < C.nx;
+  int y1 = y0 + 1;
+  bool y1ok = (fy & 1) != 0;
+  if (y1 >= C.ny) {
+    if (F.periodic) y1 -= C.ny;
+    else y1ok = false;
   }
-  if (fy & 1) {
-    ys[0] = (fy - 1) >> 1;
-    ys[1] = (fy + 1) >> 1;
-    if (ys[1] >= C.ny) ys[1] = F.periodic ? ys[1] - C.ny : -1;
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const bool ok = ys[a] >= 0 && xs[b] >= 0;
-      idx[2 * a + b] = ok ? ys[a] * C.nx + xs[b] : 0;
-      w[2 * a + b] = ok ? w1d(fy, ys[a], C.ny, F.periodic != 0) * w1d(fx, xs[b], C.nx, false) : 0.0f;
-    }
+  const float wx0 = x1ok ? 0.5f : 1.0f, wx1 = x1ok ? 0.5f : 0.0f;
+  const float wy0 = y1ok ? 0.5f : 1.0f, wy1 = y1ok ? 0.5f : 0.0f;
+  const int x1 = x1ok ? x0 + 1 : 0;
+  idx[0] = y0 * C.nx + x0;
+  idx[1] = x1ok ? y0 * C.nx + x1 : 0;
+  idx[2] = y1ok ? y1 * C.nx + x0 : 0;
+  idx[3] = (y1ok && x1ok) ? y1 * C.nx + x1 : 0;
+  w[0] = wy0 * wx0;
+  w[1] = wy0 * wx1;
+  w[2] = wy1 * wx0;
+  w[3] = wy1 * wx1;
 }
 
 // C.x must point at the coarse result (C.t, or C.x of the coarsest level).
