@@ -110,6 +110,7 @@ class BatchedSimulator:
         self.poisson_iter_log = []
         self.kernel_timing = None
         self._events = None  # count-readback events, made at the first step
+        self._ahead = []  # pool futures: the next step's beam-ion counts
         for s, p in zip(sims, params):
             self.add(s, p)
 
@@ -187,26 +188,48 @@ class BatchedSimulator:
         for _ in range(int(n_steps)):
             self._step()
 
+    @staticmethod
+    def _chunks(items, weight, nchunk):
+        """items split into nchunk lists of similar total weight (largest
+        first, each to the lightest list)."""
+        chunks = [[] for _ in range(nchunk)]
+        load = [0.0] * nchunk
+        for x in sorted(items, key=lambda x: -weight(x)):
+            j = load.index(min(load))
+            chunks[j].append(x)
+            load[j] += weight(x) + 1.0
+        return chunks
+
     def _ion_counts(self, lams):
-        """Every case's beam-ion count of the step.  The sampler costs O(lambda)
-        (one uniform per unit of the mean), so batches whose cases inject
-        thousands of ions per step run it on host threads (the C sampler
-        releases the GIL); the counts are the same either way."""
-        todo = [c for c, lam in enumerate(lams) if lam is not None]
+        """Every case's beam-ion count of the step: the sample drawn ahead
+        during the last step (_sample_ahead) where it is of this case, step
+        and rate, else drawn now.  The sampler costs O(lambda) (one uniform per
+        unit of the mean), so batches whose cases inject thousands of ions per
+        step run it on host threads (the C sampler releases the GIL); the
+        counts are the same either way."""
+        ahead = {}
+        for f in self._ahead:
+            for sim, it, lam, cnt in f.result():
+                ahead[id(sim)] = (sim, it, lam, cnt)
+        self._ahead = []
         out = [None] * len(lams)
+        todo = []
+        for c, lam in enumerate(lams):
+            if lam is None:
+                continue
+            sim = self.sims[c]
+            a = ahead.get(id(sim))
+            if a is not None and a[0] is sim and a[1] == sim.iteration and a[2] == lam:
+                out[c] = a[3]
+            else:
+                todo.append(c)
         work = sum(lams[c] for c in todo)
         if work < 20000 or len(todo) < 2:
             for c in todo:
                 out[c] = self.sims[c]._ion_injection_count(lams[c])
             return out
         pool = _count_pool()
-        nchunk = min(len(todo), pool._max_workers)
-        chunks = [[] for _ in range(nchunk)]
-        load = [0.0] * nchunk
-        for c in sorted(todo, key=lambda c: -lams[c]):  # largest first, to the lightest chunk
-            j = load.index(min(load))
-            chunks[j].append(c)
-            load[j] += lams[c] + 1.0
+        chunks = self._chunks(todo, lambda c: lams[c], min(len(todo), pool._max_workers))
 
         def run(chunk):
             return [(c, self.sims[c]._ion_injection_count(lams[c])) for c in chunk]
@@ -214,6 +237,27 @@ class BatchedSimulator:
             for c, n in res:
                 out[c] = n
         return out
+
+    def _sample_ahead(self):
+        """Draw the next step's beam-ion counts on the pool's threads.  They
+        depend only on each case's seed, step and rate, so they can be drawn
+        while this thread waits in the library (the Picard call releases the
+        GIL); the next step takes them (_ion_counts) and draws itself any it
+        does not find (a case admitted in between)."""
+        jobs = []
+        for sim, p in zip(self.sims, self.params):
+            it = sim.iteration + 1
+            lam = sim._ion_injection_lambda(p, it * sim.dt)
+            if lam is not None:
+                jobs.append((sim, it, lam))
+        if not jobs:
+            return
+        pool = _count_pool()
+
+        def run(chunk):
+            return [(sim, it, lam, sim._ion_injection_count(lam, it)) for sim, it, lam in chunk]
+        self._ahead = [pool.submit(run, ch)
+                       for ch in self._chunks(jobs, lambda j: j[2], min(len(jobs), pool._max_workers))]
 
     def _picard(self, idx):
         """Batched Picard update of the cases idx (their own even steps)."""
@@ -376,6 +420,7 @@ class BatchedSimulator:
         # ---- launches
         for dptr, cnt, max_n in inj_launch:
             check(lib.bemcs_inject_batched(dptr, cnt, max_n, s), "inject_batched")
+        self._sample_ahead()
         if poisson:
             kt = self.kernel_timing
             ev = DigitalTwinSimulator._timed_begin(kt)

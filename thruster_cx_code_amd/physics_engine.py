@@ -652,9 +652,11 @@ class DigitalTwinSimulator:
         I_ion = self.q_ion * bf * params.get('n0_plasma', 1e17) * vb * area
         return (I_ion * self.dt) / (self.q_ion * self.macro_weight)
 
-    def _ion_injection_count(self, lam):
-        """Poisson sample of the step's beam-ion count (stream P_INJ_COUNT)."""
-        return int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, self.iteration & 0xFFFFFFFF, lam))
+    def _ion_injection_count(self, lam, iteration=None):
+        """Poisson sample of the beam-ion count of this step, or of step
+        `iteration` (stream P_INJ_COUNT, keyed by the case's seed and step)."""
+        it = self.iteration if iteration is None else iteration
+        return int(self.lib.bemcs_poisson_count(self.seed, P_INJ_COUNT, it & 0xFFFFFFFF, lam))
 
     def _inject_plan(self, params, t_current, n_ion=None):
         """The step's injections as (is_ion, bemcs_inject_t, n), in launch
